@@ -1,0 +1,304 @@
+/*
+ * vr.h -- C ABI of libvrgpu.so, the MI355X (gfx950) hash-aggregation and
+ * hash-join library that sits under VecRuntime's operator API.
+ *
+ * Every entry point returns a vr_status (0 = VR_OK); on failure the message is
+ * available from vr_last_error() on the calling thread.  No torch or HIP type
+ * appears in a signature: handles are opaque, buffers are plain pointers with
+ * sizes, and a stream is passed as `void*` (a hipStream_t, or NULL for the
+ * handle's own stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   vr_agg_*   GroupKeyTable.assign + GroupedAccumulators.*.update fused
+ *              (kernels/src/main/java/io/vecruntime/kernels/GroupKeyTable.java:709,
+ *               GroupedAccumulators.java:224,380,652,1017,1212,1287), driven by
+ *              VectorGroupedAggregateIterator.consume
+ *              (spark/src/main/scala/org/apache/spark/sql/vecruntime/VectorHashAggregateExec.scala:584-602);
+ *              emission replaces GroupKeyTable.writeKeys (GroupKeyTable.java:2851)
+ *              and GroupedAggState.writeBuffer (spark/.../agg/VectorAggregates.scala:63).
+ *   vr_join_*  BuildTable.build (VectorHashJoinExec.scala:478-516), GroupKeyTable.lookup
+ *              (GroupKeyTable.java:772), VectorHashJoinIterator.emitMatches /
+ *              emitSemiAnti / flush (VectorHashJoinExec.scala:1116-1182,1605).
+ */
+#ifndef VR_H
+#define VR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+typedef int32_t vr_status;
+
+enum {
+    VR_OK = 0,
+    /* A bad argument (IllegalArgumentException in the reference). */
+    VR_ERR_INVALID_ARGUMENT = 1,
+    /* A lane or aggregate the device path does not take (IllegalStateException,
+     * GroupKeyTable.java:638).  The caller runs the CPU kernels instead; no new
+     * planning-time fallback reason is introduced. */
+    VR_ERR_UNSUPPORTED = 2,
+    /* Device memory (or the HBM budget) exhausted: the caller re-runs the batch on the CPU kernels. */
+    VR_ERR_OOM = 3,
+    /* A HIP runtime error. */
+    VR_ERR_DEVICE = 4,
+    /* A checked (ANSI) bigint SUM left the long range: Spark's ARITHMETIC_OVERFLOW
+     * (VectorAggregates.scala:142-170). */
+    VR_ERR_ARITHMETIC_OVERFLOW = 5,
+    /* An ANSI decimal result past its precision (NUMERIC_VALUE_OUT_OF_RANGE,
+     * VectorAggregates.scala:329-349). */
+    VR_ERR_DECIMAL_OVERFLOW = 6,
+    /* A caller-allocated output is too small; the size needed is reported. */
+    VR_ERR_CAPACITY = 7,
+    /* A call out of order (e.g. a probe before vr_join_build_finish). */
+    VR_ERR_STATE = 8
+};
+
+/* Message of the last failed call on this thread ("" when none). */
+const char* vr_last_error(void);
+
+/* ------------------------------------------------------------------ columns */
+
+/* Physical lanes, in the order of the reference's VecType enum (VecType.java:23-91). */
+enum vr_type {
+    VR_BOOL = 0,       /* bitmap, one bit per row, LSB first */
+    VR_INT32 = 1,
+    VR_INT64 = 2,
+    VR_FLOAT64 = 3,
+    VR_UTF8 = 4,       /* int32 offsets[n+1] + bytes, or dictionary-encoded int32 codes */
+    VR_DECIMAL128 = 5  /* 16 bytes per row: low limb at 16i, high limb at 16i+8, little endian */
+};
+
+/* Where a column's buffers live. */
+enum vr_mem { VR_MEM_HOST = 0, VR_MEM_DEVICE = 1 };
+
+/* An Arrow-layout input column (the reference's VectorBuffers, VectorBuffers.java:48-215).
+ * `validity` is NULL when no row is null.  For a dictionary-encoded UTF8 column `data` holds
+ * int32 codes and `dictionary` points at a plain UTF8 column in host memory. */
+typedef struct vr_column {
+    int32_t type;
+    int32_t mem;
+    int64_t length;
+    const uint8_t* validity;
+    const void* data;
+    const int32_t* offsets;
+    const struct vr_column* dictionary;
+} vr_column;
+
+/* A caller-allocated Arrow-layout output column.  `validity` must hold ceil(capacity/8)
+ * bytes; fixed-width `data` holds capacity * width bytes (a bitmap for VR_BOOL). */
+typedef struct vr_out_column {
+    int32_t type;
+    int32_t mem;
+    int64_t capacity;
+    uint8_t* validity;
+    void* data;
+    int32_t* offsets;
+} vr_out_column;
+
+/* ------------------------------------------------------------------ device */
+
+typedef struct vr_device vr_device;
+
+/* Opens device `ordinal` (one handle per process and device is typical; the handle is
+ * thread-safe).  `hbm_budget_bytes` = 0 means no limit beyond the device's memory. */
+vr_status vr_device_open(int32_t ordinal, int64_t hbm_budget_bytes, vr_device** out);
+vr_status vr_device_close(vr_device* dev);
+/* Bytes currently allocated by handles of this device. */
+vr_status vr_device_memory(vr_device* dev, int64_t* allocated, int64_t* budget);
+/* Waits for all work the library queued on `stream` (NULL: every stream of the device). */
+vr_status vr_device_sync(vr_device* dev, void* stream);
+
+/* Per-device counters (the reference's SQLMetrics numInputBatches/numOutputRows/time,
+ * VectorExec.scala:31-45, plus device-side figures). */
+typedef struct vr_stats {
+    int64_t calls;
+    int64_t input_rows;
+    int64_t output_rows;
+    int64_t h2d_bytes;
+    int64_t d2h_bytes;
+    int64_t kernel_launches;
+    int64_t table_grows;
+    int64_t cpu_fallbacks_signalled;
+} vr_stats;
+vr_status vr_stats_get(vr_device* dev, vr_stats* out);
+
+/* ------------------------------------------------------------------ aggregation */
+
+/* Aggregate functions in update mode (Partial / Complete), mapped from Spark's functions as
+ * VectorAggregates.compileFunction does (VectorAggregates.scala:1327-1460). */
+enum vr_agg_kind {
+    VR_AGG_COUNT_STAR = 0,   /* COUNT(*): every selected row (GroupedAccumulators.Count.updateAll) */
+    VR_AGG_COUNT = 1,        /* COUNT(x): non-null rows (Count.updateNonNull) */
+    VR_AGG_SUM_LONG = 2,     /* SUM over INT32/INT64 into bigint (SumLongAgg); VR_AGG_CHECKED = ANSI */
+    VR_AGG_SUM_DOUBLE = 3,   /* SUM over FLOAT64 (SumDoubleAgg) */
+    VR_AGG_SUM_WIDE = 4,     /* SUM over a decimal lane into Decimal(p+10,s) (WideDecimalSumAgg) */
+    VR_AGG_AVG_DOUBLE = 5,   /* AVG over FLOAT64, buffer (sum, count) (AverageAgg) */
+    VR_AGG_AVG_WIDE = 6,     /* AVG over a decimal lane, buffer (sum, count) (WideDecimalAvgAgg) */
+    VR_AGG_MIN = 7,          /* MIN over INT32/INT64/FLOAT64 (MinMaxAgg) */
+    VR_AGG_MAX = 8,          /* MAX over INT32/INT64/FLOAT64 (MinMaxAgg) */
+    /* Merge modes (PartialMerge / Final): inputs are partial buffer columns
+     * (VectorAggregates.compileMerge, VectorAggregates.scala:1141). */
+    VR_AGG_MERGE_COUNT = 16,      /* input: count bigint */
+    VR_AGG_MERGE_SUM_LONG = 17,   /* input: sum bigint (nullable) */
+    VR_AGG_MERGE_SUM_DOUBLE = 18, /* input: sum double (nullable) */
+    VR_AGG_MERGE_SUM_WIDE = 19,   /* inputs: sum DECIMAL128 (null = overflowed), isEmpty BOOL */
+    VR_AGG_MERGE_AVG_DOUBLE = 20, /* inputs: sum double, count bigint */
+    VR_AGG_MERGE_AVG_WIDE = 21,   /* inputs: sum DECIMAL128 (null = overflowed), count bigint */
+    VR_AGG_MERGE_MIN = 22,
+    VR_AGG_MERGE_MAX = 23
+};
+
+enum vr_agg_flags {
+    /* ANSI evaluation: overflow raises instead of wrapping / nulling. */
+    VR_AGG_CHECKED = 1,
+    /* Emit the final result in the first slot (Complete / Final) instead of the partial buffer. */
+    VR_AGG_RESULT = 2
+};
+
+typedef struct vr_agg_desc {
+    int32_t kind;          /* vr_agg_kind */
+    int32_t input;         /* value column index into the update's `values`, -1 for COUNT(*) */
+    int32_t input2;        /* second buffer column of a merge (isEmpty / count), else -1 */
+    int32_t flags;         /* vr_agg_flags */
+    int32_t buffer_precision; /* decimal sum buffer Decimal(p+10, s): precision; 0 otherwise */
+    int32_t buffer_scale;
+    int32_t result_precision; /* AVG over a decimal: result Decimal(p+4, s+4) */
+    int32_t result_scale;
+} vr_agg_desc;
+
+typedef struct vr_agg_spec {
+    int32_t num_keys;
+    const int32_t* key_types;     /* vr_type per key; UTF8 keys must be dictionary encoded */
+    const uint8_t* key_nullable;  /* may be NULL: every key nullable */
+    int32_t num_values;
+    const int32_t* value_types;   /* vr_type per value column */
+    int32_t num_aggs;
+    const vr_agg_desc* aggs;
+    int64_t capacity_hint;        /* expected groups; 0 = start small and grow */
+    /* 1: emit groups in first-seen order and fold double sums in row order, bit-identical to the
+     * reference's strict mode (GroupedAccumulators.DoubleSum(strict), GroupedAccumulators.java:202-206).
+     * 0: group order unspecified, double sums by device atomics (within 1e-12 relative). */
+    int32_t deterministic;
+    int32_t reserved;
+} vr_agg_spec;
+
+typedef struct vr_agg vr_agg;
+
+vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out);
+/* One batch: n rows of `keys` (num_keys columns) and `values` (num_values columns), rows not set
+ * in `selection` (a bitmap in the columns' memory, or NULL) ignored.  Asynchronous on `stream`. */
+vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* values, int64_t n,
+                        const uint8_t* selection, void* stream);
+/* Groups so far (synchronises with the handle's work). */
+vr_status vr_agg_num_groups(vr_agg* agg, int64_t* groups);
+/* Number of output columns an emission writes per aggregate, in order. */
+vr_status vr_agg_output_layout(vr_agg* agg, int32_t* num_columns, int32_t* types, int32_t max_columns);
+/* Writes groups [from, to) of the emission order: `keys` (num_keys columns, key lanes; a UTF8 key
+ * is written as its table-dictionary id in an INT32 lane, see vr_agg_key_dictionary) and `out`
+ * (the columns vr_agg_output_layout lists).  Synchronous with respect to `stream`. */
+vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys,
+                      vr_out_column* out, void* stream);
+/* The table's dictionary of UTF8 key column `c` (GroupKeyTable.dictionary, GroupKeyTable.java:2804):
+ * ids in first-seen order; offsets[count+1] and bytes, copied to host buffers. */
+vr_status vr_agg_key_dictionary(vr_agg* agg, int32_t c, int32_t* offsets, int64_t offsets_capacity,
+                                uint8_t* bytes, int64_t bytes_capacity, int64_t* count,
+                                int64_t* num_bytes);
+/* Device bytes held by the handle (GroupKeyTable.memoryBytes, GroupKeyTable.java:2596). */
+vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes);
+/* Drops every group, keeping the allocation (emit-and-reset, VectorHashAggregateExec.scala:193-246). */
+vr_status vr_agg_reset(vr_agg* agg);
+vr_status vr_agg_destroy(vr_agg* agg);
+
+/* ------------------------------------------------------------------ batched aggregation */
+
+/* Many batches (typically from concurrent partitions) folded into one handle with a single
+ * persistent launch.  Batch b's columns are keys[b*num_keys ...] and values[b*num_values ...],
+ * its length lengths[b]; a batch's optional filter is `filter_col` (an INT32 column index into
+ * its values) with predicate value < filter_threshold (filter_col < 0: none). */
+typedef struct vr_batch_set {
+    int32_t num_batches;
+    const vr_column* keys;
+    const vr_column* values;
+    const int64_t* lengths;
+    int32_t filter_col;
+    int32_t filter_threshold;
+} vr_batch_set;
+
+/* Launch shape for the persistent kernel; zeros pick the tuned defaults. */
+typedef struct vr_launch_config {
+    int32_t block_threads;
+    int32_t rows_per_thread;
+    int32_t blocks_per_cu;
+    int32_t lds_preaggregate;   /* 1: LDS pre-aggregation before global atomics, 0: off, -1: auto */
+} vr_launch_config;
+
+vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* batches,
+                                const vr_launch_config* cfg, void* stream);
+/* The same with a filter fused in front of a plain per-batch update (one launch per batch):
+ * the baseline the persistent launch is measured against. */
+vr_status vr_agg_update_filtered(vr_agg* agg, const vr_column* keys, const vr_column* values,
+                                 int64_t n, int32_t filter_col, int32_t filter_threshold,
+                                 void* stream);
+
+/* ------------------------------------------------------------------ join */
+
+enum vr_join_type { VR_JOIN_INNER = 0, VR_JOIN_LEFT_SEMI = 1, VR_JOIN_LEFT_ANTI = 2 };
+
+typedef struct vr_join_spec {
+    int32_t num_keys;
+    const int32_t* key_types;      /* INT32 / INT64 / dictionary ids */
+    int32_t num_columns;           /* build-side columns kept for the output (keys included) */
+    const int32_t* column_types;
+    const int32_t* key_columns;    /* index into the build columns of each key */
+    int64_t capacity_hint;         /* expected build rows */
+} vr_join_spec;
+
+typedef struct vr_join vr_join;
+
+vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out);
+/* Appends a build batch (ColumnBuilder.append in BuildTable.fromBatches, VectorHashJoinExec.scala:653-680). */
+vr_status vr_join_build_append(vr_join* join, const vr_column* columns, int64_t n, void* stream);
+/* Builds the table over every appended row (BuildTable.build): rows with a null key never match. */
+vr_status vr_join_build_finish(vr_join* join, void* stream);
+/* Build rows, distinct non-null keys, longest chain. */
+vr_status vr_join_info(vr_join* join, int64_t* build_rows, int64_t* distinct_keys,
+                       int64_t* max_rows_per_key);
+
+/* Output of a probe.  For VR_JOIN_INNER either `pairs` (probe_idx/build_idx int32 arrays, the
+ * reference's probeIdx/buildIdx) or materialized columns (probe_out_columns of the streamed batch
+ * then build_out_columns of the build side, in that order), or both.  For semi/anti: `pairs`'
+ * probe_idx lists the kept streamed rows and/or the streamed columns are compacted. */
+typedef struct vr_probe_output {
+    int32_t mem;                   /* vr_mem of every output buffer */
+    int64_t capacity;              /* rows the outputs hold */
+    int32_t* probe_idx;            /* may be NULL */
+    int32_t* build_idx;            /* may be NULL (always NULL for semi/anti) */
+    int32_t num_probe_out;         /* streamed columns to materialize */
+    const int32_t* probe_out_columns;
+    int32_t num_build_out;         /* build columns to materialize (inner only) */
+    const int32_t* build_out_columns;
+    vr_out_column* columns;        /* num_probe_out + num_build_out columns */
+    int64_t rows;                  /* out: rows produced (or needed, with VR_ERR_CAPACITY) */
+} vr_probe_output;
+
+/* Probes one streamed batch (`columns`: num_columns streamed columns; `key_columns`: the key
+ * column index of each join key) against the built table.  Pair order is the reference's:
+ * streamed rows in order, each row's build rows in build order.  `drop_null_keys` is the
+ * null-aware anti join (VectorHashJoinExec.scala:994,1070). */
+vr_status vr_join_probe(vr_join* join, const vr_column* columns, int32_t num_columns,
+                        const int32_t* key_columns, int64_t n, const uint8_t* selection,
+                        int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
+                        void* stream);
+vr_status vr_join_memory(vr_join* join, int64_t* bytes);
+vr_status vr_join_destroy(vr_join* join);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VR_H */

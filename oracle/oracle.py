@@ -1,0 +1,170 @@
+"""ctypes front end of the CPU oracle (oracle/libvroracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the package's smoke check and bench.py's cpu_baseline leg import this module, and
+only as the checker or the timed CPU baseline; the product path never does.  The oracle takes the
+same vr_column / vr_agg_spec / vr_join_spec structs as the device library (include/vr.h), host
+memory only.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from vecruntime_amd import abi
+from vecruntime_amd.columns import OutColumn, c_array
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvroracle.so")
+
+
+def build():
+    """Compiles the oracle (make in oracle/)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.vro_last_error.restype = C.c_char_p
+        L.vro_agg_create.argtypes = [C.POINTER(abi.VrAggSpec), C.POINTER(P)]
+        L.vro_agg_update.argtypes = [P, P, P, C.c_int64, P, P]
+        L.vro_agg_num_groups.argtypes = [P]
+        L.vro_agg_num_groups.restype = C.c_int64
+        L.vro_agg_output_layout.argtypes = [P, P, C.c_int32]
+        L.vro_agg_emit.argtypes = [P, C.c_int64, C.c_int64, P, P]
+        L.vro_agg_key_dictionary.argtypes = [P, C.c_int32, P, P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.vro_agg_destroy.argtypes = [P]
+        L.vro_join_build.argtypes = [C.POINTER(abi.VrJoinSpec), P, C.c_int64, C.POINTER(P)]
+        L.vro_join_distinct_keys.argtypes = [P]
+        L.vro_join_distinct_keys.restype = C.c_int64
+        L.vro_join_probe.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int32, C.c_int32,
+                                     C.POINTER(abi.VrProbeOutput)]
+        L.vro_join_destroy.argtypes = [P]
+        L.vro_agg_partial_threads.argtypes = [C.POINTER(abi.VrAggSpec), P, P, P, C.c_int32, C.c_int32,
+                                              C.POINTER(C.c_double)]
+        L.vro_agg_partial_threads.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def _check(st):
+    if st != abi.VR_OK:
+        raise OracleError(st, lib().vro_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleAgg:
+    """GroupKeyTable + GroupedAccumulators restated on the CPU (group ids in first-seen order)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.h = C.c_void_p()
+        _check(lib().vro_agg_create(C.byref(spec.c), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vro_agg_destroy(self.h)
+            self.h = None
+
+    def update(self, keys, values, n, selection=None):
+        ka, va = c_array(keys), c_array(values)
+        ids = np.zeros(max(1, n), np.int32)
+        _check(lib().vro_agg_update(self.h, C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p), n,
+                                    _ptr(selection), ids.ctypes.data))
+        return ids[:n]
+
+    def num_groups(self):
+        return lib().vro_agg_num_groups(self.h)
+
+    def emit(self, start=0, stop=None):
+        """Emits groups [start, stop): (key OutColumns, aggregate OutColumns)."""
+        if stop is None:
+            stop = self.num_groups()
+        m = stop - start
+        keys = [OutColumn(abi.INT32 if t == abi.UTF8 else t, m) for t in self.spec.key_types]
+        outs = [OutColumn(t, m) for t in self.spec.output_types()]
+        ka, oa = c_array(keys, abi.VrOutColumn), c_array(outs, abi.VrOutColumn)
+        _check(lib().vro_agg_emit(self.h, start, stop, C.cast(ka, C.c_void_p), C.cast(oa, C.c_void_p)))
+        return keys, outs
+
+    def key_dictionary(self, c):
+        count, nbytes = C.c_int64(), C.c_int64()
+        _check(lib().vro_agg_key_dictionary(self.h, c, None, None, C.byref(count), C.byref(nbytes)))
+        offs = np.zeros(count.value + 1, np.int32)
+        data = np.zeros(max(1, nbytes.value), np.uint8)
+        _check(lib().vro_agg_key_dictionary(self.h, c, offs.ctypes.data, data.ctypes.data,
+                                            C.byref(count), C.byref(nbytes)))
+        raw = data.tobytes()
+        return [raw[offs[i]:offs[i + 1]].decode("utf-8") for i in range(count.value)]
+
+
+class OracleJoin:
+    """BuildTable.build + the probe iterator's pair / semi / anti emission, on the CPU."""
+
+    def __init__(self, spec, columns, n):
+        self.spec = spec
+        self.columns = columns  # borrowed by the oracle: keep alive
+        self._ca = c_array(columns)
+        self.h = C.c_void_p()
+        _check(lib().vro_join_build(C.byref(spec.c), C.cast(self._ca, C.c_void_p), n, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vro_join_destroy(self.h)
+            self.h = None
+
+    def distinct_keys(self):
+        return lib().vro_join_distinct_keys(self.h)
+
+    def probe(self, columns, key_columns, n, join_type, selection=None, drop_null_keys=False,
+              probe_out=(), build_out=(), capacity=None):
+        """Returns (probe_idx, build_idx or None, output columns, rows)."""
+        if capacity is None:
+            capacity = max(1, n) * 4
+        ca = c_array(columns)
+        kc = abi.int_array(key_columns)
+        pidx = np.zeros(capacity, np.int32)
+        bidx = np.zeros(capacity, np.int32) if join_type == abi.INNER else None
+        types = [columns[c].type for c in probe_out] + [self.spec.column_types[c] for c in build_out]
+        outs = [OutColumn(t, capacity) for t in types]
+        oa = c_array(outs, abi.VrOutColumn)
+        po, bo = abi.int_array(list(probe_out)), abi.int_array(list(build_out))
+        out = abi.VrProbeOutput(abi.MEM_HOST, capacity, pidx.ctypes.data,
+                                bidx.ctypes.data if bidx is not None else None,
+                                len(probe_out), C.cast(po, C.POINTER(C.c_int32)),
+                                len(build_out), C.cast(bo, C.POINTER(C.c_int32)),
+                                C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        _check(lib().vro_join_probe(self.h, C.cast(ca, C.c_void_p), len(columns), C.cast(kc, C.c_void_p), n,
+                                    _ptr(selection), join_type, 1 if drop_null_keys else 0, C.byref(out)))
+        r = out.rows
+        return pidx[:r], (bidx[:r] if bidx is not None else None), outs, r
+
+
+def partial_threads(spec, batch_keys, batch_values, lengths, threads):
+    """Aggregates batches round-robin over `threads` independent tables (Spark tasks); returns
+    (total groups over the tables, wall seconds)."""
+    nb = len(lengths)
+    ka = c_array([c for b in batch_keys for c in b])
+    va = c_array([c for b in batch_values for c in b])
+    ln = np.asarray(lengths, np.int64)
+    secs = C.c_double()
+    g = lib().vro_agg_partial_threads(C.byref(spec.c), C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p),
+                                      ln.ctypes.data, nb, threads, C.byref(secs))
+    return g, secs.value
