@@ -89,8 +89,10 @@ typedef struct vr_column {
     const struct vr_column* dictionary;
 } vr_column;
 
-/* A caller-allocated Arrow-layout output column.  `validity` must hold ceil(capacity/8)
- * bytes; fixed-width `data` holds capacity * width bytes (a bitmap for VR_BOOL). */
+/* A caller-allocated Arrow-layout output column.  `validity` holds ceil(capacity/8) bytes padded
+ * to a multiple of 8 (Arrow pads buffers to 64 bytes; bitmaps are written in whole 8-byte words);
+ * fixed-width `data` holds capacity * width bytes (a bitmap for VR_BOOL).  A NULL `validity` means
+ * the caller needs no validity for the column. */
 typedef struct vr_out_column {
     int32_t type;
     int32_t mem;

@@ -1,0 +1,1327 @@
+// Group-by hash aggregation on gfx950: the vr_agg_* entry points of include/vr.h.
+//
+// Replaces, per batch, GroupKeyTable.assign + GroupedAccumulators.*.update (reference:
+// kernels/src/main/java/io/vecruntime/kernels/GroupKeyTable.java:709, GroupedAccumulators.java:224,
+// 380,652,1017,1212,1287) as one fused launch, and the emission GroupKeyTable.writeKeys +
+// GroupedAggState.writeBuffer (GroupKeyTable.java:2851, spark/.../agg/VectorAggregates.scala:63).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "vr_agg_kernels.h"
+
+namespace vr {
+
+// ------------------------------------------------------------------ kernels
+
+__device__ __forceinline__ int lds_slot(uint64_t* lkeys, int E, uint64_t slot) {
+    uint32_t h = (uint32_t)(fmix64(slot)) & (uint32_t)(E - 1);
+    for (int p = 0; p < 16; p++) {
+        uint64_t k = lkeys[h];
+        if (k == slot) return (int)h;
+        if (k == EMPTY_KEY) {
+            uint64_t old = atomicCAS((unsigned long long*)&lkeys[h], (unsigned long long)EMPTY_KEY,
+                                     (unsigned long long)slot);
+            if (old == EMPTY_KEY || old == slot) return (int)h;
+        }
+        h = (h + 1) & (uint32_t)(E - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void lds_reset(const UpdateArgs& a, uint64_t* lkeys, uint64_t* lstate) {
+    const int E = a.lds_entries, S = a.lds_words;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) lkeys[e] = EMPTY_KEY;
+    for (int w = threadIdx.x; w < E * S; w += blockDim.x) lstate[w] = a.ident[w % S];
+}
+
+__device__ __forceinline__ void lds_flush(const UpdateArgs& a, uint64_t* lkeys, uint64_t* lstate) {
+    const int E = a.lds_entries, S = a.lds_words;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        uint64_t slot = lkeys[e];
+        if (slot != EMPTY_KEY) flush_entry(a, lstate + (size_t)e * S, a.t.state + slot * (uint64_t)a.t.stride);
+    }
+}
+
+// Locates the batch of a tile in a batched launch (tile_begin ascending).
+__device__ __forceinline__ int find_batch(const BatchDesc* b, int nb, int64_t tile) {
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (b[mid].tile_begin <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void agg_update_kernel(UpdateArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ unsigned long long inserted_groups;
+    const int E = a.lds_entries;
+    uint64_t* lkeys = lds;
+    uint64_t* lstate = lds + E;
+    if (threadIdx.x == 0) inserted_groups = 0;
+    if (E) lds_reset(a, lkeys, lstate);
+    __syncthreads();
+    const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
+    const int64_t ntiles = MULTI ? a.total_tiles : (a.one.n + T - 1) / T;
+    const int lane = __lane_id();
+    int since_flush = 0;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const BatchDesc* bp;
+        int64_t row0;
+        if (MULTI) {
+            bp = a.batches + find_batch(a.batches, a.num_batches, tile);
+            row0 = (tile - bp->tile_begin) * T;
+        } else {
+            bp = &a.one;
+            row0 = tile * T;
+        }
+        const BatchDesc& b = *bp;
+        const int64_t n = b.n;
+        for (int r = 0; r < a.rows_per_thread; r++) {
+            int64_t i = row0 + (int64_t)r * blockDim.x + threadIdx.x;
+            bool active = i < n;
+            if (active && b.sel) active = bit_at(b.sel, i);
+            if (active && a.filter_col >= 0) {
+                const DCol& f = b.vals[a.filter_col];
+                active = is_valid(f, i) && ld_stream((const int32_t*)f.data + i) < a.filter_threshold;
+            }
+            bool ins = false;
+            uint64_t slot = ~0ull;
+            if (active) {
+                slot = row_slot_of(a, b, i, &ins, true);
+                uint64_t* grow = a.t.state + slot * (uint64_t)a.t.stride;
+                uint64_t* lrow = nullptr;
+                if (E) {
+                    int e = lds_slot(lkeys, E, slot);
+                    if (e >= 0) lrow = lstate + (size_t)e * a.lds_words;
+                }
+                apply_ops(a, b, i, lrow, grow);
+            }
+            if (!MULTI && a.row_slot && i < n) a.row_slot[i] = active ? (uint32_t)slot : 0xFFFFFFFFu;
+            uint64_t m = __ballot(ins);
+            if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&inserted_groups, (unsigned long long)__popcll(m));
+        }
+        if (E && ++since_flush >= a.flush_tiles) {
+            __syncthreads();
+            lds_flush(a, lkeys, lstate);
+            __syncthreads();
+            lds_reset(a, lkeys, lstate);
+            __syncthreads();
+            since_flush = 0;
+        }
+    }
+    __syncthreads();
+    if (E && since_flush) lds_flush(a, lkeys, lstate);
+    if (threadIdx.x == 0 && inserted_groups) atomic_add_u64(&a.t.counters[C_GROUPS], inserted_groups);
+}
+
+// Fills state rows [0, rows) with the identity row.
+__global__ void init_state_kernel(uint64_t* state, uint64_t words, int stride, UpdateArgs a) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        state[w] = a.ident[w % (uint64_t)stride];
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Moves every group of `from` into the (larger, empty) table of `a`.
+__global__ void migrate_kernel(TableView from, UpdateArgs a) {
+    const TableView& to = a.t;
+    const int W = from.width;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < from.slots + 2;
+         s += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t dst;
+        if (s >= from.slots) {
+            uint64_t bit = (s == from.slots) ? RES_SENTINEL : RES_NULL;
+            if (!(from.counters[C_RESERVED] & bit)) continue;
+            dst = to.slots + (s - from.slots);
+        } else if (a.ks.mode != KM_WIDE) {
+            uint64_t key = from.cells[s];
+            if (key == EMPTY_KEY) continue;
+            bool ins;
+            dst = find_or_insert64(to, key, &ins);
+        } else {
+            const uint64_t* c = from.cells + s * (uint64_t)W;
+            if (c[0] == 0) continue;
+            uint32_t w[4 * VR_MAX_KEYS + 1];
+            for (int i = 0; i < W; i++) w[i] = (uint32_t)c[i];
+            bool ins;
+            dst = find_or_insert_wide(to, w, W, &ins, true);
+        }
+        const uint64_t* src = from.state + s * (uint64_t)from.stride;
+        uint64_t* d = to.state + dst * (uint64_t)to.stride;
+        for (int k = 0; k < to.stride; k++) d[k] = src[k];
+    }
+}
+
+// Deterministic mode: after a radix sort of (slot, row) pairs by slot, the first position of each
+// slot's run folds that run's double values in row order onto the group's running sum -- Spark's
+// `sum += x` chain across batches (GroupedAccumulators.DoubleSum(strict), GroupedAccumulators.java:
+// 202-206,276-280).
+__global__ void ordered_fold_kernel(const uint32_t* slots, const uint32_t* rows, int64_t n, UpdateArgs a) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t s = slots[p];
+        if (s == 0xFFFFFFFFu || (p > 0 && slots[p - 1] == s)) continue;
+        uint64_t* g = a.t.state + (uint64_t)s * a.t.stride;
+        for (int k = 0; k < a.nops; k++) {
+            const Op& op = a.ops[k];
+            if (op.code != OP_SUM_F64) continue;
+            const DCol& c = a.one.vals[op.col];
+            double acc = __longlong_as_double((long long)g[op.off]);
+            for (int64_t q = p; q < n && slots[q] == s; q++) {
+                uint32_t i = rows[q];
+                if (is_valid(c, i)) acc += ((const double*)c.data)[i];
+            }
+            g[op.off] = (uint64_t)__double_as_longlong(acc);
+        }
+    }
+}
+
+__global__ void iota_u32_kernel(uint32_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+// Records the payload of each group's first NaN (the NaN a tie keeps), for the updates that saw one.
+__device__ __forceinline__ void nan_payload_rows(const UpdateArgs& a, const BatchDesc& b, int64_t i0, int64_t i1,
+                                                 int64_t step) {
+    for (int64_t i = i0; i < i1; i += step) {
+        if (b.sel && !bit_at(b.sel, i)) continue;
+        if (a.filter_col >= 0) {
+            const DCol& f = b.vals[a.filter_col];
+            if (!is_valid(f, i) || ((const int32_t*)f.data)[i] >= a.filter_threshold) continue;
+        }
+        uint64_t slot = ~0ull;
+        for (int k = 0; k < a.nops; k++) {
+            const Op& op = a.ops[k];
+            if (op.code != OP_MIN_F64 && op.code != OP_MAX_F64) continue;
+            const DCol& c = b.vals[op.col];
+            if (!is_valid(c, i)) continue;
+            uint64_t bits = ((const uint64_t*)c.data)[i];
+            if (!is_nan_bits(bits)) continue;
+            if (slot == ~0ull) {
+                bool ins;
+                slot = row_slot_of(a, b, i, &ins, false);
+            }
+            uint64_t* w = a.t.state + slot * (uint64_t)a.t.stride + op.off;
+            if (w[2] == (uint64_t)(b.row_base + i)) w[3] = bits;
+        }
+    }
+}
+
+template <bool MULTI>
+__global__ void nan_payload_kernel(UpdateArgs a) {
+    if (a.t.counters[C_NAN] != (uint64_t)a.epoch) return;
+    if (!MULTI) {
+        nan_payload_rows(a, a.one, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a.one.n,
+                         (int64_t)gridDim.x * blockDim.x);
+        return;
+    }
+    const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
+    for (int64_t tile = blockIdx.x; tile < a.total_tiles; tile += gridDim.x) {
+        const BatchDesc& b = a.batches[find_batch(a.batches, a.num_batches, tile)];
+        int64_t row0 = (tile - b.tile_begin) * T;
+        int64_t row1 = row0 + T < b.n ? row0 + T : b.n;
+        nan_payload_rows(a, b, row0 + threadIdx.x, row1, blockDim.x);
+    }
+}
+
+struct OccupiedSlot {
+    const uint64_t* cells;
+    int width;
+    __host__ __device__ bool operator()(const uint64_t& s) const {
+        uint64_t c = cells[s * (uint64_t)width];
+        return width == 1 ? c != EMPTY_KEY : c != 0;
+    }
+};
+
+__global__ void first_rows_kernel(const uint64_t* order, int64_t m, const uint64_t* state, int stride, int off,
+                                  uint64_t* out) {
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < m; o += (int64_t)gridDim.x * blockDim.x)
+        out[o] = state[order[o] * (uint64_t)stride + off];
+}
+
+// ------------------------------------------------------------------ emission kernel
+
+struct OutCol {
+    uint8_t* valid;
+    void* data;
+    int32_t type;
+    int32_t pad;
+};
+
+struct EmitAgg {
+    int32_t kind, flags, op_code, off;
+    int32_t in_type, out0, bp, bs;
+    int32_t rp, rs, bit, pad;
+};
+
+struct EmitArgs {
+    TableView t;
+    KeySpec ks;
+    const uint64_t* order;
+    int64_t from, m;
+    EmitAgg aggs[VR_MAX_AGGS];
+    int32_t naggs, flags_off;
+    OutCol keys[VR_MAX_KEYS];
+    OutCol outs[2 * VR_MAX_AGGS];
+};
+
+// Writes row o's bit of a bitmap: the wave's 64 bits are gathered with a ballot and every eighth
+// lane stores one byte (rows of a wave are consecutive and start on a multiple of 64).
+__device__ __forceinline__ void put_bit(uint8_t* bm, int64_t o, bool v, bool in_range) {
+    uint64_t mask = __ballot(v && in_range);
+    int lane = __lane_id();
+    if (bm && in_range && (lane & 7) == 0) bm[o >> 3] = (uint8_t)(mask >> lane);
+}
+
+__device__ __forceinline__ void put_i64(const OutCol& c, int64_t o, bool ok, int64_t v, bool in) {
+    put_bit(c.valid, o, ok, in);
+    if (in) ((int64_t*)c.data)[o] = ok ? v : 0;
+}
+__device__ __forceinline__ void put_d128(const OutCol& c, int64_t o, bool ok, uint64_t lo, uint64_t hi, bool in) {
+    put_bit(c.valid, o, ok, in);
+    if (in) {
+        ((uint64_t*)c.data)[2 * o] = ok ? lo : 0;
+        ((uint64_t*)c.data)[2 * o + 1] = ok ? hi : 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = o < a.m;
+    uint64_t slot = in ? a.order[a.from + o] : 0;
+    const uint64_t* g = a.t.state + slot * (uint64_t)a.t.stride;
+    // ---- keys
+    const KeySpec& ks = a.ks;
+    uint32_t w[4 * VR_MAX_KEYS + 1];
+    uint64_t key64 = 0;
+    bool null64 = false;
+    if (in) {
+        if (ks.mode == KM_WIDE) {
+            const uint64_t* c = a.t.cells + slot * (uint64_t)a.t.width;
+            for (int i = 0; i < ks.width; i++) w[i] = (uint32_t)c[i];
+        } else if (slot == a.t.slots) {
+            key64 = EMPTY_KEY;
+        } else if (slot == a.t.slots + 1) {
+            null64 = true;
+        } else {
+            key64 = a.t.cells[slot];
+        }
+    }
+    for (int c = 0; c < ks.nkeys; c++) {
+        const OutCol& out = a.keys[c];
+        bool isnull = false;
+        uint64_t lo = 0, hi = 0;
+        if (ks.mode == KM_I64) {
+            isnull = null64;
+            lo = key64;
+        } else if (ks.mode == KM_I32) {
+            isnull = key64 == (1ull << 32);
+            lo = (uint32_t)key64;
+        } else if (ks.mode == KM_2X32) {
+            lo = c == 0 ? (key64 >> 32) : (uint32_t)key64;
+        } else {
+            isnull = ks.null_word >= 0 && ((w[ks.null_word] >> c) & 1u);
+            int o2 = ks.word_off[c];
+            switch (ks.types[c]) {
+                case VR_INT64:
+                case VR_FLOAT64: lo = (uint64_t)w[o2] | ((uint64_t)w[o2 + 1] << 32); break;
+                case VR_DECIMAL128:
+                    lo = (uint64_t)w[o2] | ((uint64_t)w[o2 + 1] << 32);
+                    hi = (uint64_t)w[o2 + 2] | ((uint64_t)w[o2 + 3] << 32);
+                    break;
+                default: lo = w[o2];
+            }
+        }
+        if (!in) isnull = true;
+        put_bit(out.valid, o, !isnull, in);
+        switch (ks.types[c]) {
+            case VR_BOOL: put_bit((uint8_t*)out.data, o, !isnull && lo != 0, in); break;
+            case VR_INT32:
+            case VR_UTF8:
+                if (in) ((int32_t*)out.data)[o] = isnull ? 0 : (int32_t)(uint32_t)lo;
+                break;
+            case VR_INT64:
+            case VR_FLOAT64:
+                if (in) ((uint64_t*)out.data)[o] = isnull ? 0 : lo;
+                break;
+            case VR_DECIMAL128:
+                if (in) {
+                    ((uint64_t*)out.data)[2 * o] = isnull ? 0 : lo;
+                    ((uint64_t*)out.data)[2 * o + 1] = isnull ? 0 : hi;
+                }
+                break;
+        }
+    }
+    // ---- aggregates
+    uint64_t err = 0;
+    for (int k = 0; k < a.naggs; k++) {
+        const EmitAgg& e = a.aggs[k];
+        const OutCol& o0 = a.outs[e.out0];
+        const OutCol& o1 = a.outs[e.out0 + 1];
+        const bool result = e.flags & VR_AGG_RESULT;
+        const bool ansi = e.flags & VR_AGG_CHECKED;
+        const uint64_t* s = g + e.off;
+        switch (e.kind) {
+            case VR_AGG_COUNT_STAR:
+            case VR_AGG_COUNT: put_i64(o0, o, in, in ? (int64_t)s[0] : 0, in); break;
+            case VR_AGG_SUM_LONG: {
+                uint64_t lo = in ? s[0] : 0, hi = in ? s[1] : 0, cnt = in ? s[2] : 0;
+                bool fits = hi == (uint64_t)((int64_t)lo >> 63);
+                if (ansi && cnt && !fits) err |= ERR_LONG_OVERFLOW;
+                put_i64(o0, o, cnt > 0, (int64_t)lo, in);
+                break;
+            }
+            case VR_AGG_SUM_DOUBLE: {
+                uint64_t cnt = in ? s[1] : 0;
+                put_i64(o0, o, cnt > 0, in ? (int64_t)s[0] : 0, in);
+                break;
+            }
+            case VR_AGG_AVG_DOUBLE: {
+                uint64_t cnt = in ? s[1] : 0;
+                double sum = in ? __longlong_as_double((long long)s[0]) : 0.0;
+                if (result) {
+                    double q = cnt ? sum / (double)cnt : 0.0;
+                    put_i64(o0, o, cnt > 0, __double_as_longlong(q), in);
+                } else {
+                    put_i64(o0, o, true, __double_as_longlong(sum), in);
+                    put_i64(o1, o, true, (int64_t)cnt, in);
+                }
+                break;
+            }
+            case VR_AGG_SUM_WIDE:
+            case VR_AGG_AVG_WIDE: {
+                bool wide3 = e.op_code == OP_SUM_I192;
+                uint64_t lo = in ? s[0] : 0, hi = in ? s[1] : 0;
+                uint64_t cnt = in ? s[wide3 ? 3 : 2] : 0;
+                bool over = wide3 && in && s[2] != (uint64_t)((int64_t)hi >> 63);
+                if (!over) over = past_precision(lo, hi, e.bp);
+                if (e.kind == VR_AGG_SUM_WIDE) {
+                    if (result) {
+                        if (cnt && over && ansi) err |= ERR_DECIMAL_OVERFLOW;
+                        put_d128(o0, o, cnt && !over, lo, hi, in);
+                    } else {
+                        put_d128(o0, o, cnt == 0 || !over, cnt ? lo : 0, cnt ? hi : 0, in);
+                        put_bit(o1.valid, o, true, in);
+                        put_bit((uint8_t*)o1.data, o, cnt == 0, in);
+                    }
+                } else if (!result) {
+                    put_d128(o0, o, cnt == 0 || !over, cnt ? lo : 0, cnt ? hi : 0, in);
+                    put_i64(o1, o, true, (int64_t)cnt, in);
+                } else {
+                    bool ok = cnt > 0;
+                    uint64_t qlo = 0, qhi = 0;
+                    if (ok && over) {
+                        if (ansi) err |= ERR_AVG_OF_OVERFLOW;
+                        ok = false;
+                    }
+                    if (ok) {
+                        bool fits = decimal_avg(lo, hi, cnt, e.rs - e.bs, &qlo, &qhi);
+                        if (!fits || past_precision(qlo, qhi, e.rp)) {
+                            if (ansi) err |= ERR_DECIMAL_OVERFLOW;
+                            ok = false;
+                        }
+                    }
+                    if (e.rp > 18) put_d128(o0, o, ok, qlo, qhi, in);
+                    else put_i64(o0, o, ok, (int64_t)qlo, in);
+                }
+                break;
+            }
+            case VR_AGG_MIN:
+            case VR_AGG_MAX: {
+                bool isMin = e.kind == VR_AGG_MIN;
+                if (e.in_type == VR_INT32) {
+                    int64_t v = in ? (int64_t)s[0] : 0;
+                    bool has = in && v != (isMin ? INT64_MAX : INT64_MIN);
+                    put_bit(o0.valid, o, has, in);
+                    if (in) ((int32_t*)o0.data)[o] = has ? (int32_t)v : 0;
+                } else if (e.in_type == VR_INT64) {
+                    bool has = in && ((g[a.flags_off] >> e.bit) & 1ull);
+                    put_i64(o0, o, has, has ? (int64_t)(s[0] ^ 0x8000000000000000ull) : 0, in);
+                } else {
+                    uint64_t code = in ? s[0] : 0;
+                    bool has = in && code != (isMin ? ~0ull : 0ull);
+                    uint64_t bits = unorder_f64(code);
+                    if (has && bits == 0) bits = (s[1] & 1ull) << 63;  // the first-seen zero's sign
+                    else if (has && is_nan_bits(bits)) bits = s[3];    // the first-seen NaN's payload
+                    put_i64(o0, o, has, (int64_t)bits, in);
+                }
+                break;
+            }
+        }
+    }
+    if (err) atomicOr((unsigned long long*)&a.t.counters[C_ERROR], (unsigned long long)err);
+}
+
+// ------------------------------------------------------------------ host side
+
+static uint64_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+static int key_words(int32_t type) {
+    switch (type) {
+        case VR_INT64:
+        case VR_FLOAT64: return 2;
+        case VR_DECIMAL128: return 4;
+        default: return 1;
+    }
+}
+
+struct AggImpl {
+    Device* dev = nullptr;
+    std::vector<int32_t> key_types, value_types;
+    std::vector<uint8_t> key_nullable;
+    std::vector<vr_agg_desc> aggs;
+    bool det = false;
+
+    UpdateArgs proto{};  // ops, key spec, identity row
+    int stride = 0;
+    std::vector<EmitAgg> emits;
+    std::vector<int32_t> out_types;
+
+    // table
+    DevBuf cells, state, counters;
+    uint64_t slots = 0;
+    double max_load = 0.5;
+    int64_t groups_upper = 0;  // an upper bound of the groups (exact after a sync)
+    int64_t rows_seen = 0;
+    int32_t epoch = 0;
+    bool has_f64_minmax = false;
+
+    std::vector<HostDictionary> dicts;
+    Staging staging;
+    DevBuf ids_stage;  // host-mapped UTF8 ids (pinned copies go through `staging`)
+
+    // emission
+    DevBuf order, order_tmp, firsts, firsts_sorted, cub_tmp, out_stage;
+    int64_t order_count = -1;
+
+    // deterministic fold
+    DevBuf row_slot, sort_keys, sort_vals, sort_vals_in;
+
+    // batched launches
+    DevBuf batch_descs;
+
+    vr_launch_config cfg{256, 4, 4, -1};
+    int lds_entries = 512;
+};
+
+static vr_status compile_spec(AggImpl& h) {
+    UpdateArgs& p = h.proto;
+    std::memset(&p, 0, sizeof(p));
+    const int nk = (int)h.key_types.size();
+    if (nk < 1 || nk > VR_MAX_KEYS) return fail(VR_ERR_UNSUPPORTED, "between 1 and 8 grouping keys are supported");
+    if ((int)h.value_types.size() > VR_MAX_VALUES) return fail(VR_ERR_UNSUPPORTED, "at most 16 value columns");
+    KeySpec& ks = p.ks;
+    ks.nkeys = nk;
+    bool any_nullable = false;
+    int words = 0, narrow = 0;
+    for (int c = 0; c < nk; c++) {
+        int32_t t = h.key_types[c];
+        if (t < VR_BOOL || t > VR_DECIMAL128) return fail(VR_ERR_UNSUPPORTED, "key lane type " + std::to_string(t));
+        ks.types[c] = t;
+        ks.word_off[c] = words;
+        words += key_words(t);
+        if (key_words(t) == 1) narrow++;
+        if (h.key_nullable[c]) any_nullable = true;
+    }
+    if (nk == 1 && (h.key_types[0] == VR_INT64 || h.key_types[0] == VR_FLOAT64)) ks.mode = KM_I64;
+    else if (nk == 1 && narrow == 1) ks.mode = KM_I32;
+    else if (nk == 2 && narrow == 2 && !any_nullable) ks.mode = KM_2X32;
+    else ks.mode = KM_WIDE;
+    ks.null_word = -1;
+    if (ks.mode == KM_WIDE) {
+        if (any_nullable) ks.null_word = words++;
+        ks.width = words;
+    } else {
+        ks.width = 1;
+    }
+    // operations, shared where two aggregates need the same state
+    std::vector<Op> ops;
+    int off = 0;
+    std::vector<uint64_t> ident;
+    auto add_op = [&](int32_t code, int32_t col, int nwords) -> int {
+        for (size_t k = 0; k < ops.size(); k++)
+            if (ops[k].code == code && ops[k].col == col) return (int)k;
+        Op op{};
+        op.code = code;
+        op.col = col;
+        op.off = off;
+        op.lds = 1;
+        ops.push_back(op);
+        for (int i = 0; i < nwords; i++) ident.push_back(0);
+        switch (code) {
+            case OP_MIN_I32: ident[off] = (uint64_t)INT64_MAX; break;
+            case OP_MAX_I32: ident[off] = (uint64_t)INT64_MIN; break;
+            case OP_MIN_I64: ident[off] = ~0ull; break;
+            case OP_MIN_F64:
+                ident[off] = ~0ull;
+                ident[off + 1] = ~0ull;
+                ident[off + 2] = ~0ull;
+                break;
+            case OP_MAX_F64:
+                ident[off + 1] = ~0ull;
+                ident[off + 2] = ~0ull;
+                break;
+            case OP_FIRST_ROW: ident[off] = ~0ull; break;
+        }
+        off += nwords;
+        return (int)ops.size() - 1;
+    };
+    int min_max_i64 = 0;
+    h.emits.clear();
+    h.out_types.clear();
+    for (auto& d : h.aggs) {
+        EmitAgg e{};
+        e.kind = d.kind;
+        e.flags = d.flags;
+        e.bp = d.buffer_precision;
+        e.bs = d.buffer_scale;
+        e.rp = d.result_precision;
+        e.rs = d.result_scale;
+        e.out0 = (int)h.out_types.size();
+        int32_t in_type = -1;
+        if (d.kind != VR_AGG_COUNT_STAR) {
+            if (d.input < 0 || d.input >= (int)h.value_types.size())
+                return fail(VR_ERR_INVALID_ARGUMENT, "aggregate input column out of range");
+            in_type = h.value_types[d.input];
+        }
+        e.in_type = in_type;
+        bool result = d.flags & VR_AGG_RESULT;
+        int k = -1;
+        switch (d.kind) {
+            case VR_AGG_COUNT_STAR:
+                k = add_op(OP_COUNT_STAR, -1, 1);
+                h.out_types.push_back(VR_INT64);
+                break;
+            case VR_AGG_COUNT:
+                k = add_op(OP_COUNT, d.input, 1);
+                h.out_types.push_back(VR_INT64);
+                break;
+            case VR_AGG_SUM_LONG:
+                if (in_type != VR_INT32 && in_type != VR_INT64) return fail(VR_ERR_UNSUPPORTED, "sum over a non-integral lane");
+                k = add_op(OP_SUM_I128, d.input, 3);
+                h.out_types.push_back(VR_INT64);
+                break;
+            case VR_AGG_SUM_DOUBLE:
+            case VR_AGG_AVG_DOUBLE:
+                if (in_type != VR_FLOAT64) return fail(VR_ERR_UNSUPPORTED, "double sum over a non-double lane");
+                k = add_op(OP_SUM_F64, d.input, 2);
+                h.out_types.push_back(VR_FLOAT64);
+                if (d.kind == VR_AGG_AVG_DOUBLE && !result) h.out_types.push_back(VR_INT64);
+                break;
+            case VR_AGG_SUM_WIDE:
+            case VR_AGG_AVG_WIDE:
+                if (in_type == VR_INT32 || in_type == VR_INT64) k = add_op(OP_SUM_I128, d.input, 3);
+                else if (in_type == VR_DECIMAL128) k = add_op(OP_SUM_I192, d.input, 4);
+                else return fail(VR_ERR_UNSUPPORTED, "decimal sum over a non-integral lane");
+                if (d.kind == VR_AGG_SUM_WIDE) {
+                    h.out_types.push_back(VR_DECIMAL128);
+                    if (!result) h.out_types.push_back(VR_BOOL);
+                } else if (result) {
+                    if (d.result_scale < d.buffer_scale) return fail(VR_ERR_UNSUPPORTED, "result scale below the buffer scale");
+                    h.out_types.push_back(d.result_precision > 18 ? VR_DECIMAL128 : VR_INT64);
+                } else {
+                    h.out_types.push_back(VR_DECIMAL128);
+                    h.out_types.push_back(VR_INT64);
+                }
+                break;
+            case VR_AGG_MIN:
+            case VR_AGG_MAX: {
+                bool mn = d.kind == VR_AGG_MIN;
+                if (in_type == VR_INT32) k = add_op(mn ? OP_MIN_I32 : OP_MAX_I32, d.input, 1);
+                else if (in_type == VR_INT64) {
+                    k = add_op(mn ? OP_MIN_I64 : OP_MAX_I64, d.input, 1);
+                    if (ops[k].bit == 0 && min_max_i64 < 63) ops[k].bit = min_max_i64++;
+                } else if (in_type == VR_FLOAT64) {
+                    k = add_op(mn ? OP_MIN_F64 : OP_MAX_F64, d.input, 4);
+                    h.has_f64_minmax = true;
+                } else {
+                    return fail(VR_ERR_UNSUPPORTED, "min/max over this lane runs on the CPU kernels");
+                }
+                h.out_types.push_back(in_type);
+                break;
+            }
+            default:
+                return fail(VR_ERR_UNSUPPORTED, "aggregate kind " + std::to_string(d.kind) + " runs on the CPU kernels");
+        }
+        e.op_code = ops[k].code;
+        e.off = ops[k].off;
+        e.bit = ops[k].bit;
+        h.emits.push_back(e);
+    }
+    if ((int)h.emits.size() > VR_MAX_AGGS) return fail(VR_ERR_UNSUPPORTED, "at most 24 aggregates");
+    if (h.det) add_op(OP_FIRST_ROW, -1, 1);
+    p.flags_off = -1;
+    if (min_max_i64 > 0) {
+        p.flags_off = off;
+        ident.push_back(0);
+        off++;
+    }
+    if (off == 0) {  // keys only (a DISTINCT): one padding word keeps the rows addressable
+        ident.push_back(0);
+        off = 1;
+    }
+    if ((int)ops.size() > VR_MAX_OPS || off > 64) return fail(VR_ERR_UNSUPPORTED, "aggregate state too wide");
+    for (size_t k = 0; k < ops.size(); k++) p.ops[k] = ops[k];
+    p.nops = (int)ops.size();
+    for (int i = 0; i < off; i++) p.ident[i] = ident[i];
+    h.stride = off;
+    p.det = h.det ? 1 : 0;
+    p.filter_col = -1;
+    return VR_OK;
+}
+
+static TableView view_of(AggImpl& h) {
+    TableView t{};
+    t.cells = h.cells.as<uint64_t>();
+    t.state = h.state.as<uint64_t>();
+    t.counters = h.counters.as<uint64_t>();
+    t.slots = h.slots;
+    t.mask = h.slots - 1;
+    t.width = h.proto.ks.width;
+    t.stride = h.stride;
+    return t;
+}
+
+static int grid_for(uint64_t work, int block, int cap) {
+    uint64_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (uint64_t)cap) g = (uint64_t)cap;
+    return (int)g;
+}
+
+// Allocates an empty table of `slots` slots (cells empty, state at identity) into the given buffers.
+static vr_status alloc_table(AggImpl& h, uint64_t slots, DevBuf& cells, DevBuf& state, hipStream_t s) {
+    const int W = h.proto.ks.width;
+    VR_TRY(cells.ensure(h.dev, slots * W * sizeof(uint64_t)));
+    VR_TRY(state.ensure(h.dev, (slots + 2) * h.stride * sizeof(uint64_t)));
+    uint64_t empty = h.proto.ks.mode == KM_WIDE ? 0ull : EMPTY_KEY;
+    int cap = h.dev->num_cus * 16;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(slots * W, 256, cap)), dim3(256), 0, s, cells.as<uint64_t>(),
+                       (uint64_t)(slots * W), empty);
+    hipLaunchKernelGGL(init_state_kernel, dim3(grid_for((slots + 2) * h.stride, 256, cap)), dim3(256), 0, s,
+                       state.as<uint64_t>(), (uint64_t)((slots + 2) * h.stride), h.stride, h.proto);
+    VR_TRY_HIP(hipGetLastError());
+    return VR_OK;
+}
+
+static vr_status read_counters(AggImpl& h, uint64_t* out, hipStream_t s) {
+    VR_TRY_HIP(hipMemcpyAsync(out, h.counters.p, C_WORDS * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    return VR_OK;
+}
+
+static int64_t groups_of(const uint64_t* c) {
+    return (int64_t)c[C_GROUPS] + ((c[C_RESERVED] & RES_SENTINEL) ? 1 : 0) + ((c[C_RESERVED] & RES_NULL) ? 1 : 0);
+}
+
+// Makes room for `n` more rows: grows (and migrates) the table when the groups could pass the
+// load limit.  The group count is read back only when the cheap upper bound says it might.
+static vr_status ensure_capacity(AggImpl& h, int64_t n, hipStream_t s) {
+    if ((double)(h.groups_upper + n) <= h.max_load * (double)h.slots) return VR_OK;
+    uint64_t c[C_WORDS];
+    VR_TRY(read_counters(h, c, s));
+    int64_t g = groups_of(c);
+    h.groups_upper = g;
+    if ((double)(g + n) <= h.max_load * (double)h.slots) return VR_OK;
+    uint64_t want = next_pow2((uint64_t)((double)(g + n) / h.max_load) + 1);
+    if (want > (1ull << 31)) return fail(VR_ERR_UNSUPPORTED, "more than 2^30 groups run on the CPU kernels");
+    DevBuf ncells, nstate;
+    VR_TRY(alloc_table(h, want, ncells, nstate, s));
+    TableView from = view_of(h);
+    UpdateArgs a = h.proto;
+    a.t = from;
+    a.t.cells = ncells.as<uint64_t>();
+    a.t.state = nstate.as<uint64_t>();
+    a.t.slots = want;
+    a.t.mask = want - 1;
+    // the new table's group count is rebuilt by the migration's inserts only for bookkeeping: reset it
+    hipLaunchKernelGGL(migrate_kernel, dim3(grid_for(h.slots + 2, 256, h.dev->num_cus * 16)), dim3(256), 0, s, from, a);
+    VR_TRY_HIP(hipGetLastError());
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    std::swap(h.cells.p, ncells.p);
+    std::swap(h.cells.bytes, ncells.bytes);
+    std::swap(h.state.p, nstate.p);
+    std::swap(h.state.bytes, nstate.bytes);
+    h.slots = want;
+    h.dev->count(&vr_stats::table_grows, 1);
+    return VR_OK;
+}
+
+static vr_status init_table(AggImpl& h, int64_t capacity_hint) {
+    uint64_t want = next_pow2((uint64_t)((double)std::max<int64_t>(capacity_hint, 1024) / h.max_load) + 1);
+    if (want < 2048) want = 2048;
+    hipStream_t s = h.dev->stream;
+    VR_TRY(alloc_table(h, want, h.cells, h.state, s));
+    VR_TRY(h.counters.ensure(h.dev, C_WORDS * sizeof(uint64_t)));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
+    VR_TRY_HIP(hipGetLastError());
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    h.slots = want;
+    return VR_OK;
+}
+
+// ---- columns -> device descriptors (staging host buffers)
+
+struct StagePlan {
+    struct Piece {
+        const void* src;
+        size_t bytes;
+        const void** dst;  // device pointer to patch
+    };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    void add(const void* src, size_t bytes, const void** dst) {
+        if (!src || !bytes) return;
+        pieces.push_back({src, bytes, dst});
+        total = align_up(total, 256) + bytes;
+    }
+};
+
+static vr_status map_utf8_key(AggImpl& h, int c, const vr_column& col, int64_t n, DCol& d, StagePlan& plan,
+                              std::vector<int32_t>& ids_store) {
+    HostDictionary& dict = h.dicts[c];
+    if (col.dictionary) {
+        const vr_column& dc = *col.dictionary;
+        if (dc.mem != VR_MEM_HOST || dc.type != VR_UTF8 || !dc.offsets)
+            return fail(VR_ERR_UNSUPPORTED, "a key dictionary must be a plain UTF8 column in host memory");
+        if (dict.last_dict != dc.data || dict.last_len != dc.length || dict.last_hash != (uint64_t)(uintptr_t)dc.offsets) {
+            dict.host_remap.assign((size_t)std::max<int64_t>(dc.length, 1), -1);
+            for (int64_t e = 0; e < dc.length; e++) {
+                if (dc.validity && !((dc.validity[e >> 3] >> (e & 7)) & 1)) continue;
+                int32_t s0 = dc.offsets[e], s1 = dc.offsets[e + 1];
+                dict.host_remap[e] = dict.index_of((const char*)dc.data + s0, s1 - s0, true);
+            }
+            VR_TRY_HIP(hipStreamSynchronize(h.dev->stream));  // a kernel may still read the old map
+            VR_TRY(dict.remap.ensure(h.dev, dict.host_remap.size() * sizeof(int32_t)));
+            VR_TRY_HIP(hipMemcpy(dict.remap.p, dict.host_remap.data(), dict.host_remap.size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice));
+            dict.last_dict = dc.data;
+            dict.last_len = dc.length;
+            dict.last_hash = (uint64_t)(uintptr_t)dc.offsets;
+        }
+        d.remap = dict.remap.as<int32_t>();
+        d.remap_len = (int32_t)dict.host_remap.size();
+        d.data = col.data;
+        if (col.mem == VR_MEM_HOST) plan.add(col.data, (size_t)n * 4, &d.data);
+        return VR_OK;
+    }
+    if (col.mem != VR_MEM_HOST) return fail(VR_ERR_UNSUPPORTED, "plain UTF8 keys in device memory run on the CPU kernels");
+    ids_store.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        if (col.validity && !((col.validity[i >> 3] >> (i & 7)) & 1)) {
+            ids_store[i] = 0;
+            continue;
+        }
+        int32_t s0 = col.offsets[i], s1 = col.offsets[i + 1];
+        ids_store[i] = dict.index_of((const char*)col.data + s0, s1 - s0, true);
+    }
+    d.remap = nullptr;
+    d.data = ids_store.data();
+    plan.add(ids_store.data(), (size_t)n * 4, &d.data);
+    return VR_OK;
+}
+
+static vr_status describe(AggImpl& h, const vr_column* keys, const vr_column* values, int64_t n,
+                          const uint8_t* selection, BatchDesc& b, StagePlan& plan,
+                          std::vector<std::vector<int32_t>>& id_store, int32_t* sel_mem) {
+    std::memset(&b, 0, sizeof(b));
+    b.n = n;
+    int32_t mem = -1;
+    auto note_mem = [&](int32_t m) -> vr_status {
+        if (m != VR_MEM_HOST && m != VR_MEM_DEVICE) return fail(VR_ERR_INVALID_ARGUMENT, "bad vr_mem");
+        return VR_OK;
+    };
+    const int nk = (int)h.key_types.size(), nv = (int)h.value_types.size();
+    id_store.resize(nk);
+    for (int c = 0; c < nk; c++) {
+        const vr_column& col = keys[c];
+        VR_TRY(note_mem(col.mem));
+        if (col.type != h.key_types[c]) return fail(VR_ERR_INVALID_ARGUMENT, "key " + std::to_string(c) + " lane type differs from the spec");
+        if (col.length < n) return fail(VR_ERR_INVALID_ARGUMENT, "key column shorter than the batch");
+        if (col.validity && !h.key_nullable[c]) return fail(VR_ERR_INVALID_ARGUMENT, "nulls in a key declared not nullable");
+        DCol& d = b.keys[c];
+        d.type = col.type;
+        d.valid = col.validity;
+        if (col.mem == VR_MEM_HOST) plan.add(col.validity, bitmap_bytes(n), (const void**)&d.valid);
+        if (col.type == VR_UTF8) {
+            VR_TRY(map_utf8_key(h, c, col, n, d, plan, id_store[c]));
+        } else {
+            d.data = col.data;
+            if (col.mem == VR_MEM_HOST) plan.add(col.data, lane_bytes(col.type, n), &d.data);
+        }
+        mem = col.mem;
+    }
+    for (int c = 0; c < nv; c++) {
+        const vr_column& col = values[c];
+        VR_TRY(note_mem(col.mem));
+        if (col.type != h.value_types[c]) return fail(VR_ERR_INVALID_ARGUMENT, "value " + std::to_string(c) + " lane type differs from the spec");
+        if (col.length < n) return fail(VR_ERR_INVALID_ARGUMENT, "value column shorter than the batch");
+        DCol& d = b.vals[c];
+        d.type = col.type;
+        d.valid = col.validity;
+        d.data = col.data;
+        if (col.mem == VR_MEM_HOST) {
+            plan.add(col.validity, bitmap_bytes(n), (const void**)&d.valid);
+            plan.add(col.data, lane_bytes(col.type, n), &d.data);
+        }
+        mem = col.mem;
+    }
+    b.sel = selection;
+    if (selection && mem == VR_MEM_HOST) plan.add(selection, bitmap_bytes(n), (const void**)&b.sel);
+    *sel_mem = mem;
+    return VR_OK;
+}
+
+static vr_status stage(AggImpl& h, StagePlan& plan, hipStream_t s, int* slot) {
+    *slot = -1;
+    if (plan.pieces.empty()) return VR_OK;
+    uint8_t *host, *dev;
+    VR_TRY(h.staging.begin(h.dev, plan.total, &host, &dev, slot));
+    size_t off = 0;
+    for (auto& p : plan.pieces) {
+        off = align_up(off, 256);
+        std::memcpy(host + off, p.src, p.bytes);
+        *p.dst = dev + off;
+        off += p.bytes;
+    }
+    VR_TRY(h.staging.upload(*slot, plan.total, s));
+    return VR_OK;
+}
+
+static void launch_shape(AggImpl& h, const vr_launch_config* cfg, int64_t n, UpdateArgs& a, int* grid,
+                         int* block, size_t* lds_bytes, bool multi) {
+    vr_launch_config c = h.cfg;
+    if (cfg) {
+        if (cfg->block_threads) c.block_threads = cfg->block_threads;
+        if (cfg->rows_per_thread) c.rows_per_thread = cfg->rows_per_thread;
+        if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
+        if (cfg->lds_preaggregate >= 0) c.lds_preaggregate = cfg->lds_preaggregate;
+    }
+    *block = 256;  // the kernel's launch bounds
+    a.rows_per_thread = std::max(1, c.rows_per_thread);
+    int64_t T = (int64_t)(*block) * a.rows_per_thread;
+    int64_t tiles = multi ? a.total_tiles : (n + T - 1) / T;
+    int cap = h.dev->num_cus * std::max(1, c.blocks_per_cu);
+    *grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+    bool lds = c.lds_preaggregate != 0;
+    int E = lds ? h.lds_entries : 0;
+    size_t bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
+    while (E && bytes > 64 * 1024) {
+        E >>= 1;
+        bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
+    }
+    a.lds_entries = E;
+    a.lds_words = h.stride;
+    a.flush_tiles = multi ? 64 : 1 << 30;
+    *lds_bytes = bytes;
+}
+
+static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, const vr_launch_config* cfg) {
+    VR_TRY(ensure_capacity(h, n, s));
+    UpdateArgs a = h.proto;
+    a.t = view_of(h);
+    b.row_base = h.rows_seen;
+    b.tile_begin = 0;
+    a.one = b;
+    a.num_batches = 1;
+    a.epoch = ++h.epoch;
+    if (h.det) {
+        VR_TRY(h.row_slot.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
+        a.row_slot = h.row_slot.as<uint32_t>();
+    }
+    int grid, block;
+    size_t lds_bytes;
+    launch_shape(h, cfg, n, a, &grid, &block, &lds_bytes, false);
+    hipLaunchKernelGGL(agg_update_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    int launches = 1;
+    if (h.det) {
+        // order the batch's rows by slot (stable: row order kept inside a slot), then fold
+        VR_TRY(h.sort_keys.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
+        VR_TRY(h.sort_vals.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
+        VR_TRY(h.sort_vals_in.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
+        size_t tmp = 0;
+        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, a.row_slot, h.sort_keys.as<uint32_t>(),
+                                                      h.sort_vals_in.as<uint32_t>(), h.sort_vals.as<uint32_t>(),
+                                                      (int)n, 0, 32, s));
+        VR_TRY(h.cub_tmp.ensure(h.dev, tmp));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(iota_u32_kernel), dim3(grid_for(n, 256, h.dev->num_cus * 16)), dim3(256), 0, s,
+                           h.sort_vals_in.as<uint32_t>(), n);
+        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(h.cub_tmp.p, tmp, a.row_slot, h.sort_keys.as<uint32_t>(),
+                                                      h.sort_vals_in.as<uint32_t>(), h.sort_vals.as<uint32_t>(),
+                                                      (int)n, 0, 32, s));
+        hipLaunchKernelGGL(ordered_fold_kernel, dim3(grid_for(n, 256, h.dev->num_cus * 16)), dim3(256), 0, s,
+                           h.sort_keys.as<uint32_t>(), h.sort_vals.as<uint32_t>(), n, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches += 3;
+    }
+    if (h.has_f64_minmax) {
+        hipLaunchKernelGGL(nan_payload_kernel<false>, dim3(grid_for(n, 256, h.dev->num_cus * 8)), dim3(256), 0, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
+    }
+    h.rows_seen += n;
+    h.groups_upper += n;
+    h.order_count = -1;
+    h.dev->count(&vr_stats::kernel_launches, launches);
+    return VR_OK;
+}
+
+// The emission order: occupied slots in slot order (then the reserved rows), or by first-seen row.
+static vr_status build_order(AggImpl& h, hipStream_t s) {
+    if (h.order_count >= 0) return VR_OK;
+    uint64_t c[C_WORDS];
+    VR_TRY(read_counters(h, c, s));
+    int64_t g = groups_of(c);
+    VR_TRY(h.order.ensure(h.dev, (size_t)(g + 2) * sizeof(uint64_t)));
+    VR_TRY(h.order_tmp.ensure(h.dev, sizeof(uint64_t) * 4));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.order_tmp.as<uint64_t>(), 4ull, 0ull);
+    hipcub::CountingInputIterator<uint64_t> it(0);
+    OccupiedSlot pred{h.cells.as<uint64_t>(), h.proto.ks.width};
+    size_t tmp = 0;
+    VR_TRY_HIP(hipcub::DeviceSelect::If(nullptr, tmp, it, h.order.as<uint64_t>(), h.order_tmp.as<uint64_t>(),
+                                        (int64_t)h.slots, pred, s));
+    VR_TRY(h.cub_tmp.ensure(h.dev, tmp));
+    VR_TRY_HIP(hipcub::DeviceSelect::If(h.cub_tmp.p, tmp, it, h.order.as<uint64_t>(), h.order_tmp.as<uint64_t>(),
+                                        (int64_t)h.slots, pred, s));
+    uint64_t selected = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&selected, h.order_tmp.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    std::vector<uint64_t> extra;
+    if (c[C_RESERVED] & RES_SENTINEL) extra.push_back(h.slots);
+    if (c[C_RESERVED] & RES_NULL) extra.push_back(h.slots + 1);
+    if (!extra.empty())
+        VR_TRY_HIP(hipMemcpy(h.order.as<uint64_t>() + selected, extra.data(), extra.size() * sizeof(uint64_t),
+                             hipMemcpyHostToDevice));
+    int64_t m = (int64_t)selected + (int64_t)extra.size();
+    if (m != g) return fail(VR_ERR_DEVICE, "group count mismatch: table holds " + std::to_string(m) + ", counted " + std::to_string(g));
+    if (h.det && m > 1) {
+        int off = -1;
+        for (int k = 0; k < h.proto.nops; k++)
+            if (h.proto.ops[k].code == OP_FIRST_ROW) off = h.proto.ops[k].off;
+        VR_TRY(h.firsts.ensure(h.dev, (size_t)m * sizeof(uint64_t)));
+        VR_TRY(h.firsts_sorted.ensure(h.dev, (size_t)m * sizeof(uint64_t)));
+        VR_TRY(h.order_tmp.ensure(h.dev, (size_t)m * sizeof(uint64_t)));
+        hipLaunchKernelGGL(first_rows_kernel, dim3(grid_for(m, 256, h.dev->num_cus * 16)), dim3(256), 0, s,
+                           h.order.as<uint64_t>(), m, h.state.as<uint64_t>(), h.stride, off, h.firsts.as<uint64_t>());
+        size_t t2 = 0;
+        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, h.firsts.as<uint64_t>(), h.firsts_sorted.as<uint64_t>(),
+                                                      h.order.as<uint64_t>(), h.order_tmp.as<uint64_t>(), (int)m, 0, 64, s));
+        VR_TRY(h.cub_tmp.ensure(h.dev, t2));
+        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(h.cub_tmp.p, t2, h.firsts.as<uint64_t>(), h.firsts_sorted.as<uint64_t>(),
+                                                      h.order.as<uint64_t>(), h.order_tmp.as<uint64_t>(), (int)m, 0, 64, s));
+        std::swap(h.order.p, h.order_tmp.p);
+        std::swap(h.order.bytes, h.order_tmp.bytes);
+    }
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    h.order_count = m;
+    return VR_OK;
+}
+
+}  // namespace vr
+
+using namespace vr;
+
+struct vr_agg {
+    AggImpl h;
+};
+
+extern "C" {
+
+vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
+    if (!dev || !spec || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (spec->num_keys > 0 && !spec->key_types) return fail(VR_ERR_INVALID_ARGUMENT, "null key types");
+    std::unique_ptr<vr_agg> a(new vr_agg());
+    AggImpl& h = a->h;
+    h.dev = &dev->d;
+    VR_TRY_HIP(hipSetDevice(h.dev->ordinal));
+    h.key_types.assign(spec->key_types, spec->key_types + spec->num_keys);
+    h.key_nullable.assign(spec->num_keys, 1);
+    if (spec->key_nullable)
+        for (int c = 0; c < spec->num_keys; c++) h.key_nullable[c] = spec->key_nullable[c] ? 1 : 0;
+    if (spec->num_values > 0) h.value_types.assign(spec->value_types, spec->value_types + spec->num_values);
+    if (spec->num_aggs > 0) h.aggs.assign(spec->aggs, spec->aggs + spec->num_aggs);
+    h.det = spec->deterministic != 0;
+    VR_TRY(compile_spec(h));
+    h.dicts.resize(h.key_types.size());
+    VR_TRY(init_table(h, spec->capacity_hint));
+    *out = a.release();
+    return VR_OK;
+}
+
+vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* values, int64_t n,
+                        const uint8_t* selection, void* stream) {
+    if (!agg || (!keys && !agg->h.key_types.empty())) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
+    AggImpl& h = agg->h;
+    if (n == 0) return VR_OK;
+    if ((uint64_t)n >= (1ull << 32) && h.det) return fail(VR_ERR_UNSUPPORTED, "deterministic batches are below 2^32 rows");
+    hipStream_t s = h.dev->pick(stream);
+    BatchDesc b;
+    StagePlan plan;
+    std::vector<std::vector<int32_t>> ids;
+    int32_t mem;
+    VR_TRY(describe(h, keys, values, n, selection, b, plan, ids, &mem));
+    int slot;
+    VR_TRY(stage(h, plan, s, &slot));
+    VR_TRY(run_update(h, b, n, s, nullptr));
+    if (slot >= 0) VR_TRY(h.staging.done(slot, s));
+    h.dev->count(&vr_stats::calls, 1);
+    h.dev->count(&vr_stats::input_rows, n);
+    return VR_OK;
+}
+
+vr_status vr_agg_update_filtered(vr_agg* agg, const vr_column* keys, const vr_column* values, int64_t n,
+                                 int32_t filter_col, int32_t filter_threshold, void* stream) {
+    if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    if (filter_col >= (int)h.value_types.size() || (filter_col >= 0 && h.value_types[filter_col] != VR_INT32))
+        return fail(VR_ERR_INVALID_ARGUMENT, "the filter column must be an INT32 value column");
+    h.proto.filter_col = filter_col;
+    h.proto.filter_threshold = filter_threshold;
+    vr_status st = vr_agg_update(agg, keys, values, n, nullptr, stream);
+    h.proto.filter_col = -1;
+    return st;
+}
+
+vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_launch_config* cfg, void* stream) {
+    if (!agg || !bs) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    const int nb = bs->num_batches;
+    if (nb <= 0) return VR_OK;
+    const int nk = (int)h.key_types.size(), nv = (int)h.value_types.size();
+    if (bs->filter_col >= nv || (bs->filter_col >= 0 && h.value_types[bs->filter_col] != VR_INT32))
+        return fail(VR_ERR_INVALID_ARGUMENT, "the filter column must be an INT32 value column");
+    hipStream_t s = h.dev->pick(stream);
+    // One persistent launch needs device-resident columns and one dictionary per UTF8 key; anything
+    // else (and the deterministic mode, which orders each batch) takes the per-batch path.
+    bool fused = !h.det;
+    for (int b = 0; b < nb && fused; b++) {
+        for (int c = 0; c < nk; c++) {
+            const vr_column& k = bs->keys[(size_t)b * nk + c];
+            if (k.mem != VR_MEM_DEVICE) fused = false;
+            if (k.type == VR_UTF8 && (!k.dictionary || k.dictionary->data != bs->keys[c].dictionary->data)) fused = false;
+        }
+        for (int c = 0; c < nv; c++)
+            if (bs->values[(size_t)b * nv + c].mem != VR_MEM_DEVICE) fused = false;
+    }
+    if (!fused) {
+        for (int b = 0; b < nb; b++)
+            VR_TRY(vr_agg_update_filtered(agg, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv,
+                                          bs->lengths[b], bs->filter_col, bs->filter_threshold, stream));
+        return VR_OK;
+    }
+    UpdateArgs a = h.proto;
+    a.filter_col = bs->filter_col;
+    a.filter_threshold = bs->filter_threshold;
+    // the tile size first: batches are cut into tiles of block * rows_per_thread rows
+    int grid, block;
+    size_t lds_bytes;
+    a.total_tiles = 1;
+    launch_shape(h, cfg, 0, a, &grid, &block, &lds_bytes, true);
+    const int64_t T = (int64_t)block * a.rows_per_thread;
+    VR_TRY_HIP(hipStreamSynchronize(s));  // the previous batched launch may still read the descriptors
+    static thread_local std::vector<BatchDesc> descs;
+    descs.resize(nb);
+    int64_t tiles = 0, rows = 0;
+    std::vector<std::vector<int32_t>> ids;
+    for (int b = 0; b < nb; b++) {
+        StagePlan plan;
+        int32_t mem;
+        int64_t n = bs->lengths[b];
+        if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative batch length");
+        VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, n, nullptr, descs[b], plan, ids, &mem));
+        descs[b].row_base = h.rows_seen + rows;
+        descs[b].tile_begin = tiles;
+        tiles += (n + T - 1) / T;
+        rows += n;
+    }
+    if (rows == 0) return VR_OK;
+    VR_TRY(ensure_capacity(h, rows, s));
+    VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
+    VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs.data(), sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
+    a.t = view_of(h);
+    a.batches = h.batch_descs.as<BatchDesc>();
+    a.num_batches = nb;
+    a.total_tiles = tiles;
+    a.epoch = ++h.epoch;
+    launch_shape(h, cfg, 0, a, &grid, &block, &lds_bytes, true);
+    hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    int launches = 1;
+    if (h.has_f64_minmax) {
+        hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
+    }
+    h.rows_seen += rows;
+    h.groups_upper += rows;
+    h.order_count = -1;
+    h.dev->count(&vr_stats::kernel_launches, launches);
+    h.dev->count(&vr_stats::calls, 1);
+    h.dev->count(&vr_stats::input_rows, rows);
+    return VR_OK;
+}
+
+vr_status vr_agg_num_groups(vr_agg* agg, int64_t* groups) {
+    if (!agg || !groups) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t c[C_WORDS];
+    VR_TRY(read_counters(agg->h, c, agg->h.dev->stream));
+    *groups = groups_of(c);
+    return VR_OK;
+}
+
+vr_status vr_agg_output_layout(vr_agg* agg, int32_t* num_columns, int32_t* types, int32_t max_columns) {
+    if (!agg || !num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    *num_columns = (int32_t)agg->h.out_types.size();
+    for (int i = 0; types && i < max_columns && i < (int)agg->h.out_types.size(); i++) types[i] = agg->h.out_types[i];
+    return VR_OK;
+}
+
+vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys, vr_out_column* out, void* stream) {
+    if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    hipStream_t s = h.dev->pick(stream);
+    if (stream) VR_TRY_HIP(hipStreamSynchronize(h.dev->stream));
+    VR_TRY(build_order(h, s));
+    if (from < 0 || to < from || to > h.order_count) return fail(VR_ERR_INVALID_ARGUMENT, "emission range outside the groups");
+    const int64_t m = to - from;
+    if (m == 0) return VR_OK;
+    const int nk = (int)h.key_types.size(), no = (int)h.out_types.size();
+    EmitArgs a{};
+    a.t = view_of(h);
+    a.ks = h.proto.ks;
+    a.order = h.order.as<uint64_t>();
+    a.from = from;
+    a.m = m;
+    a.naggs = (int)h.emits.size();
+    for (int k = 0; k < a.naggs; k++) a.aggs[k] = h.emits[k];
+    a.flags_off = h.proto.flags_off;
+    // outputs: caller's device buffers directly, host ones through a device staging area
+    struct Back {
+        void* host;
+        size_t off, bytes;
+    };
+    std::vector<Back> back;
+    size_t need = 0;
+    auto plan = [&](const vr_out_column& oc, int32_t type, OutCol& d) -> vr_status {
+        if (oc.capacity < m) return fail(VR_ERR_CAPACITY, "output column holds fewer rows than emitted");
+        d.type = type;
+        size_t vb = bitmap_bytes(m), db = lane_bytes(type == VR_UTF8 ? VR_INT32 : type, m);
+        if (oc.mem == VR_MEM_DEVICE) {
+            d.valid = oc.validity;
+            d.data = oc.data;
+        } else {
+            if (oc.validity) {
+                need = align_up(need, 256);
+                back.push_back({oc.validity, need, vb});
+                need += vb;
+            }
+            need = align_up(need, 256);
+            back.push_back({oc.data, need, db});
+            need += db;
+        }
+        return VR_OK;
+    };
+    for (int c = 0; c < nk; c++) VR_TRY(plan(keys[c], h.key_types[c], a.keys[c]));
+    for (int c = 0; c < no; c++) VR_TRY(plan(out[c], h.out_types[c], a.outs[c]));
+    if (need) {
+        VR_TRY(h.out_stage.ensure(h.dev, need));
+        uint8_t* base = h.out_stage.as<uint8_t>();
+        size_t k = 0;
+        auto bind = [&](const vr_out_column& oc, OutCol& d) {
+            if (oc.mem == VR_MEM_DEVICE) return;
+            if (oc.validity) d.valid = base + back[k++].off;
+            else d.valid = nullptr;
+            d.data = base + back[k++].off;
+        };
+        for (int c = 0; c < nk; c++) bind(keys[c], a.keys[c]);
+        for (int c = 0; c < no; c++) bind(out[c], a.outs[c]);
+    }
+    hipLaunchKernelGGL(agg_emit_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    for (auto& bk : back) {
+        VR_TRY_HIP(hipMemcpyAsync(bk.host, h.out_stage.as<uint8_t>() + bk.off, bk.bytes, hipMemcpyDeviceToHost, s));
+        h.dev->count(&vr_stats::d2h_bytes, (int64_t)bk.bytes);
+    }
+    uint64_t c[C_WORDS];
+    VR_TRY(read_counters(h, c, s));
+    h.dev->count(&vr_stats::output_rows, m);
+    if (c[C_ERROR]) {
+        uint64_t e = c[C_ERROR];
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>() + C_ERROR, 1ull, 0ull);
+        VR_TRY_HIP(hipStreamSynchronize(s));
+        if (e & ERR_LONG_OVERFLOW) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "long overflow");
+        if (e & ERR_AVG_OF_OVERFLOW) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "decimal average over an overflowed sum");
+        return fail(VR_ERR_DECIMAL_OVERFLOW, "decimal value past its precision");
+    }
+    return VR_OK;
+}
+
+vr_status vr_agg_key_dictionary(vr_agg* agg, int32_t c, int32_t* offsets, int64_t offsets_capacity, uint8_t* bytes,
+                                int64_t bytes_capacity, int64_t* count, int64_t* num_bytes) {
+    if (!agg || !count || !num_bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    if (c < 0 || c >= (int)h.key_types.size() || h.key_types[c] != VR_UTF8)
+        return fail(VR_ERR_INVALID_ARGUMENT, "not a UTF8 key column");
+    const HostDictionary& d = h.dicts[c];
+    int64_t total = 0;
+    for (auto& s : d.values) total += (int64_t)s.size();
+    *count = (int64_t)d.values.size();
+    *num_bytes = total;
+    if (!offsets || !bytes) return VR_OK;
+    if (offsets_capacity < *count + 1 || bytes_capacity < total) return fail(VR_ERR_CAPACITY, "dictionary buffers too small");
+    int64_t o = 0;
+    offsets[0] = 0;
+    for (size_t i = 0; i < d.values.size(); i++) {
+        std::memcpy(bytes + o, d.values[i].data(), d.values[i].size());
+        o += (int64_t)d.values[i].size();
+        offsets[i + 1] = (int32_t)o;
+    }
+    return VR_OK;
+}
+
+vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
+    if (!agg || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    *bytes = (int64_t)(h.cells.bytes + h.state.bytes + h.counters.bytes + h.order.bytes + h.order_tmp.bytes +
+                       h.cub_tmp.bytes + h.out_stage.bytes + h.row_slot.bytes + h.sort_keys.bytes +
+                       h.sort_vals.bytes + h.sort_vals_in.bytes + h.firsts.bytes + h.firsts_sorted.bytes +
+                       h.batch_descs.bytes);
+    return VR_OK;
+}
+
+vr_status vr_agg_reset(vr_agg* agg) {
+    if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    hipStream_t s = h.dev->stream;
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    VR_TRY(alloc_table(h, h.slots, h.cells, h.state, s));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    h.groups_upper = 0;
+    h.order_count = -1;
+    return VR_OK;
+}
+
+vr_status vr_agg_destroy(vr_agg* agg) {
+    if (!agg) return VR_OK;
+    (void)hipStreamSynchronize(agg->h.dev->stream);
+    delete agg;
+    return VR_OK;
+}
+
+}  // extern "C"

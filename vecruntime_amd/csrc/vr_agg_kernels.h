@@ -1,0 +1,478 @@
+// Group-by aggregation: device data structures and kernels.
+//
+// Table.  Open addressing over 2^k slots; a group's state row has the same index as its slot, so
+// finding the slot is the whole assignment (no second indirection to a dense id).  Keys of at most
+// 64 bits (one INT64/FLOAT64, one 32-bit lane with a null flag, two 32-bit lanes) live in one
+// 64-bit cell claimed by compare-and-swap; a key equal to the empty marker and the null key of a
+// single 64-bit lane use two reserved rows after the slots.  Wider keys are W 32-bit words: cell 0
+// holds a tag (marker bit, 31 hash bits, word 0) claimed by CAS, cells 1..W-1 hold (1<<32 | word)
+// written by the claiming lane; a reader that sees a cell still 0 retries the same slot on its next
+// loop iteration, so no lane ever waits inside a branch for another lane of its wave.  Cells only
+// ever go from empty to their final value, so a plain (possibly stale) load can only return the
+// final value or "empty", and "empty" is confirmed by the CAS itself.
+//
+// State.  One row of `stride` 64-bit words per slot, array of structures, so one group's update
+// touches one or two cache lines.  Words per operation (see Op):
+//   COUNT_STAR / COUNT          count
+//   SUM_I128 (INT32/INT64 lane) lo, hi, count            -- exact 128-bit (WideLongSum, LongSum)
+//   SUM_I192 (DECIMAL128 lane)  lo, hi, sign-limb, count -- exact; total past 128 bits = overflow
+//   SUM_F64                     sum (double), count
+//   MIN/MAX_I32                 value as int64 with an impossible identity
+//   MIN/MAX_I64                 ordered code; presence bit in the row's flags word
+//   MIN/MAX_F64                 ordered code, first zero row (row<<1 | sign), first NaN row, NaN bits
+//   FIRST_ROW                   smallest global row index (first-seen emission order)
+//
+// Update.  Each workgroup walks tiles of rows; a row's slot is found (or claimed) in the global
+// table, then its values are folded into a small LDS table keyed by slot with LDS atomics, and the
+// LDS partials are flushed to the global rows with one set of global atomics per distinct slot.
+// Hot keys (Zipf) thus cost LDS atomics instead of contended global ones.  Rows whose slot finds
+// no LDS entry (the LDS table full) update the global row directly.
+#pragma once
+
+#include "vr_device.h"
+#include "vr_internal.h"
+
+namespace vr {
+
+enum KeyMode : int32_t { KM_I64 = 0, KM_I32 = 1, KM_2X32 = 2, KM_WIDE = 3 };
+
+enum OpCode : int32_t {
+    OP_COUNT_STAR = 0,
+    OP_COUNT = 1,
+    OP_SUM_I128 = 2,
+    OP_SUM_I192 = 3,
+    OP_SUM_F64 = 4,
+    OP_MIN_I32 = 5,
+    OP_MAX_I32 = 6,
+    OP_MIN_I64 = 7,
+    OP_MAX_I64 = 8,
+    OP_MIN_F64 = 9,
+    OP_MAX_F64 = 10,
+    OP_FIRST_ROW = 11
+};
+
+// counters[] words
+enum { C_GROUPS = 0, C_RESERVED = 1, C_NAN = 2, C_ERROR = 3, C_OUT = 4, C_WORDS = 8 };
+enum { RES_SENTINEL = 1, RES_NULL = 2 };
+enum { ERR_LONG_OVERFLOW = 1, ERR_DECIMAL_OVERFLOW = 2, ERR_AVG_OF_OVERFLOW = 4 };
+
+struct Op {
+    int32_t code;
+    int32_t col;   // value column, -1 for COUNT_STAR / FIRST_ROW
+    int32_t off;   // first state word
+    int32_t bit;   // presence bit (MIN/MAX_I64), else 0
+    int32_t lds;   // 1: folded through the LDS table
+    int32_t pad;
+};
+
+struct KeySpec {
+    int32_t mode;
+    int32_t width;     // 32-bit words of a wide key (null mask word included)
+    int32_t nkeys;
+    int32_t null_word; // index of the null-mask word, -1 when no key is nullable
+    int32_t types[VR_MAX_KEYS];
+    int32_t word_off[VR_MAX_KEYS];
+};
+
+struct TableView {
+    uint64_t* cells;
+    uint64_t* state;
+    uint64_t* counters;
+    uint64_t mask;   // slots - 1
+    uint64_t slots;  // reserved rows at [slots, slots + 2)
+    int32_t width;   // cells per slot (1 for the 64-bit modes)
+    int32_t stride;  // state words per row
+};
+
+struct BatchDesc {
+    DCol keys[VR_MAX_KEYS];
+    DCol vals[VR_MAX_VALUES];
+    const uint8_t* sel;
+    int64_t n;
+    int64_t row_base;    // global index of the batch's first row (first-seen order)
+    int64_t tile_begin;  // first tile of the batch in a batched launch
+};
+
+struct UpdateArgs {
+    TableView t;
+    KeySpec ks;
+    Op ops[VR_MAX_OPS];
+    int32_t nops;
+    int32_t flags_off;     // state word of the MIN/MAX_I64 presence bits, -1 when none
+    int32_t lds_entries;   // power of two, 0: no LDS pre-aggregation
+    int32_t lds_words;     // state words mirrored per LDS entry (= stride)
+    int32_t rows_per_thread;
+    int32_t flush_tiles;   // tiles between LDS flushes
+    int32_t filter_col;    // INT32 value column filtered by value < filter_threshold, -1: none
+    int32_t filter_threshold;
+    int32_t det;           // deterministic mode: double sums folded by the ordered pass
+    int32_t num_batches;   // 1: `one` below; >1: `batches` (device array)
+    int32_t epoch;         // update sequence number: the NaN flag holds the last epoch that saw one
+    int64_t total_tiles;
+    uint64_t ident[64];    // identity value of every state word
+    uint32_t* row_slot;    // deterministic: slot of every row of the batch (0xFFFFFFFF none)
+    const BatchDesc* batches;
+    BatchDesc one;
+};
+
+__device__ __forceinline__ bool is_valid(const DCol& c, int64_t i) { return !c.valid || bit_at(c.valid, i); }
+
+// A 32-bit key lane (INT32 / BOOL / UTF8 table-dictionary id).
+__device__ __forceinline__ uint32_t key32(const DCol& c, int64_t i, bool* isnull) {
+    *isnull = c.valid && !bit_at(c.valid, i);
+    if (*isnull) return 0;
+    switch (c.type) {
+        case VR_BOOL: return bit_at((const uint8_t*)c.data, i) ? 1u : 0u;
+        case VR_UTF8: {
+            int32_t code = ld_stream((const int32_t*)c.data + i);
+            if (c.remap) {
+                int32_t id = (code >= 0 && code < c.remap_len) ? c.remap[code] : -1;
+                if (id < 0) {  // a null dictionary entry: the row is null through the dictionary
+                    *isnull = true;
+                    return 0;
+                }
+                return (uint32_t)id;
+            }
+            return (uint32_t)code;
+        }
+        default: return (uint32_t)ld_stream((const int32_t*)c.data + i);
+    }
+}
+
+__device__ __forceinline__ uint64_t mark_reserved(uint64_t* counters, uint64_t bit) {
+    if ((counters[C_RESERVED] & bit) == 0) atomicOr((unsigned long long*)&counters[C_RESERVED], (unsigned long long)bit);
+    return bit;
+}
+
+// Slot of a 64-bit key, claiming an empty cell for a new key; `inserted` reports a claim.
+__device__ __forceinline__ uint64_t find_or_insert64(const TableView& t, uint64_t key, bool* inserted) {
+    *inserted = false;
+    if (key == EMPTY_KEY) {
+        mark_reserved(t.counters, RES_SENTINEL);
+        return t.slots;
+    }
+    uint64_t pos = fmix64(key) & t.mask;
+    while (true) {
+        uint64_t c = t.cells[pos];
+        if (c == key) return pos;
+        if (c == EMPTY_KEY) {
+            uint64_t old = atomic_cas_u64(&t.cells[pos], EMPTY_KEY, key);
+            if (old == EMPTY_KEY) {
+                *inserted = true;
+                return pos;
+            }
+            if (old == key) return pos;
+        }
+        pos = (pos + 1) & t.mask;
+    }
+}
+
+__device__ __forceinline__ uint64_t find64(const TableView& t, uint64_t key) {
+    if (key == EMPTY_KEY) return t.slots;
+    uint64_t pos = fmix64(key) & t.mask;
+    while (true) {
+        uint64_t c = t.cells[pos];
+        if (c == key) return pos;
+        if (c == EMPTY_KEY) {
+            c = ld_coherent(&t.cells[pos]);
+            if (c == key) return pos;
+            if (c == EMPTY_KEY) return ~0ull;
+        }
+        pos = (pos + 1) & t.mask;
+    }
+}
+
+__device__ __forceinline__ uint64_t hash_words(const uint32_t* w, int n) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)n;
+    for (int i = 0; i < n; i++) h = fmix64(h ^ ((uint64_t)w[i] * 0xC2B2AE3D27D4EB4Full));
+    return h;
+}
+
+// Slot of a wide key of `W` words.  Each loop iteration is one step for every lane still probing;
+// a lane that found the tag but a later cell not yet written stays on the slot for the next step.
+__device__ __forceinline__ uint64_t find_or_insert_wide(const TableView& t, const uint32_t* w, int W,
+                                                        bool* inserted, bool insert) {
+    *inserted = false;
+    uint64_t h = hash_words(w, W);
+    uint64_t tag = (1ull << 63) | (((h >> 32) & 0x7FFFFFFFull) << 32) | (uint64_t)w[0];
+    uint64_t pos = h & t.mask;
+    bool coherent = false;
+    while (true) {
+        uint64_t* s = t.cells + pos * (uint64_t)W;
+        uint64_t c = coherent ? ld_coherent(s) : s[0];
+        if (c == 0) {
+            if (!insert) {
+                c = ld_coherent(s);
+                if (c == 0) return ~0ull;
+            } else {
+                uint64_t old = atomic_cas_u64(s, 0, tag);
+                if (old == 0) {
+                    for (int i = 1; i < W; i++) st_coherent(s + i, (1ull << 32) | (uint64_t)w[i]);
+                    *inserted = true;
+                    return pos;
+                }
+                c = old;
+            }
+        }
+        if (c == tag) {
+            bool pending = false, same = true;
+            for (int i = 1; i < W; i++) {
+                uint64_t v = s[i];
+                if (v == 0) v = ld_coherent(s + i);
+                if (v == 0) {
+                    pending = true;
+                    break;
+                }
+                if (v != ((1ull << 32) | (uint64_t)w[i])) {
+                    same = false;
+                    break;
+                }
+            }
+            if (pending) {  // the claiming lane's words are still in flight: look again next step
+                coherent = true;
+                continue;
+            }
+            if (same) return pos;
+        }
+        coherent = false;
+        pos = (pos + 1) & t.mask;
+    }
+}
+
+// Builds the key of row i: a 64-bit key (returns the state row directly for the reserved rows) or
+// the words of a wide key.  Returns the state row index, or ~0 for a row with no group (lookup).
+__device__ __forceinline__ uint64_t row_slot_of(const UpdateArgs& a, const BatchDesc& b, int64_t i,
+                                                bool* inserted, bool insert) {
+    const KeySpec& ks = a.ks;
+    *inserted = false;
+    switch (ks.mode) {
+        case KM_I64: {
+            const DCol& c = b.keys[0];
+            if (c.valid && !bit_at(c.valid, i)) {
+                mark_reserved(a.t.counters, RES_NULL);
+                return a.t.slots + 1;
+            }
+            uint64_t key = (uint64_t)ld_stream((const int64_t*)c.data + i);
+            return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
+        }
+        case KM_I32: {
+            bool nul;
+            uint32_t v = key32(b.keys[0], i, &nul);
+            uint64_t key = nul ? (1ull << 32) : (uint64_t)v;
+            return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
+        }
+        case KM_2X32: {
+            bool n0, n1;
+            uint32_t v0 = key32(b.keys[0], i, &n0);
+            uint32_t v1 = key32(b.keys[1], i, &n1);
+            uint64_t key = ((uint64_t)v0 << 32) | (uint64_t)v1;
+            return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
+        }
+        default: {
+            uint32_t w[4 * VR_MAX_KEYS + 1];
+            int W = ks.width;
+            for (int k = 0; k < W; k++) w[k] = 0;
+            uint32_t nm = 0;
+            for (int c = 0; c < ks.nkeys; c++) {
+                const DCol& col = b.keys[c];
+                int o = ks.word_off[c];
+                switch (ks.types[c]) {
+                    case VR_INT64:
+                    case VR_FLOAT64:
+                        if (col.valid && !bit_at(col.valid, i)) {
+                            nm |= 1u << c;
+                        } else {
+                            uint64_t v = (uint64_t)ld_stream((const int64_t*)col.data + i);
+                            w[o] = (uint32_t)v;
+                            w[o + 1] = (uint32_t)(v >> 32);
+                        }
+                        break;
+                    case VR_DECIMAL128:
+                        if (col.valid && !bit_at(col.valid, i)) {
+                            nm |= 1u << c;
+                        } else {
+                            const uint64_t* p = (const uint64_t*)col.data + 2 * i;
+                            uint64_t lo = ld_stream(p), hi = ld_stream(p + 1);
+                            w[o] = (uint32_t)lo;
+                            w[o + 1] = (uint32_t)(lo >> 32);
+                            w[o + 2] = (uint32_t)hi;
+                            w[o + 3] = (uint32_t)(hi >> 32);
+                        }
+                        break;
+                    default: {
+                        bool nul;
+                        uint32_t v = key32(col, i, &nul);
+                        if (nul) nm |= 1u << c;
+                        else w[o] = v;
+                    }
+                }
+            }
+            if (ks.null_word >= 0) w[ks.null_word] = nm;
+            return find_or_insert_wide(a.t, w, W, inserted, insert);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ per-row updates
+
+__device__ __forceinline__ int64_t read_long(const DCol& c, int64_t i) {
+    return c.type == VR_INT32 ? (int64_t)ld_stream((const int32_t*)c.data + i) : ld_stream((const int64_t*)c.data + i);
+}
+
+// Folds row i of batch b into its group: each operation's words at `lrow` (the group's LDS
+// entry) when it has one and the operation goes through LDS, else at `grow` (the global row).
+// The first zero / NaN rows of a double MIN/MAX always go to the global row.
+__device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* lrow,
+                                          uint64_t* grow) {
+    for (int k = 0; k < a.nops; k++) {
+        const Op& op = a.ops[k];
+        uint64_t* row = (lrow && op.lds) ? lrow : grow;
+        uint64_t* w = row + op.off;
+        switch (op.code) {
+            case OP_COUNT_STAR: atomic_add_u64(w, 1); break;
+            case OP_COUNT:
+                if (is_valid(b.vals[op.col], i)) atomic_add_u64(w, 1);
+                break;
+            case OP_SUM_I128: {
+                const DCol& c = b.vals[op.col];
+                if (!is_valid(c, i)) break;
+                int64_t x = read_long(c, i);
+                atomic_add_i128(w, w + 1, (uint64_t)x, (uint64_t)(x >> 63));
+                atomic_add_u64(w + 2, 1);
+                break;
+            }
+            case OP_SUM_I192: {
+                const DCol& c = b.vals[op.col];
+                if (!is_valid(c, i)) break;
+                const uint64_t* p = (const uint64_t*)c.data + 2 * i;
+                atomic_add_i192(w, w + 1, w + 2, ld_stream(p), ld_stream(p + 1));
+                atomic_add_u64(w + 3, 1);
+                break;
+            }
+            case OP_SUM_F64: {
+                const DCol& c = b.vals[op.col];
+                if (!is_valid(c, i)) break;
+                if (!a.det) atomic_add_f64(w, ld_stream((const double*)c.data + i));
+                atomic_add_u64(w + 1, 1);
+                break;
+            }
+            case OP_MIN_I32:
+            case OP_MAX_I32: {
+                const DCol& c = b.vals[op.col];
+                if (!is_valid(c, i)) break;
+                int64_t x = ld_stream((const int32_t*)c.data + i);
+                if (op.code == OP_MIN_I32) atomic_min_i64(w, x);
+                else atomic_max_i64(w, x);
+                break;
+            }
+            case OP_MIN_I64:
+            case OP_MAX_I64: {
+                const DCol& c = b.vals[op.col];
+                if (!is_valid(c, i)) break;
+                uint64_t o = (uint64_t)ld_stream((const int64_t*)c.data + i) ^ 0x8000000000000000ull;
+                if (op.code == OP_MIN_I64) atomic_min_u64(w, o);
+                else atomic_max_u64(w, o);
+                uint64_t* f = row + a.flags_off;
+                if ((*f & (1ull << op.bit)) == 0) atomicOr((unsigned long long*)f, 1ull << op.bit);
+                break;
+            }
+            case OP_MIN_F64:
+            case OP_MAX_F64: {
+                const DCol& c = b.vals[op.col];
+                if (!is_valid(c, i)) break;
+                uint64_t bits = (uint64_t)ld_stream((const int64_t*)c.data + i);
+                uint64_t o = order_f64(bits);
+                if (op.code == OP_MIN_F64) atomic_min_u64(w, o);
+                else atomic_max_u64(w, o);
+                // Ties between +0.0 / -0.0 and between NaN payloads keep the first-seen value
+                // (DoubleMinMax.offer replaces only on a strict improvement): record the first
+                // zero row with its sign and the first NaN row on the global row.
+                uint64_t* gw = grow + op.off;
+                uint64_t gi = (uint64_t)(b.row_base + i);
+                if (is_zero_bits(bits)) {
+                    atomic_min_u64(gw + 1, (gi << 1) | (bits >> 63));
+                } else if (is_nan_bits(bits)) {
+                    atomic_min_u64(gw + 2, gi);
+                    uint64_t* flag = &a.t.counters[C_NAN];
+                    if (*flag != (uint64_t)a.epoch) atomic_max_u64(flag, (uint64_t)a.epoch);
+                }
+                break;
+            }
+            case OP_FIRST_ROW: {
+                uint64_t gi = (uint64_t)(b.row_base + i);
+                if (*w > gi) atomic_min_u64(w, gi);
+                break;
+            }
+        }
+    }
+}
+
+// Adds an LDS entry's partial state into the global row (one atomic per non-identity word).
+__device__ __forceinline__ void flush_entry(const UpdateArgs& a, const uint64_t* e, uint64_t* g) {
+    for (int k = 0; k < a.nops; k++) {
+        const Op& op = a.ops[k];
+        if (!op.lds) continue;
+        const uint64_t* l = e + op.off;
+        uint64_t* w = g + op.off;
+        switch (op.code) {
+            case OP_COUNT_STAR:
+            case OP_COUNT:
+                if (l[0]) atomic_add_u64(w, l[0]);
+                break;
+            case OP_SUM_I128:
+                if (l[2]) {
+                    atomic_add_i128(w, w + 1, l[0], l[1]);
+                    atomic_add_u64(w + 2, l[2]);
+                }
+                break;
+            case OP_SUM_I192:
+                if (l[3]) {
+                    uint64_t old = atomic_add_u64(w, l[0]);
+                    uint64_t c0 = (old + l[0]) < old ? 1ull : 0ull;
+                    uint64_t t = l[1] + c0;
+                    uint64_t c1 = (t < l[1]) ? 1ull : 0ull;
+                    if (t) {
+                        uint64_t oh = atomic_add_u64(w + 1, t);
+                        c1 += (oh + t) < oh ? 1ull : 0ull;
+                    }
+                    uint64_t ts = l[2] + c1;
+                    if (ts) atomic_add_u64(w + 2, ts);
+                    atomic_add_u64(w + 3, l[3]);
+                }
+                break;
+            case OP_SUM_F64:
+                if (l[1]) {
+                    if (!a.det && l[0]) atomic_add_f64(w, __longlong_as_double((long long)l[0]));
+                    atomic_add_u64(w + 1, l[1]);
+                }
+                break;
+            case OP_MIN_I32:
+                if ((int64_t)l[0] != INT64_MAX) atomic_min_i64(w, (int64_t)l[0]);
+                break;
+            case OP_MAX_I32:
+                if ((int64_t)l[0] != INT64_MIN) atomic_max_i64(w, (int64_t)l[0]);
+                break;
+            case OP_MIN_I64:
+            case OP_MAX_I64: {
+                uint64_t f = e[a.flags_off] & (1ull << op.bit);
+                if (f) {
+                    if (op.code == OP_MIN_I64) atomic_min_u64(w, l[0]);
+                    else atomic_max_u64(w, l[0]);
+                    if ((g[a.flags_off] & f) == 0) atomicOr((unsigned long long*)&g[a.flags_off], f);
+                }
+                break;
+            }
+            case OP_MIN_F64:
+                if (l[0] != ~0ull) atomic_min_u64(w, l[0]);
+                break;
+            case OP_MAX_F64:
+                if (l[0] != 0) atomic_max_u64(w, l[0]);
+                break;
+            case OP_FIRST_ROW:
+                if (l[0] != ~0ull && l[0] < w[0]) atomic_min_u64(w, l[0]);
+                break;
+        }
+    }
+}
+
+}  // namespace vr

@@ -1,0 +1,171 @@
+// Internal declarations shared by the translation units of libvrgpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vr.h"
+
+#define VR_MAX_KEYS 8
+#define VR_MAX_VALUES 16
+#define VR_MAX_OPS 24
+#define VR_MAX_AGGS 24
+
+namespace vr {
+
+// ------------------------------------------------------------------ errors
+
+vr_status fail(vr_status st, const std::string& msg);
+vr_status hip_fail(hipError_t e, const char* what);
+
+#define VR_TRY_HIP(call)                                  \
+    do {                                                  \
+        hipError_t vr_e_ = (call);                        \
+        if (vr_e_ != hipSuccess) return ::vr::hip_fail(vr_e_, #call); \
+    } while (0)
+
+#define VR_TRY(call)                      \
+    do {                                  \
+        vr_status vr_s_ = (call);         \
+        if (vr_s_ != VR_OK) return vr_s_; \
+    } while (0)
+
+// ------------------------------------------------------------------ device handle
+
+struct Device {
+    int ordinal = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;       // the handle's own compute stream
+    hipStream_t copy_stream = nullptr;  // side stream for host <-> device staging
+    std::mutex mu;
+    int64_t allocated = 0;
+    int64_t budget = 0;
+    vr_stats stats{};
+
+    vr_status alloc(void** p, size_t bytes);
+    void release(void* p, size_t bytes);
+    hipStream_t pick(void* s) const { return s ? (hipStream_t)s : stream; }
+    void count(int64_t vr_stats::*field, int64_t v) {
+        std::lock_guard<std::mutex> g(mu);
+        stats.*field += v;
+    }
+};
+
+// A device allocation owned by a handle; freed with the handle.
+struct DevBuf {
+    Device* dev = nullptr;
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : dev(o.dev), p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            dev = o.dev;
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    vr_status ensure(Device* d, size_t want) {  // grows (contents dropped) to at least `want` bytes
+        if (want <= bytes && p) return VR_OK;
+        reset();
+        dev = d;
+        size_t b = want < 256 ? 256 : want;
+        VR_TRY(d->alloc(&p, b));
+        bytes = b;
+        return VR_OK;
+    }
+    void reset() {
+        if (p) dev->release(p, bytes);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+// Pinned host staging: host columns are copied into a pinned buffer, then to the device on the
+// side stream; the compute stream waits on the copy's event.  Two slots alternate so the copy of
+// batch k+1 overlaps the kernels of batch k.
+struct Staging {
+    Device* dev = nullptr;
+    struct Slot {
+        void* pinned = nullptr;
+        size_t pinned_bytes = 0;
+        DevBuf device;
+        hipEvent_t copied = nullptr;    // H2D finished
+        hipEvent_t consumed = nullptr;  // kernels reading the slot finished
+        bool used = false;
+    } slot[2];
+    int next = 0;
+    ~Staging();
+    // Reserves `bytes` in the next slot: returns host (pinned) and device pointers.
+    vr_status begin(Device* d, size_t bytes, uint8_t** host, uint8_t** device, int* which);
+    // Uploads [0, bytes) of the slot on the side stream and makes `compute` wait for it.
+    vr_status upload(int which, size_t bytes, hipStream_t compute);
+    // Marks the slot as read by the work queued on `compute` so far.
+    vr_status done(int which, hipStream_t compute);
+};
+
+// Bytes of an Arrow buffer for n rows.
+inline size_t bitmap_bytes(int64_t n) { return (size_t)((n + 7) / 8); }
+inline size_t lane_bytes(int32_t type, int64_t n) {
+    switch (type) {
+        case VR_BOOL: return bitmap_bytes(n);
+        case VR_INT32:
+        case VR_UTF8: return (size_t)n * 4;
+        case VR_INT64:
+        case VR_FLOAT64: return (size_t)n * 8;
+        case VR_DECIMAL128: return (size_t)n * 16;
+    }
+    return 0;
+}
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// ------------------------------------------------------------------ device-side column view
+
+struct DCol {
+    const uint8_t* valid;  // nullptr: no nulls
+    const void* data;
+    const int32_t* remap;  // dictionary-encoded UTF8: input code -> table dictionary id
+    int32_t type;
+    int32_t remap_len;     // entries of `remap` (codes outside it read as null)
+};
+
+// Host-side string dictionary of one UTF8 key column (the table's own ids, first-use order), with
+// the device copy of the last input dictionary's code -> id map (GroupKeyTable.toIds,
+// GroupKeyTable.java:1308-1430).
+struct HostDictionary {
+    std::vector<std::string> values;
+    std::vector<std::pair<uint64_t, int32_t>> index;  // open addressing: (fingerprint, id)
+    size_t mask = 0;
+    // Cache of the last input dictionary mapped: its identity and the device remap array.
+    const void* last_dict = nullptr;
+    int64_t last_len = -1;
+    uint64_t last_hash = 0;
+    DevBuf remap;
+    std::vector<int32_t> host_remap;
+    int32_t index_of(const char* s, int32_t len, bool insert);
+};
+
+uint64_t hash_bytes64(const void* p, size_t n);
+
+}  // namespace vr
+
+struct vr_device {
+    vr::Device d;
+};

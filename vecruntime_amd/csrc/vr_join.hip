@@ -1,0 +1,690 @@
+// Hash join build and probe on gfx950: the vr_join_* entry points of include/vr.h.
+//
+// Replaces BuildTable.build (spark/src/main/scala/org/apache/spark/sql/vecruntime/VectorHashJoinExec.scala:
+// 478-516), GroupKeyTable.lookup (kernels/.../GroupKeyTable.java:772-826) and the probe iterator's
+// emitMatches / emitSemiAnti / gather (VectorHashJoinExec.scala:1038-1182,1605).
+//
+// Build table.  Buckets of 128 bytes: eight 64-bit keys, then eight 64-bit values, so a probe
+// reads one cache line of keys and, on a hit, one value word of the same bucket.  A value is
+// (rows of the key << 32) | payload, the payload being the build row itself for a unique key and
+// the start of the key's run in `range_rows` (build rows ordered by key, build order within a key:
+// the reference's head/next chains as a CSR) otherwise.  Keys of a single INT32/INT64 lane or two
+// INT32 lanes; a key equal to the empty marker is kept in a side entry.
+//
+// Probe.  One pass over the streamed rows in tiles of 1024: each row's match count, a block scan,
+// the tile's output offset by decoupled look-back over the preceding tiles (tile ids are taken from
+// an atomic counter, so every earlier tile is already running), then the rows write their pairs
+// or materialized columns.  The output order is therefore the reference's: streamed rows in order,
+// each row's build rows in build order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "vr_device.h"
+#include "vr_internal.h"
+
+namespace vr {
+
+constexpr int BUCKET = 8;
+constexpr int PROBE_BLOCK = 256;
+constexpr int PROBE_ITEMS = 4;
+
+struct JoinTable {
+    uint64_t* buckets;       // nbuckets * 16 words: keys[8], values[8]
+    uint64_t mask;           // nbuckets - 1
+    const uint32_t* range_rows;
+    uint64_t sentinel_value; // value of the key equal to EMPTY_KEY (0: absent)
+};
+
+__device__ __forceinline__ bool key64_of(const DCol* k, int nk, int64_t i, uint64_t* key) {
+    if (nk == 1) {
+        const DCol& c = k[0];
+        if (c.valid && !bit_at(c.valid, i)) return false;
+        if (c.type == VR_INT64) *key = (uint64_t)ld_stream((const int64_t*)c.data + i);
+        else *key = (uint64_t)(uint32_t)ld_stream((const int32_t*)c.data + i);
+        return true;
+    }
+    if ((k[0].valid && !bit_at(k[0].valid, i)) || (k[1].valid && !bit_at(k[1].valid, i))) return false;
+    uint32_t a = (uint32_t)ld_stream((const int32_t*)k[0].data + i);
+    uint32_t b = (uint32_t)ld_stream((const int32_t*)k[1].data + i);
+    *key = ((uint64_t)a << 32) | b;
+    return true;
+}
+
+__device__ __forceinline__ uint64_t bucket_of(uint64_t key, uint64_t mask) { return fmix64(key) & mask; }
+
+// Build: claims (or finds) the key's cell; returns the global cell index.
+__global__ void join_insert_kernel(JoinTable t, DCol k0, DCol k1, int nk, int64_t n, uint32_t* row_cell,
+                                   uint32_t* cell_count, uint64_t* sentinel_rows) {
+    DCol k[2] = {k0, k1};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t key;
+        if (!key64_of(k, nk, i, &key)) {  // a null key never matches (nonNullKeys)
+            row_cell[i] = 0xFFFFFFFFu;
+            continue;
+        }
+        if (key == EMPTY_KEY) {
+            row_cell[i] = 0xFFFFFFFEu;
+            atomic_add_u64(sentinel_rows, 1);
+            continue;
+        }
+        uint64_t b = bucket_of(key, t.mask);
+        uint32_t cell = 0xFFFFFFFFu;
+        while (cell == 0xFFFFFFFFu) {
+            uint64_t* keys = t.buckets + b * 16;
+            for (int j = 0; j < BUCKET; j++) {
+                uint64_t c = keys[j];
+                if (c == EMPTY_KEY) {
+                    c = atomic_cas_u64(&keys[j], EMPTY_KEY, key);
+                    if (c == EMPTY_KEY) c = key;
+                }
+                if (c == key) {
+                    cell = (uint32_t)(b * BUCKET + j);
+                    break;
+                }
+            }
+            b = (b + 1) & t.mask;
+        }
+        row_cell[i] = cell;
+        atomicAdd(&cell_count[cell], 1u);
+    }
+}
+
+// Writes each occupied cell's value from its row count and run start.
+__global__ void join_values_kernel(JoinTable t, const uint32_t* cell_count, const uint32_t* cell_start, uint64_t ncells,
+                                   const uint32_t* range_rows) {
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t cnt = cell_count[c];
+        if (!cnt) continue;
+        uint32_t start = cell_start[c];
+        uint64_t payload = cnt == 1 ? range_rows[start] : start;
+        t.buckets[(c / BUCKET) * 16 + BUCKET + (c % BUCKET)] = ((uint64_t)cnt << 32) | payload;
+    }
+}
+
+// Probe lookup: the key's value word, 0 when absent.
+__device__ __forceinline__ uint64_t lookup(const JoinTable& t, uint64_t key) {
+    if (key == EMPTY_KEY) return t.sentinel_value;
+    uint64_t b = bucket_of(key, t.mask);
+    while (true) {
+        const uint64_t* keys = t.buckets + b * 16;
+        bool open = false;
+        for (int j = 0; j < BUCKET; j++) {
+            uint64_t c = keys[j];
+            if (c == key) return keys[BUCKET + j];
+            if (c == EMPTY_KEY) open = true;
+        }
+        if (open) return 0;  // buckets fill in order: an empty cell ends the chain
+        b = (b + 1) & t.mask;
+    }
+}
+
+struct OutCol {
+    uint8_t* valid;  // nullptr: not written
+    void* data;
+    int32_t type;
+    int32_t pad;
+};
+
+struct ProbeArgs {
+    JoinTable t;
+    DCol keys[2];
+    int32_t nk;
+    int32_t join_type;
+    int32_t drop_null_keys;  // null-aware anti: a null key counts as matched
+    int32_t build_rows;      // 0: an empty build side
+    const uint8_t* sel;
+    int64_t n;
+    int64_t capacity;
+    int32_t* probe_idx;
+    int32_t* build_idx;
+    int32_t nprobe_out, nbuild_out;
+    DCol probe_src[8];
+    DCol build_src[8];
+    OutCol outs[16];
+    uint64_t* tile_state;   // per tile: (status << 62) | value, zeroed before the launch
+    uint64_t* tile_counter; // [0] next tile id, [1] total rows out
+    int64_t ntiles;
+};
+
+__device__ __forceinline__ void copy_value(const DCol& src, int64_t i, const OutCol& dst, int64_t o) {
+    if (dst.valid && (!src.valid || bit_at(src.valid, i))) atomicOr((unsigned int*)dst.valid + (o >> 5), 1u << (o & 31));
+    switch (src.type) {
+        case VR_INT32:
+        case VR_UTF8: ((int32_t*)dst.data)[o] = ((const int32_t*)src.data)[i]; break;
+        case VR_INT64:
+        case VR_FLOAT64: ((int64_t*)dst.data)[o] = ((const int64_t*)src.data)[i]; break;
+        case VR_DECIMAL128:
+            ((int64_t*)dst.data)[2 * o] = ((const int64_t*)src.data)[2 * i];
+            ((int64_t*)dst.data)[2 * o + 1] = ((const int64_t*)src.data)[2 * i + 1];
+            break;
+        case VR_BOOL:
+            if (bit_at((const uint8_t*)src.data, i)) atomicOr((unsigned int*)dst.data + (o >> 5), 1u << (o & 31));
+            break;
+    }
+}
+
+constexpr uint64_t ST_AGG = 1ull << 62, ST_PREFIX = 2ull << 62, ST_MASK = 3ull << 62;
+
+__global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
+    typedef hipcub::BlockScan<uint32_t, PROBE_BLOCK> Scan;
+    __shared__ typename Scan::TempStorage scan_tmp;
+    __shared__ int64_t tile_id_sh;
+    __shared__ uint64_t base_sh;
+    __shared__ uint32_t sub_total[PROBE_ITEMS];
+    if (threadIdx.x == 0) tile_id_sh = (int64_t)atomic_add_u64(&a.tile_counter[0], 1);
+    __syncthreads();
+    const int64_t tile = tile_id_sh;
+    const int64_t row0 = tile * PROBE_BLOCK * PROBE_ITEMS;
+    uint32_t cnt[PROBE_ITEMS], off[PROBE_ITEMS];
+    uint32_t payload[PROBE_ITEMS];
+    for (int r = 0; r < PROBE_ITEMS; r++) {
+        int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+        uint32_t c = 0;
+        uint32_t pl = 0;
+        if (i < a.n && (!a.sel || bit_at(a.sel, i))) {
+            uint64_t key;
+            bool nonnull = key64_of(a.keys, a.nk, i, &key);
+            uint64_t v = (nonnull && a.build_rows) ? lookup(a.t, key) : 0;
+            uint32_t matches = (uint32_t)(v >> 32);
+            pl = (uint32_t)v;
+            if (a.join_type == VR_JOIN_INNER) {
+                c = matches;
+            } else {
+                bool matched = matches > 0 || (a.drop_null_keys && a.build_rows && !nonnull);
+                c = (matched == (a.join_type == VR_JOIN_LEFT_SEMI)) ? 1u : 0u;
+            }
+        }
+        cnt[r] = c;
+        payload[r] = pl;
+        uint32_t total;
+        Scan(scan_tmp).ExclusiveSum(c, off[r], total);
+        if (threadIdx.x == 0) sub_total[r] = total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint64_t local = 0;
+        for (int r = 0; r < PROBE_ITEMS; r++) local += sub_total[r];
+        uint64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&a.tile_state[0], ST_PREFIX | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&a.tile_state[tile], ST_AGG | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = tile - 1;
+            while (true) {
+                uint64_t st = ld_coherent(&a.tile_state[j]);
+                if ((st & ST_MASK) == 0) continue;  // tile j has its id and publishes without waiting on us
+                excl += st & ~ST_MASK;
+                if ((st & ST_MASK) == ST_PREFIX) break;
+                j--;
+            }
+            __hip_atomic_store(&a.tile_state[tile], ST_PREFIX | (excl + local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tile == a.ntiles - 1) __hip_atomic_store(&a.tile_counter[1], excl + local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        base_sh = excl;
+    }
+    __syncthreads();
+    uint64_t base = base_sh;
+    for (int r = 0; r < PROBE_ITEMS; r++) {
+        int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+        uint64_t o = base + off[r];
+        for (uint32_t m = 0; m < cnt[r]; m++, o++) {
+            if ((int64_t)o >= a.capacity) break;
+            int64_t brow = -1;
+            if (a.join_type == VR_JOIN_INNER) brow = cnt[r] == 1 ? (int64_t)payload[r] : (int64_t)a.t.range_rows[payload[r] + m];
+            if (a.probe_idx) a.probe_idx[o] = (int32_t)i;
+            if (a.build_idx) a.build_idx[o] = (int32_t)brow;
+            for (int c = 0; c < a.nprobe_out; c++) copy_value(a.probe_src[c], i, a.outs[c], (int64_t)o);
+            for (int c = 0; c < a.nbuild_out; c++) copy_value(a.build_src[c], brow, a.outs[a.nprobe_out + c], (int64_t)o);
+        }
+        base += sub_total[r];
+    }
+}
+
+__global__ void zero_u64_kernel(uint64_t* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+__global__ void fill_keys_kernel(uint64_t* buckets, uint64_t nbuckets) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbuckets * 16; i += (uint64_t)gridDim.x * blockDim.x)
+        buckets[i] = (i % 16) < BUCKET ? EMPTY_KEY : 0ull;
+}
+__global__ void iota_kernel(uint32_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
+}
+__global__ void sentinel_rows_kernel(const uint32_t* row_cell, int64_t n, uint32_t* out, uint64_t* count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (row_cell[i] == 0xFFFFFFFEu) out[atomic_add_u64(count, 1)] = (uint32_t)i;
+}
+
+static int grid_of(uint64_t work, int block, int cap) {
+    uint64_t g = (work + block - 1) / block;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cap));
+}
+
+struct JoinImpl {
+    Device* dev = nullptr;
+    std::vector<int32_t> key_types, col_types, key_cols;
+    // build columns, device resident, grown by appends
+    struct BuildCol {
+        DevBuf data, valid;
+        bool has_valid = false;
+        size_t cap_rows = 0;
+    };
+    std::vector<BuildCol> cols;
+    int64_t rows = 0;
+    bool finished = false;
+    // table
+    DevBuf buckets, range_rows, row_cell, cell_count, cell_start, sorted_cells, rows_in, cub_tmp, small;
+    uint64_t nbuckets = 0;
+    uint64_t sentinel_value = 0;
+    int64_t distinct = 0, max_chain = 0;
+    // probe scratch
+    DevBuf tile_state, probe_stage, out_stage;
+    Staging staging;
+};
+
+static vr_status grow_col(JoinImpl& j, JoinImpl::BuildCol& c, int32_t type, size_t want_rows, hipStream_t s) {
+    if (want_rows <= c.cap_rows) return VR_OK;
+    size_t rows = std::max<size_t>(want_rows, c.cap_rows * 2);
+    rows = std::max<size_t>(rows, 1024);
+    DevBuf nd, nv;
+    VR_TRY(nd.ensure(j.dev, lane_bytes(type, (int64_t)rows) + 8));
+    VR_TRY(nv.ensure(j.dev, align_up(bitmap_bytes((int64_t)rows), 8) + 8));
+    VR_TRY_HIP(hipMemsetAsync(nv.p, 0xFF, nv.bytes, s));
+    if (j.rows) {
+        VR_TRY_HIP(hipMemcpyAsync(nd.p, c.data.p, lane_bytes(type, j.rows), hipMemcpyDeviceToDevice, s));
+        VR_TRY_HIP(hipMemcpyAsync(nv.p, c.valid.p, bitmap_bytes(j.rows), hipMemcpyDeviceToDevice, s));
+    }
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    c.data = std::move(nd);
+    c.valid = std::move(nv);
+    c.cap_rows = rows;
+    return VR_OK;
+}
+
+__global__ void append_bits_kernel(uint8_t* dst, int64_t at, const uint8_t* src, int64_t n) {
+    // dst bits [at, at + n) = src bits [0, n) (src NULL: all valid)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        bool v = !src || bit_at(src, i);
+        int64_t o = at + i;
+        unsigned int* w = (unsigned int*)dst + (o >> 5);
+        if (v) atomicOr(w, 1u << (o & 31));
+        else atomicAnd(w, ~(1u << (o & 31)));
+    }
+}
+
+}  // namespace vr
+
+using namespace vr;
+
+struct vr_join {
+    JoinImpl j;
+};
+
+extern "C" {
+
+vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out) {
+    if (!dev || !spec || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (spec->num_keys < 1 || spec->num_keys > 2) return fail(VR_ERR_UNSUPPORTED, "one or two join keys run on the device");
+    if (spec->num_columns < 1 || spec->num_columns > 8) return fail(VR_ERR_UNSUPPORTED, "at most 8 build columns");
+    std::unique_ptr<vr_join> h(new vr_join());
+    JoinImpl& j = h->j;
+    j.dev = &dev->d;
+    j.key_types.assign(spec->key_types, spec->key_types + spec->num_keys);
+    j.col_types.assign(spec->column_types, spec->column_types + spec->num_columns);
+    j.key_cols.assign(spec->key_columns, spec->key_columns + spec->num_keys);
+    for (int k = 0; k < spec->num_keys; k++) {
+        int32_t t = j.key_types[k];
+        if (j.key_cols[k] < 0 || j.key_cols[k] >= spec->num_columns || j.col_types[j.key_cols[k]] != t)
+            return fail(VR_ERR_INVALID_ARGUMENT, "join key column does not match the build columns");
+        if (t != VR_INT32 && t != VR_INT64) return fail(VR_ERR_UNSUPPORTED, "join keys other than INT32/INT64 run on the CPU kernels");
+        if (spec->num_keys == 2 && t != VR_INT32) return fail(VR_ERR_UNSUPPORTED, "two-key joins take two INT32 keys");
+    }
+    for (int32_t t : j.col_types)
+        if (t == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "UTF8 payloads run on the CPU kernels");
+    j.cols.resize(spec->num_columns);
+    VR_TRY_HIP(hipSetDevice(j.dev->ordinal));
+    if (spec->capacity_hint > 0)
+        for (int c = 0; c < spec->num_columns; c++)
+            VR_TRY(grow_col(j, j.cols[c], j.col_types[c], (size_t)spec->capacity_hint, j.dev->stream));
+    *out = h.release();
+    return VR_OK;
+}
+
+vr_status vr_join_build_append(vr_join* h, const vr_column* columns, int64_t n, void* stream) {
+    if (!h || (!columns && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    if (j.finished) return fail(VR_ERR_STATE, "the build side is already finished");
+    if (n <= 0) return n < 0 ? fail(VR_ERR_INVALID_ARGUMENT, "negative row count") : VR_OK;
+    if (j.rows + n >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "build sides past 2^31 rows run on the CPU kernels");
+    hipStream_t s = j.dev->pick(stream);
+    for (size_t c = 0; c < j.cols.size(); c++) {
+        const vr_column& col = columns[c];
+        if (col.type != j.col_types[c]) return fail(VR_ERR_INVALID_ARGUMENT, "build column lane type differs from the spec");
+        if (col.length < n) return fail(VR_ERR_INVALID_ARGUMENT, "build column shorter than the batch");
+        auto& bc = j.cols[c];
+        VR_TRY(grow_col(j, bc, col.type, (size_t)(j.rows + n), s));
+        hipMemcpyKind kind = col.mem == VR_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (col.type == VR_BOOL) {
+            return fail(VR_ERR_UNSUPPORTED, "BOOL build payloads run on the CPU kernels");
+        }
+        size_t w = lane_bytes(col.type, 1);
+        VR_TRY_HIP(hipMemcpyAsync(bc.data.as<uint8_t>() + (size_t)j.rows * w, col.data, (size_t)n * w, kind, s));
+        if (col.validity) {
+            const uint8_t* src = col.validity;
+            DevBuf tmp;
+            if (col.mem == VR_MEM_HOST) {
+                VR_TRY(tmp.ensure(j.dev, bitmap_bytes(n)));
+                VR_TRY_HIP(hipMemcpyAsync(tmp.p, col.validity, bitmap_bytes(n), hipMemcpyHostToDevice, s));
+                src = tmp.as<uint8_t>();
+            }
+            hipLaunchKernelGGL(append_bits_kernel, dim3(grid_of(n, 256, 1024)), dim3(256), 0, s, bc.valid.as<uint8_t>(),
+                               j.rows, src, n);
+            VR_TRY_HIP(hipGetLastError());
+            if (col.mem == VR_MEM_HOST) VR_TRY_HIP(hipStreamSynchronize(s));
+            bc.has_valid = true;
+        } else if (bc.has_valid) {
+            hipLaunchKernelGGL(append_bits_kernel, dim3(grid_of(n, 256, 1024)), dim3(256), 0, s, bc.valid.as<uint8_t>(),
+                               j.rows, (const uint8_t*)nullptr, n);
+        }
+        if (col.mem == VR_MEM_HOST) j.dev->count(&vr_stats::h2d_bytes, (int64_t)((size_t)n * w));
+    }
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    j.rows += n;
+    j.dev->count(&vr_stats::input_rows, n);
+    return VR_OK;
+}
+
+vr_status vr_join_build_finish(vr_join* h, void* stream) {
+    if (!h) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    if (j.finished) return fail(VR_ERR_STATE, "the build side is already finished");
+    hipStream_t s = j.dev->pick(stream);
+    const int64_t n = j.rows;
+    uint64_t cells_wanted = (uint64_t)((double)std::max<int64_t>(n, 1) / 0.75) + 1;  // at most 3/4 of the cells
+    uint64_t nb = 1;
+    while (nb * BUCKET < cells_wanted) nb <<= 1;
+    j.nbuckets = nb;
+    const uint64_t ncells = nb * BUCKET;
+    int cap = j.dev->num_cus * 16;
+    VR_TRY(j.buckets.ensure(j.dev, nb * 16 * sizeof(uint64_t)));
+    hipLaunchKernelGGL(fill_keys_kernel, dim3(grid_of(nb * 16, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(), nb);
+    VR_TRY(j.row_cell.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+    VR_TRY(j.cell_count.ensure(j.dev, ncells * 4 + 8));
+    VR_TRY(j.cell_start.ensure(j.dev, ncells * 4 + 8));
+    VR_TRY(j.small.ensure(j.dev, 64));
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(ncells / 2 + 1, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint64_t>(),
+                       ncells / 2 + 1);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>(), 8ull);
+    JoinTable t{j.buckets.as<uint64_t>(), nb - 1, nullptr, 0};
+    DCol k[2]{};
+    for (size_t q = 0; q < j.key_types.size(); q++) {
+        auto& bc = j.cols[j.key_cols[q]];
+        k[q].type = j.key_types[q];
+        k[q].data = bc.data.p;
+        k[q].valid = bc.has_valid ? bc.valid.as<uint8_t>() : nullptr;
+    }
+    if (n > 0)
+        hipLaunchKernelGGL(join_insert_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, t, k[0], k[1],
+                           (int)j.key_types.size(), n, j.row_cell.as<uint32_t>(), j.cell_count.as<uint32_t>(),
+                           j.small.as<uint64_t>());
+    VR_TRY_HIP(hipGetLastError());
+    // run starts, then the build rows ordered by cell (stable: build order inside a key)
+    size_t tmp = 0;
+    VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, j.cell_count.as<uint32_t>(), j.cell_start.as<uint32_t>(),
+                                                (int64_t)ncells, s));
+    VR_TRY(j.cub_tmp.ensure(j.dev, tmp));
+    VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(j.cub_tmp.p, tmp, j.cell_count.as<uint32_t>(), j.cell_start.as<uint32_t>(),
+                                                (int64_t)ncells, s));
+    VR_TRY(j.range_rows.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+    VR_TRY(j.sorted_cells.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+    VR_TRY(j.rows_in.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+    if (n > 0) {
+        hipLaunchKernelGGL(iota_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, j.rows_in.as<uint32_t>(), n);
+        size_t t2 = 0;
+        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, j.row_cell.as<uint32_t>(), j.sorted_cells.as<uint32_t>(),
+                                                      j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
+        VR_TRY(j.cub_tmp.ensure(j.dev, std::max(tmp, t2)));
+        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(j.cub_tmp.p, t2, j.row_cell.as<uint32_t>(), j.sorted_cells.as<uint32_t>(),
+                                                      j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
+    }
+    hipLaunchKernelGGL(join_values_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, t, j.cell_count.as<uint32_t>(),
+                       j.cell_start.as<uint32_t>(), ncells, j.range_rows.as<uint32_t>());
+    VR_TRY_HIP(hipGetLastError());
+    // the key equal to the empty marker: its rows follow every cell's rows in range_rows
+    uint64_t sentinel_rows = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&sentinel_rows, j.small.p, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    j.sentinel_value = 0;
+    if (sentinel_rows) {
+        // sorted by cell, the 0xFFFFFFFE rows sit just before the 0xFFFFFFFF (null) rows, in build order
+        uint32_t total_cells_rows = 0;
+        uint32_t last_start = 0, last_count = 0;
+        VR_TRY_HIP(hipMemcpy(&last_start, j.cell_start.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost));
+        VR_TRY_HIP(hipMemcpy(&last_count, j.cell_count.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost));
+        total_cells_rows = last_start + last_count;
+        uint64_t payload = total_cells_rows;
+        if (sentinel_rows == 1) {
+            uint32_t row = 0;
+            VR_TRY_HIP(hipMemcpy(&row, j.range_rows.as<uint32_t>() + total_cells_rows, 4, hipMemcpyDeviceToHost));
+            payload = row;
+        }
+        j.sentinel_value = (sentinel_rows << 32) | payload;
+    }
+    // statistics: distinct keys and the longest chain
+    std::vector<uint32_t> counts(ncells);
+    VR_TRY_HIP(hipMemcpy(counts.data(), j.cell_count.p, ncells * 4, hipMemcpyDeviceToHost));
+    int64_t distinct = sentinel_rows ? 1 : 0, mx = (int64_t)sentinel_rows;
+    for (uint32_t c : counts) {
+        if (c) distinct++;
+        mx = std::max<int64_t>(mx, c);
+    }
+    j.distinct = distinct;
+    j.max_chain = mx;
+    j.row_cell.reset();
+    j.sorted_cells.reset();
+    j.rows_in.reset();
+    j.cell_start.reset();
+    j.cell_count.reset();
+    j.finished = true;
+    j.dev->count(&vr_stats::kernel_launches, 7);
+    return VR_OK;
+}
+
+vr_status vr_join_info(vr_join* h, int64_t* build_rows, int64_t* distinct_keys, int64_t* max_rows_per_key) {
+    if (!h) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (build_rows) *build_rows = h->j.rows;
+    if (distinct_keys) *distinct_keys = h->j.distinct;
+    if (max_rows_per_key) *max_rows_per_key = h->j.max_chain;
+    return VR_OK;
+}
+
+vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_columns, const int32_t* key_columns, int64_t n,
+                        const uint8_t* selection, int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
+                        void* stream) {
+    if (!h || !out || (!columns && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    if (!j.finished) return fail(VR_ERR_STATE, "probe before vr_join_build_finish");
+    if (join_type < VR_JOIN_INNER || join_type > VR_JOIN_LEFT_ANTI) return fail(VR_ERR_INVALID_ARGUMENT, "join type");
+    if (join_type != VR_JOIN_INNER && out->num_build_out) return fail(VR_ERR_INVALID_ARGUMENT, "semi/anti joins output streamed columns only");
+    if (out->num_probe_out + out->num_build_out > 16 || out->num_probe_out > 8 || out->num_build_out > 8)
+        return fail(VR_ERR_UNSUPPORTED, "at most 8 + 8 output columns");
+    out->rows = 0;
+    if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
+    if (n == 0) return VR_OK;
+    if (n >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "probe batches past 2^31 rows");
+    hipStream_t s = j.dev->pick(stream);
+    ProbeArgs a{};
+    a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets - 1, j.range_rows.as<uint32_t>(), j.sentinel_value};
+    a.nk = (int)j.key_types.size();
+    a.join_type = join_type;
+    a.drop_null_keys = drop_null_keys;
+    a.build_rows = j.rows > 0 ? 1 : 0;
+    a.n = n;
+    a.capacity = out->capacity;
+    // streamed columns: device pointers, or staged from host memory
+    std::vector<std::pair<const void*, size_t>> pieces;
+    std::vector<const void**> patch;
+    auto want = [&](const void* p, size_t bytes, const void** dst, int32_t mem) {
+        *dst = p;
+        if (p && bytes && mem == VR_MEM_HOST) {
+            pieces.push_back({p, bytes});
+            patch.push_back(dst);
+        }
+    };
+    int32_t mem = VR_MEM_DEVICE;
+    for (int q = 0; q < a.nk; q++) {
+        int kc = key_columns[q];
+        if (kc < 0 || kc >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "probe key column out of range");
+        const vr_column& c = columns[kc];
+        if (c.type != j.key_types[q]) return fail(VR_ERR_INVALID_ARGUMENT, "probe key lane differs from the build key");
+        a.keys[q].type = c.type;
+        want(c.validity, bitmap_bytes(n), (const void**)&a.keys[q].valid, c.mem);
+        want(c.data, lane_bytes(c.type, n), &a.keys[q].data, c.mem);
+        mem = c.mem;
+    }
+    a.nprobe_out = out->num_probe_out;
+    a.nbuild_out = out->num_build_out;
+    for (int q = 0; q < a.nprobe_out; q++) {
+        int pc = out->probe_out_columns[q];
+        if (pc < 0 || pc >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "probe output column out of range");
+        const vr_column& c = columns[pc];
+        if (c.type == VR_UTF8 || c.type == VR_BOOL) return fail(VR_ERR_UNSUPPORTED, "UTF8/BOOL outputs run on the CPU kernels");
+        a.probe_src[q].type = c.type;
+        want(c.validity, bitmap_bytes(n), (const void**)&a.probe_src[q].valid, c.mem);
+        want(c.data, lane_bytes(c.type, n), &a.probe_src[q].data, c.mem);
+    }
+    for (int q = 0; q < a.nbuild_out; q++) {
+        int bc = out->build_out_columns[q];
+        if (bc < 0 || bc >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "build output column out of range");
+        a.build_src[q].type = j.col_types[bc];
+        a.build_src[q].data = j.cols[bc].data.p;
+        a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
+    }
+    a.sel = selection;
+    if (selection && mem == VR_MEM_HOST) want(selection, bitmap_bytes(n), (const void**)&a.sel, VR_MEM_HOST);
+    int slot = -1;
+    if (!pieces.empty()) {
+        size_t total = 0;
+        for (auto& p : pieces) total = align_up(total, 256) + p.second;
+        uint8_t *host, *dev;
+        VR_TRY(j.staging.begin(j.dev, total, &host, &dev, &slot));
+        size_t off = 0;
+        for (size_t q = 0; q < pieces.size(); q++) {
+            off = align_up(off, 256);
+            std::memcpy(host + off, pieces[q].first, pieces[q].second);
+            *patch[q] = dev + off;
+            off += pieces[q].second;
+        }
+        VR_TRY(j.staging.upload(slot, total, s));
+    }
+    // outputs: device buffers directly, host buffers through a staging area
+    const int nout = a.nprobe_out + a.nbuild_out;
+    int64_t cap = out->capacity;
+    struct Back {
+        void* host;
+        size_t off;
+        int32_t type;
+        bool bits;
+    };
+    std::vector<Back> back;
+    size_t need = 0;
+    auto place = [&](size_t bytes) {
+        need = align_up(need, 256);
+        size_t o = need;
+        need += bytes;
+        return o;
+    };
+    size_t pidx_off = 0, bidx_off = 0;
+    bool host_out = out->mem == VR_MEM_HOST;
+    if (host_out) {
+        if (out->probe_idx) pidx_off = place((size_t)cap * 4);
+        if (out->build_idx) bidx_off = place((size_t)cap * 4);
+    }
+    std::vector<size_t> voff(nout), doff(nout);
+    for (int q = 0; q < nout; q++) {
+        const vr_out_column& oc = out->columns[q];
+        int32_t t = q < a.nprobe_out ? a.probe_src[q].type : a.build_src[q - a.nprobe_out].type;
+        if (oc.type != t) return fail(VR_ERR_INVALID_ARGUMENT, "output column lane differs from its source");
+        if (host_out) {
+            if (oc.validity) voff[q] = place(align_up(bitmap_bytes(cap), 4));
+            doff[q] = place(lane_bytes(t, cap));
+        }
+    }
+    uint8_t* ob = nullptr;
+    if (need) {
+        VR_TRY(j.out_stage.ensure(j.dev, need));
+        ob = j.out_stage.as<uint8_t>();
+    }
+    a.probe_idx = host_out ? (out->probe_idx ? (int32_t*)(ob + pidx_off) : nullptr) : out->probe_idx;
+    a.build_idx = join_type == VR_JOIN_INNER ? (host_out ? (out->build_idx ? (int32_t*)(ob + bidx_off) : nullptr) : out->build_idx) : nullptr;
+    for (int q = 0; q < nout; q++) {
+        const vr_out_column& oc = out->columns[q];
+        OutCol& d = a.outs[q];
+        d.type = oc.type;
+        if (host_out) {
+            d.valid = oc.validity ? ob + voff[q] : nullptr;
+            d.data = ob + doff[q];
+        } else {
+            d.valid = oc.validity;
+            d.data = oc.data;
+        }
+        if (d.valid) {  // bits are OR-ed in: clear the words first
+            uint64_t words = (bitmap_bytes(cap) + 7) / 8;  // output bitmaps are padded to 8 bytes
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(words, 256, 1024)), dim3(256), 0, s, (uint64_t*)d.valid, words);
+        }
+    }
+    // tiles and their look-back state
+    const int64_t T = (int64_t)PROBE_BLOCK * PROBE_ITEMS;
+    a.ntiles = (n + T - 1) / T;
+    VR_TRY(j.tile_state.ensure(j.dev, (size_t)(a.ntiles + 2) * 8));
+    a.tile_counter = j.tile_state.as<uint64_t>();
+    a.tile_state = j.tile_state.as<uint64_t>() + 2;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(a.ntiles + 2, 256, 1024)), dim3(256), 0, s, j.tile_state.as<uint64_t>(),
+                       (uint64_t)(a.ntiles + 2));
+    hipLaunchKernelGGL(join_probe_kernel, dim3((unsigned)a.ntiles), dim3(PROBE_BLOCK), 0, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t rows = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&rows, j.tile_state.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    if (slot >= 0) VR_TRY(j.staging.done(slot, s));
+    int64_t written = std::min<int64_t>((int64_t)rows, cap);
+    if (host_out && written > 0) {
+        if (out->probe_idx) VR_TRY_HIP(hipMemcpy(out->probe_idx, ob + pidx_off, (size_t)written * 4, hipMemcpyDeviceToHost));
+        if (out->build_idx && join_type == VR_JOIN_INNER)
+            VR_TRY_HIP(hipMemcpy(out->build_idx, ob + bidx_off, (size_t)written * 4, hipMemcpyDeviceToHost));
+        for (int q = 0; q < nout; q++) {
+            const vr_out_column& oc = out->columns[q];
+            if (oc.validity) VR_TRY_HIP(hipMemcpy(oc.validity, ob + voff[q], bitmap_bytes(written), hipMemcpyDeviceToHost));
+            VR_TRY_HIP(hipMemcpy(oc.data, ob + doff[q], lane_bytes(oc.type, written), hipMemcpyDeviceToHost));
+        }
+    }
+    out->rows = (int64_t)rows;
+    j.dev->count(&vr_stats::calls, 1);
+    j.dev->count(&vr_stats::input_rows, n);
+    j.dev->count(&vr_stats::output_rows, written);
+    j.dev->count(&vr_stats::kernel_launches, 2);
+    if ((int64_t)rows > cap) return fail(VR_ERR_CAPACITY, "probe output needs " + std::to_string(rows) + " rows");
+    return VR_OK;
+}
+
+vr_status vr_join_memory(vr_join* h, int64_t* bytes) {
+    if (!h || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.tile_state.bytes +
+                          j.out_stage.bytes);
+    for (auto& c : j.cols) b += (int64_t)(c.data.bytes + c.valid.bytes);
+    *bytes = b;
+    return VR_OK;
+}
+
+vr_status vr_join_destroy(vr_join* h) {
+    if (!h) return VR_OK;
+    (void)hipStreamSynchronize(h->j.dev->stream);
+    delete h;
+    return VR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+// Device handle, errors, allocation accounting and pinned staging for libvrgpu.so.
+#include <cstring>
+
+#include "vr_internal.h"
+
+namespace vr {
+
+static thread_local std::string g_last_error;
+
+vr_status fail(vr_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+vr_status hip_fail(hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(VR_ERR_OOM, std::string("device memory exhausted in ") + what);
+    return fail(VR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+vr_status Device::alloc(void** p, size_t bytes) {
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (budget > 0 && allocated + (int64_t)bytes > budget)
+            return fail(VR_ERR_OOM, "HBM budget of " + std::to_string(budget) + " bytes exceeded");
+        allocated += (int64_t)bytes;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> g(mu);
+        allocated -= (int64_t)bytes;
+        (void)hipGetLastError();
+        *p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    return VR_OK;
+}
+
+void Device::release(void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    std::lock_guard<std::mutex> g(mu);
+    allocated -= (int64_t)bytes;
+}
+
+Staging::~Staging() {
+    for (auto& s : slot) {
+        if (s.pinned) (void)hipHostFree(s.pinned);
+        if (s.copied) (void)hipEventDestroy(s.copied);
+        if (s.consumed) (void)hipEventDestroy(s.consumed);
+    }
+}
+
+vr_status Staging::begin(Device* d, size_t bytes, uint8_t** host, uint8_t** device, int* which) {
+    dev = d;
+    int w = next;
+    next ^= 1;
+    Slot& s = slot[w];
+    if (!s.copied) {
+        VR_TRY_HIP(hipEventCreateWithFlags(&s.copied, hipEventDisableTiming));
+        VR_TRY_HIP(hipEventCreateWithFlags(&s.consumed, hipEventDisableTiming));
+    }
+    if (s.used) VR_TRY_HIP(hipEventSynchronize(s.consumed));  // the kernels reading it are done
+    if (s.pinned_bytes < bytes) {
+        if (s.pinned) VR_TRY_HIP(hipHostFree(s.pinned));
+        s.pinned = nullptr;
+        size_t b = align_up(bytes + bytes / 4, 1 << 20);
+        VR_TRY_HIP(hipHostMalloc(&s.pinned, b, hipHostMallocDefault));
+        s.pinned_bytes = b;
+    }
+    VR_TRY(s.device.ensure(d, align_up(bytes + bytes / 4, 1 << 20)));
+    *host = (uint8_t*)s.pinned;
+    *device = s.device.as<uint8_t>();
+    *which = w;
+    return VR_OK;
+}
+
+vr_status Staging::upload(int which, size_t bytes, hipStream_t compute) {
+    Slot& s = slot[which];
+    if (bytes) {
+        VR_TRY_HIP(hipMemcpyAsync(s.device.p, s.pinned, bytes, hipMemcpyHostToDevice, dev->copy_stream));
+        dev->count(&vr_stats::h2d_bytes, (int64_t)bytes);
+    }
+    VR_TRY_HIP(hipEventRecord(s.copied, dev->copy_stream));
+    VR_TRY_HIP(hipStreamWaitEvent(compute, s.copied, 0));
+    return VR_OK;
+}
+
+vr_status Staging::done(int which, hipStream_t compute) {
+    Slot& s = slot[which];
+    VR_TRY_HIP(hipEventRecord(s.consumed, compute));
+    s.used = true;
+    return VR_OK;
+}
+
+uint64_t hash_bytes64(const void* p, size_t n) {
+    // FNV-1a over the bytes, then a 64-bit finalizer (murmur3 fmix64).
+    const uint8_t* b = (const uint8_t*)p;
+    uint64_t h = 0xcbf29ce484222325ull ^ n;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+int32_t HostDictionary::index_of(const char* s, int32_t len, bool insert) {
+    uint64_t fp = hash_bytes64(s, (size_t)len);
+    if (index.empty()) {
+        index.assign(1024, {0, -1});
+        mask = 1023;
+    }
+    size_t pos = fp & mask;
+    while (true) {
+        auto& e = index[pos];
+        if (e.second < 0) break;
+        if (e.first == fp && values[e.second].size() == (size_t)len &&
+            std::memcmp(values[e.second].data(), s, (size_t)len) == 0)
+            return e.second;
+        pos = (pos + 1) & mask;
+    }
+    if (!insert) return -1;
+    int32_t id = (int32_t)values.size();
+    values.emplace_back(s, (size_t)len);
+    index[pos] = {fp, id};
+    if (values.size() * 2 > index.size()) {  // grow at half load
+        std::vector<std::pair<uint64_t, int32_t>> old;
+        old.swap(index);
+        index.assign(old.size() * 2, {0, -1});
+        mask = index.size() - 1;
+        for (auto& e : old) {
+            if (e.second < 0) continue;
+            size_t p = e.first & mask;
+            while (index[p].second >= 0) p = (p + 1) & mask;
+            index[p] = e;
+        }
+    }
+    return id;
+}
+
+}  // namespace vr
+
+extern "C" {
+
+const char* vr_last_error(void) { return vr::g_last_error.c_str(); }
+
+vr_status vr_device_open(int32_t ordinal, int64_t hbm_budget_bytes, vr_device** out) {
+    if (!out) return vr::fail(VR_ERR_INVALID_ARGUMENT, "null output handle");
+    int count = 0;
+    VR_TRY_HIP(hipGetDeviceCount(&count));
+    if (ordinal < 0 || ordinal >= count)
+        return vr::fail(VR_ERR_INVALID_ARGUMENT, "no device " + std::to_string(ordinal));
+    VR_TRY_HIP(hipSetDevice(ordinal));
+    auto* h = new vr_device();
+    h->d.ordinal = ordinal;
+    h->d.budget = hbm_budget_bytes;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ordinal) == hipSuccess) h->d.num_cus = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->d.copy_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return vr::hip_fail(e, "hipStreamCreate");
+    }
+    *out = h;
+    return VR_OK;
+}
+
+vr_status vr_device_close(vr_device* dev) {
+    if (!dev) return VR_OK;
+    (void)hipStreamSynchronize(dev->d.stream);
+    (void)hipStreamSynchronize(dev->d.copy_stream);
+    (void)hipStreamDestroy(dev->d.stream);
+    (void)hipStreamDestroy(dev->d.copy_stream);
+    delete dev;
+    return VR_OK;
+}
+
+vr_status vr_device_memory(vr_device* dev, int64_t* allocated, int64_t* budget) {
+    if (!dev) return vr::fail(VR_ERR_INVALID_ARGUMENT, "null device");
+    std::lock_guard<std::mutex> g(dev->d.mu);
+    if (allocated) *allocated = dev->d.allocated;
+    if (budget) *budget = dev->d.budget;
+    return VR_OK;
+}
+
+vr_status vr_device_sync(vr_device* dev, void* stream) {
+    if (!dev) return vr::fail(VR_ERR_INVALID_ARGUMENT, "null device");
+    if (stream) {
+        VR_TRY_HIP(hipStreamSynchronize((hipStream_t)stream));
+        return VR_OK;
+    }
+    VR_TRY_HIP(hipStreamSynchronize(dev->d.stream));
+    VR_TRY_HIP(hipStreamSynchronize(dev->d.copy_stream));
+    return VR_OK;
+}
+
+vr_status vr_stats_get(vr_device* dev, vr_stats* out) {
+    if (!dev || !out) return vr::fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> g(dev->d.mu);
+    *out = dev->d.stats;
+    return VR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,298 @@
+"""Python front end of libvrgpu.so (include/vr.h) through ctypes.
+
+The classes mirror the reference's operator-level use of the kernels module
+(kernels/src/main/java/io/vecruntime/kernels/GroupKeyTable.java, GroupedAccumulators.java and
+spark/.../sql/vecruntime/VectorHashJoinExec.scala's BuildTable / probe iterator): a
+GroupedAggregate is fed batches and emits keys plus Spark's partial buffers (or results); a
+BuildTable is appended to, finished, then probed batch by batch.
+
+The device library is required: importing this module on a machine where libvrgpu.so has not
+been built raises, and there is no CPU fallback here.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .columns import OutColumn, c_array
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvrgpu.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libvrgpu.so is not built (run __graft_entry__.build() or python -m vecruntime_amd.build)")
+
+_P = C.c_void_p
+_lib = C.CDLL(LIB_PATH)
+
+
+def _sig(name, args, res=C.c_int32):
+    f = getattr(_lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_sig("vr_last_error", [], C.c_char_p)
+_sig("vr_device_open", [C.c_int32, C.c_int64, C.POINTER(_P)])
+_sig("vr_device_close", [_P])
+_sig("vr_device_memory", [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+_sig("vr_device_sync", [_P, _P])
+_sig("vr_stats_get", [_P, C.POINTER(abi.VrStats)])
+_sig("vr_agg_create", [_P, C.POINTER(abi.VrAggSpec), C.POINTER(_P)])
+_sig("vr_agg_update", [_P, _P, _P, C.c_int64, _P, _P])
+_sig("vr_agg_update_filtered", [_P, _P, _P, C.c_int64, C.c_int32, C.c_int32, _P])
+_sig("vr_agg_update_batches", [_P, C.POINTER(abi.VrBatchSet), C.POINTER(abi.VrLaunchConfig), _P])
+_sig("vr_agg_num_groups", [_P, C.POINTER(C.c_int64)])
+_sig("vr_agg_output_layout", [_P, C.POINTER(C.c_int32), _P, C.c_int32])
+_sig("vr_agg_emit", [_P, C.c_int64, C.c_int64, _P, _P, _P])
+_sig("vr_agg_key_dictionary", [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int64)])
+_sig("vr_agg_memory", [_P, C.POINTER(C.c_int64)])
+_sig("vr_agg_reset", [_P])
+_sig("vr_agg_destroy", [_P])
+_sig("vr_join_create", [_P, C.POINTER(abi.VrJoinSpec), C.POINTER(_P)])
+_sig("vr_join_build_append", [_P, _P, C.c_int64, _P])
+_sig("vr_join_build_finish", [_P, _P])
+_sig("vr_join_info", [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+_sig("vr_join_probe", [_P, _P, C.c_int32, _P, C.c_int64, _P, C.c_int32, C.c_int32,
+                       C.POINTER(abi.VrProbeOutput), _P])
+_sig("vr_join_memory", [_P, C.POINTER(C.c_int64)])
+_sig("vr_join_destroy", [_P])
+
+
+def lib():
+    return _lib
+
+
+def lib_path():
+    return LIB_PATH
+
+
+class VrError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def check(st):
+    if st != abi.VR_OK:
+        raise VrError(st, _lib.vr_last_error().decode())
+
+
+def _stream(s):
+    """A HIP stream handle: None (the handle's own stream), an int, or a torch stream."""
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    return s.cuda_stream
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+class Device:
+    def __init__(self, ordinal=0, hbm_budget=0):
+        self.h = _P()
+        check(_lib.vr_device_open(ordinal, int(hbm_budget), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            _lib.vr_device_close(self.h)
+            self.h = None
+
+    def memory(self):
+        a, b = C.c_int64(), C.c_int64()
+        check(_lib.vr_device_memory(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def sync(self, stream=None):
+        check(_lib.vr_device_sync(self.h, _stream(stream)))
+
+    def stats(self):
+        s = abi.VrStats()
+        check(_lib.vr_stats_get(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in abi.VrStats._fields_}
+
+
+class GroupedAggregate:
+    """Group-by aggregation handle: GroupKeyTable.assign + GroupedAccumulators updates per batch,
+    writeKeys + the aggregate buffers (or results) at emission."""
+
+    def __init__(self, device, spec):
+        self.device = device
+        self.spec = spec
+        self.h = _P()
+        check(_lib.vr_agg_create(device.h, C.byref(spec.c), C.byref(self.h)))
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            _lib.vr_agg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, keys, values, n, selection=None, stream=None):
+        ka, va = c_array(keys), c_array(values)
+        check(_lib.vr_agg_update(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), _ptr(selection), _stream(stream)))
+
+    def update_filtered(self, keys, values, n, filter_col, threshold, stream=None):
+        ka, va = c_array(keys), c_array(values)
+        check(_lib.vr_agg_update_filtered(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), filter_col, threshold,
+                                          _stream(stream)))
+
+    def update_batches(self, batches, filter_col=-1, threshold=0, config=None, stream=None):
+        """batches: list of (keys, values, n).  One persistent launch when every column is
+        device resident."""
+        ka = c_array([c for b in batches for c in b[0]])
+        va = c_array([c for b in batches for c in b[1]])
+        ln = np.asarray([b[2] for b in batches], np.int64)
+        bs = abi.VrBatchSet(len(batches), C.cast(ka, C.POINTER(abi.VrColumn)), C.cast(va, C.POINTER(abi.VrColumn)),
+                            ln.ctypes.data_as(C.POINTER(C.c_int64)), filter_col, threshold)
+        cfg = config if config is not None else abi.VrLaunchConfig(0, 0, 0, -1)
+        check(_lib.vr_agg_update_batches(self.h, C.byref(bs), C.byref(cfg), _stream(stream)))
+        self._keep = [ka, va, ln, batches]
+
+    def num_groups(self):
+        g = C.c_int64()
+        check(_lib.vr_agg_num_groups(self.h, C.byref(g)))
+        return g.value
+
+    def output_types(self):
+        n = C.c_int32()
+        types = (C.c_int32 * 64)()
+        check(_lib.vr_agg_output_layout(self.h, C.byref(n), types, 64))
+        return list(types[: n.value])
+
+    def emit(self, start=0, stop=None, keys=None, outs=None, stream=None):
+        """Emits groups [start, stop) into host OutColumns (or the given ones); returns (keys, outs)."""
+        if stop is None:
+            stop = self.num_groups()
+        m = stop - start
+        if keys is None:
+            keys = [OutColumn(abi.INT32 if t == abi.UTF8 else t, m) for t in self.spec.key_types]
+        if outs is None:
+            outs = [OutColumn(t, m) for t in self.output_types()]
+        ka, oa = c_array(keys, abi.VrOutColumn), c_array(outs, abi.VrOutColumn)
+        check(_lib.vr_agg_emit(self.h, start, stop, C.cast(ka, _P), C.cast(oa, _P), _stream(stream)))
+        return keys, outs
+
+    def key_dictionary(self, c):
+        count, nbytes = C.c_int64(), C.c_int64()
+        check(_lib.vr_agg_key_dictionary(self.h, c, None, 0, None, 0, C.byref(count), C.byref(nbytes)))
+        offs = np.zeros(count.value + 1, np.int32)
+        data = np.zeros(max(1, nbytes.value), np.uint8)
+        check(_lib.vr_agg_key_dictionary(self.h, c, offs.ctypes.data, len(offs), data.ctypes.data, len(data),
+                                         C.byref(count), C.byref(nbytes)))
+        raw = data.tobytes()
+        return [raw[offs[i]:offs[i + 1]].decode("utf-8") for i in range(count.value)]
+
+    def memory(self):
+        b = C.c_int64()
+        check(_lib.vr_agg_memory(self.h, C.byref(b)))
+        return b.value
+
+    def reset(self):
+        check(_lib.vr_agg_reset(self.h))
+
+
+class BuildTable:
+    """A join's build side (BuildTable): appended batches, then a finished, read-only table that
+    streamed batches probe."""
+
+    def __init__(self, device, spec):
+        self.device = device
+        self.spec = spec
+        self.h = _P()
+        check(_lib.vr_join_create(device.h, C.byref(spec.c), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            _lib.vr_join_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def append(self, columns, n, stream=None):
+        ca = c_array(columns)
+        check(_lib.vr_join_build_append(self.h, C.cast(ca, _P), int(n), _stream(stream)))
+
+    def finish(self, stream=None):
+        check(_lib.vr_join_build_finish(self.h, _stream(stream)))
+
+    def info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(_lib.vr_join_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def memory(self):
+        b = C.c_int64()
+        check(_lib.vr_join_memory(self.h, C.byref(b)))
+        return b.value
+
+    def probe(self, columns, key_columns, n, join_type, selection=None, drop_null_keys=False, probe_out=(),
+              build_out=(), capacity=None, outputs=None, probe_idx=None, build_idx=None, mem=abi.MEM_HOST,
+              stream=None, want_pairs=True):
+        """Returns (probe_idx, build_idx or None, output columns, rows).  Host outputs are
+        allocated here unless given; device outputs (mem=MEM_DEVICE) must be given."""
+        if capacity is None:
+            _, _, maxc = self.info()
+            capacity = max(1, n) * max(1, maxc if join_type == abi.INNER else 1)
+        ca = c_array(columns)
+        kc = abi.int_array(list(key_columns))
+        if mem == abi.MEM_HOST:
+            if want_pairs and probe_idx is None:
+                probe_idx = np.zeros(capacity, np.int32)
+            if want_pairs and build_idx is None and join_type == abi.INNER:
+                build_idx = np.zeros(capacity, np.int32)
+            if outputs is None:
+                types = [columns[c].type for c in probe_out] + [self.spec.column_types[c] for c in build_out]
+                outputs = [OutColumn(t, capacity) for t in types]
+        outputs = outputs or []
+        oa = c_array(outputs, abi.VrOutColumn)
+        po, bo = abi.int_array(list(probe_out)), abi.int_array(list(build_out))
+        out = abi.VrProbeOutput(mem, int(capacity), _ptr(probe_idx), _ptr(build_idx) if join_type == abi.INNER else None,
+                                len(probe_out), C.cast(po, C.POINTER(C.c_int32)), len(build_out),
+                                C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        st = _lib.vr_join_probe(self.h, C.cast(ca, _P), len(columns), C.cast(kc, _P), int(n), _ptr(selection),
+                                join_type, 1 if drop_null_keys else 0, C.byref(out), _stream(stream))
+        check(st)
+        r = out.rows
+        if mem == abi.MEM_HOST:
+            return (probe_idx[:r] if probe_idx is not None else None,
+                    build_idx[:r] if build_idx is not None else None, outputs, r)
+        return probe_idx, build_idx, outputs, r
+
+
+def device_column(type, data, validity=None, length=None, dictionary=None):
+    """An input column over device tensors (torch); `dictionary` is a host Column."""
+    from .columns import Column
+    return Column(type, data, validity, dictionary=dictionary, length=length, mem=abi.MEM_DEVICE)
+
+
+def device_out_column(type, capacity, data, validity=None):
+    """An output column over device tensors (torch)."""
+    oc = OutColumn.__new__(OutColumn)
+    oc.type = type
+    oc.capacity = int(capacity)
+    oc.mem = abi.MEM_DEVICE
+    oc.validity = validity
+    oc.data = data
+    oc._c = abi.VrOutColumn(type, abi.MEM_DEVICE, int(capacity), _ptr(validity), _ptr(data), None)
+    return oc
