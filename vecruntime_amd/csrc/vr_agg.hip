@@ -1036,6 +1036,7 @@ static vr_status build_order(AggImpl& h, hipStream_t s) {
 using namespace vr;
 
 struct vr_agg {
+    DeviceRef owner;  // destroyed after the state below
     AggImpl h;
 };
 
@@ -1045,6 +1046,7 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     if (!dev || !spec || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     if (spec->num_keys > 0 && !spec->key_types) return fail(VR_ERR_INVALID_ARGUMENT, "null key types");
     std::unique_ptr<vr_agg> a(new vr_agg());
+    a->owner.hold(dev);
     AggImpl& h = a->h;
     h.dev = &dev->d;
     VR_TRY_HIP(hipSetDevice(h.dev->ordinal));

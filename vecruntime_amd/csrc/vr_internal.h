@@ -46,6 +46,9 @@ struct Device {
     int64_t allocated = 0;
     int64_t budget = 0;
     vr_stats stats{};
+    // Open handles (aggregates, joins) plus one for the vr_device_open reference: the device is
+    // freed when the last of them lets go, so a handle destroyed after vr_device_close stays valid.
+    std::atomic<int64_t> refs{1};
 
     vr_status alloc(void** p, size_t bytes);
     void release(void* p, size_t bytes);
@@ -169,3 +172,24 @@ uint64_t hash_bytes64(const void* p, size_t n);
 struct vr_device {
     vr::Device d;
 };
+
+namespace vr {
+void device_retain(vr_device* dev);
+void device_release(vr_device* dev);
+
+// Holds a reference on the device for a handle; declared before the handle's state so that the
+// state's device buffers are freed first.
+struct DeviceRef {
+    vr_device* dev = nullptr;
+    DeviceRef() = default;
+    DeviceRef(const DeviceRef&) = delete;
+    DeviceRef& operator=(const DeviceRef&) = delete;
+    void hold(vr_device* d) {
+        device_retain(d);
+        dev = d;
+    }
+    ~DeviceRef() {
+        if (dev) device_release(dev);
+    }
+};
+}  // namespace vr

@@ -320,6 +320,7 @@ __global__ void append_bits_kernel(uint8_t* dst, int64_t at, const uint8_t* src,
 using namespace vr;
 
 struct vr_join {
+    DeviceRef owner;  // destroyed after the state below
     JoinImpl j;
 };
 
@@ -330,6 +331,7 @@ vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out
     if (spec->num_keys < 1 || spec->num_keys > 2) return fail(VR_ERR_UNSUPPORTED, "one or two join keys run on the device");
     if (spec->num_columns < 1 || spec->num_columns > 8) return fail(VR_ERR_UNSUPPORTED, "at most 8 build columns");
     std::unique_ptr<vr_join> h(new vr_join());
+    h->owner.hold(dev);
     JoinImpl& j = h->j;
     j.dev = &dev->d;
     j.key_types.assign(spec->key_types, spec->key_types + spec->num_keys);

@@ -140,6 +140,20 @@ int32_t HostDictionary::index_of(const char* s, int32_t len, bool insert) {
     return id;
 }
 
+static void device_free(vr_device* dev) {
+    (void)hipStreamSynchronize(dev->d.stream);
+    (void)hipStreamSynchronize(dev->d.copy_stream);
+    (void)hipStreamDestroy(dev->d.stream);
+    (void)hipStreamDestroy(dev->d.copy_stream);
+    delete dev;
+}
+
+void device_retain(vr_device* dev) { dev->d.refs.fetch_add(1); }
+
+void device_release(vr_device* dev) {
+    if (dev->d.refs.fetch_sub(1) == 1) device_free(dev);
+}
+
 }  // namespace vr
 
 extern "C" {
@@ -170,11 +184,7 @@ vr_status vr_device_open(int32_t ordinal, int64_t hbm_budget_bytes, vr_device** 
 
 vr_status vr_device_close(vr_device* dev) {
     if (!dev) return VR_OK;
-    (void)hipStreamSynchronize(dev->d.stream);
-    (void)hipStreamSynchronize(dev->d.copy_stream);
-    (void)hipStreamDestroy(dev->d.stream);
-    (void)hipStreamDestroy(dev->d.copy_stream);
-    delete dev;
+    vr::device_release(dev);  // freed now, or when its last handle is destroyed
     return VR_OK;
 }
 
