@@ -113,6 +113,41 @@ def emit_buffers(agg, groups):
     return keys, outs
 
 
+def unpack_bits_torch(bm, n):
+    import torch
+    bits = (bm.view(-1, 1) >> torch.arange(8, device=bm.device, dtype=torch.uint8)) & 1
+    return bits.view(-1)[:n].bool()
+
+
+def verify_groupby(keys, dec, dbl, i32, valid, n, groups, ek, eo):
+    """Size-independent properties of the full-size result against the input, computed by torch:
+    distinct keys, total rows, exact decimal total and non-null count, global MIN/MAX, the double
+    total.  Outputs: [SUM dec (d128), isEmpty, AVG dec sum (d128), AVG dec count, AVG f64 sum,
+    AVG f64 count, MIN i32, MAX i32, COUNT(*)]."""
+    import torch
+    v = [unpack_bits_torch(b, n) for b in valid]
+    out = {}
+    out["distinct_keys"] = int(torch.unique(keys).numel()) == groups
+    cnt = eo[8].data[:groups]
+    out["count_star_total"] = int(cnt.sum().item()) == n
+    dsum = eo[0].data[:2 * groups].view(groups, 2)
+    dok = unpack_bits_torch(eo[0].validity, groups)
+    # every per-group sum fits 64 bits here (< 2^63): compare the low limbs' total exactly
+    exact_total = int(dec[v[0]].sum().item())
+    out["decimal_sum_total"] = bool((dsum[:, 1][dok] == 0).all().item()) and int(dsum[:, 0][dok].sum().item()) == exact_total
+    out["decimal_count_total"] = int(eo[3].data[:groups].sum().item()) == int(v[0].sum().item())
+    out["f64_count_total"] = int(eo[5].data[:groups].sum().item()) == int(v[1].sum().item())
+    fs = eo[4].data[:groups].sum().item()
+    ft = dbl[v[1]].sum().item()
+    out["f64_sum_rel_err"] = abs(fs - ft) / abs(ft)
+    mn_ok = unpack_bits_torch(eo[6].validity, groups)
+    mx_ok = unpack_bits_torch(eo[7].validity, groups)
+    out["min_i32"] = int(eo[6].data[:groups][mn_ok].min().item()) == int(i32[v[2]].min().item())
+    out["max_i32"] = int(eo[7].data[:groups][mx_ok].max().item()) == int(i32[v[2]].max().item())
+    out["ok"] = all(x for k, x in out.items() if k != "f64_sum_rel_err") and out["f64_sum_rel_err"] < 1e-9
+    return out
+
+
 def run_groupby(dev, args, n):
     import torch
     from vecruntime_amd import abi, vr
@@ -121,7 +156,8 @@ def run_groupby(dev, args, n):
     keys, dec, dbl, i32, valid = gen_groupby(n)
     batches = groupby_batches(keys, dec, dbl, i32, valid, n)
     log("generated in %.1f s: %d batches" % (time.time() - t0, len(batches)))
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000))
 
     def step_batched(emit_cols):
@@ -129,10 +165,12 @@ def run_groupby(dev, args, n):
         agg.update_batches(batches, stream=stream)
         agg.emit(0, emit_cols[2], emit_cols[0], emit_cols[1], stream=stream)
 
+    prepared = [(vr.GroupedAggregate.prepare(k, v), m) for k, v, m in batches]
+
     def step_plain(emit_cols):
         agg.reset()
-        for k, v, m in batches:
-            agg.update(k, v, m, stream=stream)
+        for prep, m in prepared:
+            agg.update_prepared(prep, m, stream=stream)
         agg.emit(0, emit_cols[2], emit_cols[0], emit_cols[1], stream=stream)
 
     agg.update_batches(batches, stream=stream)
@@ -140,6 +178,9 @@ def run_groupby(dev, args, n):
     ek, eo = emit_buffers(agg, groups)
     cols = (ek, eo, groups)
     log("groups: %d" % groups)
+    agg.emit(0, groups, ek, eo, stream=stream)
+    checks = verify_groupby(keys, dec, dbl, i32, valid, n, groups, ek, eo)
+    log("checks: %s" % json.dumps(checks))
     res = {}
     for name, fn, steps in (("batched", step_batched, args.steps), ("plain", step_plain, max(1, args.steps // 2))):
         for _ in range(args.warmup):
@@ -167,6 +208,7 @@ def run_groupby(dev, args, n):
         kt.append(e0.elapsed_time(e1))
     res["update_kernel_ms"] = float(np.mean(kt))
     res["groups"] = groups
+    res["checks"] = checks
     # a bounded sample copied to the host for the CPU baseline
     res["_sample"] = None
     if not args.skip_cpu:
@@ -177,6 +219,142 @@ def run_groupby(dev, args, n):
         res["_sample"] = (host, s_batches)
     agg.close()
     del batches, keys, dec, dbl, i32, valid
+    torch.cuda.empty_cache()
+    return res
+
+
+def gen_uneven(seed=104, scale=1.0):
+    """64 partitions x U{1..200} batches of round(log-uniform[1e3, 1e6]) rows, interleaved across
+    partitions; columns f (filter int32), k1 int32 1..1000, k2 int64 Zipf(1.2) over 1M (scrambled),
+    k3 dictionary codes over 10,000 strings of 8-24 ASCII chars, dec(15,2) unscaled
+    [-1e13, 1e13), f64 uniform [-1e6, 1e6)."""
+    import torch
+    from vecruntime_amd import abi
+    from vecruntime_amd.columns import Column
+    rng = np.random.default_rng(seed)
+    per_part = rng.integers(1, 201, 64)
+    sizes = [[int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)) ))) for _ in range(nb)] for nb in per_part]
+    if scale != 1.0:
+        sizes = [[max(1, int(x * scale)) for x in p] for p in sizes]
+    order = []
+    for j in range(int(per_part.max())):
+        for p in range(64):
+            if j < len(sizes[p]):
+                order.append(sizes[p][j])
+    total = int(sum(order))
+    words = ["".join(chr(c) for c in rng.integers(33, 127, int(rng.integers(8, 25)))) for _ in range(10_000)]
+    dictionary = Column.strings(words)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    f = torch.randint(0, 2 ** 31, (total,), dtype=torch.int32, device="cuda", generator=g)
+    k1 = torch.randint(1, 1001, (total,), dtype=torch.int32, device="cuda", generator=g)
+    k2 = scramble(zipf_ranks(g, total, 1_000_000, 1.2))
+    k3 = torch.randint(0, 10_000, (total,), dtype=torch.int32, device="cuda", generator=g)
+    dec = torch.randint(-10 ** 13, 10 ** 13, (total,), dtype=torch.int64, device="cuda", generator=g)
+    mx = torch.rand(total, dtype=torch.float64, device="cuda", generator=g) * 2e6 - 1e6
+    torch.cuda.synchronize()
+    return order, total, dictionary, (f, k1, k2, k3, dec, mx)
+
+
+def uneven_spec(capacity_hint, max_load_percent=0):
+    from vecruntime_amd import abi
+    return abi.AggSpec([abi.INT32, abi.INT64, abi.UTF8], [abi.INT32, abi.INT64, abi.FLOAT64],
+                       [abi.agg_desc(abi.SUM_WIDE, 1, buffer_precision=25, buffer_scale=2),
+                        abi.agg_desc(abi.COUNT_STAR), abi.agg_desc(abi.MAX, 2)],
+                       key_nullable=[False] * 3, capacity_hint=capacity_hint, max_load_percent=max_load_percent)
+
+
+def uneven_batches(order, dictionary, cols, lo=0, hi=None):
+    from vecruntime_amd import abi, vr
+    f, k1, k2, k3, dec, mx = cols
+    out = []
+    off = 0
+    for bi, n in enumerate(order):
+        if hi is not None and bi >= hi:
+            break
+        if bi >= lo:
+            sl = slice(off, off + n)
+            out.append(([vr.device_column(abi.INT32, k1[sl]), vr.device_column(abi.INT64, k2[sl]),
+                         vr.device_column(abi.UTF8, k3[sl], dictionary=dictionary, length=n)],
+                        [vr.device_column(abi.INT32, f[sl]), vr.device_column(abi.INT64, dec[sl]),
+                         vr.device_column(abi.FLOAT64, mx[sl])], n))
+        off += n
+    return out
+
+
+def run_uneven(dev, args):
+    """Batched uneven workload: one persistent launch sequence vs one launch per batch, with the
+    autotuner's launch shape for the batched path."""
+    import torch
+    from vecruntime_amd import abi, autotune, vr
+    log("generating the uneven batched input")
+    t0 = time.time()
+    order, total, dictionary, cols = gen_uneven(scale=args.uneven_scale)
+    batches = uneven_batches(order, dictionary, cols)
+    thr = int(0.3 * 2 ** 31)
+    passing = int((cols[0] < thr).sum().item())
+    log("generated %d rows in %d batches (%d pass the filter) in %.1f s" % (total, len(batches), passing, time.time() - t0))
+    stream = torch.cuda.current_stream()
+    # autotune the batched launch on a sample (the first ~6% of the batches)
+    sample = batches[: max(8, len(batches) // 16)]
+    srows = sum(b[2] for b in sample)
+    handles = {}
+
+    def run(c):
+        m = c["max_load_percent"]
+        if m not in handles:
+            handles[m] = vr.GroupedAggregate(dev, uneven_spec(int(srows * 0.3), m))
+        h = handles[m]
+        h.reset()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        h.update_batches(sample, filter_col=0, threshold=thr, config=autotune.to_launch_config(c), stream=stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    props = torch.cuda.get_device_properties(0)
+    key = autotune.signature(props.name, uneven_spec(0), "uneven-filter30")
+    t0 = time.time()
+    best, results = autotune.tuned(key, run, autotune.candidates(), reps=2, budget_s=args.tune_budget)
+    for h in handles.values():
+        h.close()
+    log("autotune: %s (%.1f s%s)" % (best, time.time() - t0, "" if results else ", cached"))
+    default_cfg = abi.launch_config()
+    tuned_cfg = autotune.to_launch_config(best)
+    agg = vr.GroupedAggregate(dev, uneven_spec(int(passing * 0.8), best["max_load_percent"]))
+    agg.set_launch_config(tuned_cfg)  # the per-batch launches use the tuned shape too
+    prepared = [(vr.GroupedAggregate.prepare(k, v), n) for k, v, n in batches]
+    res = {"rows": total, "batches": len(batches), "passing_rows": passing, "tuned": best}
+
+    def batched(cfg):
+        agg.reset()
+        agg.update_batches(batches, filter_col=0, threshold=thr, config=cfg, stream=stream)
+        torch.cuda.synchronize()
+
+    def plain():
+        agg.reset()
+        for prep, n in prepared:
+            agg.update_prepared(prep, n, 0, thr, stream=stream)
+        torch.cuda.synchronize()
+
+    groups = {}
+    for name, fn in (("batched_tuned", lambda: batched(tuned_cfg)), ("batched_default", lambda: batched(default_cfg)),
+                     ("plain", plain)):
+        fn()  # warm (and sizes the table)
+        ts = []
+        for _ in range(max(1, args.uneven_steps)):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        dt = min(ts)
+        groups[name] = agg.num_groups()
+        res[name] = dict(ms=dt * 1e3, rows_per_s=total / dt)
+        log("uneven %s: %.1f ms, %.3e rows/s, %d groups" % (name, dt * 1e3, total / dt, groups[name]))
+    res["batched_over_plain"] = res["batched_tuned"]["rows_per_s"] / res["plain"]["rows_per_s"]
+    res["groups"] = groups["plain"]
+    res["paths_agree"] = len(set(groups.values())) == 1
+    agg.close()
+    del batches, prepared, cols
     torch.cuda.empty_cache()
     return res
 
@@ -263,6 +441,10 @@ def main():
     ap.add_argument("--cpu-batches", type=int, default=600)
     ap.add_argument("--join-build", type=int, default=10_000_000)
     ap.add_argument("--join-probe", type=int, default=40_000_000)
+    ap.add_argument("--skip-uneven", action="store_true")
+    ap.add_argument("--uneven-scale", type=float, default=1.0)
+    ap.add_argument("--uneven-steps", type=int, default=2)
+    ap.add_argument("--tune-budget", type=float, default=60.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -275,6 +457,7 @@ def main():
     n = args.rows
     gb = run_groupby(dev, args, n)
     join = None if args.skip_join else run_join(dev, args)
+    uneven = None if args.skip_uneven else run_uneven(dev, args)
     # roofline of the dominant kernel: algorithmic bytes per launch / its measured duration
     bytes_per_row = 8 + 8 + 8 + 4 + 3 / 8.0
     alg_bytes = n * bytes_per_row
@@ -311,6 +494,7 @@ def main():
         "plain": {"rows_per_s": gb["plain"]["rows_per_s"], "ms_per_step": gb["plain"]["ms_per_step"],
                   "batched_over_plain": gb["batched"]["rows_per_s"] / gb["plain"]["rows_per_s"]},
         "join": join,
+        "batched_uneven": uneven,
     }
     dev.close()
     print(json.dumps(line), flush=True)

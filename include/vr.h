@@ -186,7 +186,8 @@ typedef struct vr_agg_spec {
      * reference's strict mode (GroupedAccumulators.DoubleSum(strict), GroupedAccumulators.java:202-206).
      * 0: group order unspecified, double sums by device atomics (within 1e-12 relative). */
     int32_t deterministic;
-    int32_t reserved;
+    /* Largest fraction of the table's slots in use before it grows, in percent (0: 50). */
+    int32_t max_load_percent;
 } vr_agg_spec;
 
 typedef struct vr_agg vr_agg;
@@ -231,13 +232,19 @@ typedef struct vr_batch_set {
     int32_t filter_threshold;
 } vr_batch_set;
 
-/* Launch shape for the persistent kernel; zeros pick the tuned defaults. */
+/* Launch shape of the update kernel; zeros pick the defaults. */
 typedef struct vr_launch_config {
-    int32_t block_threads;
-    int32_t rows_per_thread;
-    int32_t blocks_per_cu;
-    int32_t lds_preaggregate;   /* 1: LDS pre-aggregation before global atomics, 0: off, -1: auto */
+    int32_t block_threads;      /* fixed at 256 (the kernel's launch bounds); reserved */
+    int32_t rows_per_thread;    /* rows per thread per tile */
+    int32_t blocks_per_cu;      /* workgroups per CU in the persistent grid */
+    int32_t lds_preaggregate;   /* 1: the LDS hot-group cache, 0: off, -1: default (on) */
+    int32_t lds_entries;        /* hot-group cache entries (a power of two, <= 2048) */
+    int32_t flush_tiles;        /* tiles between flushes of the cache */
 } vr_launch_config;
+
+/* Sets the handle's default launch shape (what vr_agg_update uses, and what
+ * vr_agg_update_batches uses for fields its own config leaves at zero). */
+vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg);
 
 vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* batches,
                                 const vr_launch_config* cfg, void* stream);

@@ -94,7 +94,7 @@ class VrAggSpec(C.Structure):
         ("aggs", C.POINTER(VrAggDesc)),
         ("capacity_hint", C.c_int64),
         ("deterministic", C.c_int32),
-        ("reserved", C.c_int32),
+        ("max_load_percent", C.c_int32),
     ]
 
 
@@ -128,7 +128,13 @@ class VrLaunchConfig(C.Structure):
         ("rows_per_thread", C.c_int32),
         ("blocks_per_cu", C.c_int32),
         ("lds_preaggregate", C.c_int32),
+        ("lds_entries", C.c_int32),
+        ("flush_tiles", C.c_int32),
     ]
+
+
+def launch_config(rows_per_thread=0, blocks_per_cu=0, lds=-1, lds_entries=0, flush_tiles=0):
+    return VrLaunchConfig(0, rows_per_thread, blocks_per_cu, lds, lds_entries, flush_tiles)
 
 
 class VrJoinSpec(C.Structure):
@@ -173,7 +179,7 @@ class AggSpec:
     """Owns a VrAggSpec and the arrays it points at."""
 
     def __init__(self, key_types, value_types, aggs, key_nullable=None, capacity_hint=0,
-                 deterministic=False):
+                 deterministic=False, max_load_percent=0):
         self.key_types = list(key_types)
         self.value_types = list(value_types)
         self.aggs = list(aggs)
@@ -188,7 +194,7 @@ class AggSpec:
         self.c = VrAggSpec(len(self.key_types), C.cast(self._kt, C.POINTER(C.c_int32)), kn,
                            len(self.value_types), C.cast(self._vt, C.POINTER(C.c_int32)),
                            len(self.aggs), C.cast(self._aggs, C.POINTER(VrAggDesc)),
-                           int(capacity_hint), 1 if deterministic else 0, 0)
+                           int(capacity_hint), 1 if deterministic else 0, int(max_load_percent))
 
     def output_types(self):
         """The lanes an emission writes, aggregate by aggregate (Spark's buffer or result)."""

@@ -16,33 +16,59 @@ namespace vr {
 
 // ------------------------------------------------------------------ kernels
 
-__device__ __forceinline__ int lds_slot(uint64_t* lkeys, int E, uint64_t slot) {
-    uint32_t h = (uint32_t)(fmix64(slot)) & (uint32_t)(E - 1);
+constexpr int AGG_BLOCK = 256;
+constexpr int MAX_KEEP = 8;  // LDS entries one thread re-inserts after an eviction (E <= 2048)
+
+// Index of `key` in the LDS cache (claiming an empty entry when `insert`), -1 when absent or full.
+__device__ __forceinline__ int lds_find(uint64_t* lkeys, int E, uint64_t key, bool insert) {
+    uint32_t h = (uint32_t)(fmix64(key)) & (uint32_t)(E - 1);
     for (int p = 0; p < 16; p++) {
         uint64_t k = lkeys[h];
-        if (k == slot) return (int)h;
+        if (k == key) return (int)h;
         if (k == EMPTY_KEY) {
+            if (!insert) return -1;
             uint64_t old = atomicCAS((unsigned long long*)&lkeys[h], (unsigned long long)EMPTY_KEY,
-                                     (unsigned long long)slot);
-            if (old == EMPTY_KEY || old == slot) return (int)h;
+                                     (unsigned long long)key);
+            if (old == EMPTY_KEY || old == key) return (int)h;
         }
         h = (h + 1) & (uint32_t)(E - 1);
     }
     return -1;
 }
 
-__device__ __forceinline__ void lds_reset(const UpdateArgs& a, uint64_t* lkeys, uint64_t* lstate) {
-    const int E = a.lds_entries, S = a.lds_words;
-    for (int e = threadIdx.x; e < E; e += blockDim.x) lkeys[e] = EMPTY_KEY;
+__device__ __forceinline__ void lds_clear_state(const UpdateArgs& a, uint64_t* lstate, int E, int S) {
     for (int w = threadIdx.x; w < E * S; w += blockDim.x) lstate[w] = a.ident[w % S];
 }
 
-__device__ __forceinline__ void lds_flush(const UpdateArgs& a, uint64_t* lkeys, uint64_t* lstate) {
-    const int E = a.lds_entries, S = a.lds_words;
+// Flushes every LDS entry into its global row.  With `evict`, entries that took fewer than two
+// rows since the last flush give their place up; the others stay cached (with cleared partials).
+// `by_key`: entries are keyed by the group's key (the global slot is found here), else by slot.
+__device__ __forceinline__ void lds_flush(const UpdateArgs& a, uint64_t* lkeys, uint64_t* lstate, int E, bool by_key,
+                                          bool evict, unsigned int* inserted) {
+    const int S = a.t.stride;
+    uint64_t keep[MAX_KEEP];
+    int nkeep = 0;
     for (int e = threadIdx.x; e < E; e += blockDim.x) {
-        uint64_t slot = lkeys[e];
-        if (slot != EMPTY_KEY) flush_entry(a, lstate + (size_t)e * S, a.t.state + slot * (uint64_t)a.t.stride);
+        uint64_t k = lkeys[e];
+        if (k == EMPTY_KEY) continue;
+        uint64_t* l = lstate + (size_t)e * S;
+        uint64_t slot = k;
+        if (by_key) {
+            bool ins;
+            slot = find_or_insert64(a.t, k, &ins);
+            if (ins) atomicAdd(inserted, 1u);
+        }
+        bool hot = l[0] >= 2;  // word 0 is the group's ROWS count
+        if (l[0]) flush_entry(a, l, a.t.state + slot * (uint64_t)S);
+        if (evict && hot && nkeep < MAX_KEEP) keep[nkeep++] = k;
     }
+    if (!evict) return;
+    __syncthreads();
+    for (int e = threadIdx.x; e < E; e += blockDim.x) lkeys[e] = EMPTY_KEY;
+    lds_clear_state(a, lstate, E, S);
+    __syncthreads();
+    for (int q = 0; q < nkeep; q++) (void)lds_find(lkeys, E, keep[q], true);
+    __syncthreads();
 }
 
 // Locates the batch of a tile in a batched launch (tile_begin ascending).
@@ -57,20 +83,26 @@ __device__ __forceinline__ int find_batch(const BatchDesc* b, int nb, int64_t ti
 }
 
 template <bool MULTI>
-__global__ __launch_bounds__(256) void agg_update_kernel(UpdateArgs a) {
+__global__ __launch_bounds__(AGG_BLOCK) void agg_update_kernel(UpdateArgs a) {
     extern __shared__ uint64_t lds[];
-    __shared__ unsigned long long inserted_groups;
+    __shared__ unsigned int inserted;
     const int E = a.lds_entries;
+    const int S = a.t.stride;
     uint64_t* lkeys = lds;
     uint64_t* lstate = lds + E;
-    if (threadIdx.x == 0) inserted_groups = 0;
-    if (E) lds_reset(a, lkeys, lstate);
+    // 64-bit keys are cached by key (an LDS hit touches no global memory); wide keys, and every
+    // key in the deterministic mode (which records each row's slot), by slot.
+    const bool by_key = a.ks.mode != KM_WIDE && !a.det;
+    if (threadIdx.x == 0) inserted = 0;
+    if (E) {
+        for (int e = threadIdx.x; e < E; e += blockDim.x) lkeys[e] = EMPTY_KEY;
+        lds_clear_state(a, lstate, E, S);
+    }
     __syncthreads();
     const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
     const int64_t ntiles = MULTI ? a.total_tiles : (a.one.n + T - 1) / T;
-    const int lane = __lane_id();
     int since_flush = 0;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int64_t tile = (MULTI ? a.tile_lo : 0) + blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const BatchDesc* bp;
         int64_t row0;
         if (MULTI) {
@@ -90,40 +122,73 @@ __global__ __launch_bounds__(256) void agg_update_kernel(UpdateArgs a) {
                 const DCol& f = b.vals[a.filter_col];
                 active = is_valid(f, i) && ld_stream((const int32_t*)f.data + i) < a.filter_threshold;
             }
-            bool ins = false;
             uint64_t slot = ~0ull;
             if (active) {
-                slot = row_slot_of(a, b, i, &ins, true);
-                uint64_t* grow = a.t.state + slot * (uint64_t)a.t.stride;
-                uint64_t* lrow = nullptr;
-                if (E) {
-                    int e = lds_slot(lkeys, E, slot);
-                    if (e >= 0) lrow = lstate + (size_t)e * a.lds_words;
+                bool ins = false;
+                uint64_t* row = nullptr;
+                if (by_key) {
+                    uint64_t key;
+                    bool keyed = key64_of(a, b, i, &key, &slot);
+                    if (keyed && key != EMPTY_KEY && E) {
+                        int e = lds_find(lkeys, E, key, true);
+                        if (e >= 0) row = lstate + (size_t)e * S;
+                    }
+                    if (!row) {
+                        if (keyed) slot = find_or_insert64(a.t, key, &ins);
+                        else mark_reserved(a.t.counters, RES_NULL);
+                        row = a.t.state + slot * (uint64_t)S;
+                    }
+                } else {
+                    slot = row_slot_of(a, b, i, &ins, true);
+                    if (E) {
+                        int e = lds_find(lkeys, E, slot, true);
+                        if (e >= 0) row = lstate + (size_t)e * S;
+                    }
+                    if (!row) row = a.t.state + slot * (uint64_t)S;
                 }
-                apply_ops(a, b, i, lrow, grow);
+                if (ins) atomicAdd(&inserted, 1u);
+                apply_ops(a, b, i, row);
             }
             if (!MULTI && a.row_slot && i < n) a.row_slot[i] = active ? (uint32_t)slot : 0xFFFFFFFFu;
-            uint64_t m = __ballot(ins);
-            if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&inserted_groups, (unsigned long long)__popcll(m));
         }
         if (E && ++since_flush >= a.flush_tiles) {
             __syncthreads();
-            lds_flush(a, lkeys, lstate);
-            __syncthreads();
-            lds_reset(a, lkeys, lstate);
-            __syncthreads();
+            lds_flush(a, lkeys, lstate, E, by_key, true, &inserted);
             since_flush = 0;
         }
     }
     __syncthreads();
-    if (E && since_flush) lds_flush(a, lkeys, lstate);
-    if (threadIdx.x == 0 && inserted_groups) atomic_add_u64(&a.t.counters[C_GROUPS], inserted_groups);
+    if (E && since_flush) lds_flush(a, lkeys, lstate, E, by_key, false, &inserted);
+    __syncthreads();
+    if (threadIdx.x == 0 && inserted) atomic_add_u64(&a.t.counters[C_GROUPS], inserted);
 }
 
-// Fills state rows [0, rows) with the identity row.
-__global__ void init_state_kernel(uint64_t* state, uint64_t words, int stride, UpdateArgs a) {
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
-        state[w] = a.ident[w % (uint64_t)stride];
+// Fills state rows [0, rows) with the identity row.  A wave covers 64 consecutive rows and its
+// lanes store consecutive words, so the stores coalesce whatever the row width.
+__global__ void init_state_kernel(uint64_t* state, uint64_t rows, UpdateArgs a) {
+    const int S = a.t.stride;
+    const int lane = threadIdx.x & 63;
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t wave = (((uint64_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6;
+    for (uint64_t r0 = wave * 64; r0 < rows; r0 += waves * 64) {
+        const uint64_t nrows = rows - r0 < 64 ? rows - r0 : 64;
+        uint64_t* d = state + r0 * (uint64_t)S;
+        const int words = (int)nrows * S;
+        for (int j = lane; j < words; j += 64) d[j] = a.ident[j % S];
+    }
+}
+
+// Resets the listed rows (an emission order): key cells empty, state at identity.
+__global__ void reset_rows_kernel(const uint64_t* order, int64_t m, uint64_t* cells, int W, uint64_t slots, uint64_t empty,
+                                  UpdateArgs a) {
+    const int S = a.t.stride;
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < m; o += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t r = order[o];
+        if (r < slots)
+            for (int k = 0; k < W; k++) cells[r * (uint64_t)W + k] = empty;
+        uint64_t* d = a.t.state + r * (uint64_t)S;
+        for (int k = 0; k < S; k++) d[k] = a.ident[k];
+    }
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
@@ -225,7 +290,7 @@ __global__ void nan_payload_kernel(UpdateArgs a) {
         return;
     }
     const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
-    for (int64_t tile = blockIdx.x; tile < a.total_tiles; tile += gridDim.x) {
+    for (int64_t tile = a.tile_lo + blockIdx.x; tile < a.total_tiles; tile += gridDim.x) {
         const BatchDesc& b = a.batches[find_batch(a.batches, a.num_batches, tile)];
         int64_t row0 = (tile - b.tile_begin) * T;
         int64_t row1 = row0 + T < b.n ? row0 + T : b.n;
@@ -233,14 +298,45 @@ __global__ void nan_payload_kernel(UpdateArgs a) {
     }
 }
 
-struct OccupiedSlot {
-    const uint64_t* cells;
-    int width;
-    __host__ __device__ bool operator()(const uint64_t& s) const {
-        uint64_t c = cells[s * (uint64_t)width];
-        return width == 1 ? c != EMPTY_KEY : c != 0;
+// ---- emission order: occupied slots in slot order, by a count / scan / write compaction
+
+constexpr int ORDER_CHUNK = 4096;  // slots per block
+
+__device__ __forceinline__ bool occupied(const uint64_t* cells, int W, uint64_t s) {
+    uint64_t c = cells[s * (uint64_t)W];
+    return W == 1 ? c != EMPTY_KEY : c != 0;
+}
+
+__global__ __launch_bounds__(256) void count_occupied_kernel(const uint64_t* cells, int W, uint64_t slots,
+                                                             uint64_t* block_counts) {
+    typedef hipcub::BlockReduce<uint32_t, 256> Reduce;
+    __shared__ typename Reduce::TempStorage tmp;
+    uint64_t base = (uint64_t)blockIdx.x * ORDER_CHUNK;
+    uint32_t c = 0;
+    for (int k = threadIdx.x; k < ORDER_CHUNK; k += 256) {
+        uint64_t s = base + k;
+        if (s < slots && occupied(cells, W, s)) c++;
     }
-};
+    uint32_t total = Reduce(tmp).Sum(c);
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void write_occupied_kernel(const uint64_t* cells, int W, uint64_t slots,
+                                                             const uint64_t* block_offsets, uint64_t* order) {
+    typedef hipcub::BlockScan<uint32_t, 256> Scan;
+    __shared__ typename Scan::TempStorage tmp;
+    uint64_t base = (uint64_t)blockIdx.x * ORDER_CHUNK;
+    uint64_t out = block_offsets[blockIdx.x];
+    for (int k0 = 0; k0 < ORDER_CHUNK; k0 += 256) {  // slot order: a 256-slot window at a time
+        uint64_t s = base + k0 + threadIdx.x;
+        uint32_t flag = (s < slots && occupied(cells, W, s)) ? 1u : 0u;
+        uint32_t pos, total;
+        Scan(tmp).ExclusiveSum(flag, pos, total);
+        if (flag) order[out + pos] = s;
+        out += total;
+        __syncthreads();
+    }
+}
 
 __global__ void first_rows_kernel(const uint64_t* order, int64_t m, const uint64_t* state, int stride, int off,
                                   uint64_t* out) {
@@ -259,8 +355,8 @@ struct OutCol {
 
 struct EmitAgg {
     int32_t kind, flags, op_code, off;
-    int32_t in_type, out0, bp, bs;
-    int32_t rp, rs, bit, pad;
+    int32_t nulls_off, in_type, out0, bp;
+    int32_t bs, rp, rs, bit;
 };
 
 struct EmitArgs {
@@ -292,6 +388,48 @@ __device__ __forceinline__ void put_d128(const OutCol& c, int64_t o, bool ok, ui
         ((uint64_t*)c.data)[2 * o] = ok ? lo : 0;
         ((uint64_t*)c.data)[2 * o + 1] = ok ? hi : 0;
     }
+}
+
+// t (192 bits, little-endian limbs) += v * 2^shift, v unsigned or sign-extended.
+__device__ __forceinline__ void add_shifted(uint64_t* t, uint64_t v, bool is_signed, int shift) {
+    uint64_t ext = (is_signed && (int64_t)v < 0) ? ~0ull : 0ull;
+    uint64_t x[3] = {v, ext, ext};  // v as 192 bits
+    if (shift >= 64) {
+        x[2] = x[1];
+        x[1] = x[0];
+        x[0] = 0;
+        shift -= 64;
+    }
+    if (shift) {
+        x[2] = (x[2] << shift) | (x[1] >> (64 - shift));
+        x[1] = (x[1] << shift) | (x[0] >> (64 - shift));
+        x[0] <<= shift;
+    }
+    uint64_t c = 0;
+    for (int j = 0; j < 3; j++) {
+        uint64_t s1 = t[j] + x[j];
+        uint64_t c1 = s1 < t[j] ? 1ull : 0ull;
+        uint64_t s2 = s1 + c;
+        c = c1 + (s2 < s1 ? 1ull : 0ull);
+        t[j] = s2;
+    }
+}
+
+// The exact total of a sum state as 192 bits; `fits` = it is a signed 128-bit value.
+__device__ __forceinline__ void sum_total(const uint64_t* w, bool d128, uint64_t* lo, uint64_t* hi, bool* fits) {
+    uint64_t t[3] = {0, 0, 0};
+    if (d128) {
+        add_shifted(t, w[0], false, 0);
+        add_shifted(t, w[1], false, 32);
+        add_shifted(t, w[2], false, 64);
+        add_shifted(t, w[3], true, 96);
+    } else {
+        add_shifted(t, w[0], false, 0);
+        add_shifted(t, w[1], true, 32);
+    }
+    *lo = t[0];
+    *hi = t[1];
+    *fits = t[2] == (uint64_t)((int64_t)t[1] >> 63);
 }
 
 __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
@@ -328,7 +466,7 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
             lo = (uint32_t)key64;
         } else if (ks.mode == KM_2X32) {
             lo = c == 0 ? (key64 >> 32) : (uint32_t)key64;
-        } else {
+        } else if (in) {
             isnull = ks.null_word >= 0 && ((w[ks.null_word] >> c) & 1u);
             int o2 = ks.word_off[c];
             switch (ks.types[c]) {
@@ -363,6 +501,7 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
     }
     // ---- aggregates
     uint64_t err = 0;
+    const uint64_t rows = in ? g[0] : 0;
     for (int k = 0; k < a.naggs; k++) {
         const EmitAgg& e = a.aggs[k];
         const OutCol& o0 = a.outs[e.out0];
@@ -370,23 +509,21 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
         const bool result = e.flags & VR_AGG_RESULT;
         const bool ansi = e.flags & VR_AGG_CHECKED;
         const uint64_t* s = g + e.off;
+        const uint64_t cnt = rows - ((in && e.nulls_off >= 0) ? g[e.nulls_off] : 0);  // non-null rows
         switch (e.kind) {
-            case VR_AGG_COUNT_STAR:
-            case VR_AGG_COUNT: put_i64(o0, o, in, in ? (int64_t)s[0] : 0, in); break;
+            case VR_AGG_COUNT_STAR: put_i64(o0, o, in, (int64_t)rows, in); break;
+            case VR_AGG_COUNT: put_i64(o0, o, in, (int64_t)cnt, in); break;
             case VR_AGG_SUM_LONG: {
-                uint64_t lo = in ? s[0] : 0, hi = in ? s[1] : 0, cnt = in ? s[2] : 0;
-                bool fits = hi == (uint64_t)((int64_t)lo >> 63);
-                if (ansi && cnt && !fits) err |= ERR_LONG_OVERFLOW;
+                uint64_t lo = 0, hi = 0;
+                bool fits = true;
+                if (in) sum_total(s, false, &lo, &hi, &fits);
+                bool long_fits = fits && hi == (uint64_t)((int64_t)lo >> 63);
+                if (ansi && cnt && !long_fits) err |= ERR_LONG_OVERFLOW;
                 put_i64(o0, o, cnt > 0, (int64_t)lo, in);
                 break;
             }
-            case VR_AGG_SUM_DOUBLE: {
-                uint64_t cnt = in ? s[1] : 0;
-                put_i64(o0, o, cnt > 0, in ? (int64_t)s[0] : 0, in);
-                break;
-            }
+            case VR_AGG_SUM_DOUBLE: put_i64(o0, o, cnt > 0, in ? (int64_t)s[0] : 0, in); break;
             case VR_AGG_AVG_DOUBLE: {
-                uint64_t cnt = in ? s[1] : 0;
                 double sum = in ? __longlong_as_double((long long)s[0]) : 0.0;
                 if (result) {
                     double q = cnt ? sum / (double)cnt : 0.0;
@@ -399,11 +536,10 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
             }
             case VR_AGG_SUM_WIDE:
             case VR_AGG_AVG_WIDE: {
-                bool wide3 = e.op_code == OP_SUM_I192;
-                uint64_t lo = in ? s[0] : 0, hi = in ? s[1] : 0;
-                uint64_t cnt = in ? s[wide3 ? 3 : 2] : 0;
-                bool over = wide3 && in && s[2] != (uint64_t)((int64_t)hi >> 63);
-                if (!over) over = past_precision(lo, hi, e.bp);
+                uint64_t lo = 0, hi = 0;
+                bool fits = true;
+                if (in) sum_total(s, e.op_code == OP_SUM_D128, &lo, &hi, &fits);
+                bool over = !fits || past_precision(lo, hi, e.bp);
                 if (e.kind == VR_AGG_SUM_WIDE) {
                     if (result) {
                         if (cnt && over && ansi) err |= ERR_DECIMAL_OVERFLOW;
@@ -424,8 +560,8 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
                         ok = false;
                     }
                     if (ok) {
-                        bool fits = decimal_avg(lo, hi, cnt, e.rs - e.bs, &qlo, &qhi);
-                        if (!fits || past_precision(qlo, qhi, e.rp)) {
+                        bool q_fits = decimal_avg(lo, hi, cnt, e.rs - e.bs, &qlo, &qhi);
+                        if (!q_fits || past_precision(qlo, qhi, e.rp)) {
                             if (ansi) err |= ERR_DECIMAL_OVERFLOW;
                             ok = false;
                         }
@@ -513,8 +649,19 @@ struct AggImpl {
     // batched launches
     DevBuf batch_descs;
 
-    vr_launch_config cfg{256, 4, 4, -1};
-    int lds_entries = 512;
+    vr_launch_config cfg{256, 4, 4, -1, 512, 4};
+    int lds_budget = 64 * 1024;  // LDS bytes per workgroup for the hot-key cache
+
+    // the stream of the last work queued on the handle, and its completion event: work queued on
+    // another stream (or a host read) first waits for it
+    hipStream_t last_stream = nullptr;
+    hipEvent_t last_ev = nullptr;
+    AggImpl() = default;
+    AggImpl(const AggImpl&) = delete;
+    AggImpl& operator=(const AggImpl&) = delete;
+    ~AggImpl() {
+        if (last_ev) (void)hipEventDestroy(last_ev);
+    }
 };
 
 static vr_status compile_spec(AggImpl& h) {
@@ -547,7 +694,7 @@ static vr_status compile_spec(AggImpl& h) {
     } else {
         ks.width = 1;
     }
-    // operations, shared where two aggregates need the same state
+    // operations, shared where two aggregates need the same state; ROWS is word 0 of every row
     std::vector<Op> ops;
     int off = 0;
     std::vector<uint64_t> ident;
@@ -558,7 +705,6 @@ static vr_status compile_spec(AggImpl& h) {
         op.code = code;
         op.col = col;
         op.off = off;
-        op.lds = 1;
         ops.push_back(op);
         for (int i = 0; i < nwords; i++) ident.push_back(0);
         switch (code) {
@@ -579,6 +725,7 @@ static vr_status compile_spec(AggImpl& h) {
         off += nwords;
         return (int)ops.size() - 1;
     };
+    add_op(OP_ROWS, -1, 1);
     int min_max_i64 = 0;
     h.emits.clear();
     h.out_types.clear();
@@ -591,6 +738,7 @@ static vr_status compile_spec(AggImpl& h) {
         e.rp = d.result_precision;
         e.rs = d.result_scale;
         e.out0 = (int)h.out_types.size();
+        e.nulls_off = -1;
         int32_t in_type = -1;
         if (d.kind != VR_AGG_COUNT_STAR) {
             if (d.input < 0 || d.input >= (int)h.value_types.size())
@@ -599,33 +747,34 @@ static vr_status compile_spec(AggImpl& h) {
         }
         e.in_type = in_type;
         bool result = d.flags & VR_AGG_RESULT;
-        int k = -1;
+        int k = 0;
+        auto nulls = [&]() { e.nulls_off = ops[add_op(OP_NULLS, d.input, 1)].off; };
         switch (d.kind) {
-            case VR_AGG_COUNT_STAR:
-                k = add_op(OP_COUNT_STAR, -1, 1);
-                h.out_types.push_back(VR_INT64);
-                break;
+            case VR_AGG_COUNT_STAR: h.out_types.push_back(VR_INT64); break;
             case VR_AGG_COUNT:
-                k = add_op(OP_COUNT, d.input, 1);
+                nulls();
                 h.out_types.push_back(VR_INT64);
                 break;
             case VR_AGG_SUM_LONG:
                 if (in_type != VR_INT32 && in_type != VR_INT64) return fail(VR_ERR_UNSUPPORTED, "sum over a non-integral lane");
-                k = add_op(OP_SUM_I128, d.input, 3);
+                k = add_op(OP_SUM_I64, d.input, 2);
+                nulls();
                 h.out_types.push_back(VR_INT64);
                 break;
             case VR_AGG_SUM_DOUBLE:
             case VR_AGG_AVG_DOUBLE:
                 if (in_type != VR_FLOAT64) return fail(VR_ERR_UNSUPPORTED, "double sum over a non-double lane");
-                k = add_op(OP_SUM_F64, d.input, 2);
+                k = add_op(OP_SUM_F64, d.input, 1);
+                nulls();
                 h.out_types.push_back(VR_FLOAT64);
                 if (d.kind == VR_AGG_AVG_DOUBLE && !result) h.out_types.push_back(VR_INT64);
                 break;
             case VR_AGG_SUM_WIDE:
             case VR_AGG_AVG_WIDE:
-                if (in_type == VR_INT32 || in_type == VR_INT64) k = add_op(OP_SUM_I128, d.input, 3);
-                else if (in_type == VR_DECIMAL128) k = add_op(OP_SUM_I192, d.input, 4);
+                if (in_type == VR_INT32 || in_type == VR_INT64) k = add_op(OP_SUM_I64, d.input, 2);
+                else if (in_type == VR_DECIMAL128) k = add_op(OP_SUM_D128, d.input, 4);
                 else return fail(VR_ERR_UNSUPPORTED, "decimal sum over a non-integral lane");
+                nulls();
                 if (d.kind == VR_AGG_SUM_WIDE) {
                     h.out_types.push_back(VR_DECIMAL128);
                     if (!result) h.out_types.push_back(VR_BOOL);
@@ -642,8 +791,12 @@ static vr_status compile_spec(AggImpl& h) {
                 bool mn = d.kind == VR_AGG_MIN;
                 if (in_type == VR_INT32) k = add_op(mn ? OP_MIN_I32 : OP_MAX_I32, d.input, 1);
                 else if (in_type == VR_INT64) {
+                    size_t before = ops.size();
                     k = add_op(mn ? OP_MIN_I64 : OP_MAX_I64, d.input, 1);
-                    if (ops[k].bit == 0 && min_max_i64 < 63) ops[k].bit = min_max_i64++;
+                    if (ops.size() != before) {
+                        if (min_max_i64 >= 64) return fail(VR_ERR_UNSUPPORTED, "at most 64 bigint min/max");
+                        ops[k].bit = min_max_i64++;
+                    }
                 } else if (in_type == VR_FLOAT64) {
                     k = add_op(mn ? OP_MIN_F64 : OP_MAX_F64, d.input, 4);
                     h.has_f64_minmax = true;
@@ -668,10 +821,6 @@ static vr_status compile_spec(AggImpl& h) {
         p.flags_off = off;
         ident.push_back(0);
         off++;
-    }
-    if (off == 0) {  // keys only (a DISTINCT): one padding word keeps the rows addressable
-        ident.push_back(0);
-        off = 1;
     }
     if ((int)ops.size() > VR_MAX_OPS || off > 64) return fail(VR_ERR_UNSUPPORTED, "aggregate state too wide");
     for (size_t k = 0; k < ops.size(); k++) p.ops[k] = ops[k];
@@ -711,8 +860,10 @@ static vr_status alloc_table(AggImpl& h, uint64_t slots, DevBuf& cells, DevBuf& 
     int cap = h.dev->num_cus * 16;
     hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(slots * W, 256, cap)), dim3(256), 0, s, cells.as<uint64_t>(),
                        (uint64_t)(slots * W), empty);
-    hipLaunchKernelGGL(init_state_kernel, dim3(grid_for((slots + 2) * h.stride, 256, cap)), dim3(256), 0, s,
-                       state.as<uint64_t>(), (uint64_t)((slots + 2) * h.stride), h.stride, h.proto);
+    UpdateArgs a = h.proto;
+    a.t.stride = h.stride;
+    hipLaunchKernelGGL(init_state_kernel, dim3(grid_for((slots + 2 + 63) / 64 * 64, 256, cap)), dim3(256), 0, s,
+                       state.as<uint64_t>(), (uint64_t)(slots + 2), a);
     VR_TRY_HIP(hipGetLastError());
     return VR_OK;
 }
@@ -905,28 +1056,37 @@ static void launch_shape(AggImpl& h, const vr_launch_config* cfg, int64_t n, Upd
                          int* block, size_t* lds_bytes, bool multi) {
     vr_launch_config c = h.cfg;
     if (cfg) {
-        if (cfg->block_threads) c.block_threads = cfg->block_threads;
         if (cfg->rows_per_thread) c.rows_per_thread = cfg->rows_per_thread;
         if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
         if (cfg->lds_preaggregate >= 0) c.lds_preaggregate = cfg->lds_preaggregate;
+        if (cfg->lds_entries) c.lds_entries = cfg->lds_entries;
+        if (cfg->flush_tiles) c.flush_tiles = cfg->flush_tiles;
     }
-    *block = 256;  // the kernel's launch bounds
-    a.rows_per_thread = std::max(1, c.rows_per_thread);
+    *block = AGG_BLOCK;  // the kernel's launch bounds
+    if (multi || (cfg && cfg->rows_per_thread)) {
+        a.rows_per_thread = std::max(1, c.rows_per_thread);
+    } else {
+        // one batch: enough tiles to give every CU a couple of workgroups
+        int64_t want_tiles = (int64_t)h.dev->num_cus * 2;
+        int64_t rpt = n / ((int64_t)AGG_BLOCK * want_tiles);
+        a.rows_per_thread = (int)std::max<int64_t>(1, std::min<int64_t>(rpt, c.rows_per_thread));
+    }
     int64_t T = (int64_t)(*block) * a.rows_per_thread;
     int64_t tiles = multi ? a.total_tiles : (n + T - 1) / T;
     int cap = h.dev->num_cus * std::max(1, c.blocks_per_cu);
     *grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
     bool lds = c.lds_preaggregate != 0;
-    int E = lds ? h.lds_entries : 0;
+    int E = lds ? std::max(64, c.lds_entries) : 0;
+    while (E & (E - 1)) E &= E - 1;  // a power of two
+    if (E > 2048) E = 2048;
     size_t bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
-    while (E && bytes > 64 * 1024) {
+    while (E > 64 && bytes > (size_t)h.lds_budget) {
         E >>= 1;
         bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
     }
     a.lds_entries = E;
-    a.lds_words = h.stride;
-    a.flush_tiles = multi ? 64 : 1 << 30;
-    *lds_bytes = bytes;
+    a.flush_tiles = std::max(1, c.flush_tiles);
+    *lds_bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
 }
 
 static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, const vr_launch_config* cfg) {
@@ -987,19 +1147,25 @@ static vr_status build_order(AggImpl& h, hipStream_t s) {
     VR_TRY(read_counters(h, c, s));
     int64_t g = groups_of(c);
     VR_TRY(h.order.ensure(h.dev, (size_t)(g + 2) * sizeof(uint64_t)));
-    VR_TRY(h.order_tmp.ensure(h.dev, sizeof(uint64_t) * 4));
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.order_tmp.as<uint64_t>(), 4ull, 0ull);
-    hipcub::CountingInputIterator<uint64_t> it(0);
-    OccupiedSlot pred{h.cells.as<uint64_t>(), h.proto.ks.width};
+    const uint64_t nblocks = (h.slots + ORDER_CHUNK - 1) / ORDER_CHUNK;
+    VR_TRY(h.order_tmp.ensure(h.dev, (size_t)(2 * nblocks + 2) * sizeof(uint64_t)));
+    uint64_t* counts = h.order_tmp.as<uint64_t>();
+    uint64_t* offsets = counts + nblocks + 1;
+    const int W = h.proto.ks.width;
+    hipLaunchKernelGGL(count_occupied_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, h.cells.as<uint64_t>(), W,
+                       h.slots, counts);
     size_t tmp = 0;
-    VR_TRY_HIP(hipcub::DeviceSelect::If(nullptr, tmp, it, h.order.as<uint64_t>(), h.order_tmp.as<uint64_t>(),
-                                        (int64_t)h.slots, pred, s));
+    VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts, offsets, (int)nblocks, s));
     VR_TRY(h.cub_tmp.ensure(h.dev, tmp));
-    VR_TRY_HIP(hipcub::DeviceSelect::If(h.cub_tmp.p, tmp, it, h.order.as<uint64_t>(), h.order_tmp.as<uint64_t>(),
-                                        (int64_t)h.slots, pred, s));
-    uint64_t selected = 0;
-    VR_TRY_HIP(hipMemcpyAsync(&selected, h.order_tmp.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(h.cub_tmp.p, tmp, counts, offsets, (int)nblocks, s));
+    hipLaunchKernelGGL(write_occupied_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, h.cells.as<uint64_t>(), W,
+                       h.slots, offsets, h.order.as<uint64_t>());
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t tail[2] = {0, 0};
+    VR_TRY_HIP(hipMemcpyAsync(tail, counts + nblocks - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipMemcpyAsync(tail + 1, offsets + nblocks - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     VR_TRY_HIP(hipStreamSynchronize(s));
+    uint64_t selected = tail[0] + tail[1];
     std::vector<uint64_t> extra;
     if (c[C_RESERVED] & RES_SENTINEL) extra.push_back(h.slots);
     if (c[C_RESERVED] & RES_NULL) extra.push_back(h.slots + 1);
@@ -1031,6 +1197,18 @@ static vr_status build_order(AggImpl& h, hipStream_t s) {
     return VR_OK;
 }
 
+static vr_status enter(AggImpl& h, hipStream_t s) {
+    if (h.last_stream && h.last_stream != s) VR_TRY_HIP(hipStreamWaitEvent(s, h.last_ev, 0));
+    return VR_OK;
+}
+
+static vr_status leave(AggImpl& h, hipStream_t s) {
+    if (!h.last_ev) VR_TRY_HIP(hipEventCreateWithFlags(&h.last_ev, hipEventDisableTiming));
+    VR_TRY_HIP(hipEventRecord(h.last_ev, s));
+    h.last_stream = s;
+    return VR_OK;
+}
+
 }  // namespace vr
 
 using namespace vr;
@@ -1057,6 +1235,9 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     if (spec->num_values > 0) h.value_types.assign(spec->value_types, spec->value_types + spec->num_values);
     if (spec->num_aggs > 0) h.aggs.assign(spec->aggs, spec->aggs + spec->num_aggs);
     h.det = spec->deterministic != 0;
+    if (spec->max_load_percent < 0 || spec->max_load_percent > 90)
+        return fail(VR_ERR_INVALID_ARGUMENT, "max_load_percent outside [0, 90]");
+    if (spec->max_load_percent) h.max_load = spec->max_load_percent / 100.0;
     VR_TRY(compile_spec(h));
     h.dicts.resize(h.key_types.size());
     VR_TRY(init_table(h, spec->capacity_hint));
@@ -1072,6 +1253,7 @@ vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* val
     if (n == 0) return VR_OK;
     if ((uint64_t)n >= (1ull << 32) && h.det) return fail(VR_ERR_UNSUPPORTED, "deterministic batches are below 2^32 rows");
     hipStream_t s = h.dev->pick(stream);
+    VR_TRY(enter(h, s));
     BatchDesc b;
     StagePlan plan;
     std::vector<std::vector<int32_t>> ids;
@@ -1081,6 +1263,7 @@ vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* val
     VR_TRY(stage(h, plan, s, &slot));
     VR_TRY(run_update(h, b, n, s, nullptr));
     if (slot >= 0) VR_TRY(h.staging.done(slot, s));
+    VR_TRY(leave(h, s));
     h.dev->count(&vr_stats::calls, 1);
     h.dev->count(&vr_stats::input_rows, n);
     return VR_OK;
@@ -1115,7 +1298,9 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
         for (int c = 0; c < nk; c++) {
             const vr_column& k = bs->keys[(size_t)b * nk + c];
             if (k.mem != VR_MEM_DEVICE) fused = false;
-            if (k.type == VR_UTF8 && (!k.dictionary || k.dictionary->data != bs->keys[c].dictionary->data)) fused = false;
+            if (k.type == VR_UTF8 && (!k.dictionary || !bs->keys[c].dictionary ||
+                                      k.dictionary->data != bs->keys[c].dictionary->data))
+                fused = false;
         }
         for (int c = 0; c < nv; c++)
             if (bs->values[(size_t)b * nv + c].mem != VR_MEM_DEVICE) fused = false;
@@ -1135,6 +1320,7 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     a.total_tiles = 1;
     launch_shape(h, cfg, 0, a, &grid, &block, &lds_bytes, true);
     const int64_t T = (int64_t)block * a.rows_per_thread;
+    VR_TRY(enter(h, s));
     VR_TRY_HIP(hipStreamSynchronize(s));  // the previous batched launch may still read the descriptors
     static thread_local std::vector<BatchDesc> descs;
     descs.resize(nb);
@@ -1152,36 +1338,70 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
         rows += n;
     }
     if (rows == 0) return VR_OK;
-    VR_TRY(ensure_capacity(h, rows, s));
     VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
     VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs.data(), sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
-    a.t = view_of(h);
-    a.batches = h.batch_descs.as<BatchDesc>();
-    a.num_batches = nb;
-    a.total_tiles = tiles;
-    a.epoch = ++h.epoch;
-    launch_shape(h, cfg, 0, a, &grid, &block, &lds_bytes, true);
-    hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
-    VR_TRY_HIP(hipGetLastError());
-    int launches = 1;
-    if (h.has_f64_minmax) {
-        hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
-        VR_TRY_HIP(hipGetLastError());
-        launches++;
+    // Persistent launches over runs of batches that fit the table's headroom: every input row
+    // may be a new group, so a run never holds more rows than the table can take before growing.
+    int launches = 0;
+    int b0 = 0;
+    while (b0 < nb) {
+        VR_TRY(ensure_capacity(h, bs->lengths[b0], s));
+        double headroom = h.max_load * (double)h.slots - (double)h.groups_upper;
+        int b1 = b0;
+        int64_t run = 0;
+        while (b1 < nb && (double)(run + bs->lengths[b1]) <= headroom) run += bs->lengths[b1++];
+        if (b1 == b0) run = bs->lengths[b1++];
+        int64_t tile_hi = b1 < nb ? descs[b1].tile_begin : tiles;
+        a.t = view_of(h);
+        a.batches = h.batch_descs.as<BatchDesc>();
+        a.num_batches = nb;
+        a.tile_lo = descs[b0].tile_begin;
+        a.total_tiles = tile_hi;
+        a.epoch = ++h.epoch;
+        int64_t span = tile_hi - a.tile_lo;
+        int cap = h.dev->num_cus * std::max(1, (cfg && cfg->blocks_per_cu) ? cfg->blocks_per_cu : h.cfg.blocks_per_cu);
+        grid = (int)std::max<int64_t>(1, std::min<int64_t>(span, cap));
+        if (span > 0) {
+            hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+            VR_TRY_HIP(hipGetLastError());
+            launches++;
+            if (h.has_f64_minmax) {
+                hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
+                VR_TRY_HIP(hipGetLastError());
+                launches++;
+            }
+        }
+        h.groups_upper += run;
+        b0 = b1;
     }
     h.rows_seen += rows;
-    h.groups_upper += rows;
     h.order_count = -1;
     h.dev->count(&vr_stats::kernel_launches, launches);
     h.dev->count(&vr_stats::calls, 1);
     h.dev->count(&vr_stats::input_rows, rows);
+    return leave(h, s);
+}
+
+vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg) {
+    if (!agg || !cfg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    vr_launch_config& c = agg->h.cfg;
+    if (cfg->rows_per_thread < 0 || cfg->rows_per_thread > 64 || cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 32 ||
+        cfg->lds_entries < 0 || cfg->lds_entries > 2048 || cfg->flush_tiles < 0)
+        return fail(VR_ERR_INVALID_ARGUMENT, "launch config out of range");
+    if (cfg->rows_per_thread) c.rows_per_thread = cfg->rows_per_thread;
+    if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
+    if (cfg->lds_preaggregate >= 0) c.lds_preaggregate = cfg->lds_preaggregate;
+    if (cfg->lds_entries) c.lds_entries = cfg->lds_entries;
+    if (cfg->flush_tiles) c.flush_tiles = cfg->flush_tiles;
     return VR_OK;
 }
 
 vr_status vr_agg_num_groups(vr_agg* agg, int64_t* groups) {
     if (!agg || !groups) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     uint64_t c[C_WORDS];
+    VR_TRY(enter(agg->h, agg->h.dev->stream));
     VR_TRY(read_counters(agg->h, c, agg->h.dev->stream));
+    VR_TRY(leave(agg->h, agg->h.dev->stream));
     *groups = groups_of(c);
     return VR_OK;
 }
@@ -1197,7 +1417,7 @@ vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys
     if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     AggImpl& h = agg->h;
     hipStream_t s = h.dev->pick(stream);
-    if (stream) VR_TRY_HIP(hipStreamSynchronize(h.dev->stream));
+    VR_TRY(enter(h, s));
     VR_TRY(build_order(h, s));
     if (from < 0 || to < from || to > h.order_count) return fail(VR_ERR_INVALID_ARGUMENT, "emission range outside the groups");
     const int64_t m = to - from;
@@ -1261,6 +1481,7 @@ vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys
     }
     uint64_t c[C_WORDS];
     VR_TRY(read_counters(h, c, s));
+    VR_TRY(leave(h, s));
     h.dev->count(&vr_stats::output_rows, m);
     if (c[C_ERROR]) {
         uint64_t e = c[C_ERROR];
@@ -1310,13 +1531,25 @@ vr_status vr_agg_reset(vr_agg* agg) {
     if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     AggImpl& h = agg->h;
     hipStream_t s = h.dev->stream;
-    VR_TRY_HIP(hipStreamSynchronize(s));
-    VR_TRY(alloc_table(h, h.slots, h.cells, h.state, s));
+    VR_TRY(enter(h, s));
+    if (h.order_count >= 0 && (uint64_t)h.order_count < h.slots / 4) {
+        // only the rows the last emission listed hold anything: reset those
+        UpdateArgs a = h.proto;
+        a.t = view_of(h);
+        uint64_t empty = h.proto.ks.mode == KM_WIDE ? 0ull : EMPTY_KEY;
+        if (h.order_count > 0)
+            hipLaunchKernelGGL(reset_rows_kernel, dim3(grid_for(h.order_count, 256, h.dev->num_cus * 8)), dim3(256), 0, s,
+                               h.order.as<uint64_t>(), h.order_count, h.cells.as<uint64_t>(), h.proto.ks.width, h.slots,
+                               empty, a);
+        VR_TRY_HIP(hipGetLastError());
+    } else {
+        VR_TRY(alloc_table(h, h.slots, h.cells, h.state, s));
+    }
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
     VR_TRY_HIP(hipStreamSynchronize(s));
     h.groups_upper = 0;
     h.order_count = -1;
-    return VR_OK;
+    return leave(h, s);
 }
 
 vr_status vr_agg_destroy(vr_agg* agg) {

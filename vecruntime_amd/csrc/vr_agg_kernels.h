@@ -1,4 +1,4 @@
-// Group-by aggregation: device data structures and kernels.
+// Group-by aggregation: device data structures and per-row operations.
 //
 // Table.  Open addressing over 2^k slots; a group's state row has the same index as its slot, so
 // finding the slot is the whole assignment (no second indirection to a dense id).  Keys of at most
@@ -11,22 +11,30 @@
 // ever go from empty to their final value, so a plain (possibly stale) load can only return the
 // final value or "empty", and "empty" is confirmed by the CAS itself.
 //
-// State.  One row of `stride` 64-bit words per slot, array of structures, so one group's update
-// touches one or two cache lines.  Words per operation (see Op):
-//   COUNT_STAR / COUNT          count
-//   SUM_I128 (INT32/INT64 lane) lo, hi, count            -- exact 128-bit (WideLongSum, LongSum)
-//   SUM_I192 (DECIMAL128 lane)  lo, hi, sign-limb, count -- exact; total past 128 bits = overflow
-//   SUM_F64                     sum (double), count
-//   MIN/MAX_I32                 value as int64 with an impossible identity
-//   MIN/MAX_I64                 ordered code; presence bit in the row's flags word
-//   MIN/MAX_F64                 ordered code, first zero row (row<<1 | sign), first NaN row, NaN bits
-//   FIRST_ROW                   smallest global row index (first-seen emission order)
+// State.  One row of `stride` 64-bit words per slot (array of structures: one group's update
+// touches one or two cache lines), built so that a typical row costs few atomics and none of them
+// needs its old value back:
+//   ROWS                     rows of the group (COUNT(*)); word 0 of every row
+//   NULLS(col)               null rows of `col`; COUNT(col) = ROWS - NULLS, touched only by nulls
+//   SUM_I64(col)             sum of the low 32-bit halves (u64) and of the high halves (i64):
+//                            the exact 128-bit total is hi * 2^32 + lo for groups below 2^32 rows,
+//                            and a zero half (e.g. every non-negative value below 2^32) costs nothing
+//   SUM_D128(col)            the same over the four 32-bit limbs of a DECIMAL128 lane
+//   SUM_F64(col)             double sum
+//   MIN/MAX_I32(col)         value as int64 with an impossible identity
+//   MIN/MAX_I64(col)         order-preserving code; presence bit in the row's flags word
+//   MIN/MAX_F64(col)         order-preserving code, first zero row (row<<1 | sign), first NaN row,
+//                            first NaN's bits
+//   FIRST_ROW                smallest global row index (first-seen emission order)
+// A MIN/MAX row first compares against a plain read of the current best and skips the atomic
+// when it cannot improve it (the best only moves one way, so a stale read is conservative).
 //
-// Update.  Each workgroup walks tiles of rows; a row's slot is found (or claimed) in the global
-// table, then its values are folded into a small LDS table keyed by slot with LDS atomics, and the
-// LDS partials are flushed to the global rows with one set of global atomics per distinct slot.
-// Hot keys (Zipf) thus cost LDS atomics instead of contended global ones.  Rows whose slot finds
-// no LDS entry (the LDS table full) update the global row directly.
+// Update.  Workgroups walk tiles of rows.  An LDS table caches hot groups: keyed by the key itself
+// for 64-bit keys (a hit needs no global access at all) and by slot for wide keys.  A row whose
+// group has an LDS entry is folded in with LDS atomics; other rows update their global row
+// directly.  Every few tiles the entries are flushed to the global rows (one set of atomics per
+// entry), entries seen at least twice since the last flush keep their place, and the rest are
+// evicted -- an adaptive hot-key cache for skewed (Zipf) keys.
 #pragma once
 
 #include "vr_device.h"
@@ -37,10 +45,10 @@ namespace vr {
 enum KeyMode : int32_t { KM_I64 = 0, KM_I32 = 1, KM_2X32 = 2, KM_WIDE = 3 };
 
 enum OpCode : int32_t {
-    OP_COUNT_STAR = 0,
-    OP_COUNT = 1,
-    OP_SUM_I128 = 2,
-    OP_SUM_I192 = 3,
+    OP_ROWS = 0,
+    OP_NULLS = 1,
+    OP_SUM_I64 = 2,
+    OP_SUM_D128 = 3,
     OP_SUM_F64 = 4,
     OP_MIN_I32 = 5,
     OP_MAX_I32 = 6,
@@ -58,18 +66,16 @@ enum { ERR_LONG_OVERFLOW = 1, ERR_DECIMAL_OVERFLOW = 2, ERR_AVG_OF_OVERFLOW = 4 
 
 struct Op {
     int32_t code;
-    int32_t col;   // value column, -1 for COUNT_STAR / FIRST_ROW
-    int32_t off;   // first state word
-    int32_t bit;   // presence bit (MIN/MAX_I64), else 0
-    int32_t lds;   // 1: folded through the LDS table
-    int32_t pad;
+    int32_t col;  // value column, -1 for ROWS / FIRST_ROW
+    int32_t off;  // first state word
+    int32_t bit;  // presence bit (MIN/MAX_I64), else 0
 };
 
 struct KeySpec {
     int32_t mode;
-    int32_t width;     // 32-bit words of a wide key (null mask word included)
+    int32_t width;      // 32-bit words of a wide key (null mask word included)
     int32_t nkeys;
-    int32_t null_word; // index of the null-mask word, -1 when no key is nullable
+    int32_t null_word;  // index of the null-mask word, -1 when no key is nullable
     int32_t types[VR_MAX_KEYS];
     int32_t word_off[VR_MAX_KEYS];
 };
@@ -99,8 +105,7 @@ struct UpdateArgs {
     Op ops[VR_MAX_OPS];
     int32_t nops;
     int32_t flags_off;     // state word of the MIN/MAX_I64 presence bits, -1 when none
-    int32_t lds_entries;   // power of two, 0: no LDS pre-aggregation
-    int32_t lds_words;     // state words mirrored per LDS entry (= stride)
+    int32_t lds_entries;   // power of two, 0: no LDS cache
     int32_t rows_per_thread;
     int32_t flush_tiles;   // tiles between LDS flushes
     int32_t filter_col;    // INT32 value column filtered by value < filter_threshold, -1: none
@@ -108,6 +113,8 @@ struct UpdateArgs {
     int32_t det;           // deterministic mode: double sums folded by the ordered pass
     int32_t num_batches;   // 1: `one` below; >1: `batches` (device array)
     int32_t epoch;         // update sequence number: the NaN flag holds the last epoch that saw one
+    int32_t pad;
+    int64_t tile_lo;       // batched launch: tiles [tile_lo, total_tiles) of `batches`
     int64_t total_tiles;
     uint64_t ident[64];    // identity value of every state word
     uint32_t* row_slot;    // deterministic: slot of every row of the batch (0xFFFFFFFF none)
@@ -139,9 +146,8 @@ __device__ __forceinline__ uint32_t key32(const DCol& c, int64_t i, bool* isnull
     }
 }
 
-__device__ __forceinline__ uint64_t mark_reserved(uint64_t* counters, uint64_t bit) {
+__device__ __forceinline__ void mark_reserved(uint64_t* counters, uint64_t bit) {
     if ((counters[C_RESERVED] & bit) == 0) atomicOr((unsigned long long*)&counters[C_RESERVED], (unsigned long long)bit);
-    return bit;
 }
 
 // Slot of a 64-bit key, claiming an empty cell for a new key; `inserted` reports a claim.
@@ -239,78 +245,92 @@ __device__ __forceinline__ uint64_t find_or_insert_wide(const TableView& t, cons
     }
 }
 
-// Builds the key of row i: a 64-bit key (returns the state row directly for the reserved rows) or
-// the words of a wide key.  Returns the state row index, or ~0 for a row with no group (lookup).
-__device__ __forceinline__ uint64_t row_slot_of(const UpdateArgs& a, const BatchDesc& b, int64_t i,
-                                                bool* inserted, bool insert) {
-    const KeySpec& ks = a.ks;
-    *inserted = false;
-    switch (ks.mode) {
+// The 64-bit key of row i (64-bit key modes).  Returns false for a row that lives in a reserved
+// row without a key (the null of a single 64-bit lane): then *slot is that row.
+__device__ __forceinline__ bool key64_of(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* key,
+                                         uint64_t* slot) {
+    switch (a.ks.mode) {
         case KM_I64: {
             const DCol& c = b.keys[0];
             if (c.valid && !bit_at(c.valid, i)) {
-                mark_reserved(a.t.counters, RES_NULL);
-                return a.t.slots + 1;
+                *slot = a.t.slots + 1;
+                return false;
             }
-            uint64_t key = (uint64_t)ld_stream((const int64_t*)c.data + i);
-            return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
+            *key = (uint64_t)ld_stream((const int64_t*)c.data + i);
+            return true;
         }
         case KM_I32: {
             bool nul;
             uint32_t v = key32(b.keys[0], i, &nul);
-            uint64_t key = nul ? (1ull << 32) : (uint64_t)v;
-            return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
+            *key = nul ? (1ull << 32) : (uint64_t)v;
+            return true;
         }
-        case KM_2X32: {
+        default: {
             bool n0, n1;
             uint32_t v0 = key32(b.keys[0], i, &n0);
             uint32_t v1 = key32(b.keys[1], i, &n1);
-            uint64_t key = ((uint64_t)v0 << 32) | (uint64_t)v1;
-            return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
-        }
-        default: {
-            uint32_t w[4 * VR_MAX_KEYS + 1];
-            int W = ks.width;
-            for (int k = 0; k < W; k++) w[k] = 0;
-            uint32_t nm = 0;
-            for (int c = 0; c < ks.nkeys; c++) {
-                const DCol& col = b.keys[c];
-                int o = ks.word_off[c];
-                switch (ks.types[c]) {
-                    case VR_INT64:
-                    case VR_FLOAT64:
-                        if (col.valid && !bit_at(col.valid, i)) {
-                            nm |= 1u << c;
-                        } else {
-                            uint64_t v = (uint64_t)ld_stream((const int64_t*)col.data + i);
-                            w[o] = (uint32_t)v;
-                            w[o + 1] = (uint32_t)(v >> 32);
-                        }
-                        break;
-                    case VR_DECIMAL128:
-                        if (col.valid && !bit_at(col.valid, i)) {
-                            nm |= 1u << c;
-                        } else {
-                            const uint64_t* p = (const uint64_t*)col.data + 2 * i;
-                            uint64_t lo = ld_stream(p), hi = ld_stream(p + 1);
-                            w[o] = (uint32_t)lo;
-                            w[o + 1] = (uint32_t)(lo >> 32);
-                            w[o + 2] = (uint32_t)hi;
-                            w[o + 3] = (uint32_t)(hi >> 32);
-                        }
-                        break;
-                    default: {
-                        bool nul;
-                        uint32_t v = key32(col, i, &nul);
-                        if (nul) nm |= 1u << c;
-                        else w[o] = v;
-                    }
-                }
-            }
-            if (ks.null_word >= 0) w[ks.null_word] = nm;
-            return find_or_insert_wide(a.t, w, W, inserted, insert);
+            *key = ((uint64_t)v0 << 32) | (uint64_t)v1;
+            return true;
         }
     }
+}
+
+__device__ __forceinline__ void wide_words(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint32_t* w) {
+    const KeySpec& ks = a.ks;
+    for (int k = 0; k < ks.width; k++) w[k] = 0;
+    uint32_t nm = 0;
+    for (int c = 0; c < ks.nkeys; c++) {
+        const DCol& col = b.keys[c];
+        int o = ks.word_off[c];
+        switch (ks.types[c]) {
+            case VR_INT64:
+            case VR_FLOAT64:
+                if (col.valid && !bit_at(col.valid, i)) {
+                    nm |= 1u << c;
+                } else {
+                    uint64_t v = (uint64_t)ld_stream((const int64_t*)col.data + i);
+                    w[o] = (uint32_t)v;
+                    w[o + 1] = (uint32_t)(v >> 32);
+                }
+                break;
+            case VR_DECIMAL128:
+                if (col.valid && !bit_at(col.valid, i)) {
+                    nm |= 1u << c;
+                } else {
+                    const uint64_t* p = (const uint64_t*)col.data + 2 * i;
+                    uint64_t lo = ld_stream(p), hi = ld_stream(p + 1);
+                    w[o] = (uint32_t)lo;
+                    w[o + 1] = (uint32_t)(lo >> 32);
+                    w[o + 2] = (uint32_t)hi;
+                    w[o + 3] = (uint32_t)(hi >> 32);
+                }
+                break;
+            default: {
+                bool nul;
+                uint32_t v = key32(col, i, &nul);
+                if (nul) nm |= 1u << c;
+                else w[o] = v;
+            }
+        }
+    }
+    if (ks.null_word >= 0) w[ks.null_word] = nm;
+}
+
+// The state row of row i (global table), claiming a new group when `insert`; ~0 when absent.
+__device__ __forceinline__ uint64_t row_slot_of(const UpdateArgs& a, const BatchDesc& b, int64_t i, bool* inserted,
+                                                bool insert) {
+    *inserted = false;
+    if (a.ks.mode != KM_WIDE) {
+        uint64_t key, slot;
+        if (!key64_of(a, b, i, &key, &slot)) {
+            if (insert) mark_reserved(a.t.counters, RES_NULL);
+            return slot;
+        }
+        return insert ? find_or_insert64(a.t, key, inserted) : find64(a.t, key);
+    }
+    uint32_t w[4 * VR_MAX_KEYS + 1];
+    wide_words(a, b, i, w);
+    return find_or_insert_wide(a.t, w, a.ks.width, inserted, insert);
 }
 
 // ------------------------------------------------------------------ per-row updates
@@ -319,41 +339,43 @@ __device__ __forceinline__ int64_t read_long(const DCol& c, int64_t i) {
     return c.type == VR_INT32 ? (int64_t)ld_stream((const int32_t*)c.data + i) : ld_stream((const int64_t*)c.data + i);
 }
 
-// Folds row i of batch b into its group: each operation's words at `lrow` (the group's LDS
-// entry) when it has one and the operation goes through LDS, else at `grow` (the global row).
-// The first zero / NaN rows of a double MIN/MAX always go to the global row.
-__device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* lrow,
-                                          uint64_t* grow) {
+__device__ __forceinline__ void add_if(uint64_t* w, uint64_t v) {
+    if (v) atomic_add_u64(w, v);
+}
+
+// Folds row i of batch b into the state words at `row` (an LDS entry or a global row).
+__device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* row) {
     for (int k = 0; k < a.nops; k++) {
         const Op& op = a.ops[k];
-        uint64_t* row = (lrow && op.lds) ? lrow : grow;
         uint64_t* w = row + op.off;
         switch (op.code) {
-            case OP_COUNT_STAR: atomic_add_u64(w, 1); break;
-            case OP_COUNT:
-                if (is_valid(b.vals[op.col], i)) atomic_add_u64(w, 1);
+            case OP_ROWS: atomic_add_u64(w, 1); break;
+            case OP_NULLS:
+                if (!is_valid(b.vals[op.col], i)) atomic_add_u64(w, 1);
                 break;
-            case OP_SUM_I128: {
+            case OP_SUM_I64: {
                 const DCol& c = b.vals[op.col];
                 if (!is_valid(c, i)) break;
                 int64_t x = read_long(c, i);
-                atomic_add_i128(w, w + 1, (uint64_t)x, (uint64_t)(x >> 63));
-                atomic_add_u64(w + 2, 1);
+                add_if(w, (uint64_t)(uint32_t)x);
+                add_if(w + 1, (uint64_t)(x >> 32));  // the signed high half
                 break;
             }
-            case OP_SUM_I192: {
+            case OP_SUM_D128: {
                 const DCol& c = b.vals[op.col];
                 if (!is_valid(c, i)) break;
                 const uint64_t* p = (const uint64_t*)c.data + 2 * i;
-                atomic_add_i192(w, w + 1, w + 2, ld_stream(p), ld_stream(p + 1));
-                atomic_add_u64(w + 3, 1);
+                uint64_t lo = ld_stream(p), hi = ld_stream(p + 1);
+                add_if(w, lo & 0xFFFFFFFFull);
+                add_if(w + 1, lo >> 32);
+                add_if(w + 2, hi & 0xFFFFFFFFull);
+                add_if(w + 3, (uint64_t)((int64_t)hi >> 32));
                 break;
             }
             case OP_SUM_F64: {
                 const DCol& c = b.vals[op.col];
-                if (!is_valid(c, i)) break;
-                if (!a.det) atomic_add_f64(w, ld_stream((const double*)c.data + i));
-                atomic_add_u64(w + 1, 1);
+                if (a.det || !is_valid(c, i)) break;
+                atomic_add_f64(w, ld_stream((const double*)c.data + i));
                 break;
             }
             case OP_MIN_I32:
@@ -361,8 +383,11 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
                 const DCol& c = b.vals[op.col];
                 if (!is_valid(c, i)) break;
                 int64_t x = ld_stream((const int32_t*)c.data + i);
-                if (op.code == OP_MIN_I32) atomic_min_i64(w, x);
-                else atomic_max_i64(w, x);
+                if (op.code == OP_MIN_I32) {
+                    if (x < (int64_t)*w) atomic_min_i64(w, x);
+                } else if (x > (int64_t)*w) {
+                    atomic_max_i64(w, x);
+                }
                 break;
             }
             case OP_MIN_I64:
@@ -370,10 +395,15 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
                 const DCol& c = b.vals[op.col];
                 if (!is_valid(c, i)) break;
                 uint64_t o = (uint64_t)ld_stream((const int64_t*)c.data + i) ^ 0x8000000000000000ull;
-                if (op.code == OP_MIN_I64) atomic_min_u64(w, o);
-                else atomic_max_u64(w, o);
                 uint64_t* f = row + a.flags_off;
-                if ((*f & (1ull << op.bit)) == 0) atomicOr((unsigned long long*)f, 1ull << op.bit);
+                uint64_t bit = 1ull << op.bit;
+                bool has = (*f & bit) != 0;
+                if (op.code == OP_MIN_I64) {
+                    if (!has || o < *w) atomic_min_u64(w, o);
+                } else if (!has || o > *w) {
+                    atomic_max_u64(w, o);
+                }
+                if (!has) atomicOr((unsigned long long*)f, bit);
                 break;
             }
             case OP_MIN_F64:
@@ -382,17 +412,19 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
                 if (!is_valid(c, i)) break;
                 uint64_t bits = (uint64_t)ld_stream((const int64_t*)c.data + i);
                 uint64_t o = order_f64(bits);
-                if (op.code == OP_MIN_F64) atomic_min_u64(w, o);
-                else atomic_max_u64(w, o);
+                if (op.code == OP_MIN_F64) {
+                    if (o < *w) atomic_min_u64(w, o);
+                } else if (o > *w) {
+                    atomic_max_u64(w, o);
+                }
                 // Ties between +0.0 / -0.0 and between NaN payloads keep the first-seen value
                 // (DoubleMinMax.offer replaces only on a strict improvement): record the first
-                // zero row with its sign and the first NaN row on the global row.
-                uint64_t* gw = grow + op.off;
+                // zero row with its sign and the first NaN row.
                 uint64_t gi = (uint64_t)(b.row_base + i);
                 if (is_zero_bits(bits)) {
-                    atomic_min_u64(gw + 1, (gi << 1) | (bits >> 63));
+                    atomic_min_u64(w + 1, (gi << 1) | (bits >> 63));
                 } else if (is_nan_bits(bits)) {
-                    atomic_min_u64(gw + 2, gi);
+                    atomic_min_u64(w + 2, gi);
                     uint64_t* flag = &a.t.counters[C_NAN];
                     if (*flag != (uint64_t)a.epoch) atomic_max_u64(flag, (uint64_t)a.epoch);
                 }
@@ -411,46 +443,26 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
 __device__ __forceinline__ void flush_entry(const UpdateArgs& a, const uint64_t* e, uint64_t* g) {
     for (int k = 0; k < a.nops; k++) {
         const Op& op = a.ops[k];
-        if (!op.lds) continue;
         const uint64_t* l = e + op.off;
         uint64_t* w = g + op.off;
         switch (op.code) {
-            case OP_COUNT_STAR:
-            case OP_COUNT:
-                if (l[0]) atomic_add_u64(w, l[0]);
+            case OP_ROWS:
+            case OP_NULLS: add_if(w, l[0]); break;
+            case OP_SUM_I64:
+                add_if(w, l[0]);
+                add_if(w + 1, l[1]);
                 break;
-            case OP_SUM_I128:
-                if (l[2]) {
-                    atomic_add_i128(w, w + 1, l[0], l[1]);
-                    atomic_add_u64(w + 2, l[2]);
-                }
-                break;
-            case OP_SUM_I192:
-                if (l[3]) {
-                    uint64_t old = atomic_add_u64(w, l[0]);
-                    uint64_t c0 = (old + l[0]) < old ? 1ull : 0ull;
-                    uint64_t t = l[1] + c0;
-                    uint64_t c1 = (t < l[1]) ? 1ull : 0ull;
-                    if (t) {
-                        uint64_t oh = atomic_add_u64(w + 1, t);
-                        c1 += (oh + t) < oh ? 1ull : 0ull;
-                    }
-                    uint64_t ts = l[2] + c1;
-                    if (ts) atomic_add_u64(w + 2, ts);
-                    atomic_add_u64(w + 3, l[3]);
-                }
+            case OP_SUM_D128:
+                for (int q = 0; q < 4; q++) add_if(w + q, l[q]);
                 break;
             case OP_SUM_F64:
-                if (l[1]) {
-                    if (!a.det && l[0]) atomic_add_f64(w, __longlong_as_double((long long)l[0]));
-                    atomic_add_u64(w + 1, l[1]);
-                }
+                if (!a.det && l[0]) atomic_add_f64(w, __longlong_as_double((long long)l[0]));
                 break;
             case OP_MIN_I32:
-                if ((int64_t)l[0] != INT64_MAX) atomic_min_i64(w, (int64_t)l[0]);
+                if ((int64_t)l[0] < (int64_t)*w) atomic_min_i64(w, (int64_t)l[0]);
                 break;
             case OP_MAX_I32:
-                if ((int64_t)l[0] != INT64_MIN) atomic_max_i64(w, (int64_t)l[0]);
+                if ((int64_t)l[0] > (int64_t)*w) atomic_max_i64(w, (int64_t)l[0]);
                 break;
             case OP_MIN_I64:
             case OP_MAX_I64: {
@@ -463,13 +475,16 @@ __device__ __forceinline__ void flush_entry(const UpdateArgs& a, const uint64_t*
                 break;
             }
             case OP_MIN_F64:
-                if (l[0] != ~0ull) atomic_min_u64(w, l[0]);
-                break;
             case OP_MAX_F64:
-                if (l[0] != 0) atomic_max_u64(w, l[0]);
+                if (op.code == OP_MIN_F64 ? l[0] < *w : l[0] > *w) {
+                    if (op.code == OP_MIN_F64) atomic_min_u64(w, l[0]);
+                    else atomic_max_u64(w, l[0]);
+                }
+                if (l[1] < w[1]) atomic_min_u64(w + 1, l[1]);
+                if (l[2] < w[2]) atomic_min_u64(w + 2, l[2]);
                 break;
             case OP_FIRST_ROW:
-                if (l[0] != ~0ull && l[0] < w[0]) atomic_min_u64(w, l[0]);
+                if (l[0] < *w) atomic_min_u64(w, l[0]);
                 break;
         }
     }

@@ -45,6 +45,7 @@ _sig("vr_agg_update", [_P, _P, _P, C.c_int64, _P, _P])
 _sig("vr_agg_update_filtered", [_P, _P, _P, C.c_int64, C.c_int32, C.c_int32, _P])
 _sig("vr_agg_update_batches", [_P, C.POINTER(abi.VrBatchSet), C.POINTER(abi.VrLaunchConfig), _P])
 _sig("vr_agg_num_groups", [_P, C.POINTER(C.c_int64)])
+_sig("vr_agg_set_launch_config", [_P, C.POINTER(abi.VrLaunchConfig)])
 _sig("vr_agg_output_layout", [_P, C.POINTER(C.c_int32), _P, C.c_int32])
 _sig("vr_agg_emit", [_P, C.c_int64, C.c_int64, _P, _P, _P])
 _sig("vr_agg_key_dictionary", [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64),
@@ -148,6 +149,16 @@ class GroupedAggregate:
         ka, va = c_array(keys), c_array(values)
         check(_lib.vr_agg_update(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), _ptr(selection), _stream(stream)))
 
+    @staticmethod
+    def prepare(keys, values):
+        """The ctypes arrays of one batch's columns, built once for repeated updates."""
+        ka, va = c_array(keys), c_array(values)
+        return (ka, va, C.cast(ka, _P), C.cast(va, _P), keys, values)
+
+    def update_prepared(self, prep, n, filter_col=-1, threshold=0, stream=None):
+        """vr_agg_update_filtered over prepare()d columns (filter_col -1: no filter)."""
+        check(_lib.vr_agg_update_filtered(self.h, prep[2], prep[3], int(n), filter_col, threshold, _stream(stream)))
+
     def update_filtered(self, keys, values, n, filter_col, threshold, stream=None):
         ka, va = c_array(keys), c_array(values)
         check(_lib.vr_agg_update_filtered(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), filter_col, threshold,
@@ -161,7 +172,7 @@ class GroupedAggregate:
         ln = np.asarray([b[2] for b in batches], np.int64)
         bs = abi.VrBatchSet(len(batches), C.cast(ka, C.POINTER(abi.VrColumn)), C.cast(va, C.POINTER(abi.VrColumn)),
                             ln.ctypes.data_as(C.POINTER(C.c_int64)), filter_col, threshold)
-        cfg = config if config is not None else abi.VrLaunchConfig(0, 0, 0, -1)
+        cfg = config if config is not None else abi.launch_config()
         check(_lib.vr_agg_update_batches(self.h, C.byref(bs), C.byref(cfg), _stream(stream)))
         self._keep = [ka, va, ln, batches]
 
@@ -169,6 +180,9 @@ class GroupedAggregate:
         g = C.c_int64()
         check(_lib.vr_agg_num_groups(self.h, C.byref(g)))
         return g.value
+
+    def set_launch_config(self, cfg):
+        check(_lib.vr_agg_set_launch_config(self.h, C.byref(cfg)))
 
     def output_types(self):
         n = C.c_int32()
