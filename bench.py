@@ -430,6 +430,172 @@ def run_join(dev, args):
     return res
 
 
+SERVE_VALUE_TYPES = ("int32", "int64", "decimal(12,2)", "float64")
+
+
+def gen_serve_pools(rows, seed=105):
+    """Device pools every serving query takes a window of: int64 and int32 keys, uniform and
+    Zipf(1.1) over 1e3..1e7 groups; int32 / int64 / decimal(12,2) / float64 values; unique build
+    keys scramble(0..rows) and probe keys scramble(j) with j log-uniform over [1, 1.1 * rows], so a
+    probe of a build over keys 0..b hits with probability ln(b) / ln(1.1 * rows)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    pools = {"keys": {}}
+    for e in range(3, 8):
+        G = 10 ** e
+        for dist in ("uniform", "zipf"):
+            r = (torch.randint(0, G, (rows,), dtype=torch.int64, device="cuda", generator=g) if dist == "uniform"
+                 else zipf_ranks(g, rows, G, 1.1))
+            k32 = ((r * 2654435761) & 0xFFFFFFFF).to(torch.int32)
+            pools["keys"][(G, dist)] = (scramble(r), k32)
+            del r
+    pools["int32"] = torch.randint(-2 ** 31, 2 ** 31, (rows,), dtype=torch.int32, device="cuda", generator=g)
+    pools["int64"] = torch.randint(-10 ** 6, 10 ** 6 + 1, (rows,), dtype=torch.int64, device="cuda", generator=g)
+    pools["decimal(12,2)"] = torch.randint(0, 10 ** 8, (rows,), dtype=torch.int64, device="cuda", generator=g)
+    pools["float64"] = torch.empty(rows, dtype=torch.float64, device="cuda").log_normal_(0.0, 1.0, generator=g)
+    pools["build_keys"] = scramble(torch.arange(rows, dtype=torch.int64, device="cuda"))
+    u = torch.rand(rows, dtype=torch.float64, device="cuda", generator=g)
+    j = torch.exp(u * float(np.log(1.1 * rows))).to(torch.int64) - 1
+    pools["probe_keys"] = scramble(j)
+    del u, j
+    pools["build_dec"] = torch.randint(0, 10 ** 7, (rows,), dtype=torch.int64, device="cuda", generator=g)
+    torch.cuda.synchronize()
+    return pools
+
+
+def serve_queries(pools, rows, seconds, rate, seed=106):
+    """The query stream: Poisson arrivals, 70% group-by / 30% join, sizes log-uniform over
+    [1e4, 1e8] rows (capped by the pools)."""
+    from vecruntime_amd import abi, vr
+    rng = np.random.default_rng(seed)
+    qs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        n = int(min(rows, np.exp(rng.uniform(np.log(1e4), np.log(1e8)))))
+        off = int(rng.integers(0, rows - n + 1))
+        sl = slice(off, off + n)
+        if rng.random() < 0.7:
+            G = 10 ** int(rng.integers(3, 8))
+            dist = "uniform" if rng.random() < 0.5 else "zipf"
+            k64, k32 = pools["keys"][(G, dist)]
+            kt = abi.INT64 if rng.random() < 0.5 else abi.INT32
+            key = vr.device_column(kt, (k64 if kt == abi.INT64 else k32)[sl])
+            vt = SERVE_VALUE_TYPES[int(rng.integers(0, 4))]
+            lane = {"int32": abi.INT32, "int64": abi.INT64, "decimal(12,2)": abi.INT64, "float64": abi.FLOAT64}[vt]
+            val = vr.device_column(lane, pools[vt][sl])
+            aggs = [abi.agg_desc(abi.COUNT_STAR)]
+            if vt == "int32":
+                aggs += [abi.agg_desc(abi.SUM_LONG, 0), abi.agg_desc(abi.MIN, 0), abi.agg_desc(abi.MAX, 0)]
+            elif vt == "int64":
+                aggs += [abi.agg_desc(abi.SUM_LONG, 0), abi.agg_desc(abi.MAX, 0)]
+            elif vt == "decimal(12,2)":
+                aggs += [abi.agg_desc(abi.SUM_WIDE, 0, buffer_precision=22, buffer_scale=2),
+                         abi.agg_desc(abi.AVG_WIDE, 0, buffer_precision=22, buffer_scale=2)]
+            else:
+                aggs += [abi.agg_desc(abi.SUM_DOUBLE, 0), abi.agg_desc(abi.AVG_DOUBLE, 0)]
+            spec = abi.AggSpec([kt], [lane], aggs, key_nullable=[False])
+            qs.append(dict(at=t, kind="agg", n=n, groups=min(G, n), spec=spec, batches=[([key], [val], n)],
+                           desc="%s %s G=%d %s" % ("int64" if kt == abi.INT64 else "int32", dist, G, vt)))
+        else:
+            nb = max(1000, n // 5)
+            npr = max(1, n - nb)
+            poff = int(rng.integers(0, rows - npr + 1))
+            ps = slice(poff, poff + npr)
+            jt = [abi.INNER, abi.INNER, abi.LEFT_SEMI, abi.LEFT_ANTI][int(rng.integers(0, 4))]
+            spec = abi.JoinSpec([abi.INT64], [abi.INT64, abi.INT32, abi.INT64], [0], capacity_hint=nb)
+            build = [vr.device_column(abi.INT64, pools["build_keys"][:nb]),
+                     vr.device_column(abi.INT32, pools["int32"][:nb]),
+                     vr.device_column(abi.INT64, pools["build_dec"][:nb])]
+            probe = [vr.device_column(abi.INT64, pools["probe_keys"][ps]),
+                     vr.device_column(abi.INT32, pools["int32"][ps]),
+                     vr.device_column(abi.INT32, pools["int32"][rows - npr - poff:rows - poff])]
+            qs.append(dict(at=t, kind="join", n=nb + npr, nb=nb, npr=npr, jt=jt, spec=spec, build=build,
+                           probe=probe, build_out=[1, 2] if jt == abi.INNER else [],
+                           desc="join %s build=%d probe=%d" % (["inner", "semi", "anti"][jt], nb, npr)))
+    return qs
+
+
+def run_serve(dev, args):
+    """The serving loop: a Poisson stream of group-bys and joins through the query server."""
+    import threading
+    import torch
+    from vecruntime_amd import abi
+    from vecruntime_amd.serve import Server
+    rows = args.serve_pool_rows
+    log("serving: generating device pools of %d rows" % rows)
+    t0 = time.time()
+    pools = gen_serve_pools(rows)
+    qs = serve_queries(pools, rows, args.serve_seconds, args.serve_rate)
+    log("serving: pools in %.1f s; %d queries over %.1f s (%d group-by, %d join), %.3e input rows"
+        % (time.time() - t0, len(qs), args.serve_seconds, sum(q["kind"] == "agg" for q in qs),
+           sum(q["kind"] == "join" for q in qs), sum(q["n"] for q in qs)))
+    srv = Server(dev, workers=args.serve_workers, hbm_budget=int(args.serve_budget_gb * 1e9))
+
+    def submit(q):
+        if q["kind"] == "agg":
+            return srv.submit_agg(q["spec"], q["batches"], expected_groups=q["groups"])
+        return srv.submit_join(q["spec"], q["build"], q["nb"], q["probe"], [0], q["npr"], q["jt"],
+                               probe_out=[0, 1, 2], build_out=q["build_out"], expected_rows=q["npr"])
+
+    # warm-up: a few queries of each kind, untimed
+    for q in [x for x in qs if x["kind"] == "agg"][:3] + [x for x in qs if x["kind"] == "join"][:2]:
+        t = submit(q)
+        r = srv.wait(t)
+        assert r.status == abi.VR_OK, (q["desc"], r.status)
+        srv.release(t)
+    tickets = [None] * len(qs)
+    submitted_at = [0.0] * len(qs)
+    ready = [threading.Event() for _ in qs]
+    results = [None] * len(qs)
+    count_ok = [True]
+
+    def waiter():
+        for i, q in enumerate(qs):
+            ready[i].wait()
+            r = srv.wait(tickets[i])
+            done = time.perf_counter()
+            results[i] = (r.status, r.latency_ms, r.queue_ms, r.run_ms, r.rows, done)
+            if r.status == abi.VR_OK and q["kind"] == "agg" and r.rows <= 100_000:
+                cnt = srv.fetch(tickets[i], r)[1]
+                count_ok[0] &= int(cnt.data[:r.rows].sum()) == q["n"]
+            srv.release(tickets[i])
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for i, q in enumerate(qs):
+        delay = start + q["at"] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        submitted_at[i] = time.perf_counter()
+        tickets[i] = submit(q)
+        ready[i].set()
+    th.join()
+    end = max(r[5] for r in results)
+    st = srv.stats()
+    srv.close()
+    failed = sum(1 for r in results if r[0] != abi.VR_OK)
+    # latency from the scheduled arrival: any lag of the submitting thread counts against it
+    lat = np.array([(submitted_at[i] - (start + q["at"])) * 1e3 + results[i][1] for i, q in enumerate(qs)])
+    total_rows = sum(q["n"] for q in qs)
+    res = {"queries": len(qs), "failed": failed, "seconds": end - start, "rows": total_rows,
+           "rows_per_s": total_rows / (end - start),
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_queue_ms": float(np.mean([r[2] for r in results])),
+           "agg_p50_ms": float(np.percentile([lat[i] for i, q in enumerate(qs) if q["kind"] == "agg"], 50)),
+           "join_p50_ms": float(np.percentile([lat[i] for i, q in enumerate(qs) if q["kind"] == "join"], 50)),
+           "peak_inflight": st["peak_inflight"], "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9,
+           "workers": args.serve_workers, "rate_qps": args.serve_rate, "count_checks_ok": count_ok[0]}
+    log("serving: %s" % json.dumps(res))
+    del pools, qs
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,6 +611,12 @@ def main():
     ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--uneven-steps", type=int, default=2)
     ap.add_argument("--tune-budget", type=float, default=60.0)
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-pool-rows", type=int, default=100_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -458,6 +630,7 @@ def main():
     gb = run_groupby(dev, args, n)
     join = None if args.skip_join else run_join(dev, args)
     uneven = None if args.skip_uneven else run_uneven(dev, args)
+    serve = None if args.skip_serve else run_serve(dev, args)
     # roofline of the dominant kernel: algorithmic bytes per launch / its measured duration
     bytes_per_row = 8 + 8 + 8 + 4 + 3 / 8.0
     alg_bytes = n * bytes_per_row
@@ -495,6 +668,7 @@ def main():
                   "batched_over_plain": gb["batched"]["rows_per_s"] / gb["plain"]["rows_per_s"]},
         "join": join,
         "batched_uneven": uneven,
+        "serving": serve,
     }
     dev.close()
     print(json.dumps(line), flush=True)

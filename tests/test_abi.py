@@ -40,8 +40,11 @@ def test_struct_layouts_match_c():
     structs = {"vr_column": abi.VrColumn, "vr_out_column": abi.VrOutColumn, "vr_agg_desc": abi.VrAggDesc,
                "vr_agg_spec": abi.VrAggSpec, "vr_stats": abi.VrStats, "vr_batch_set": abi.VrBatchSet,
                "vr_launch_config": abi.VrLaunchConfig, "vr_join_spec": abi.VrJoinSpec,
-               "vr_probe_output": abi.VrProbeOutput}
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vr.h"', "int main(void) {"]
+               "vr_probe_output": abi.VrProbeOutput, "vr_server_config": abi.VrServerConfig,
+               "vr_query": abi.VrQuery, "vr_query_result": abi.VrQueryResult,
+               "vr_server_stats": abi.VrServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vr.h"', '#include "vr_serve.h"',
+             "int main(void) {"]
     for cname, py in structs.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
         for fname, _ in py._fields_:

@@ -375,3 +375,35 @@ def test_composite_key_filter_batched_matches_plain_and_oracle(dev):
     gb.update_batches(dev_batches, filter_col=0, threshold=thr)
     for g in (gb, gp):
         compare(g, o, spec)
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_long_batch_over_few_groups_is_split(dev, batched):
+    """One 9M-row batch over 300 groups: the table stays sized by its groups and the batch is cut
+    into several launches (double MIN/MAX with NaN, infinities and signed zeros, sums): results
+    equal the oracle's as for one launch."""
+    import torch
+    from vecruntime_amd import vr
+    rng = np.random.default_rng(107)
+    n = 9_000_000
+    pool = rng.integers(-2 ** 63, 2 ** 63 - 1, 300, dtype=np.int64)
+    keys = pool[rng.integers(0, 300, n)]
+    dbl = rng.normal(0, 1, n)
+    special = rng.random(n) < 0.001
+    dbl[special] = rng.choice([np.nan, 0.0, -0.0, np.inf, -np.inf], int(special.sum()))
+    i64 = rng.integers(-10 ** 12, 10 ** 12, n)
+    spec = abi.AggSpec([abi.INT64], [abi.FLOAT64, abi.INT64],
+                       [abi.agg_desc(abi.COUNT_STAR), abi.agg_desc(abi.MIN, 0), abi.agg_desc(abi.MAX, 0),
+                        abi.agg_desc(abi.SUM_LONG, 1), abi.agg_desc(abi.SUM_DOUBLE, 0)],
+                       key_nullable=[False], capacity_hint=300)
+    tk, td, ti = (torch.from_numpy(a).cuda() for a in (keys, dbl, i64))
+    g = vr.GroupedAggregate(dev, spec)
+    dk, dv = [vr.device_column(abi.INT64, tk)], [vr.device_column(abi.FLOAT64, td), vr.device_column(abi.INT64, ti)]
+    if batched:
+        g.update_batches([(dk, dv, n)])
+    else:
+        g.update(dk, dv, n)
+    o = OracleAgg(spec)
+    o.update([Column.longs(keys)], [Column.doubles(dbl), Column.longs(i64)], n)
+    compare(g, o, spec, rel=1e-9)
+    assert g.memory() < 2 ** 30  # sized by its groups plus one launch's room, not by 9M rows

@@ -163,6 +163,67 @@ class VrProbeOutput(C.Structure):
     ]
 
 
+# vr_query_kind (include/vr_serve.h)
+QUERY_AGG, QUERY_JOIN = 0, 1
+MAX_RESULT_COLUMNS = 40
+
+
+class VrServerConfig(C.Structure):
+    _fields_ = [
+        ("workers", C.c_int32),
+        ("reserved", C.c_int32),
+        ("hbm_budget_bytes", C.c_int64),
+    ]
+
+
+class VrQuery(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("join_type", C.c_int32),
+        ("agg", C.POINTER(VrAggSpec)),
+        ("batches", C.POINTER(VrBatchSet)),
+        ("expected_groups", C.c_int64),
+        ("join", C.POINTER(VrJoinSpec)),
+        ("build_columns", C.POINTER(VrColumn)),
+        ("build_rows", C.c_int64),
+        ("probe_columns", C.POINTER(VrColumn)),
+        ("num_probe_columns", C.c_int32),
+        ("num_probe_out", C.c_int32),
+        ("probe_key_columns", C.POINTER(C.c_int32)),
+        ("probe_out_columns", C.POINTER(C.c_int32)),
+        ("probe_rows", C.c_int64),
+        ("num_build_out", C.c_int32),
+        ("reserved", C.c_int32),
+        ("build_out_columns", C.POINTER(C.c_int32)),
+        ("expected_rows", C.c_int64),
+    ]
+
+
+class VrQueryResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("num_columns", C.c_int32),
+        ("rows", C.c_int64),
+        ("columns", VrOutColumn * MAX_RESULT_COLUMNS),
+        ("queue_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("latency_ms", C.c_double),
+        ("device_bytes", C.c_int64),
+        ("agg", C.c_void_p),
+    ]
+
+
+class VrServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("completed", C.c_int64),
+        ("failed", C.c_int64),
+        ("peak_inflight", C.c_int64),
+        ("peak_reserved_bytes", C.c_int64),
+        ("reserved_bytes", C.c_int64),
+    ]
+
+
 def int_array(values):
     """A ctypes int32 array (kept alive by the caller)."""
     arr = (C.c_int32 * max(1, len(values)))(*values)

@@ -19,12 +19,13 @@ ARCH = os.environ.get("VR_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["vr_runtime.hip", "vr_agg.hip", "vr_join.hip"]
+SOURCES = ["vr_runtime.hip", "vr_agg.hip", "vr_join.hip", "vr_serve.hip"]
 
 
 def _headers():
     hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
-    hs.append(os.path.join(os.path.dirname(HERE), "include", "vr.h"))
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    hs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
     return hs
 
 

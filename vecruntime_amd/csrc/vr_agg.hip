@@ -100,9 +100,8 @@ __global__ __launch_bounds__(AGG_BLOCK) void agg_update_kernel(UpdateArgs a) {
     }
     __syncthreads();
     const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
-    const int64_t ntiles = MULTI ? a.total_tiles : (a.one.n + T - 1) / T;
     int since_flush = 0;
-    for (int64_t tile = (MULTI ? a.tile_lo : 0) + blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int64_t tile = a.tile_lo + blockIdx.x; tile < a.total_tiles; tile += gridDim.x) {
         const BatchDesc* bp;
         int64_t row0;
         if (MULTI) {
@@ -284,8 +283,10 @@ __device__ __forceinline__ void nan_payload_rows(const UpdateArgs& a, const Batc
 template <bool MULTI>
 __global__ void nan_payload_kernel(UpdateArgs a) {
     if (a.t.counters[C_NAN] != (uint64_t)a.epoch) return;
-    if (!MULTI) {
-        nan_payload_rows(a, a.one, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, a.one.n,
+    if (!MULTI) {  // the rows of tiles [tile_lo, total_tiles) of the update launch
+        const int64_t T = (int64_t)AGG_BLOCK * a.rows_per_thread;
+        const int64_t hi = a.total_tiles * T < a.one.n ? a.total_tiles * T : a.one.n;
+        nan_payload_rows(a, a.one, a.tile_lo * T + (int64_t)blockIdx.x * blockDim.x + threadIdx.x, hi,
                          (int64_t)gridDim.x * blockDim.x);
         return;
     }
@@ -1089,26 +1090,59 @@ static void launch_shape(AggImpl& h, const vr_launch_config* cfg, int64_t n, Upd
     *lds_bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
 }
 
+// Every input row may be a new group, so a launch never takes more rows than the table has room
+// for.  The table grows only to leave room for CHUNK_ROWS more groups; a longer input is cut into
+// launches of at most its headroom, so a long input over few groups keeps a table sized by its
+// groups, not by its rows.
+constexpr int64_t CHUNK_ROWS = 3ll << 20;
+
+// Tiles of T rows, from `t0` of `tiles`, that the next launch may cover (growing the table first).
+static vr_status next_span(AggImpl& h, int64_t t0, int64_t tiles, int64_t T, hipStream_t s, int64_t* span) {
+    VR_TRY(ensure_capacity(h, std::min<int64_t>((tiles - t0) * T, CHUNK_ROWS), s));
+    double headroom = h.max_load * (double)h.slots - (double)h.groups_upper;
+    *span = std::max<int64_t>(1, std::min<int64_t>(tiles - t0, (int64_t)(headroom / (double)T)));
+    return VR_OK;
+}
+
 static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, const vr_launch_config* cfg) {
-    VR_TRY(ensure_capacity(h, n, s));
     UpdateArgs a = h.proto;
-    a.t = view_of(h);
     b.row_base = h.rows_seen;
     b.tile_begin = 0;
     a.one = b;
     a.num_batches = 1;
-    a.epoch = ++h.epoch;
-    if (h.det) {
-        VR_TRY(h.row_slot.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
-        a.row_slot = h.row_slot.as<uint32_t>();
-    }
     int grid, block;
     size_t lds_bytes;
     launch_shape(h, cfg, n, a, &grid, &block, &lds_bytes, false);
-    hipLaunchKernelGGL(agg_update_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
-    VR_TRY_HIP(hipGetLastError());
-    int launches = 1;
+    const int64_t T = (int64_t)block * a.rows_per_thread;
+    const int64_t tiles = (n + T - 1) / T;
+    int launches = 0;
     if (h.det) {
+        // one launch over the whole batch: the recorded slots must outlive no table growth
+        VR_TRY(ensure_capacity(h, n, s));
+        VR_TRY(h.row_slot.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
+        a.row_slot = h.row_slot.as<uint32_t>();
+    }
+    for (int64_t t0 = 0; t0 < tiles;) {
+        int64_t span = tiles - t0;
+        if (!h.det) VR_TRY(next_span(h, t0, tiles, T, s, &span));
+        a.t = view_of(h);
+        a.tile_lo = t0;
+        a.total_tiles = t0 + span;
+        a.epoch = ++h.epoch;
+        hipLaunchKernelGGL(agg_update_kernel<false>, dim3((unsigned)std::min<int64_t>(grid, span)), dim3(block),
+                           lds_bytes, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
+        if (h.has_f64_minmax) {
+            int64_t rows = std::min<int64_t>(span * T, n - t0 * T);
+            hipLaunchKernelGGL(nan_payload_kernel<false>, dim3(grid_for(rows, 256, h.dev->num_cus * 8)), dim3(256), 0, s, a);
+            VR_TRY_HIP(hipGetLastError());
+            launches++;
+        }
+        h.groups_upper += std::min<int64_t>(span * T, n - t0 * T);
+        t0 += span;
+    }
+    if (h.det && n > 0) {
         // order the batch's rows by slot (stable: row order kept inside a slot), then fold
         VR_TRY(h.sort_keys.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
         VR_TRY(h.sort_vals.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
@@ -1128,13 +1162,7 @@ static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, 
         VR_TRY_HIP(hipGetLastError());
         launches += 3;
     }
-    if (h.has_f64_minmax) {
-        hipLaunchKernelGGL(nan_payload_kernel<false>, dim3(grid_for(n, 256, h.dev->num_cus * 8)), dim3(256), 0, s, a);
-        VR_TRY_HIP(hipGetLastError());
-        launches++;
-    }
     h.rows_seen += n;
-    h.groups_upper += n;
     h.order_count = -1;
     h.dev->count(&vr_stats::kernel_launches, launches);
     return VR_OK;
@@ -1340,39 +1368,29 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     if (rows == 0) return VR_OK;
     VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
     VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs.data(), sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
-    // Persistent launches over runs of batches that fit the table's headroom: every input row
-    // may be a new group, so a run never holds more rows than the table can take before growing.
+    // Persistent launches over runs of tiles that fit the table's headroom (batches may be split).
     int launches = 0;
-    int b0 = 0;
-    while (b0 < nb) {
-        VR_TRY(ensure_capacity(h, bs->lengths[b0], s));
-        double headroom = h.max_load * (double)h.slots - (double)h.groups_upper;
-        int b1 = b0;
-        int64_t run = 0;
-        while (b1 < nb && (double)(run + bs->lengths[b1]) <= headroom) run += bs->lengths[b1++];
-        if (b1 == b0) run = bs->lengths[b1++];
-        int64_t tile_hi = b1 < nb ? descs[b1].tile_begin : tiles;
+    const int cap = h.dev->num_cus * std::max(1, (cfg && cfg->blocks_per_cu) ? cfg->blocks_per_cu : h.cfg.blocks_per_cu);
+    for (int64_t t0 = 0; t0 < tiles;) {
+        int64_t span;
+        VR_TRY(next_span(h, t0, tiles, T, s, &span));
         a.t = view_of(h);
         a.batches = h.batch_descs.as<BatchDesc>();
         a.num_batches = nb;
-        a.tile_lo = descs[b0].tile_begin;
-        a.total_tiles = tile_hi;
+        a.tile_lo = t0;
+        a.total_tiles = t0 + span;
         a.epoch = ++h.epoch;
-        int64_t span = tile_hi - a.tile_lo;
-        int cap = h.dev->num_cus * std::max(1, (cfg && cfg->blocks_per_cu) ? cfg->blocks_per_cu : h.cfg.blocks_per_cu);
         grid = (int)std::max<int64_t>(1, std::min<int64_t>(span, cap));
-        if (span > 0) {
-            hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+        hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
+        if (h.has_f64_minmax) {
+            hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
             VR_TRY_HIP(hipGetLastError());
             launches++;
-            if (h.has_f64_minmax) {
-                hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
-                VR_TRY_HIP(hipGetLastError());
-                launches++;
-            }
         }
-        h.groups_upper += run;
-        b0 = b1;
+        h.groups_upper += span * T;
+        t0 += span;
     }
     h.rows_seen += rows;
     h.order_count = -1;
@@ -1555,6 +1573,7 @@ vr_status vr_agg_reset(vr_agg* agg) {
 vr_status vr_agg_destroy(vr_agg* agg) {
     if (!agg) return VR_OK;
     (void)hipStreamSynchronize(agg->h.dev->stream);
+    if (agg->h.last_ev) (void)hipEventSynchronize(agg->h.last_ev);  // work queued on a caller's stream
     delete agg;
     return VR_OK;
 }

@@ -204,14 +204,7 @@ class GroupedAggregate:
         return keys, outs
 
     def key_dictionary(self, c):
-        count, nbytes = C.c_int64(), C.c_int64()
-        check(_lib.vr_agg_key_dictionary(self.h, c, None, 0, None, 0, C.byref(count), C.byref(nbytes)))
-        offs = np.zeros(count.value + 1, np.int32)
-        data = np.zeros(max(1, nbytes.value), np.uint8)
-        check(_lib.vr_agg_key_dictionary(self.h, c, offs.ctypes.data, len(offs), data.ctypes.data, len(data),
-                                         C.byref(count), C.byref(nbytes)))
-        raw = data.tobytes()
-        return [raw[offs[i]:offs[i + 1]].decode("utf-8") for i in range(count.value)]
+        return agg_key_dictionary(self.h, c)
 
     def memory(self):
         b = C.c_int64()
@@ -292,6 +285,18 @@ class BuildTable:
             return (probe_idx[:r] if probe_idx is not None else None,
                     build_idx[:r] if build_idx is not None else None, outputs, r)
         return probe_idx, build_idx, outputs, r
+
+
+def agg_key_dictionary(handle, c):
+    """The strings of UTF8 key column `c` of an aggregate handle, in id order."""
+    count, nbytes = C.c_int64(), C.c_int64()
+    check(_lib.vr_agg_key_dictionary(handle, c, None, 0, None, 0, C.byref(count), C.byref(nbytes)))
+    offs = np.zeros(count.value + 1, np.int32)
+    data = np.zeros(max(1, nbytes.value), np.uint8)
+    check(_lib.vr_agg_key_dictionary(handle, c, offs.ctypes.data, len(offs), data.ctypes.data, len(data),
+                                     C.byref(count), C.byref(nbytes)))
+    raw = data.tobytes()
+    return [raw[offs[i]:offs[i + 1]].decode("utf-8") for i in range(count.value)]
 
 
 def device_column(type, data, validity=None, length=None, dictionary=None):
