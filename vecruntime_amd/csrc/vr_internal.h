@@ -5,8 +5,10 @@
 
 #include <atomic>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/vr.h"
@@ -43,15 +45,26 @@ struct Device {
     hipStream_t stream = nullptr;       // the handle's own compute stream
     hipStream_t copy_stream = nullptr;  // side stream for host <-> device staging
     std::mutex mu;
-    int64_t allocated = 0;
+    int64_t allocated = 0;  // bytes of the blocks handed out
     int64_t budget = 0;
     vr_stats stats{};
     // Open handles (aggregates, joins) plus one for the vr_device_open reference: the device is
     // freed when the last of them lets go, so a handle destroyed after vr_device_close stays valid.
     std::atomic<int64_t> refs{1};
+    // Block cache: hipFree waits for the whole device, so blocks no queued work reads any more are
+    // kept (by size) and handed out again instead; a request takes the smallest cached block of at
+    // least its size and at most 1.5x.  Every block's size is recorded for the release.
+    std::multimap<size_t, void*> cache;
+    std::unordered_map<void*, size_t> block_bytes;
+    int64_t cached = 0;
+    int64_t cache_limit = 48ll << 30;
 
     vr_status alloc(void** p, size_t bytes);
+    // A block no queued work reads any more (its owner synchronized): kept for reuse.
     void release(void* p, size_t bytes);
+    // A block queued work may still read: freed now (hipFree waits for the device).
+    void release_now(void* p, size_t bytes);
+    void trim();  // frees every cached block
     hipStream_t pick(void* s) const { return s ? (hipStream_t)s : stream; }
     void count(int64_t vr_stats::*field, int64_t v) {
         std::lock_guard<std::mutex> g(mu);
@@ -85,14 +98,16 @@ struct DevBuf {
     ~DevBuf() { reset(); }
     vr_status ensure(Device* d, size_t want) {  // grows (contents dropped) to at least `want` bytes
         if (want <= bytes && p) return VR_OK;
-        reset();
+        if (p) dev->release_now(p, bytes);  // the old block may still be read by queued work
+        p = nullptr;
+        bytes = 0;
         dev = d;
         size_t b = want < 256 ? 256 : want;
         VR_TRY(d->alloc(&p, b));
         bytes = b;
         return VR_OK;
     }
-    void reset() {
+    void reset() {  // the owner has synchronized: no queued work reads the block
         if (p) dev->release(p, bytes);
         p = nullptr;
         bytes = 0;

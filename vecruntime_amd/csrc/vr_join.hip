@@ -250,6 +250,25 @@ __global__ void fill_keys_kernel(uint64_t* buckets, uint64_t nbuckets) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbuckets * 16; i += (uint64_t)gridDim.x * blockDim.x)
         buckets[i] = (i % 16) < BUCKET ? EMPTY_KEY : 0ull;
 }
+// Build statistics: stats[1] += cells holding rows (distinct keys), stats[2] = max rows of a cell.
+__global__ __launch_bounds__(256) void cell_stats_kernel(const uint32_t* cell_count, uint64_t ncells, uint64_t* stats) {
+    typedef hipcub::BlockReduce<uint32_t, 256> Reduce;
+    __shared__ typename Reduce::TempStorage tmp;
+    uint32_t used = 0, mx = 0;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t k = cell_count[c];
+        used += k != 0;
+        mx = k > mx ? k : mx;
+    }
+    uint32_t u = Reduce(tmp).Sum(used);
+    __syncthreads();
+    uint32_t m = Reduce(tmp).Reduce(mx, hipcub::Max());
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long*)&stats[1], (unsigned long long)u);
+        atomicMax((unsigned long long*)&stats[2], (unsigned long long)m);
+    }
+}
+
 __global__ void iota_kernel(uint32_t* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
 }
@@ -455,36 +474,33 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     hipLaunchKernelGGL(join_values_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, t, j.cell_count.as<uint32_t>(),
                        j.cell_start.as<uint32_t>(), ncells, j.range_rows.as<uint32_t>());
     VR_TRY_HIP(hipGetLastError());
+    // statistics (distinct keys, longest chain) reduced on the device into small[1], small[2]
+    hipLaunchKernelGGL(cell_stats_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint32_t>(),
+                       ncells, j.small.as<uint64_t>());
+    VR_TRY_HIP(hipGetLastError());
     // the key equal to the empty marker: its rows follow every cell's rows in range_rows
-    uint64_t sentinel_rows = 0;
-    VR_TRY_HIP(hipMemcpyAsync(&sentinel_rows, j.small.p, 8, hipMemcpyDeviceToHost, s));
+    uint64_t small[3] = {0, 0, 0};
+    uint32_t last[2] = {0, 0};
+    VR_TRY_HIP(hipMemcpyAsync(small, j.small.p, sizeof(small), hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipMemcpyAsync(&last[0], j.cell_start.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipMemcpyAsync(&last[1], j.cell_count.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
     VR_TRY_HIP(hipStreamSynchronize(s));
+    const uint64_t sentinel_rows = small[0];
     j.sentinel_value = 0;
     if (sentinel_rows) {
         // sorted by cell, the 0xFFFFFFFE rows sit just before the 0xFFFFFFFF (null) rows, in build order
-        uint32_t total_cells_rows = 0;
-        uint32_t last_start = 0, last_count = 0;
-        VR_TRY_HIP(hipMemcpy(&last_start, j.cell_start.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost));
-        VR_TRY_HIP(hipMemcpy(&last_count, j.cell_count.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost));
-        total_cells_rows = last_start + last_count;
+        uint32_t total_cells_rows = last[0] + last[1];
         uint64_t payload = total_cells_rows;
         if (sentinel_rows == 1) {
             uint32_t row = 0;
-            VR_TRY_HIP(hipMemcpy(&row, j.range_rows.as<uint32_t>() + total_cells_rows, 4, hipMemcpyDeviceToHost));
+            VR_TRY_HIP(hipMemcpyAsync(&row, j.range_rows.as<uint32_t>() + total_cells_rows, 4, hipMemcpyDeviceToHost, s));
+            VR_TRY_HIP(hipStreamSynchronize(s));
             payload = row;
         }
         j.sentinel_value = (sentinel_rows << 32) | payload;
     }
-    // statistics: distinct keys and the longest chain
-    std::vector<uint32_t> counts(ncells);
-    VR_TRY_HIP(hipMemcpy(counts.data(), j.cell_count.p, ncells * 4, hipMemcpyDeviceToHost));
-    int64_t distinct = sentinel_rows ? 1 : 0, mx = (int64_t)sentinel_rows;
-    for (uint32_t c : counts) {
-        if (c) distinct++;
-        mx = std::max<int64_t>(mx, c);
-    }
-    j.distinct = distinct;
-    j.max_chain = mx;
+    j.distinct = (int64_t)small[1] + (sentinel_rows ? 1 : 0);
+    j.max_chain = std::max<int64_t>((int64_t)small[2], (int64_t)sentinel_rows);
     j.row_cell.reset();
     j.sorted_cells.reset();
     j.rows_in.reset();

@@ -18,29 +18,90 @@ vr_status hip_fail(hipError_t e, const char* what) {
     return fail(VR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+static size_t size_class(size_t bytes) {
+    if (bytes <= (1u << 20)) {  // powers of two up to 1 MiB
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    size_t top = 1ull << (63 - __builtin_clzll(bytes));  // eighths of the power-of-two band above
+    size_t step = top / 8;
+    return (bytes + step - 1) / step * step;
+}
+
 vr_status Device::alloc(void** p, size_t bytes) {
+    const size_t want = size_class(bytes);
     {
         std::lock_guard<std::mutex> g(mu);
-        if (budget > 0 && allocated + (int64_t)bytes > budget)
+        if (budget > 0 && allocated + (int64_t)want > budget)
             return fail(VR_ERR_OOM, "HBM budget of " + std::to_string(budget) + " bytes exceeded");
-        allocated += (int64_t)bytes;
+        auto it = cache.lower_bound(want);
+        if (it != cache.end() && it->first <= want + want / 2) {
+            *p = it->second;
+            allocated += (int64_t)it->first;
+            cached -= (int64_t)it->first;
+            cache.erase(it);
+            return VR_OK;
+        }
+        allocated += (int64_t)want;
     }
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = hipMalloc(p, want);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {  // give the cache back and retry
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(p, want);
+    }
+    std::lock_guard<std::mutex> g(mu);
     if (e != hipSuccess) {
-        std::lock_guard<std::mutex> g(mu);
-        allocated -= (int64_t)bytes;
+        allocated -= (int64_t)want;
         (void)hipGetLastError();
         *p = nullptr;
         return hip_fail(e, "hipMalloc");
     }
+    block_bytes[*p] = want;
     return VR_OK;
 }
 
-void Device::release(void* p, size_t bytes) {
+void Device::release(void* p, size_t) {
+    if (!p) return;
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        size_t b = block_bytes[p];
+        allocated -= (int64_t)b;
+        cache.emplace(b, p);
+        cached += (int64_t)b;
+        while (cached > cache_limit && !cache.empty()) {  // the largest blocks go first
+            auto it = std::prev(cache.end());
+            cached -= (int64_t)it->first;
+            block_bytes.erase(it->second);
+            drop.push_back(it->second);
+            cache.erase(it);
+        }
+    }
+    for (void* q : drop) (void)hipFree(q);
+}
+
+void Device::release_now(void* p, size_t) {
     if (!p) return;
     (void)hipFree(p);
     std::lock_guard<std::mutex> g(mu);
-    allocated -= (int64_t)bytes;
+    allocated -= (int64_t)block_bytes[p];
+    block_bytes.erase(p);
+}
+
+void Device::trim() {
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (auto& kv : cache) {
+            drop.push_back(kv.second);
+            block_bytes.erase(kv.second);
+        }
+        cache.clear();
+        cached = 0;
+    }
+    for (void* q : drop) (void)hipFree(q);
 }
 
 Staging::~Staging() {
@@ -143,6 +204,7 @@ int32_t HostDictionary::index_of(const char* s, int32_t len, bool insert) {
 static void device_free(vr_device* dev) {
     (void)hipStreamSynchronize(dev->d.stream);
     (void)hipStreamSynchronize(dev->d.copy_stream);
+    dev->d.trim();
     (void)hipStreamDestroy(dev->d.stream);
     (void)hipStreamDestroy(dev->d.copy_stream);
     delete dev;
