@@ -193,20 +193,18 @@ def run_groupby(dev, args, n):
         dt = (time.perf_counter() - t0) / steps
         res[name] = dict(ms_per_step=dt * 1e3, rows_per_s=n / dt, steps=steps)
         log("%s: %.2f ms/step, %.3e rows/s" % (name, dt * 1e3, n / dt))
-    # the dominant kernel, timed with HIP events on the stream it runs on
-    agg.reset()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the dominant kernel, timed with HIP events the library records around each of its launches
     reps = max(1, args.steps)
-    kt = []
+    agg.set_timing(True)
     for _ in range(reps):
-        agg.reset()
-        e0.record(stream)
-        agg.update_batches(batches, stream=stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        kt.append(e0.elapsed_time(e1))
-    res["update_kernel_ms"] = float(np.mean(kt))
+        step_batched(cols)
+    torch.cuda.synchronize()
+    kms, kl = agg.kernel_time()
+    agg.set_timing(False)
+    res["update_kernel_ms"] = kms / reps
+    res["update_launches_per_step"] = kl / reps
+    res["update_kernel_avg_launch_ms"] = kms / max(1, kl)
+    log("update kernel: %.3f ms/step over %.1f launches/step" % (kms / reps, kl / reps))
     res["groups"] = groups
     res["checks"] = checks
     # a bounded sample copied to the host for the CPU baseline
@@ -604,7 +602,7 @@ def main():
     ap.add_argument("--rows", type=int, default=200_000_000)
     ap.add_argument("--skip-join", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-batches", type=int, default=600)
+    ap.add_argument("--cpu-batches", type=int, default=3052)
     ap.add_argument("--join-build", type=int, default=10_000_000)
     ap.add_argument("--join-probe", type=int, default=40_000_000)
     ap.add_argument("--skip-uneven", action="store_true")
@@ -617,7 +615,18 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-pool-rows", type=int, default=100_000_000)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only copy this many MiB device to device three times (scales the PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(args.copy_calibration << 17, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU path (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
@@ -661,8 +670,9 @@ def main():
                    "groups": gb["groups"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "agg_update_kernel<true>", "kernel_ms": gb["update_kernel_ms"],
-                     "algorithmic_bytes": alg_bytes},
+                     "kernel": "agg_update_kernel<true>", "kernel_ms_per_step": gb["update_kernel_ms"],
+                     "launches_per_step": gb["update_launches_per_step"],
+                     "avg_launch_ms": gb["update_kernel_avg_launch_ms"], "algorithmic_bytes_per_step": alg_bytes},
         "cpu_baseline": cpu,
         "plain": {"rows_per_s": gb["plain"]["rows_per_s"], "ms_per_step": gb["plain"]["ms_per_step"],
                   "batched_over_plain": gb["batched"]["rows_per_s"] / gb["plain"]["rows_per_s"]},

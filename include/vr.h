@@ -248,6 +248,11 @@ vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg);
 
 vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* batches,
                                 const vr_launch_config* cfg, void* stream);
+/* Measurement aid (no reference counterpart): while timing is on, every update-kernel launch of the
+ * handle is bracketed by HIP events on its stream; vr_agg_kernel_time waits for them and returns
+ * the summed device time and the launch count since timing was switched on or last read. */
+vr_status vr_agg_set_timing(vr_agg* agg, int32_t enable);
+vr_status vr_agg_kernel_time(vr_agg* agg, double* ms, int64_t* launches);
 /* The same with a filter fused in front of a plain per-batch update (one launch per batch):
  * the baseline the persistent launch is measured against. */
 vr_status vr_agg_update_filtered(vr_agg* agg, const vr_column* keys, const vr_column* values,

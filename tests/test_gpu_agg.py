@@ -407,3 +407,35 @@ def test_long_batch_over_few_groups_is_split(dev, batched):
     o.update([Column.longs(keys)], [Column.doubles(dbl), Column.longs(i64)], n)
     compare(g, o, spec, rel=1e-9)
     assert g.memory() < 2 ** 30  # sized by its groups plus one launch's room, not by 9M rows
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_guarded_launch_stops_grows_and_resumes(dev, batched):
+    """6M rows over 4.5M distinct keys in one call: the launch passes the table's load limit part way,
+    its workgroups stop, the table grows and the same grid resumes where each stopped -- every row
+    folded once (COUNT(*) total, sums, double MIN/MAX with NaNs as the oracle)."""
+    import torch
+    from vecruntime_amd import vr
+    rng = np.random.default_rng(108)
+    n, distinct = 6_000_000, 4_500_000
+    ids = np.concatenate([rng.permutation(distinct), rng.integers(0, distinct, n - distinct)])
+    keys = (ids.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15) ^ np.uint64(0x5851F42D4C957F2D)).astype(np.int64)
+    dbl = rng.normal(0, 1, n)
+    dbl[rng.random(n) < 0.01] = np.nan
+    i64 = rng.integers(-10 ** 12, 10 ** 12, n)
+    spec = abi.AggSpec([abi.INT64], [abi.FLOAT64, abi.INT64],
+                       [abi.agg_desc(abi.COUNT_STAR), abi.agg_desc(abi.MIN, 0), abi.agg_desc(abi.MAX, 0),
+                        abi.agg_desc(abi.SUM_LONG, 1)],
+                       key_nullable=[False], capacity_hint=1000)
+    tk, td, ti = (torch.from_numpy(a).cuda() for a in (keys, dbl, i64))
+    grows0 = dev.stats()["table_grows"]
+    g = vr.GroupedAggregate(dev, spec)
+    dk, dv = [vr.device_column(abi.INT64, tk)], [vr.device_column(abi.FLOAT64, td), vr.device_column(abi.INT64, ti)]
+    if batched:
+        g.update_batches([(dk, dv, n)])
+    else:
+        g.update(dk, dv, n)
+    assert dev.stats()["table_grows"] - grows0 >= 2  # sized for the guard's room, then again mid-launch
+    o = OracleAgg(spec)
+    o.update([Column.longs(keys)], [Column.doubles(dbl), Column.longs(i64)], n)
+    compare(g, o, spec)

@@ -100,8 +100,21 @@ __global__ __launch_bounds__(AGG_BLOCK) void agg_update_kernel(UpdateArgs a) {
     }
     __syncthreads();
     const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
+    const bool guard = a.resume != nullptr;
+    __shared__ int stop;
     int since_flush = 0;
-    for (int64_t tile = a.tile_lo + blockIdx.x; tile < a.total_tiles; tile += gridDim.x) {
+    int64_t tile = a.resumed ? a.resume[blockIdx.x] : a.tile_lo + blockIdx.x;
+    for (; tile < a.total_tiles; tile += gridDim.x) {
+        if (guard) {
+            __syncthreads();  // the previous tile's inserts are all in `inserted`
+            if (threadIdx.x == 0) {
+                if (inserted) atomic_add_u64(&a.t.counters[C_GROUPS], inserted);
+                inserted = 0;
+                stop = ld_coherent(&a.t.counters[C_GROUPS]) >= a.soft_limit;
+            }
+            __syncthreads();
+            if (stop) break;
+        }
         const BatchDesc* bp;
         int64_t row0;
         if (MULTI) {
@@ -159,7 +172,13 @@ __global__ __launch_bounds__(AGG_BLOCK) void agg_update_kernel(UpdateArgs a) {
     __syncthreads();
     if (E && since_flush) lds_flush(a, lkeys, lstate, E, by_key, false, &inserted);
     __syncthreads();
-    if (threadIdx.x == 0 && inserted) atomic_add_u64(&a.t.counters[C_GROUPS], inserted);
+    if (threadIdx.x == 0) {
+        if (inserted) atomic_add_u64(&a.t.counters[C_GROUPS], inserted);
+        if (guard) {
+            a.resume[blockIdx.x] = tile;
+            if (tile < a.total_tiles) atomicOr((unsigned long long*)&a.t.counters[C_STOP], 1ull);
+        }
+    }
 }
 
 // Fills state rows [0, rows) with the identity row.  A wave covers 64 consecutive rows and its
@@ -649,6 +668,7 @@ struct AggImpl {
 
     // batched launches
     DevBuf batch_descs;
+    DevBuf resume;  // guarded launches: the tile each workgroup resumes at
 
     vr_launch_config cfg{256, 4, 4, -1, 512, 4};
     int lds_budget = 64 * 1024;  // LDS bytes per workgroup for the hot-key cache
@@ -657,13 +677,39 @@ struct AggImpl {
     // another stream (or a host read) first waits for it
     hipStream_t last_stream = nullptr;
     hipEvent_t last_ev = nullptr;
+    // measurement aid: events around every update-kernel launch while `timing` is on
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timer_events;
+    size_t timer_used = 0;
     AggImpl() = default;
     AggImpl(const AggImpl&) = delete;
     AggImpl& operator=(const AggImpl&) = delete;
     ~AggImpl() {
         if (last_ev) (void)hipEventDestroy(last_ev);
+        for (auto& e : timer_events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
     }
 };
+
+static vr_status timer_begin(AggImpl& h, hipStream_t s) {
+    if (!h.timing) return VR_OK;
+    if (h.timer_used == h.timer_events.size()) {
+        std::pair<hipEvent_t, hipEvent_t> e{};
+        VR_TRY_HIP(hipEventCreate(&e.first));
+        VR_TRY_HIP(hipEventCreate(&e.second));
+        h.timer_events.push_back(e);
+    }
+    VR_TRY_HIP(hipEventRecord(h.timer_events[h.timer_used].first, s));
+    return VR_OK;
+}
+
+static vr_status timer_end(AggImpl& h, hipStream_t s) {
+    if (!h.timing) return VR_OK;
+    VR_TRY_HIP(hipEventRecord(h.timer_events[h.timer_used++].second, s));
+    return VR_OK;
+}
 
 static vr_status compile_spec(AggImpl& h) {
     UpdateArgs& p = h.proto;
@@ -879,16 +925,8 @@ static int64_t groups_of(const uint64_t* c) {
     return (int64_t)c[C_GROUPS] + ((c[C_RESERVED] & RES_SENTINEL) ? 1 : 0) + ((c[C_RESERVED] & RES_NULL) ? 1 : 0);
 }
 
-// Makes room for `n` more rows: grows (and migrates) the table when the groups could pass the
-// load limit.  The group count is read back only when the cheap upper bound says it might.
-static vr_status ensure_capacity(AggImpl& h, int64_t n, hipStream_t s) {
-    if ((double)(h.groups_upper + n) <= h.max_load * (double)h.slots) return VR_OK;
-    uint64_t c[C_WORDS];
-    VR_TRY(read_counters(h, c, s));
-    int64_t g = groups_of(c);
-    h.groups_upper = g;
-    if ((double)(g + n) <= h.max_load * (double)h.slots) return VR_OK;
-    uint64_t want = next_pow2((uint64_t)((double)(g + n) / h.max_load) + 1);
+// Moves the groups into a new table of `want` slots.
+static vr_status grow_table(AggImpl& h, uint64_t want, hipStream_t s) {
     if (want > (1ull << 31)) return fail(VR_ERR_UNSUPPORTED, "more than 2^30 groups run on the CPU kernels");
     DevBuf ncells, nstate;
     VR_TRY(alloc_table(h, want, ncells, nstate, s));
@@ -899,7 +937,6 @@ static vr_status ensure_capacity(AggImpl& h, int64_t n, hipStream_t s) {
     a.t.state = nstate.as<uint64_t>();
     a.t.slots = want;
     a.t.mask = want - 1;
-    // the new table's group count is rebuilt by the migration's inserts only for bookkeeping: reset it
     hipLaunchKernelGGL(migrate_kernel, dim3(grid_for(h.slots + 2, 256, h.dev->num_cus * 16)), dim3(256), 0, s, from, a);
     VR_TRY_HIP(hipGetLastError());
     VR_TRY_HIP(hipStreamSynchronize(s));
@@ -910,6 +947,18 @@ static vr_status ensure_capacity(AggImpl& h, int64_t n, hipStream_t s) {
     h.slots = want;
     h.dev->count(&vr_stats::table_grows, 1);
     return VR_OK;
+}
+
+// Makes room for `n` more rows: grows (and migrates) the table when the groups could pass the
+// load limit.  The group count is read back only when the cheap upper bound says it might.
+static vr_status ensure_capacity(AggImpl& h, int64_t n, hipStream_t s) {
+    if ((double)(h.groups_upper + n) <= h.max_load * (double)h.slots) return VR_OK;
+    uint64_t c[C_WORDS];
+    VR_TRY(read_counters(h, c, s));
+    int64_t g = groups_of(c);
+    h.groups_upper = g;
+    if ((double)(g + n) <= h.max_load * (double)h.slots) return VR_OK;
+    return grow_table(h, next_pow2((uint64_t)((double)(g + n) / h.max_load) + 1), s);
 }
 
 static vr_status init_table(AggImpl& h, int64_t capacity_hint) {
@@ -1090,18 +1139,45 @@ static void launch_shape(AggImpl& h, const vr_launch_config* cfg, int64_t n, Upd
     *lds_bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
 }
 
-// Every input row may be a new group, so a launch never takes more rows than the table has room
-// for.  The table grows only to leave room for CHUNK_ROWS more groups; a longer input is cut into
-// launches of at most its headroom, so a long input over few groups keeps a table sized by its
-// groups, not by its rows.
-constexpr int64_t CHUNK_ROWS = 3ll << 20;
+// Inputs up to GUARD_ROWS rows are launched once the table has room for every row to be a new
+// group (no read-back while the upper bound allows it); longer inputs take the guarded launch, so a
+// long input over few groups keeps a table sized by its groups, not by its rows.
+constexpr int64_t GUARD_ROWS = 1ll << 22;
+// Share of the slots a guarded launch may fill: the load limit plus what the workgroups in flight
+// can still insert after they see it.
+constexpr double GUARD_FULL = 0.95;
 
-// Tiles of T rows, from `t0` of `tiles`, that the next launch may cover (growing the table first).
-static vr_status next_span(AggImpl& h, int64_t t0, int64_t tiles, int64_t T, hipStream_t s, int64_t* span) {
-    VR_TRY(ensure_capacity(h, std::min<int64_t>((tiles - t0) * T, CHUNK_ROWS), s));
-    double headroom = h.max_load * (double)h.slots - (double)h.groups_upper;
-    *span = std::max<int64_t>(1, std::min<int64_t>(tiles - t0, (int64_t)(headroom / (double)T)));
-    return VR_OK;
+// Runs the update over tiles [a.tile_lo, a.total_tiles) whatever the table's headroom: workgroups
+// stop at the load limit, the table grows, and the same grid resumes where each one stopped.
+template <bool MULTI>
+static vr_status launch_guarded(AggImpl& h, UpdateArgs& a, int grid, int block, size_t lds_bytes, hipStream_t s,
+                                int* launches) {
+    const int64_t T = (int64_t)block * a.rows_per_thread;
+    // Groups the table may gain beyond the limit: between two looks at the shared count a workgroup
+    // inserts at most a tile's rows and its LDS entries, and the count it sees lags by as much.
+    const int64_t margin = 2 * (int64_t)grid * (T + a.lds_entries);
+    VR_TRY(h.resume.ensure(h.dev, (size_t)grid * sizeof(int64_t)));
+    a.resume = h.resume.as<int64_t>();
+    a.resumed = 0;
+    while (true) {
+        VR_TRY(ensure_capacity(h, margin, s));
+        if ((GUARD_FULL - h.max_load) * (double)h.slots < (double)margin)
+            VR_TRY(grow_table(h, next_pow2((uint64_t)((double)margin / (GUARD_FULL - h.max_load)) + 1), s));
+        a.t = view_of(h);
+        a.soft_limit = (uint64_t)(h.max_load * (double)h.slots);
+        VR_TRY(timer_begin(h, s));
+        hipLaunchKernelGGL(agg_update_kernel<MULTI>, dim3(grid), dim3(block), lds_bytes, s, a);
+        VR_TRY(timer_end(h, s));
+        VR_TRY_HIP(hipGetLastError());
+        (*launches)++;
+        uint64_t c[C_WORDS];
+        VR_TRY(read_counters(h, c, s));
+        h.groups_upper = groups_of(c);
+        if (!c[C_STOP]) return VR_OK;
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>() + C_STOP, 1ull, 0ull);
+        VR_TRY_HIP(hipGetLastError());
+        a.resumed = 1;
+    }
 }
 
 static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, const vr_launch_config* cfg) {
@@ -1114,33 +1190,26 @@ static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, 
     size_t lds_bytes;
     launch_shape(h, cfg, n, a, &grid, &block, &lds_bytes, false);
     const int64_t T = (int64_t)block * a.rows_per_thread;
-    const int64_t tiles = (n + T - 1) / T;
+    a.tile_lo = 0;
+    a.total_tiles = (n + T - 1) / T;
+    a.epoch = ++h.epoch;
     int launches = 0;
-    if (h.det) {
-        // one launch over the whole batch: the recorded slots must outlive no table growth
+    if (h.det || n <= GUARD_ROWS) {
+        // the deterministic mode records each row's slot: one launch, no growth inside it
         VR_TRY(ensure_capacity(h, n, s));
-        VR_TRY(h.row_slot.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
-        a.row_slot = h.row_slot.as<uint32_t>();
-    }
-    for (int64_t t0 = 0; t0 < tiles;) {
-        int64_t span = tiles - t0;
-        if (!h.det) VR_TRY(next_span(h, t0, tiles, T, s, &span));
+        if (h.det) {
+            VR_TRY(h.row_slot.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
+            a.row_slot = h.row_slot.as<uint32_t>();
+        }
         a.t = view_of(h);
-        a.tile_lo = t0;
-        a.total_tiles = t0 + span;
-        a.epoch = ++h.epoch;
-        hipLaunchKernelGGL(agg_update_kernel<false>, dim3((unsigned)std::min<int64_t>(grid, span)), dim3(block),
-                           lds_bytes, s, a);
+        VR_TRY(timer_begin(h, s));
+        hipLaunchKernelGGL(agg_update_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
+        VR_TRY(timer_end(h, s));
         VR_TRY_HIP(hipGetLastError());
         launches++;
-        if (h.has_f64_minmax) {
-            int64_t rows = std::min<int64_t>(span * T, n - t0 * T);
-            hipLaunchKernelGGL(nan_payload_kernel<false>, dim3(grid_for(rows, 256, h.dev->num_cus * 8)), dim3(256), 0, s, a);
-            VR_TRY_HIP(hipGetLastError());
-            launches++;
-        }
-        h.groups_upper += std::min<int64_t>(span * T, n - t0 * T);
-        t0 += span;
+        h.groups_upper += n;
+    } else {
+        VR_TRY(launch_guarded<false>(h, a, grid, block, lds_bytes, s, &launches));
     }
     if (h.det && n > 0) {
         // order the batch's rows by slot (stable: row order kept inside a slot), then fold
@@ -1161,6 +1230,12 @@ static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, 
                            h.sort_keys.as<uint32_t>(), h.sort_vals.as<uint32_t>(), n, a);
         VR_TRY_HIP(hipGetLastError());
         launches += 3;
+    }
+    if (h.has_f64_minmax) {  // over the final table: after any growth
+        a.t = view_of(h);
+        hipLaunchKernelGGL(nan_payload_kernel<false>, dim3(grid_for(n, 256, h.dev->num_cus * 8)), dim3(256), 0, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
     }
     h.rows_seen += n;
     h.order_count = -1;
@@ -1368,29 +1443,32 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     if (rows == 0) return VR_OK;
     VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
     VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs.data(), sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
-    // Persistent launches over runs of tiles that fit the table's headroom (batches may be split).
+    // One persistent launch over every tile; a long input takes the guarded launch.
     int launches = 0;
     const int cap = h.dev->num_cus * std::max(1, (cfg && cfg->blocks_per_cu) ? cfg->blocks_per_cu : h.cfg.blocks_per_cu);
-    for (int64_t t0 = 0; t0 < tiles;) {
-        int64_t span;
-        VR_TRY(next_span(h, t0, tiles, T, s, &span));
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+    a.batches = h.batch_descs.as<BatchDesc>();
+    a.num_batches = nb;
+    a.tile_lo = 0;
+    a.total_tiles = tiles;
+    a.epoch = ++h.epoch;
+    if (rows <= GUARD_ROWS) {
+        VR_TRY(ensure_capacity(h, rows, s));
         a.t = view_of(h);
-        a.batches = h.batch_descs.as<BatchDesc>();
-        a.num_batches = nb;
-        a.tile_lo = t0;
-        a.total_tiles = t0 + span;
-        a.epoch = ++h.epoch;
-        grid = (int)std::max<int64_t>(1, std::min<int64_t>(span, cap));
+        VR_TRY(timer_begin(h, s));
         hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+        VR_TRY(timer_end(h, s));
         VR_TRY_HIP(hipGetLastError());
         launches++;
-        if (h.has_f64_minmax) {
-            hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
-            VR_TRY_HIP(hipGetLastError());
-            launches++;
-        }
-        h.groups_upper += span * T;
-        t0 += span;
+        h.groups_upper += rows;
+    } else {
+        VR_TRY(launch_guarded<true>(h, a, grid, block, lds_bytes, s, &launches));
+    }
+    if (h.has_f64_minmax) {  // over the final table: after any growth
+        a.t = view_of(h);
+        hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
     }
     h.rows_seen += rows;
     h.order_count = -1;
@@ -1568,6 +1646,29 @@ vr_status vr_agg_reset(vr_agg* agg) {
     h.groups_upper = 0;
     h.order_count = -1;
     return leave(h, s);
+}
+
+vr_status vr_agg_set_timing(vr_agg* agg, int32_t enable) {
+    if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    agg->h.timing = enable != 0;
+    agg->h.timer_used = 0;
+    return VR_OK;
+}
+
+vr_status vr_agg_kernel_time(vr_agg* agg, double* ms, int64_t* launches) {
+    if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    double total = 0;
+    for (size_t i = 0; i < h.timer_used; i++) {
+        VR_TRY_HIP(hipEventSynchronize(h.timer_events[i].second));
+        float t = 0;
+        VR_TRY_HIP(hipEventElapsedTime(&t, h.timer_events[i].first, h.timer_events[i].second));
+        total += t;
+    }
+    if (ms) *ms = total;
+    if (launches) *launches = (int64_t)h.timer_used;
+    h.timer_used = 0;
+    return VR_OK;
 }
 
 vr_status vr_agg_destroy(vr_agg* agg) {

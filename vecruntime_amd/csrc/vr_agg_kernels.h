@@ -60,7 +60,7 @@ enum OpCode : int32_t {
 };
 
 // counters[] words
-enum { C_GROUPS = 0, C_RESERVED = 1, C_NAN = 2, C_ERROR = 3, C_OUT = 4, C_WORDS = 8 };
+enum { C_GROUPS = 0, C_RESERVED = 1, C_NAN = 2, C_ERROR = 3, C_OUT = 4, C_STOP = 5, C_WORDS = 8 };
 enum { RES_SENTINEL = 1, RES_NULL = 2 };
 enum { ERR_LONG_OVERFLOW = 1, ERR_DECIMAL_OVERFLOW = 2, ERR_AVG_OF_OVERFLOW = 4 };
 
@@ -114,8 +114,16 @@ struct UpdateArgs {
     int32_t num_batches;   // 1: `one` below; >1: `batches` (device array)
     int32_t epoch;         // update sequence number: the NaN flag holds the last epoch that saw one
     int32_t pad;
-    int64_t tile_lo;       // batched launch: tiles [tile_lo, total_tiles) of `batches`
+    int64_t tile_lo;       // tiles [tile_lo, total_tiles) of `batches` (or of `one`)
     int64_t total_tiles;
+    // Load guard (resume != nullptr): before each tile a workgroup publishes its new groups and
+    // stops once the table holds soft_limit groups, leaving in resume[blockIdx.x] the tile it would
+    // have taken next (total_tiles or beyond when it ran out of tiles) and setting C_STOP.  A resumed
+    // launch (resumed != 0, same grid) starts each workgroup at its resume entry.
+    int64_t* resume;
+    uint64_t soft_limit;
+    int32_t resumed;
+    int32_t pad2;
     uint64_t ident[64];    // identity value of every state word
     uint32_t* row_slot;    // deterministic: slot of every row of the batch (0xFFFFFFFF none)
     const BatchDesc* batches;

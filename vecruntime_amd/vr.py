@@ -46,6 +46,8 @@ _sig("vr_agg_update_filtered", [_P, _P, _P, C.c_int64, C.c_int32, C.c_int32, _P]
 _sig("vr_agg_update_batches", [_P, C.POINTER(abi.VrBatchSet), C.POINTER(abi.VrLaunchConfig), _P])
 _sig("vr_agg_num_groups", [_P, C.POINTER(C.c_int64)])
 _sig("vr_agg_set_launch_config", [_P, C.POINTER(abi.VrLaunchConfig)])
+_sig("vr_agg_set_timing", [_P, C.c_int32])
+_sig("vr_agg_kernel_time", [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)])
 _sig("vr_agg_output_layout", [_P, C.POINTER(C.c_int32), _P, C.c_int32])
 _sig("vr_agg_emit", [_P, C.c_int64, C.c_int64, _P, _P, _P])
 _sig("vr_agg_key_dictionary", [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64),
@@ -175,6 +177,16 @@ class GroupedAggregate:
         cfg = config if config is not None else abi.launch_config()
         check(_lib.vr_agg_update_batches(self.h, C.byref(bs), C.byref(cfg), _stream(stream)))
         self._keep = [ka, va, ln, batches]
+
+    def set_timing(self, on):
+        """Brackets every update-kernel launch with HIP events (a measurement aid)."""
+        check(_lib.vr_agg_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self):
+        """(device ms summed over the timed update-kernel launches, launches) since the last read."""
+        ms, n = C.c_double(), C.c_int64()
+        check(_lib.vr_agg_kernel_time(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def num_groups(self):
         g = C.c_int64()
