@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BATCH = 65536
+# HBM bytes per launch of the batched update kernel on the 200M-row group-by, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a 1 GiB device copy
+# (profiles/agg_update_groupby200M_pmc_traffic.md).  Measured for the kernel as committed with it.
+PMC_TRAFFIC_BYTES = {200_000_000: 41.34e9}
 
 
 def log(*a):
@@ -669,7 +673,7 @@ def main():
                                % n, "global_batch": n, "seq_len": BATCH, "parallelism": "single GPU",
                    "groups": gb["groups"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get(n),
                      "kernel": "agg_update_kernel<true>", "kernel_ms_per_step": gb["update_kernel_ms"],
                      "launches_per_step": gb["update_launches_per_step"],
                      "avg_launch_ms": gb["update_kernel_avg_launch_ms"], "algorithmic_bytes_per_step": alg_bytes},
