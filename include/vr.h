@@ -126,6 +126,7 @@ typedef struct vr_stats {
     int64_t kernel_launches;
     int64_t table_grows;
     int64_t cpu_fallbacks_signalled;
+    int64_t partitioned_updates;  /* aggregate calls that took the partitioned update */
 } vr_stats;
 vr_status vr_stats_get(vr_device* dev, vr_stats* out);
 
@@ -240,6 +241,9 @@ typedef struct vr_launch_config {
     int32_t lds_preaggregate;   /* 1: the LDS hot-group cache, 0: off, -1: default (on) */
     int32_t lds_entries;        /* hot-group cache entries (a power of two, <= 2048) */
     int32_t flush_tiles;        /* tiles between flushes of the cache */
+    int32_t partitioned;        /* keys of at most 64 bits: 1 takes the partitioned update (hot-set
+                                   scatter, then one merge workgroup per table region) whatever the
+                                   input size, -1 never, 0 from 4M rows per call */
 } vr_launch_config;
 
 /* Sets the handle's default launch shape (what vr_agg_update uses, and what

@@ -398,9 +398,11 @@ def test_long_batch_over_few_groups_is_split(dev, batched):
                        key_nullable=[False], capacity_hint=300)
     tk, td, ti = (torch.from_numpy(a).cuda() for a in (keys, dbl, i64))
     g = vr.GroupedAggregate(dev, spec)
+    per_tile = abi.launch_config(partitioned=-1)  # the per-tile kernel's launch splitting
+    g.set_launch_config(per_tile)
     dk, dv = [vr.device_column(abi.INT64, tk)], [vr.device_column(abi.FLOAT64, td), vr.device_column(abi.INT64, ti)]
     if batched:
-        g.update_batches([(dk, dv, n)])
+        g.update_batches([(dk, dv, n)], config=per_tile)
     else:
         g.update(dk, dv, n)
     o = OracleAgg(spec)
@@ -430,9 +432,11 @@ def test_guarded_launch_stops_grows_and_resumes(dev, batched):
     tk, td, ti = (torch.from_numpy(a).cuda() for a in (keys, dbl, i64))
     grows0 = dev.stats()["table_grows"]
     g = vr.GroupedAggregate(dev, spec)
+    per_tile = abi.launch_config(partitioned=-1)  # the guarded launch of the per-tile kernel
+    g.set_launch_config(per_tile)
     dk, dv = [vr.device_column(abi.INT64, tk)], [vr.device_column(abi.FLOAT64, td), vr.device_column(abi.INT64, ti)]
     if batched:
-        g.update_batches([(dk, dv, n)])
+        g.update_batches([(dk, dv, n)], config=per_tile)
     else:
         g.update(dk, dv, n)
     assert dev.stats()["table_grows"] - grows0 >= 2  # sized for the guard's room, then again mid-launch

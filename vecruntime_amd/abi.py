@@ -108,6 +108,7 @@ class VrStats(C.Structure):
         ("kernel_launches", C.c_int64),
         ("table_grows", C.c_int64),
         ("cpu_fallbacks_signalled", C.c_int64),
+        ("partitioned_updates", C.c_int64),
     ]
 
 
@@ -130,11 +131,13 @@ class VrLaunchConfig(C.Structure):
         ("lds_preaggregate", C.c_int32),
         ("lds_entries", C.c_int32),
         ("flush_tiles", C.c_int32),
+        ("partitioned", C.c_int32),
     ]
 
 
-def launch_config(rows_per_thread=0, blocks_per_cu=0, lds=-1, lds_entries=0, flush_tiles=0):
-    return VrLaunchConfig(0, rows_per_thread, blocks_per_cu, lds, lds_entries, flush_tiles)
+def launch_config(rows_per_thread=0, blocks_per_cu=0, lds=-1, lds_entries=0, flush_tiles=0, partitioned=0):
+    """partitioned: 1 takes the partitioned update at any size, -1 never, 0 from 4M rows per call."""
+    return VrLaunchConfig(0, rows_per_thread, blocks_per_cu, lds, lds_entries, flush_tiles, partitioned)
 
 
 class VrJoinSpec(C.Structure):

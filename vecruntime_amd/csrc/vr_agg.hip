@@ -10,7 +10,7 @@
 #include <cstring>
 #include <memory>
 
-#include "vr_agg_kernels.h"
+#include "vr_agg_part.h"
 
 namespace vr {
 
@@ -646,9 +646,10 @@ struct AggImpl {
     std::vector<EmitAgg> emits;
     std::vector<int32_t> out_types;
 
-    // table
-    DevBuf cells, state, counters;
+    // table: 2^pbits regions of slots >> pbits slots each (rgroups: groups per region)
+    DevBuf cells, state, counters, rgroups;
     uint64_t slots = 0;
+    int pbits = 0;
     double max_load = 0.5;
     int64_t groups_upper = 0;  // an upper bound of the groups (exact after a sync)
     int64_t rows_seen = 0;
@@ -670,7 +671,12 @@ struct AggImpl {
     DevBuf batch_descs;
     DevBuf resume;  // guarded launches: the tile each workgroup resumes at
 
-    vr_launch_config cfg{256, 4, 4, -1, 512, 4};
+    // partitioned update: hot-set sampling, record pool and chains, merge resume points
+    DevBuf p_cand, p_cand_count, p_hist, p_hot_keys, p_hot_part, p_hot_total;
+    DevBuf p_pool, p_next, p_cnt, p_heads, p_resume;
+    int part_min_rows = 1 << 22;  // inputs from this many rows take the partitioned update
+
+    vr_launch_config cfg{256, 4, 4, -1, 512, 4, 0};
     int lds_budget = 64 * 1024;  // LDS bytes per workgroup for the hot-key cache
 
     // the stream of the last work queued on the handle, and its completion event: work queued on
@@ -745,6 +751,7 @@ static vr_status compile_spec(AggImpl& h) {
     std::vector<Op> ops;
     int off = 0;
     std::vector<uint64_t> ident;
+    std::vector<uint8_t> comb;  // how two partial states combine, word by word
     auto add_op = [&](int32_t code, int32_t col, int nwords) -> int {
         for (size_t k = 0; k < ops.size(); k++)
             if (ops[k].code == code && ops[k].col == col) return (int)k;
@@ -753,21 +760,40 @@ static vr_status compile_spec(AggImpl& h) {
         op.col = col;
         op.off = off;
         ops.push_back(op);
-        for (int i = 0; i < nwords; i++) ident.push_back(0);
+        for (int i = 0; i < nwords; i++) {
+            ident.push_back(0);
+            comb.push_back(CB_ADD);
+        }
         switch (code) {
-            case OP_MIN_I32: ident[off] = (uint64_t)INT64_MAX; break;
-            case OP_MAX_I32: ident[off] = (uint64_t)INT64_MIN; break;
-            case OP_MIN_I64: ident[off] = ~0ull; break;
-            case OP_MIN_F64:
+            case OP_SUM_F64: comb[off] = CB_FADD; break;
+            case OP_MIN_I32:
+                ident[off] = (uint64_t)INT64_MAX;
+                comb[off] = CB_SMIN;
+                break;
+            case OP_MAX_I32:
+                ident[off] = (uint64_t)INT64_MIN;
+                comb[off] = CB_SMAX;
+                break;
+            case OP_MIN_I64:
                 ident[off] = ~0ull;
-                ident[off + 1] = ~0ull;
-                ident[off + 2] = ~0ull;
+                comb[off] = CB_UMIN;
                 break;
+            case OP_MAX_I64: comb[off] = CB_UMAX; break;
+            case OP_MIN_F64:
             case OP_MAX_F64:
+                // order code; first zero row; first NaN row; the NaN's bits (set on the final table)
+                if (code == OP_MIN_F64) ident[off] = ~0ull;
                 ident[off + 1] = ~0ull;
                 ident[off + 2] = ~0ull;
+                comb[off] = code == OP_MIN_F64 ? CB_UMIN : CB_UMAX;
+                comb[off + 1] = CB_UMIN;
+                comb[off + 2] = CB_UMIN;
+                comb[off + 3] = CB_OR;
                 break;
-            case OP_FIRST_ROW: ident[off] = ~0ull; break;
+            case OP_FIRST_ROW:
+                ident[off] = ~0ull;
+                comb[off] = CB_UMIN;
+                break;
         }
         off += nwords;
         return (int)ops.size() - 1;
@@ -867,25 +893,42 @@ static vr_status compile_spec(AggImpl& h) {
     if (min_max_i64 > 0) {
         p.flags_off = off;
         ident.push_back(0);
+        comb.push_back(CB_OR);
         off++;
     }
     if ((int)ops.size() > VR_MAX_OPS || off > 64) return fail(VR_ERR_UNSUPPORTED, "aggregate state too wide");
     for (size_t k = 0; k < ops.size(); k++) p.ops[k] = ops[k];
     p.nops = (int)ops.size();
-    for (int i = 0; i < off; i++) p.ident[i] = ident[i];
+    for (int i = 0; i < off; i++) {
+        p.ident[i] = ident[i];
+        p.comb[i] = comb[i];
+    }
     h.stride = off;
     p.det = h.det ? 1 : 0;
     p.filter_col = -1;
     return VR_OK;
 }
 
+static int log2_of(uint64_t v) {
+    int l = 0;
+    while ((1ull << l) < v) l++;
+    return l;
+}
+
+// Regions of at least 2048 slots, at most 1024 of them.
+static int choose_pbits(uint64_t slots) { return std::max(0, std::min(10, log2_of(slots) - 11)); }
+
 static TableView view_of(AggImpl& h) {
     TableView t{};
     t.cells = h.cells.as<uint64_t>();
     t.state = h.state.as<uint64_t>();
     t.counters = h.counters.as<uint64_t>();
+    t.rgroups = h.rgroups.as<uint32_t>();
     t.slots = h.slots;
     t.mask = h.slots - 1;
+    t.pbits = h.pbits;
+    t.rbits = log2_of(h.slots) - h.pbits;
+    t.rmask = (1ull << t.rbits) - 1;
     t.width = h.proto.ks.width;
     t.stride = h.stride;
     return t;
@@ -898,26 +941,41 @@ static int grid_for(uint64_t work, int block, int cap) {
     return (int)g;
 }
 
-// Allocates an empty table of `slots` slots (cells empty, state at identity) into the given buffers.
-static vr_status alloc_table(AggImpl& h, uint64_t slots, DevBuf& cells, DevBuf& state, hipStream_t s) {
+// Zeroes the groups-per-region counts of a table with 2^pbits regions.
+static vr_status clear_rgroups(AggImpl& h, DevBuf& rgroups, int pbits, hipStream_t s) {
+    VR_TRY(rgroups.ensure(h.dev, sizeof(uint32_t) << pbits));
+    const uint64_t words = ((1ull << pbits) + 1) / 2;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(words, 256, 64)), dim3(256), 0, s, rgroups.as<uint64_t>(), words, 0ull);
+    VR_TRY_HIP(hipGetLastError());
+    return VR_OK;
+}
+
+// Allocates an empty table of `slots` slots in 2^pbits regions (cells empty, state at identity,
+// region counts zero) into the given buffers.  State rows: the slots, two reserved rows, and the
+// row that takes the rows of a full region.
+static vr_status alloc_table(AggImpl& h, uint64_t slots, int pbits, DevBuf& cells, DevBuf& state, DevBuf& rgroups,
+                             hipStream_t s) {
     const int W = h.proto.ks.width;
     VR_TRY(cells.ensure(h.dev, slots * W * sizeof(uint64_t)));
-    VR_TRY(state.ensure(h.dev, (slots + 2) * h.stride * sizeof(uint64_t)));
+    VR_TRY(state.ensure(h.dev, (slots + 3) * h.stride * sizeof(uint64_t)));
     uint64_t empty = h.proto.ks.mode == KM_WIDE ? 0ull : EMPTY_KEY;
     int cap = h.dev->num_cus * 16;
     hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(slots * W, 256, cap)), dim3(256), 0, s, cells.as<uint64_t>(),
                        (uint64_t)(slots * W), empty);
     UpdateArgs a = h.proto;
     a.t.stride = h.stride;
-    hipLaunchKernelGGL(init_state_kernel, dim3(grid_for((slots + 2 + 63) / 64 * 64, 256, cap)), dim3(256), 0, s,
-                       state.as<uint64_t>(), (uint64_t)(slots + 2), a);
+    hipLaunchKernelGGL(init_state_kernel, dim3(grid_for((slots + 3 + 63) / 64 * 64, 256, cap)), dim3(256), 0, s,
+                       state.as<uint64_t>(), (uint64_t)(slots + 3), a);
     VR_TRY_HIP(hipGetLastError());
+    VR_TRY(clear_rgroups(h, rgroups, pbits, s));
     return VR_OK;
 }
 
 static vr_status read_counters(AggImpl& h, uint64_t* out, hipStream_t s) {
     VR_TRY_HIP(hipMemcpyAsync(out, h.counters.p, C_WORDS * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     VR_TRY_HIP(hipStreamSynchronize(s));
+    if (out[C_ERROR] & ERR_TABLE_FULL)
+        return fail(VR_ERR_OOM, "a hash table region is full (keys that do not spread over the table)");
     return VR_OK;
 }
 
@@ -925,26 +983,27 @@ static int64_t groups_of(const uint64_t* c) {
     return (int64_t)c[C_GROUPS] + ((c[C_RESERVED] & RES_SENTINEL) ? 1 : 0) + ((c[C_RESERVED] & RES_NULL) ? 1 : 0);
 }
 
-// Moves the groups into a new table of `want` slots.
-static vr_status grow_table(AggImpl& h, uint64_t want, hipStream_t s) {
+// Moves the groups into a new table of `want` slots in 2^pbits regions (pbits < 0: the default
+// for the size).
+static vr_status grow_table(AggImpl& h, uint64_t want, hipStream_t s, int pbits = -1) {
     if (want > (1ull << 31)) return fail(VR_ERR_UNSUPPORTED, "more than 2^30 groups run on the CPU kernels");
-    DevBuf ncells, nstate;
-    VR_TRY(alloc_table(h, want, ncells, nstate, s));
+    if (pbits < 0) pbits = choose_pbits(want);
+    DevBuf ncells, nstate, nrg;
+    VR_TRY(alloc_table(h, want, pbits, ncells, nstate, nrg, s));
     TableView from = view_of(h);
     UpdateArgs a = h.proto;
-    a.t = from;
-    a.t.cells = ncells.as<uint64_t>();
-    a.t.state = nstate.as<uint64_t>();
-    a.t.slots = want;
-    a.t.mask = want - 1;
-    hipLaunchKernelGGL(migrate_kernel, dim3(grid_for(h.slots + 2, 256, h.dev->num_cus * 16)), dim3(256), 0, s, from, a);
-    VR_TRY_HIP(hipGetLastError());
-    VR_TRY_HIP(hipStreamSynchronize(s));
     std::swap(h.cells.p, ncells.p);
     std::swap(h.cells.bytes, ncells.bytes);
     std::swap(h.state.p, nstate.p);
     std::swap(h.state.bytes, nstate.bytes);
+    std::swap(h.rgroups.p, nrg.p);
+    std::swap(h.rgroups.bytes, nrg.bytes);
     h.slots = want;
+    h.pbits = pbits;
+    a.t = view_of(h);
+    hipLaunchKernelGGL(migrate_kernel, dim3(grid_for(from.slots + 2, 256, h.dev->num_cus * 16)), dim3(256), 0, s, from, a);
+    VR_TRY_HIP(hipGetLastError());
+    VR_TRY_HIP(hipStreamSynchronize(s));
     h.dev->count(&vr_stats::table_grows, 1);
     return VR_OK;
 }
@@ -965,7 +1024,8 @@ static vr_status init_table(AggImpl& h, int64_t capacity_hint) {
     uint64_t want = next_pow2((uint64_t)((double)std::max<int64_t>(capacity_hint, 1024) / h.max_load) + 1);
     if (want < 2048) want = 2048;
     hipStream_t s = h.dev->stream;
-    VR_TRY(alloc_table(h, want, h.cells, h.state, s));
+    h.pbits = choose_pbits(want);
+    VR_TRY(alloc_table(h, want, h.pbits, h.cells, h.state, h.rgroups, s));
     VR_TRY(h.counters.ensure(h.dev, C_WORDS * sizeof(uint64_t)));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
     VR_TRY_HIP(hipGetLastError());
@@ -1180,7 +1240,236 @@ static vr_status launch_guarded(AggImpl& h, UpdateArgs& a, int grid, int block, 
     }
 }
 
+// ---- partitioned update (keys of at most 64 bits; the kernels' header describes the scheme)
+
+static int lane_bytes_of(int32_t type) {
+    switch (type) {
+        case VR_INT32: return 4;
+        case VR_INT64:
+        case VR_FLOAT64: return 8;
+        case VR_DECIMAL128: return 16;
+    }
+    return 0;
+}
+
+// The record layout of this spec: key (words 0-1), 16-byte lanes, 8-byte lanes, the row index,
+// 4-byte lanes, the validity word.  False when the partitioned update does not apply to the spec.
+static bool record_layout(const AggImpl& h, RecLayout& L) {
+    std::memset(&L, 0, sizeof(L));
+    const int nv = (int)h.value_types.size();
+    L.nvals = nv;
+    for (int c = 0; c < VR_MAX_VALUES; c++) {
+        L.val_off[c] = -1;
+        L.val_type[c] = c < nv ? h.value_types[c] : 0;
+    }
+    bool need_row = false;
+    std::vector<int> carry(nv, 0);
+    for (int k = 0; k < h.proto.nops; k++) {
+        const Op& op = h.proto.ops[k];
+        if (op.code == OP_FIRST_ROW) return false;
+        if (op.col < 0) continue;
+        L.vmask |= 1u << op.col;
+        if (op.code == OP_NULLS) continue;
+        if (op.code == OP_MIN_F64 || op.code == OP_MAX_F64) need_row = true;
+        carry[op.col] = lane_bytes_of(h.value_types[op.col]);
+    }
+    int w = 2;
+    for (int size : {16, 8})
+        for (int c = 0; c < nv; c++)
+            if (carry[c] == size) {
+                L.val_off[c] = w;
+                w += size / 4;
+            }
+    L.row_off = -1;
+    if (need_row) {
+        L.row_off = w;
+        w += 2;
+    }
+    for (int c = 0; c < nv; c++)
+        if (carry[c] == 4) L.val_off[c] = w++;
+    L.valid_off = w++;
+    L.words = (w + 3) / 4 * 4;
+    return L.words <= PART_REC_MAX;
+}
+
+// LDS cache entries of the merge (0: the state rows are too wide for it).
+static int merge_cache_entries(const AggImpl& h) {
+    const int64_t budget = h.dev->lds_per_block - 1024;
+    for (int E = 2048; E >= 2 * PB_BLOCK; E >>= 1)
+        if ((int64_t)E * (8 + 8 * h.stride) <= budget) return E;
+    return 0;
+}
+
+static bool partitioned_applies(const AggImpl& h, int64_t rows, RecLayout* L, const vr_launch_config* cfg) {
+    const int32_t mode = (cfg && cfg->partitioned) ? cfg->partitioned : h.cfg.partitioned;
+    if (mode < 0 || (mode == 0 && rows < h.part_min_rows)) return false;
+    if (h.det || h.proto.ks.mode == KM_WIDE || rows <= 0) return false;
+    if (!record_layout(h, *L)) return false;
+    return merge_cache_entries(h) > 0;
+}
+
+template <class K>
+static vr_status allow_lds(K kernel, size_t bytes) {
+    VR_TRY_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return VR_OK;
+}
+
+// Runs the partitioned update over the tiles [a.tile_lo, a.total_tiles) of the batches in `a`
+// (tiles of PA_BLOCK * a.rows_per_thread rows, columns on the device).
+static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const RecLayout& L, hipStream_t s,
+                                 int* launches) {
+    const int S = h.stride;
+    // enough regions for the merge to spread over every CU
+    const int want_pbits = std::max(0, std::min(10, log2_of((uint64_t)(rows >> 17) + 1)));
+    if (h.pbits < want_pbits) VR_TRY(grow_table(h, std::max<uint64_t>(h.slots, 2048ull << want_pbits), s, want_pbits));
+    const int P = 1 << h.pbits;
+    const int E = merge_cache_entries(h);
+    const int64_t budget = h.dev->lds_per_block - 1024;
+    int K = 0, H = 0;
+    for (int k = 2048; k >= 256; k -= 256) {
+        const int hs = (int)next_pow2(2 * (uint64_t)k);
+        if ((int64_t)scatter_lds_bytes(k, hs, S, P) <= budget) {
+            K = k;
+            H = hs;
+            break;
+        }
+    }
+    if ((int64_t)scatter_lds_bytes(K, H, S, P) > budget) return fail(VR_ERR_UNSUPPORTED, "state rows too wide for the scatter");
+    const int grid_a = h.dev->num_cus;
+    const int cap = h.dev->num_cus * 8;
+    const uint64_t empty = EMPTY_KEY;
+    a.t = view_of(h);
+    PartArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.rl = L;
+    pa.nparts = P;
+    pa.nhot = K;
+    pa.hot_slots = H;
+    pa.grid_a = grid_a;
+    pa.cache_e = E;
+
+    VR_TRY(timer_begin(h, s));
+    // the hot set, from a sample of the input
+    if (K) {
+        const uint64_t cand_n = 1ull << 17;
+        VR_TRY(h.p_cand.ensure(h.dev, cand_n * sizeof(uint64_t)));
+        VR_TRY(h.p_cand_count.ensure(h.dev, cand_n * sizeof(uint32_t)));
+        VR_TRY(h.p_hist.ensure(h.dev, 264 * sizeof(uint32_t)));
+        VR_TRY(h.p_hot_keys.ensure(h.dev, (size_t)K * sizeof(uint64_t)));
+        VR_TRY(h.p_hot_part.ensure(h.dev, (size_t)grid_a * K * S * sizeof(uint64_t)));
+        VR_TRY(h.p_hot_total.ensure(h.dev, (size_t)K * S * sizeof(uint64_t)));
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(cand_n, 256, cap)), dim3(256), 0, s, h.p_cand.as<uint64_t>(), cand_n, empty);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(cand_n / 2, 256, cap)), dim3(256), 0, s, h.p_cand_count.as<uint64_t>(),
+                           cand_n / 2, 0ull);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, s, h.p_hist.as<uint64_t>(), (uint64_t)132, 0ull);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(K, 256, cap)), dim3(256), 0, s, h.p_hot_keys.as<uint64_t>(),
+                           (uint64_t)K, empty);
+        const int64_t tiles = a.total_tiles - a.tile_lo;
+        const int64_t nsamp = std::min<int64_t>(512, tiles);
+        const int per = std::min(1024, PA_BLOCK * a.rows_per_thread);
+        hipLaunchKernelGGL(hot_sample_kernel, dim3((unsigned)nsamp), dim3(256), 0, s, a, per, nsamp, h.p_cand.as<uint64_t>(),
+                           h.p_cand_count.as<uint32_t>(), cand_n - 1);
+        hipLaunchKernelGGL(hot_hist_kernel, dim3(64), dim3(256), 0, s, h.p_cand_count.as<uint32_t>(), cand_n,
+                           h.p_hist.as<uint32_t>());
+        hipLaunchKernelGGL(hot_pick_kernel, dim3(64), dim3(256), 0, s, h.p_cand.as<uint64_t>(), h.p_cand_count.as<uint32_t>(),
+                           cand_n, h.p_hist.as<uint32_t>(), K, h.p_hot_keys.as<uint64_t>(), h.p_hist.as<uint32_t>() + 256);
+        VR_TRY_HIP(hipGetLastError());
+        *launches += 7;
+        pa.hot_keys = h.p_hot_keys.as<uint64_t>();
+        pa.hot_part = h.p_hot_part.as<uint64_t>();
+        pa.hot_total = h.p_hot_total.as<uint64_t>();
+    }
+    // the record pool: every active row could be a record, plus one open block per chain
+    const uint64_t max_blocks = (uint64_t)(rows + PART_BLK - 1) / PART_BLK + (uint64_t)grid_a * P + 1;
+    if (max_blocks >= (uint64_t)NO_BLOCK) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
+    VR_TRY(h.p_pool.ensure(h.dev, max_blocks * PART_BLK * L.words * sizeof(uint32_t)));
+    VR_TRY(h.p_next.ensure(h.dev, max_blocks * sizeof(uint32_t)));
+    VR_TRY(h.p_cnt.ensure(h.dev, max_blocks * sizeof(uint32_t)));
+    VR_TRY(h.p_heads.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
+    VR_TRY(h.p_resume.ensure(h.dev, (size_t)P * sizeof(int32_t)));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>() + C_POOL, 2ull, 0ull);
+    pa.pool = h.p_pool.as<uint32_t>();
+    pa.block_next = h.p_next.as<uint32_t>();
+    pa.block_cnt = h.p_cnt.as<uint32_t>();
+    pa.heads = h.p_heads.as<uint32_t>();
+    pa.resume_step = h.p_resume.as<int32_t>();
+    pa.max_blocks = max_blocks;
+    pa.u = a;
+    const size_t lds_a = scatter_lds_bytes(K, H, S, P);
+    const size_t lds_b = (size_t)E * (8 + 8 * S);
+    VR_TRY(allow_lds(part_scatter_kernel, lds_a));
+    VR_TRY(allow_lds(part_merge_kernel, lds_b));
+    hipLaunchKernelGGL(part_scatter_kernel, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+    if (K) hipLaunchKernelGGL(hot_reduce_kernel, dim3(grid_for((uint64_t)K * S, 256, cap)), dim3(256), 0, s, pa);
+    VR_TRY_HIP(hipGetLastError());
+    *launches += 3;
+    // the merge: one workgroup per region; stopped regions resume after the table grows
+    pa.resumed = 0;
+    for (int round = 0;; round++) {
+        pa.u.t = view_of(h);
+        pa.region_limit = (uint32_t)((pa.u.t.rmask + 1) * 3 / 4);
+        if (round) VR_TRY(timer_begin(h, s));
+        hipLaunchKernelGGL(part_merge_kernel, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+        VR_TRY(timer_end(h, s));
+        VR_TRY_HIP(hipGetLastError());
+        (*launches)++;
+        uint64_t c[C_WORDS];
+        VR_TRY(read_counters(h, c, s));
+        h.groups_upper = groups_of(c);
+        if (!c[C_PSTOP]) break;
+        if (round >= 40) return fail(VR_ERR_DEVICE, "the partitioned merge did not finish");
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>() + C_PSTOP, 1ull, 0ull);
+        VR_TRY(grow_table(h, h.slots * 2, s, h.pbits));
+        pa.resumed = 1;
+    }
+    return VR_OK;
+}
+
+// The partitioned update of nb device-resident batches (host descriptors: columns, selection, n).
+static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_t rows, int32_t filter_col,
+                                    int32_t filter_threshold, const RecLayout& L, const vr_launch_config* cfg,
+                                    hipStream_t s) {
+    UpdateArgs a = h.proto;
+    a.filter_col = filter_col;
+    a.filter_threshold = filter_threshold;
+    int rpt = (cfg && cfg->rows_per_thread) ? cfg->rows_per_thread : h.cfg.rows_per_thread;
+    a.rows_per_thread = std::max(1, std::min(PA_MAX_RPT, rpt));
+    const int64_t T = (int64_t)PA_BLOCK * a.rows_per_thread;
+    int64_t tiles = 0, seen = 0;
+    for (int b = 0; b < nb; b++) {
+        descs[b].row_base = h.rows_seen + seen;
+        descs[b].tile_begin = tiles;
+        tiles += (descs[b].n + T - 1) / T;
+        seen += descs[b].n;
+    }
+    VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
+    VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs, sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
+    a.batches = h.batch_descs.as<BatchDesc>();
+    a.num_batches = nb;
+    a.one = descs[0];
+    a.tile_lo = 0;
+    a.total_tiles = tiles;
+    a.epoch = ++h.epoch;
+    int launches = 0;
+    VR_TRY(run_partitioned(h, a, rows, L, s, &launches));
+    if (h.has_f64_minmax) {  // over the final table
+        a.t = view_of(h);
+        hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid_for(tiles, 1, h.dev->num_cus * 4)), dim3(PA_BLOCK), 0, s, a);
+        VR_TRY_HIP(hipGetLastError());
+        launches++;
+    }
+    VR_TRY_HIP(hipStreamSynchronize(s));  // the descriptors are rewritten by the next call
+    h.dev->count(&vr_stats::partitioned_updates, 1);
+    h.rows_seen += rows;
+    h.order_count = -1;
+    h.dev->count(&vr_stats::kernel_launches, launches);
+    return VR_OK;
+}
+
 static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, const vr_launch_config* cfg) {
+    RecLayout layout;
+    if (partitioned_applies(h, n, &layout, cfg))
+        return update_partitioned(h, &b, 1, n, h.proto.filter_col, h.proto.filter_threshold, layout, cfg, s);
     UpdateArgs a = h.proto;
     b.row_base = h.rows_seen;
     b.tile_begin = 0;
@@ -1414,6 +1703,28 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
                                           bs->lengths[b], bs->filter_col, bs->filter_threshold, stream));
         return VR_OK;
     }
+    int64_t total = 0;
+    for (int b = 0; b < nb; b++) {
+        if (bs->lengths[b] < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative batch length");
+        total += bs->lengths[b];
+    }
+    RecLayout layout;
+    if (partitioned_applies(h, total, &layout, cfg)) {
+        VR_TRY(enter(h, s));
+        static thread_local std::vector<BatchDesc> pdescs;
+        pdescs.resize(nb);
+        std::vector<std::vector<int32_t>> ids;
+        for (int b = 0; b < nb; b++) {
+            StagePlan plan;
+            int32_t mem;
+            VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, bs->lengths[b], nullptr, pdescs[b],
+                            plan, ids, &mem));
+        }
+        VR_TRY(update_partitioned(h, pdescs.data(), nb, total, bs->filter_col, bs->filter_threshold, layout, cfg, s));
+        h.dev->count(&vr_stats::calls, 1);
+        h.dev->count(&vr_stats::input_rows, total);
+        return leave(h, s);
+    }
     UpdateArgs a = h.proto;
     a.filter_col = bs->filter_col;
     a.filter_threshold = bs->filter_threshold;
@@ -1482,13 +1793,15 @@ vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg) {
     if (!agg || !cfg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     vr_launch_config& c = agg->h.cfg;
     if (cfg->rows_per_thread < 0 || cfg->rows_per_thread > 64 || cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 32 ||
-        cfg->lds_entries < 0 || cfg->lds_entries > 2048 || cfg->flush_tiles < 0)
+        cfg->lds_entries < 0 || cfg->lds_entries > 2048 || cfg->flush_tiles < 0 || cfg->partitioned < -1 ||
+        cfg->partitioned > 1)
         return fail(VR_ERR_INVALID_ARGUMENT, "launch config out of range");
     if (cfg->rows_per_thread) c.rows_per_thread = cfg->rows_per_thread;
     if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
     if (cfg->lds_preaggregate >= 0) c.lds_preaggregate = cfg->lds_preaggregate;
     if (cfg->lds_entries) c.lds_entries = cfg->lds_entries;
     if (cfg->flush_tiles) c.flush_tiles = cfg->flush_tiles;
+    if (cfg->partitioned) c.partitioned = cfg->partitioned;
     return VR_OK;
 }
 
@@ -1619,7 +1932,10 @@ vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
     *bytes = (int64_t)(h.cells.bytes + h.state.bytes + h.counters.bytes + h.order.bytes + h.order_tmp.bytes +
                        h.cub_tmp.bytes + h.out_stage.bytes + h.row_slot.bytes + h.sort_keys.bytes +
                        h.sort_vals.bytes + h.sort_vals_in.bytes + h.firsts.bytes + h.firsts_sorted.bytes +
-                       h.batch_descs.bytes);
+                       h.batch_descs.bytes + h.rgroups.bytes + h.resume.bytes + h.p_cand.bytes +
+                       h.p_cand_count.bytes + h.p_hist.bytes + h.p_hot_keys.bytes + h.p_hot_part.bytes +
+                       h.p_hot_total.bytes + h.p_pool.bytes + h.p_next.bytes + h.p_cnt.bytes + h.p_heads.bytes +
+                       h.p_resume.bytes);
     return VR_OK;
 }
 
@@ -1638,8 +1954,9 @@ vr_status vr_agg_reset(vr_agg* agg) {
                                h.order.as<uint64_t>(), h.order_count, h.cells.as<uint64_t>(), h.proto.ks.width, h.slots,
                                empty, a);
         VR_TRY_HIP(hipGetLastError());
+        VR_TRY(clear_rgroups(h, h.rgroups, h.pbits, s));
     } else {
-        VR_TRY(alloc_table(h, h.slots, h.cells, h.state, s));
+        VR_TRY(alloc_table(h, h.slots, h.pbits, h.cells, h.state, h.rgroups, s));
     }
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
     VR_TRY_HIP(hipStreamSynchronize(s));

@@ -60,9 +60,12 @@ enum OpCode : int32_t {
 };
 
 // counters[] words
-enum { C_GROUPS = 0, C_RESERVED = 1, C_NAN = 2, C_ERROR = 3, C_OUT = 4, C_STOP = 5, C_WORDS = 8 };
+enum { C_GROUPS = 0, C_RESERVED = 1, C_NAN = 2, C_ERROR = 3, C_OUT = 4, C_STOP = 5, C_POOL = 6, C_PSTOP = 7, C_WORDS = 8 };
 enum { RES_SENTINEL = 1, RES_NULL = 2 };
-enum { ERR_LONG_OVERFLOW = 1, ERR_DECIMAL_OVERFLOW = 2, ERR_AVG_OF_OVERFLOW = 4 };
+enum { ERR_LONG_OVERFLOW = 1, ERR_DECIMAL_OVERFLOW = 2, ERR_AVG_OF_OVERFLOW = 4, ERR_TABLE_FULL = 8 };
+
+// How two partial states of a group combine, word by word.
+enum CombCode : uint8_t { CB_ADD = 0, CB_FADD = 1, CB_SMIN = 2, CB_SMAX = 3, CB_UMIN = 4, CB_UMAX = 5, CB_OR = 6 };
 
 struct Op {
     int32_t code;
@@ -80,15 +83,31 @@ struct KeySpec {
     int32_t word_off[VR_MAX_KEYS];
 };
 
+// The slots are cut into 2^pbits regions of 2^rbits slots.  A key's region is the top pbits bits
+// of its 64-bit hash and its home slot in the region the low rbits bits; probing wraps inside the
+// region, so one workgroup can own a region (the partitioned update) while every other kernel sees
+// an ordinary open-addressing table.
 struct TableView {
     uint64_t* cells;
     uint64_t* state;
     uint64_t* counters;
-    uint64_t mask;   // slots - 1
-    uint64_t slots;  // reserved rows at [slots, slots + 2)
-    int32_t width;   // cells per slot (1 for the 64-bit modes)
-    int32_t stride;  // state words per row
+    uint32_t* rgroups;  // groups per region (nullptr: not counted by this view's inserts)
+    uint64_t mask;      // slots - 1
+    uint64_t slots;     // reserved rows at [slots, slots + 2); row slots + 2 takes rows of a full region
+    uint64_t rmask;     // region slots - 1
+    int32_t pbits;
+    int32_t rbits;
+    int32_t width;      // cells per slot (1 for the 64-bit modes)
+    int32_t stride;     // state words per row
 };
+
+__device__ __forceinline__ uint64_t region_of_hash(int pbits, uint64_t h) { return pbits ? (h >> (64 - pbits)) : 0; }
+__device__ __forceinline__ uint64_t home_slot(const TableView& t, uint64_t h) {
+    return (region_of_hash(t.pbits, h) << t.rbits) | (h & t.rmask);
+}
+__device__ __forceinline__ uint64_t next_slot(const TableView& t, uint64_t pos) {
+    return (pos & ~t.rmask) | ((pos + 1) & t.rmask);
+}
 
 struct BatchDesc {
     DCol keys[VR_MAX_KEYS];
@@ -125,6 +144,7 @@ struct UpdateArgs {
     int32_t resumed;
     int32_t pad2;
     uint64_t ident[64];    // identity value of every state word
+    uint8_t comb[64];      // CombCode of every state word
     uint32_t* row_slot;    // deterministic: slot of every row of the batch (0xFFFFFFFF none)
     const BatchDesc* batches;
     BatchDesc one;
@@ -158,6 +178,18 @@ __device__ __forceinline__ void mark_reserved(uint64_t* counters, uint64_t bit) 
     if ((counters[C_RESERVED] & bit) == 0) atomicOr((unsigned long long*)&counters[C_RESERVED], (unsigned long long)bit);
 }
 
+// A region with no empty slot left: the row goes to the spare row slots + 2 and the update fails
+// with ERR_TABLE_FULL (the load limits keep this from happening to keys that hash evenly).
+__device__ __forceinline__ uint64_t region_full(const TableView& t) {
+    if ((t.counters[C_ERROR] & ERR_TABLE_FULL) == 0)
+        atomicOr((unsigned long long*)&t.counters[C_ERROR], (unsigned long long)ERR_TABLE_FULL);
+    return t.slots + 2;
+}
+
+__device__ __forceinline__ void count_insert(const TableView& t, uint64_t slot) {
+    if (t.rgroups) atomicAdd(&t.rgroups[slot >> t.rbits], 1u);
+}
+
 // Slot of a 64-bit key, claiming an empty cell for a new key; `inserted` reports a claim.
 __device__ __forceinline__ uint64_t find_or_insert64(const TableView& t, uint64_t key, bool* inserted) {
     *inserted = false;
@@ -165,26 +197,28 @@ __device__ __forceinline__ uint64_t find_or_insert64(const TableView& t, uint64_
         mark_reserved(t.counters, RES_SENTINEL);
         return t.slots;
     }
-    uint64_t pos = fmix64(key) & t.mask;
-    while (true) {
+    uint64_t pos = home_slot(t, fmix64(key));
+    for (uint64_t probes = 0; probes <= t.rmask; probes++) {
         uint64_t c = t.cells[pos];
         if (c == key) return pos;
         if (c == EMPTY_KEY) {
             uint64_t old = atomic_cas_u64(&t.cells[pos], EMPTY_KEY, key);
             if (old == EMPTY_KEY) {
                 *inserted = true;
+                count_insert(t, pos);
                 return pos;
             }
             if (old == key) return pos;
         }
-        pos = (pos + 1) & t.mask;
+        pos = next_slot(t, pos);
     }
+    return region_full(t);
 }
 
 __device__ __forceinline__ uint64_t find64(const TableView& t, uint64_t key) {
     if (key == EMPTY_KEY) return t.slots;
-    uint64_t pos = fmix64(key) & t.mask;
-    while (true) {
+    uint64_t pos = home_slot(t, fmix64(key));
+    for (uint64_t probes = 0; probes <= t.rmask; probes++) {
         uint64_t c = t.cells[pos];
         if (c == key) return pos;
         if (c == EMPTY_KEY) {
@@ -192,8 +226,9 @@ __device__ __forceinline__ uint64_t find64(const TableView& t, uint64_t key) {
             if (c == key) return pos;
             if (c == EMPTY_KEY) return ~0ull;
         }
-        pos = (pos + 1) & t.mask;
+        pos = next_slot(t, pos);
     }
+    return ~0ull;
 }
 
 __device__ __forceinline__ uint64_t hash_words(const uint32_t* w, int n) {
@@ -209,9 +244,11 @@ __device__ __forceinline__ uint64_t find_or_insert_wide(const TableView& t, cons
     *inserted = false;
     uint64_t h = hash_words(w, W);
     uint64_t tag = (1ull << 63) | (((h >> 32) & 0x7FFFFFFFull) << 32) | (uint64_t)w[0];
-    uint64_t pos = h & t.mask;
+    uint64_t pos = home_slot(t, h);
     bool coherent = false;
+    uint64_t probes = 0;
     while (true) {
+        if (probes > t.rmask) return insert ? region_full(t) : ~0ull;
         uint64_t* s = t.cells + pos * (uint64_t)W;
         uint64_t c = coherent ? ld_coherent(s) : s[0];
         if (c == 0) {
@@ -223,6 +260,7 @@ __device__ __forceinline__ uint64_t find_or_insert_wide(const TableView& t, cons
                 if (old == 0) {
                     for (int i = 1; i < W; i++) st_coherent(s + i, (1ull << 32) | (uint64_t)w[i]);
                     *inserted = true;
+                    count_insert(t, pos);
                     return pos;
                 }
                 c = old;
@@ -249,7 +287,8 @@ __device__ __forceinline__ uint64_t find_or_insert_wide(const TableView& t, cons
             if (same) return pos;
         }
         coherent = false;
-        pos = (pos + 1) & t.mask;
+        pos = next_slot(t, pos);
+        probes++;
     }
 }
 
@@ -351,29 +390,44 @@ __device__ __forceinline__ void add_if(uint64_t* w, uint64_t v) {
     if (v) atomic_add_u64(w, v);
 }
 
-// Folds row i of batch b into the state words at `row` (an LDS entry or a global row).
-__device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* row) {
+// A row of an input batch, read from its columns.
+struct ColSrc {
+    const BatchDesc& b;
+    int64_t i;
+    __device__ __forceinline__ bool valid(int c) const { return is_valid(b.vals[c], i); }
+    __device__ __forceinline__ int64_t lng(int c) const { return read_long(b.vals[c], i); }
+    __device__ __forceinline__ int32_t i32(int c) const { return ld_stream((const int32_t*)b.vals[c].data + i); }
+    __device__ __forceinline__ uint64_t u64(int c) const { return (uint64_t)ld_stream((const int64_t*)b.vals[c].data + i); }
+    __device__ __forceinline__ void d128(int c, uint64_t* lo, uint64_t* hi) const {
+        const uint64_t* p = (const uint64_t*)b.vals[c].data + 2 * i;
+        *lo = ld_stream(p);
+        *hi = ld_stream(p + 1);
+    }
+    __device__ __forceinline__ uint64_t row() const { return (uint64_t)(b.row_base + i); }
+};
+
+// Folds one row into the state words at `row` (an LDS entry or a global row) with atomics.
+template <class Src>
+__device__ __forceinline__ void apply_src(const UpdateArgs& a, const Src& src, uint64_t* row) {
     for (int k = 0; k < a.nops; k++) {
         const Op& op = a.ops[k];
         uint64_t* w = row + op.off;
         switch (op.code) {
             case OP_ROWS: atomic_add_u64(w, 1); break;
             case OP_NULLS:
-                if (!is_valid(b.vals[op.col], i)) atomic_add_u64(w, 1);
+                if (!src.valid(op.col)) atomic_add_u64(w, 1);
                 break;
             case OP_SUM_I64: {
-                const DCol& c = b.vals[op.col];
-                if (!is_valid(c, i)) break;
-                int64_t x = read_long(c, i);
+                if (!src.valid(op.col)) break;
+                int64_t x = src.lng(op.col);
                 add_if(w, (uint64_t)(uint32_t)x);
                 add_if(w + 1, (uint64_t)(x >> 32));  // the signed high half
                 break;
             }
             case OP_SUM_D128: {
-                const DCol& c = b.vals[op.col];
-                if (!is_valid(c, i)) break;
-                const uint64_t* p = (const uint64_t*)c.data + 2 * i;
-                uint64_t lo = ld_stream(p), hi = ld_stream(p + 1);
+                if (!src.valid(op.col)) break;
+                uint64_t lo, hi;
+                src.d128(op.col, &lo, &hi);
                 add_if(w, lo & 0xFFFFFFFFull);
                 add_if(w + 1, lo >> 32);
                 add_if(w + 2, hi & 0xFFFFFFFFull);
@@ -381,16 +435,14 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
                 break;
             }
             case OP_SUM_F64: {
-                const DCol& c = b.vals[op.col];
-                if (a.det || !is_valid(c, i)) break;
-                atomic_add_f64(w, ld_stream((const double*)c.data + i));
+                if (a.det || !src.valid(op.col)) break;
+                atomic_add_f64(w, __longlong_as_double((long long)src.u64(op.col)));
                 break;
             }
             case OP_MIN_I32:
             case OP_MAX_I32: {
-                const DCol& c = b.vals[op.col];
-                if (!is_valid(c, i)) break;
-                int64_t x = ld_stream((const int32_t*)c.data + i);
+                if (!src.valid(op.col)) break;
+                int64_t x = src.i32(op.col);
                 if (op.code == OP_MIN_I32) {
                     if (x < (int64_t)*w) atomic_min_i64(w, x);
                 } else if (x > (int64_t)*w) {
@@ -400,9 +452,8 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
             }
             case OP_MIN_I64:
             case OP_MAX_I64: {
-                const DCol& c = b.vals[op.col];
-                if (!is_valid(c, i)) break;
-                uint64_t o = (uint64_t)ld_stream((const int64_t*)c.data + i) ^ 0x8000000000000000ull;
+                if (!src.valid(op.col)) break;
+                uint64_t o = src.u64(op.col) ^ 0x8000000000000000ull;
                 uint64_t* f = row + a.flags_off;
                 uint64_t bit = 1ull << op.bit;
                 bool has = (*f & bit) != 0;
@@ -416,9 +467,8 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
             }
             case OP_MIN_F64:
             case OP_MAX_F64: {
-                const DCol& c = b.vals[op.col];
-                if (!is_valid(c, i)) break;
-                uint64_t bits = (uint64_t)ld_stream((const int64_t*)c.data + i);
+                if (!src.valid(op.col)) break;
+                uint64_t bits = src.u64(op.col);
                 uint64_t o = order_f64(bits);
                 if (op.code == OP_MIN_F64) {
                     if (o < *w) atomic_min_u64(w, o);
@@ -428,7 +478,7 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
                 // Ties between +0.0 / -0.0 and between NaN payloads keep the first-seen value
                 // (DoubleMinMax.offer replaces only on a strict improvement): record the first
                 // zero row with its sign and the first NaN row.
-                uint64_t gi = (uint64_t)(b.row_base + i);
+                uint64_t gi = src.row();
                 if (is_zero_bits(bits)) {
                     atomic_min_u64(w + 1, (gi << 1) | (bits >> 63));
                 } else if (is_nan_bits(bits)) {
@@ -439,11 +489,35 @@ __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& 
                 break;
             }
             case OP_FIRST_ROW: {
-                uint64_t gi = (uint64_t)(b.row_base + i);
+                uint64_t gi = src.row();
                 if (*w > gi) atomic_min_u64(w, gi);
                 break;
             }
         }
+    }
+}
+
+// Folds row i of batch b into the state words at `row` (an LDS entry or a global row).
+__device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* row) {
+    apply_src(a, ColSrc{b, i}, row);
+}
+
+// Combines a partial state into a row that no other thread updates meanwhile (plain stores).
+__device__ __forceinline__ void merge_plain(const UpdateArgs& a, const uint64_t* l, uint64_t* g, int S) {
+    for (int k = 0; k < S; k++) {
+        uint64_t x = l[k];
+        if (x == a.ident[k]) continue;
+        uint64_t y = g[k], r;
+        switch (a.comb[k]) {
+            case CB_ADD: r = y + x; break;
+            case CB_FADD: r = (uint64_t)__double_as_longlong(__longlong_as_double((long long)y) + __longlong_as_double((long long)x)); break;
+            case CB_SMIN: r = (int64_t)x < (int64_t)y ? x : y; break;
+            case CB_SMAX: r = (int64_t)x > (int64_t)y ? x : y; break;
+            case CB_UMIN: r = x < y ? x : y; break;
+            case CB_UMAX: r = x > y ? x : y; break;
+            default: r = x | y; break;
+        }
+        g[k] = r;
     }
 }
 

@@ -42,6 +42,7 @@ vr_status hip_fail(hipError_t e, const char* what);
 struct Device {
     int ordinal = 0;
     int num_cus = 256;
+    int64_t lds_per_block = 64 * 1024;  // LDS bytes a workgroup may use
     hipStream_t stream = nullptr;       // the handle's own compute stream
     hipStream_t copy_stream = nullptr;  // side stream for host <-> device staging
     std::mutex mu;

@@ -233,7 +233,10 @@ vr_status vr_device_open(int32_t ordinal, int64_t hbm_budget_bytes, vr_device** 
     h->d.ordinal = ordinal;
     h->d.budget = hbm_budget_bytes;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ordinal) == hipSuccess) h->d.num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, ordinal) == hipSuccess) {
+        h->d.num_cus = prop.multiProcessorCount;
+        h->d.lds_per_block = (int64_t)prop.sharedMemPerBlock;
+    }
     hipError_t e = hipStreamCreateWithFlags(&h->d.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->d.copy_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
