@@ -76,7 +76,7 @@ __device__ __forceinline__ int find_batch(const BatchDesc* b, int nb, int64_t ti
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (b[mid].tile_begin <= tile) lo = mid;
+        if (batch_at(b, mid).tile_begin <= tile) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void agg_update_kernel(UpdateArgs a) {
         const BatchDesc* bp;
         int64_t row0;
         if (MULTI) {
-            bp = a.batches + find_batch(a.batches, a.num_batches, tile);
+            bp = &batch_at(a.batches, find_batch(a.batches, a.num_batches, tile));
             row0 = (tile - bp->tile_begin) * T;
         } else {
             bp = &a.one;
@@ -311,7 +311,7 @@ __global__ void nan_payload_kernel(UpdateArgs a) {
     }
     const int64_t T = (int64_t)blockDim.x * a.rows_per_thread;
     for (int64_t tile = a.tile_lo + blockIdx.x; tile < a.total_tiles; tile += gridDim.x) {
-        const BatchDesc& b = a.batches[find_batch(a.batches, a.num_batches, tile)];
+        const BatchDesc& b = batch_at(a.batches, find_batch(a.batches, a.num_batches, tile));
         int64_t row0 = (tile - b.tile_begin) * T;
         int64_t row1 = row0 + T < b.n ? row0 + T : b.n;
         nan_payload_rows(a, b, row0 + threadIdx.x, row1, blockDim.x);
@@ -1289,7 +1289,48 @@ static bool record_layout(const AggImpl& h, RecLayout& L) {
         if (carry[c] == 4) L.val_off[c] = w++;
     L.valid_off = w++;
     L.words = (w + 3) / 4 * 4;
-    return L.words <= PART_REC_MAX;
+    return L.words <= PART_REC_WORDS;
+}
+
+static_assert(sizeof(PartArgs) <= 4096, "kernel arguments are at most 4 KiB");
+
+// The aggregates of each carried or counted value column as a program (ColProg).
+static int column_programs(const AggImpl& h, const RecLayout& L, ColProg* prog) {
+    int n = 0;
+    for (int c = 0; c < L.nvals; c++) {
+        if (!((L.vmask >> c) & 1u)) continue;
+        ColProg cp;
+        std::memset(&cp, 0, sizeof(cp));
+        cp.col = (int16_t)c;
+        cp.type = (int16_t)h.value_types[c];
+        cp.rec_off = (int16_t)L.val_off[c];
+        cp.nulls_off = cp.sum_off = cp.min_off = cp.max_off = -1;
+        for (int k = 0; k < h.proto.nops; k++) {
+            const Op& op = h.proto.ops[k];
+            if (op.col != c) continue;
+            switch (op.code) {
+                case OP_NULLS: cp.flags |= CP_NULLS; cp.nulls_off = (int16_t)op.off; break;
+                case OP_SUM_I64: cp.flags |= CP_SUM_I64; cp.sum_off = (int16_t)op.off; break;
+                case OP_SUM_D128: cp.flags |= CP_SUM_D128; cp.sum_off = (int16_t)op.off; break;
+                case OP_SUM_F64: cp.flags |= CP_SUM_F64; cp.sum_off = (int16_t)op.off; break;
+                case OP_MIN_I32: cp.flags |= CP_MIN_I32; cp.min_off = (int16_t)op.off; break;
+                case OP_MAX_I32: cp.flags |= CP_MAX_I32; cp.max_off = (int16_t)op.off; break;
+                case OP_MIN_I64:
+                    cp.flags |= CP_MIN_I64;
+                    cp.min_off = (int16_t)op.off;
+                    cp.min_bit = (int16_t)op.bit;
+                    break;
+                case OP_MAX_I64:
+                    cp.flags |= CP_MAX_I64;
+                    cp.max_off = (int16_t)op.off;
+                    cp.max_bit = (int16_t)op.bit;
+                    break;
+                default: cp.flags |= CP_GENERIC; break;  // double MIN/MAX: first-seen zero and NaN rows
+            }
+        }
+        prog[n++] = cp;
+    }
+    return n;
 }
 
 // LDS cache entries of the merge (0: the state rows are too wide for it).
@@ -1342,6 +1383,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     PartArgs pa;
     std::memset(&pa, 0, sizeof(pa));
     pa.rl = L;
+    pa.nprog = column_programs(h, L, pa.prog);
     pa.nparts = P;
     pa.nhot = K;
     pa.hot_slots = H;
@@ -1351,7 +1393,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     VR_TRY(timer_begin(h, s));
     // the hot set, from a sample of the input
     if (K) {
-        const uint64_t cand_n = 1ull << 17;
+        const uint64_t cand_n = 1ull << 20;
         VR_TRY(h.p_cand.ensure(h.dev, cand_n * sizeof(uint64_t)));
         VR_TRY(h.p_cand_count.ensure(h.dev, cand_n * sizeof(uint32_t)));
         VR_TRY(h.p_hist.ensure(h.dev, 264 * sizeof(uint32_t)));
@@ -1366,8 +1408,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
                            (uint64_t)K, empty);
         const int64_t tiles = a.total_tiles - a.tile_lo;
         const int64_t nsamp = std::min<int64_t>(512, tiles);
-        const int per = std::min(1024, PA_BLOCK * a.rows_per_thread);
-        hipLaunchKernelGGL(hot_sample_kernel, dim3((unsigned)nsamp), dim3(256), 0, s, a, per, nsamp, h.p_cand.as<uint64_t>(),
+        hipLaunchKernelGGL(hot_sample_kernel, dim3((unsigned)nsamp), dim3(HS_BLOCK), 0, s, a, nsamp, h.p_cand.as<uint64_t>(),
                            h.p_cand_count.as<uint32_t>(), cand_n - 1);
         hipLaunchKernelGGL(hot_hist_kernel, dim3(64), dim3(256), 0, s, h.p_cand_count.as<uint32_t>(), cand_n,
                            h.p_hist.as<uint32_t>());
@@ -1379,8 +1420,10 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
         pa.hot_part = h.p_hot_part.as<uint64_t>();
         pa.hot_total = h.p_hot_total.as<uint64_t>();
     }
-    // the record pool: every active row could be a record, plus one open block per chain
-    const uint64_t max_blocks = (uint64_t)(rows + PART_BLK - 1) / PART_BLK + (uint64_t)grid_a * P + 1;
+    // the record pool: every active row could be a record, plus one open block per chain and the
+    // unused rest of each workgroup's last share of the pool
+    const uint64_t max_blocks =
+        (uint64_t)(rows + PART_BLK - 1) / PART_BLK + (uint64_t)grid_a * P + (uint64_t)grid_a * PA_CHUNK + 1;
     if (max_blocks >= (uint64_t)NO_BLOCK) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
     VR_TRY(h.p_pool.ensure(h.dev, max_blocks * PART_BLK * L.words * sizeof(uint32_t)));
     VR_TRY(h.p_next.ensure(h.dev, max_blocks * sizeof(uint32_t)));
@@ -1397,9 +1440,24 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.u = a;
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
     const size_t lds_b = (size_t)E * (8 + 8 * S);
-    VR_TRY(allow_lds(part_scatter_kernel, lds_a));
-    VR_TRY(allow_lds(part_merge_kernel, lds_b));
-    hipLaunchKernelGGL(part_scatter_kernel, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+    switch (L.words) {
+        case 4:
+            VR_TRY(allow_lds(part_scatter_kernel<4>, lds_a));
+            hipLaunchKernelGGL(part_scatter_kernel<4>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+            break;
+        case 8:
+            VR_TRY(allow_lds(part_scatter_kernel<8>, lds_a));
+            hipLaunchKernelGGL(part_scatter_kernel<8>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+            break;
+        case 12:
+            VR_TRY(allow_lds(part_scatter_kernel<12>, lds_a));
+            hipLaunchKernelGGL(part_scatter_kernel<12>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+            break;
+        default:
+            VR_TRY(allow_lds(part_scatter_kernel<16>, lds_a));
+            hipLaunchKernelGGL(part_scatter_kernel<16>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+            break;
+    }
     if (K) hipLaunchKernelGGL(hot_reduce_kernel, dim3(grid_for((uint64_t)K * S, 256, cap)), dim3(256), 0, s, pa);
     VR_TRY_HIP(hipGetLastError());
     *launches += 3;
@@ -1409,7 +1467,24 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
         pa.u.t = view_of(h);
         pa.region_limit = (uint32_t)((pa.u.t.rmask + 1) * 3 / 4);
         if (round) VR_TRY(timer_begin(h, s));
-        hipLaunchKernelGGL(part_merge_kernel, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+        switch (L.words) {
+            case 4:
+                VR_TRY(allow_lds(part_merge_kernel<4>, lds_b));
+                hipLaunchKernelGGL(part_merge_kernel<4>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                break;
+            case 8:
+                VR_TRY(allow_lds(part_merge_kernel<8>, lds_b));
+                hipLaunchKernelGGL(part_merge_kernel<8>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                break;
+            case 12:
+                VR_TRY(allow_lds(part_merge_kernel<12>, lds_b));
+                hipLaunchKernelGGL(part_merge_kernel<12>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                break;
+            default:
+                VR_TRY(allow_lds(part_merge_kernel<16>, lds_b));
+                hipLaunchKernelGGL(part_merge_kernel<16>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                break;
+        }
         VR_TRY(timer_end(h, s));
         VR_TRY_HIP(hipGetLastError());
         (*launches)++;

@@ -150,6 +150,23 @@ struct UpdateArgs {
     BatchDesc one;
 };
 
+// Batch descriptor `i` of a device array that no kernel writes while it runs: read through the
+// constant address space, so its fields come in scalar loads that the compiler may keep across
+// the LDS atomics of a row loop.
+__device__ __forceinline__ const BatchDesc& batch_at(const BatchDesc* batches, int64_t i) {
+    const __attribute__((address_space(4))) BatchDesc* c = (const __attribute__((address_space(4))) BatchDesc*)batches;
+    return *(const BatchDesc*)(c + i);
+}
+
+// The same for a descriptor every lane of the wave reads (a tile's batch): its address is made
+// uniform, so the fields come in scalar loads once per tile rather than per row and operation.
+__device__ __forceinline__ const BatchDesc& uniform_batch(const BatchDesc* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return batch_at((const BatchDesc*)(((uint64_t)hi << 32) | lo), 0);
+}
+
 __device__ __forceinline__ bool is_valid(const DCol& c, int64_t i) { return !c.valid || bit_at(c.valid, i); }
 
 // A 32-bit key lane (INT32 / BOOL / UTF8 table-dictionary id).
@@ -406,11 +423,28 @@ struct ColSrc {
     __device__ __forceinline__ uint64_t row() const { return (uint64_t)(b.row_base + i); }
 };
 
-// Folds one row into the state words at `row` (an LDS entry or a global row) with atomics.
+// A value already in registers: validity, the lane's 64-bit word(s) (a DECIMAL128 in x0/x1, an
+// INT32 in the low half of x0) and the global row index.
+struct RegSrc {
+    bool v;
+    uint64_t x0, x1, gi;
+    int32_t type;
+    __device__ __forceinline__ bool valid(int) const { return v; }
+    __device__ __forceinline__ uint64_t u64(int) const { return x0; }
+    __device__ __forceinline__ int32_t i32(int) const { return (int32_t)(uint32_t)x0; }
+    __device__ __forceinline__ int64_t lng(int) const { return type == VR_INT32 ? (int64_t)(int32_t)(uint32_t)x0 : (int64_t)x0; }
+    __device__ __forceinline__ void d128(int, uint64_t* lo, uint64_t* hi) const {
+        *lo = x0;
+        *hi = x1;
+    }
+    __device__ __forceinline__ uint64_t row() const { return gi; }
+};
+
+// Folds one operation of one row into the state words at `row` (an LDS entry or a global row)
+// with atomics.
 template <class Src>
-__device__ __forceinline__ void apply_src(const UpdateArgs& a, const Src& src, uint64_t* row) {
-    for (int k = 0; k < a.nops; k++) {
-        const Op& op = a.ops[k];
+__device__ __forceinline__ void apply_one(const UpdateArgs& a, const Op& op, const Src& src, uint64_t* row) {
+    {
         uint64_t* w = row + op.off;
         switch (op.code) {
             case OP_ROWS: atomic_add_u64(w, 1); break;
@@ -497,27 +531,44 @@ __device__ __forceinline__ void apply_src(const UpdateArgs& a, const Src& src, u
     }
 }
 
+// Folds one row (every operation) into the state words at `row` with atomics.
+template <class Src>
+__device__ __forceinline__ void apply_src(const UpdateArgs& a, const Src& src, uint64_t* row) {
+    for (int k = 0; k < a.nops; k++) apply_one(a, a.ops[k], src, row);
+}
+
 // Folds row i of batch b into the state words at `row` (an LDS entry or a global row).
 __device__ __forceinline__ void apply_ops(const UpdateArgs& a, const BatchDesc& b, int64_t i, uint64_t* row) {
     apply_src(a, ColSrc{b, i}, row);
 }
 
-// Combines a partial state into a row that no other thread updates meanwhile (plain stores).
+__device__ __forceinline__ uint64_t combine_word(uint8_t code, uint64_t acc, uint64_t x) {
+    switch (code) {
+        case CB_ADD: return acc + x;
+        case CB_FADD: return (uint64_t)__double_as_longlong(__longlong_as_double((long long)acc) + __longlong_as_double((long long)x));
+        case CB_SMIN: return (int64_t)x < (int64_t)acc ? x : acc;
+        case CB_SMAX: return (int64_t)x > (int64_t)acc ? x : acc;
+        case CB_UMIN: return x < acc ? x : acc;
+        case CB_UMAX: return x > acc ? x : acc;
+        default: return x | acc;
+    }
+}
+
+// Combines a partial state into a row that no other thread updates meanwhile (plain stores): the
+// row is read eight words at a time, so its loads are in flight together.
 __device__ __forceinline__ void merge_plain(const UpdateArgs& a, const uint64_t* l, uint64_t* g, int S) {
-    for (int k = 0; k < S; k++) {
-        uint64_t x = l[k];
-        if (x == a.ident[k]) continue;
-        uint64_t y = g[k], r;
-        switch (a.comb[k]) {
-            case CB_ADD: r = y + x; break;
-            case CB_FADD: r = (uint64_t)__double_as_longlong(__longlong_as_double((long long)y) + __longlong_as_double((long long)x)); break;
-            case CB_SMIN: r = (int64_t)x < (int64_t)y ? x : y; break;
-            case CB_SMAX: r = (int64_t)x > (int64_t)y ? x : y; break;
-            case CB_UMIN: r = x < y ? x : y; break;
-            case CB_UMAX: r = x > y ? x : y; break;
-            default: r = x | y; break;
+    for (int k0 = 0; k0 < S; k0 += 8) {
+        uint64_t y[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (k0 + j < S) y[j] = g[k0 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + j;
+            if (k >= S) break;
+            const uint64_t x = l[k];
+            if (x != a.ident[k]) g[k] = combine_word(a.comb[k], y[j], x);
         }
-        g[k] = r;
     }
 }
 

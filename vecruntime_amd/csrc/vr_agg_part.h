@@ -32,11 +32,15 @@
 namespace vr {
 
 constexpr int PART_BLK = 64;         // records per block: one per lane of a wave
-constexpr int PART_REC_MAX = 32;     // 32-bit words per record
+constexpr int PART_REC_WORDS = 16;  // 32-bit words per record at most (a 64-byte record)
 constexpr uint32_t NO_BLOCK = 0xFFFFFFFFu;
-constexpr int PA_BLOCK = 512;        // scatter workgroup
-constexpr int PA_MAX_RPT = 8;        // rows per thread per scatter tile
-constexpr int PB_BLOCK = 512;        // merge workgroup
+constexpr int PA_BLOCK = 1024;       // scatter workgroup
+constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile
+constexpr int PA_CHUNK = 256;        // pool blocks a scatter workgroup takes at a time
+constexpr int PB_BLOCK = 1024;       // merge workgroup
+constexpr int HS_BLOCK = 256;        // sampling workgroup
+constexpr int HS_ROWS = 2048;        // rows per sample
+constexpr int HS_SLOTS = 4096;       // LDS counting slots of a sample
 constexpr int PB_PROBES = 16;        // LDS cache probes before a record waits for the next merge
 constexpr uint32_t HOT_EMPTY = 0xFFFFFFFFu;
 
@@ -51,9 +55,41 @@ struct RecLayout {
     int32_t val_type[VR_MAX_VALUES];
 };
 
+// The aggregates of one value column, compiled by the host: which state words a row's value
+// updates (offsets into the state row; -1 when absent).  Columns with a double MIN/MAX (first-seen
+// zero and NaN rows) are folded op by op instead (CP_GENERIC).
+enum ColProgFlags : uint16_t {
+    CP_NULLS = 1,
+    CP_SUM_I64 = 2,
+    CP_SUM_D128 = 4,
+    CP_SUM_F64 = 8,
+    CP_MIN_I32 = 16,
+    CP_MAX_I32 = 32,
+    CP_MIN_I64 = 64,
+    CP_MAX_I64 = 128,
+    CP_GENERIC = 256
+};
+
+struct ColProg {
+    uint16_t flags;
+    int16_t col;
+    int16_t type;
+    int16_t nulls_off;
+    int16_t sum_off;
+    int16_t min_off;
+    int16_t max_off;
+    int16_t min_bit;  // presence bit of MIN/MAX_I64 in the flags word
+    int16_t max_bit;
+    int16_t rec_off;  // word offset of the lane in the record, -1: not carried
+    int16_t pad[2];
+};
+
 struct PartArgs {
     UpdateArgs u;
     RecLayout rl;
+    ColProg prog[VR_MAX_VALUES];
+    int32_t nprog;
+    int32_t pad2;
     int32_t nparts;         // regions = 2^u.t.pbits
     int32_t nhot;           // hot-set entries (unused entries hold EMPTY_KEY)
     int32_t hot_slots;      // LDS index slots of the hot set (power of two >= 2 * nhot)
@@ -73,53 +109,57 @@ struct PartArgs {
     uint64_t max_blocks;
 };
 
-// A record in the pool, read field by field by the merge.
-struct RecSrc {
-    const uint32_t* r;
-    const RecLayout& L;
-    __device__ __forceinline__ bool valid(int c) const { return (r[L.valid_off] >> c) & 1u; }
-    __device__ __forceinline__ uint64_t u64(int c) const { return *(const uint64_t*)(r + L.val_off[c]); }
-    __device__ __forceinline__ int32_t i32(int c) const { return (int32_t)r[L.val_off[c]]; }
-    __device__ __forceinline__ int64_t lng(int c) const {
-        return L.val_type[c] == VR_INT32 ? (int64_t)i32(c) : (int64_t)u64(c);
+// Folds one value of a column into the state words at `e` (an LDS entry: LDS atomics), as its
+// program says.  v: valid; x0/x1: the lane's words (INT32 in the low half of x0).
+__device__ __forceinline__ void apply_prog(const UpdateArgs& u, const ColProg& cp, bool v, uint64_t x0, uint64_t x1,
+                                           uint64_t* e) {
+    const uint16_t fl = cp.flags;
+    if ((fl & CP_NULLS) && !v) atomic_add_u64(e + cp.nulls_off, 1);
+    if (!v) return;
+    if (fl & CP_SUM_I64) {
+        const int64_t x = cp.type == VR_INT32 ? (int64_t)(int32_t)(uint32_t)x0 : (int64_t)x0;
+        atomic_add_u64(e + cp.sum_off, (uint64_t)(uint32_t)x);
+        const uint64_t hi = (uint64_t)(x >> 32);
+        if (hi) atomic_add_u64(e + cp.sum_off + 1, hi);
     }
-    __device__ __forceinline__ void d128(int c, uint64_t* lo, uint64_t* hi) const {
-        const uint64_t* p = (const uint64_t*)(r + L.val_off[c]);
-        *lo = p[0];
-        *hi = p[1];
+    if (fl & CP_SUM_D128) {
+        add_if(e + cp.sum_off, x0 & 0xFFFFFFFFull);
+        add_if(e + cp.sum_off + 1, x0 >> 32);
+        add_if(e + cp.sum_off + 2, x1 & 0xFFFFFFFFull);
+        add_if(e + cp.sum_off + 3, (uint64_t)((int64_t)x1 >> 32));
     }
-    __device__ __forceinline__ uint64_t row() const { return *(const uint64_t*)(r + L.row_off); }
-};
-
-// Writes the record of row i to `dst` (8-byte aligned): key, carried value lanes, row index,
-// validity bits.  Lanes of null values are left unwritten (never read).
-__device__ __forceinline__ void write_record(const RecLayout& L, const BatchDesc& b, int64_t i, uint64_t key,
-                                             uint32_t* dst) {
-    *(uint64_t*)dst = key;
-    uint32_t vbits = 0;
-    for (int c = 0; c < L.nvals; c++) {
-        if (!((L.vmask >> c) & 1u)) continue;
-        const DCol& col = b.vals[c];
-        if (!is_valid(col, i)) continue;
-        vbits |= 1u << c;
-        const int o = L.val_off[c];
-        if (o < 0) continue;
-        switch (L.val_type[c]) {
-            case VR_INT32: dst[o] = (uint32_t)ld_stream((const int32_t*)col.data + i); break;
-            case VR_INT64:
-            case VR_FLOAT64: *(uint64_t*)(dst + o) = (uint64_t)ld_stream((const int64_t*)col.data + i); break;
-            case VR_DECIMAL128: {
-                const uint64_t* p = (const uint64_t*)col.data + 2 * i;
-                uint64_t lo = ld_stream(p), hi = ld_stream(p + 1);
-                *(uint64_t*)(dst + o) = lo;
-                *(uint64_t*)(dst + o + 2) = hi;
-                break;
-            }
-            default: break;
+    if (fl & CP_SUM_F64) atomic_add_f64(e + cp.sum_off, __longlong_as_double((long long)x0));
+    if (fl & (CP_MIN_I32 | CP_MAX_I32)) {
+        const int64_t x = (int64_t)(int32_t)(uint32_t)x0;
+        if ((fl & CP_MIN_I32) && x < (int64_t)e[cp.min_off]) atomic_min_i64(e + cp.min_off, x);
+        if ((fl & CP_MAX_I32) && x > (int64_t)e[cp.max_off]) atomic_max_i64(e + cp.max_off, x);
+    }
+    if (fl & (CP_MIN_I64 | CP_MAX_I64)) {
+        const uint64_t o = x0 ^ 0x8000000000000000ull;
+        uint64_t* f = e + u.flags_off;
+        if (fl & CP_MIN_I64) {
+            const uint64_t bit = 1ull << cp.min_bit;
+            const bool has = (*f & bit) != 0;
+            if (!has || o < e[cp.min_off]) atomic_min_u64(e + cp.min_off, o);
+            if (!has) atomicOr((unsigned long long*)f, bit);
+        }
+        if (fl & CP_MAX_I64) {
+            const uint64_t bit = 1ull << cp.max_bit;
+            const bool has = (*f & bit) != 0;
+            if (!has || o > e[cp.max_off]) atomic_max_u64(e + cp.max_off, o);
+            if (!has) atomicOr((unsigned long long*)f, bit);
         }
     }
-    if (L.row_off >= 0) *(uint64_t*)(dst + L.row_off) = (uint64_t)(b.row_base + i);
-    dst[L.valid_off] = vbits;
+}
+
+// Word o of a record held in registers (o uniform): a select per word, no indexed register access.
+template <int RW>
+__device__ __forceinline__ uint32_t get_word(const uint32_t (&w)[RW], int o) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int j = 0; j < RW; j++)
+        if (j == o) x = w[j];
+    return x;
 }
 
 __device__ __forceinline__ const BatchDesc& batch_of_tile(const UpdateArgs& u, int64_t tile) {
@@ -127,60 +167,65 @@ __device__ __forceinline__ const BatchDesc& batch_of_tile(const UpdateArgs& u, i
     int lo = 0, hi = u.num_batches - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (u.batches[mid].tile_begin <= tile) lo = mid;
+        if (batch_at(u.batches, mid).tile_begin <= tile) lo = mid;
         else hi = mid - 1;
     }
-    return u.batches[lo];
-}
-
-__device__ __forceinline__ uint64_t combine_word(uint8_t code, uint64_t acc, uint64_t x) {
-    switch (code) {
-        case CB_ADD: return acc + x;
-        case CB_FADD: return (uint64_t)__double_as_longlong(__longlong_as_double((long long)acc) + __longlong_as_double((long long)x));
-        case CB_SMIN: return (int64_t)x < (int64_t)acc ? x : acc;
-        case CB_SMAX: return (int64_t)x > (int64_t)acc ? x : acc;
-        case CB_UMIN: return x < acc ? x : acc;
-        case CB_UMAX: return x > acc ? x : acc;
-        default: return x | acc;
-    }
+    return batch_at(u.batches, lo);
 }
 
 // ------------------------------------------------------------------ hot set
 
-// Counts the keys of `per` consecutive rows at the start of `nsamp` evenly spaced tiles (tiles of
-// PA_BLOCK * rows_per_thread rows) in `cand`: open addressing over cand_mask + 1 entries of (key,
-// count).  Keys past a full candidate table are dropped (the hot set is only a hint).
-__global__ __launch_bounds__(256) void hot_sample_kernel(UpdateArgs u, int per, int64_t nsamp, uint64_t* cand,
-                                                         uint32_t* cand_count, uint64_t cand_mask) {
+// Counts the keys of HS_ROWS consecutive rows at the start of `nsamp` evenly spaced tiles (tiles
+// of PA_BLOCK * rows_per_thread rows): each workgroup counts its sample in LDS, then adds each
+// distinct key's count to `cand` (open addressing over cand_mask + 1 entries of key, count).  Keys
+// past a full LDS or candidate table are dropped: the hot set is only a hint.
+__global__ __launch_bounds__(HS_BLOCK) void hot_sample_kernel(UpdateArgs u, int64_t nsamp, uint64_t* cand,
+                                                              uint32_t* cand_count, uint64_t cand_mask) {
+    __shared__ uint64_t lkeys[HS_SLOTS];
+    __shared__ uint32_t lcount[HS_SLOTS];
     const int64_t T = (int64_t)u.rows_per_thread * PA_BLOCK;
     for (int64_t s = blockIdx.x; s < nsamp; s += gridDim.x) {
+        for (int q = threadIdx.x; q < HS_SLOTS; q += blockDim.x) {
+            lkeys[q] = EMPTY_KEY;
+            lcount[q] = 0;
+        }
+        __syncthreads();
         const int64_t tile = u.tile_lo + s * ((u.total_tiles - u.tile_lo) / nsamp);
         const BatchDesc& b = batch_of_tile(u, tile);
         const int64_t row0 = (tile - b.tile_begin) * T;
-        for (int r = threadIdx.x; r < per; r += blockDim.x) {
+        for (int r = threadIdx.x; r < HS_ROWS; r += blockDim.x) {
             const int64_t i = row0 + r;
             if (i >= b.n) break;
             if (b.sel && !bit_at(b.sel, i)) continue;
             uint64_t key, slot;
             if (!key64_of(u, b, i, &key, &slot) || key == EMPTY_KEY) continue;
-            uint64_t pos = fmix64(key) & cand_mask;
-            for (uint64_t q = 0; q <= cand_mask; q++) {
-                uint64_t c = cand[pos];
-                if (c == EMPTY_KEY) {
-                    uint64_t old = atomic_cas_u64(&cand[pos], EMPTY_KEY, key);
-                    if (old == EMPTY_KEY || old == key) {
-                        atomicAdd(&cand_count[pos], 1u);
-                        break;
-                    }
-                    c = old;
+            uint32_t pos = (uint32_t)fmix64(key) & (HS_SLOTS - 1);
+            for (int q = 0; q < HS_SLOTS; q++) {
+                const uint64_t old = atomicCAS((unsigned long long*)&lkeys[pos], (unsigned long long)EMPTY_KEY,
+                                               (unsigned long long)key);
+                if (old == EMPTY_KEY || old == key) {
+                    atomicAdd(&lcount[pos], 1u);
+                    break;
                 }
-                if (c == key) {
-                    atomicAdd(&cand_count[pos], 1u);
+                pos = (pos + 1) & (HS_SLOTS - 1);
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < HS_SLOTS; q += blockDim.x) {
+            const uint64_t key = lkeys[q];
+            if (key == EMPTY_KEY) continue;
+            uint64_t pos = fmix64(key) & cand_mask;
+            for (uint64_t t = 0; t <= cand_mask; t++) {
+                uint64_t c = cand[pos];
+                if (c == EMPTY_KEY) c = atomic_cas_u64(&cand[pos], EMPTY_KEY, key);
+                if (c == EMPTY_KEY || c == key) {
+                    atomicAdd(&cand_count[pos], lcount[q]);
                     break;
                 }
                 pos = (pos + 1) & cand_mask;
             }
         }
+        __syncthreads();
     }
 }
 
@@ -242,6 +287,17 @@ __host__ __device__ inline size_t scatter_lds_bytes(int nhot, int hot_slots, int
     return (size_t)nhot * 8 + (size_t)nhot * stride * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 16;
 }
 
+// Word o of a record held in registers (o uniform): a select per word, no indexed register access.
+template <int RW>
+__device__ __forceinline__ void put_word(uint32_t (&w)[RW], int o, uint32_t x) {
+#pragma unroll
+    for (int j = 0; j < RW; j++)
+        if (j == o) w[j] = x;
+}
+
+// RW: 32-bit words per record (the layout's `words`), so that a record is assembled in registers
+// and stored whole (16-byte stores): a record's line then leaves L2 in full sectors.
+template <int RW>
 __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     extern __shared__ uint64_t lds[];
     const UpdateArgs& u = a.u;
@@ -257,6 +313,8 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     uint32_t* start = fill + P;  // `fill` at the start of the tile
     uint32_t* cur = start + P;   // the chain's newest block
     uint32_t* first = cur + P;   // first block taken for the tile's records
+    // blocks taken from the pool and not handed out yet: [s_next, s_end)
+    __shared__ uint32_t s_next, s_end, s_need;
 
     for (int k = threadIdx.x; k < K; k += blockDim.x) hkeys[k] = a.hot_keys[k];
     for (int w = threadIdx.x; w < K * S; w += blockDim.x) hstate[w] = u.ident[w % S];
@@ -266,6 +324,11 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         start[p] = 0;
         cur[p] = NO_BLOCK;
         first[p] = NO_BLOCK;
+    }
+    if (threadIdx.x == 0) {
+        s_next = 0;
+        s_end = 0;
+        s_need = 0;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
@@ -279,16 +342,50 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     const int RPT = u.rows_per_thread;
     const int64_t T = (int64_t)PA_BLOCK * RPT;
     const int pbits = u.t.pbits;
-    const int RW = a.rl.words;
-    for (int64_t tile = u.tile_lo + blockIdx.x; tile < u.total_tiles; tile += gridDim.x) {
-        const BatchDesc& b = batch_of_tile(u, tile);
+    const RecLayout& L = a.rl;
+    // the current batch's descriptor, copied to LDS when the batch changes: its column pointers are
+    // read there once per column and tile
+    __shared__ BatchDesc lb;
+    // a contiguous range of tiles per workgroup: the batch only moves forward
+    const int64_t ntiles = u.total_tiles - u.tile_lo;
+    const int64_t per_wg = (ntiles + gridDim.x - 1) / gridDim.x;
+    const int64_t t_begin = u.tile_lo + (int64_t)blockIdx.x * per_wg;
+    const int64_t t_end = t_begin + per_wg < u.total_tiles ? t_begin + per_wg : u.total_tiles;
+    int bi = 0;
+    if (u.num_batches > 1 && t_begin < t_end) {
+        int lo = 0, hi = u.num_batches - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (batch_at(u.batches, mid).tile_begin <= t_begin) lo = mid;
+            else hi = mid - 1;
+        }
+        bi = lo;
+    }
+    int have_bi = -1;
+    for (int64_t tile = t_begin; tile < t_end; tile++) {
+        if (u.num_batches > 1)
+            while (bi + 1 < u.num_batches && batch_at(u.batches, bi + 1).tile_begin <= tile) bi++;
+        if (bi != have_bi) {  // uniform: the last tile's readers are past the barrier ending it
+            const uint64_t* src = (const uint64_t*)(u.num_batches > 1 ? &batch_at(u.batches, bi) : &u.one);
+            for (int w = threadIdx.x; w < (int)(sizeof(BatchDesc) / 8); w += blockDim.x) ((uint64_t*)&lb)[w] = src[w];
+            __syncthreads();
+            have_bi = bi;
+        }
+        const BatchDesc& b = lb;
         const int64_t row0 = (tile - b.tile_begin) * T;
-        uint32_t cold_v[PA_MAX_RPT];  // position in the region's chain
-        uint32_t cold_p[PA_MAX_RPT];  // region, NO_BLOCK: the row is not a record
-        uint64_t cold_k[PA_MAX_RPT];
+        // per row: key, and its fate -- ent >= 0: hot entry; -1: nothing (inactive or folded);
+        // -2: a record of region pv at chain position vv
+        uint64_t kk[PA_MAX_RPT];
+        int32_t ent[PA_MAX_RPT];
+        uint32_t pv[PA_MAX_RPT], vv[PA_MAX_RPT];
+        bool nul[PA_MAX_RPT];
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++) {
-            cold_p[r] = NO_BLOCK;
+            ent[r] = -1;
+            kk[r] = 0;
+            nul[r] = false;
+            pv[r] = 0;
+            vv[r] = 0;
             if (r >= RPT) continue;
             const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
             bool active = i < b.n;
@@ -298,66 +395,170 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
                 active = is_valid(f, i) && ld_stream((const int32_t*)f.data + i) < u.filter_threshold;
             }
             if (!active) continue;
-            uint64_t key, slot;
-            const bool keyed = key64_of(u, b, i, &key, &slot);
-            if (!keyed || key == EMPTY_KEY) {
+            uint64_t slot;
+            nul[r] = !key64_of(u, b, i, &kk[r], &slot);
+            ent[r] = -3;  // active, not classified yet
+        }
+#pragma unroll
+        for (int r = 0; r < PA_MAX_RPT; r++) {
+            if (ent[r] != -3) continue;
+            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+            const uint64_t key = kk[r];
+            if (nul[r] || key == EMPTY_KEY) {
                 // a reserved row (the null of a 64-bit lane, or a key equal to the empty marker)
-                if (keyed) slot = u.t.slots;
-                mark_reserved(u.t.counters, keyed ? RES_SENTINEL : RES_NULL);
+                const uint64_t slot = nul[r] ? u.t.slots + 1 : u.t.slots;
+                mark_reserved(u.t.counters, nul[r] ? RES_NULL : RES_SENTINEL);
                 apply_ops(u, b, i, u.t.state + slot * (uint64_t)S);
+                ent[r] = -1;
                 continue;
             }
             const uint64_t h = fmix64(key);
             if (K) {
                 const uint32_t k = hot_find(hidx, hkeys, (uint32_t)(H - 1), h, key);
                 if (k != HOT_EMPTY) {
-                    apply_ops(u, b, i, hstate + (size_t)k * S);
+                    ent[r] = (int32_t)k;
                     continue;
                 }
             }
             const uint32_t p = (uint32_t)region_of_hash(pbits, h);
-            cold_p[r] = p;
-            cold_k[r] = key;
-            cold_v[r] = atomicAdd(&fill[p], 1u);
+            pv[r] = p;
+            vv[r] = atomicAdd(&fill[p], 1u);
+            ent[r] = -2;
         }
         __syncthreads();
-        // blocks for the positions [start, fill) the tile took in each chain
+        // blocks for the positions [start, fill) the tile took in each chain, from the
+        // workgroup's share of the pool (refilled with one device atomic per PA_CHUNK blocks)
         for (int p = threadIdx.x; p < P; p += blockDim.x) {
-            const uint32_t s0 = start[p], f = fill[p];
-            if (f == s0) continue;
-            const uint32_t have = (s0 + PART_BLK - 1) / PART_BLK;
-            const uint32_t need = (f + PART_BLK - 1) / PART_BLK - have;
+            const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - (start[p] + PART_BLK - 1) / PART_BLK;
+            if (need) atomicAdd(&s_need, need);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t need = s_need;
+            if (s_end - s_next < need) {
+                const uint32_t take = need > (uint32_t)PA_CHUNK ? need : (uint32_t)PA_CHUNK;
+                const uint64_t b0 = atomic_add_u64(&u.t.counters[C_POOL], take);
+                if (b0 + take > a.max_blocks) {
+                    atomicOr((unsigned long long*)&u.t.counters[C_ERROR], (unsigned long long)ERR_TABLE_FULL);
+                    s_next = s_end = 0;  // nothing handed out: the launch fails
+                } else {
+                    s_next = (uint32_t)b0;
+                    s_end = (uint32_t)(b0 + take);
+                }
+            }
+            s_need = 0;
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < P; p += blockDim.x) {
+            const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
+            const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - have;
             if (need == 0) continue;
-            const uint64_t b0 = atomic_add_u64(&u.t.counters[C_POOL], need);
-            if (b0 + need > a.max_blocks) {
-                atomicOr((unsigned long long*)&u.t.counters[C_ERROR], (unsigned long long)ERR_TABLE_FULL);
+            if (s_end == 0) {
                 first[p] = NO_BLOCK;
                 continue;
             }
-            first[p] = (uint32_t)b0;
+            const uint32_t b0 = atomicAdd(&s_next, need);
+            first[p] = b0;
             for (uint32_t j = 0; j < need; j++) {
-                a.block_next[b0 + j] = j ? (uint32_t)(b0 + j - 1) : cur[p];
+                a.block_next[b0 + j] = j ? b0 + j - 1 : cur[p];
                 a.block_cnt[b0 + j] = PART_BLK;
             }
         }
         __syncthreads();
+        // record addresses of the cold rows; the records are assembled in registers
+        uint32_t* rec[PA_MAX_RPT];
+        uint32_t recw[PA_MAX_RPT][RW];
+        uint32_t vbits[PA_MAX_RPT];
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++) {
-            const uint32_t p = cold_p[r];
-            if (p == NO_BLOCK) continue;
-            const uint32_t v = cold_v[r];
+            rec[r] = nullptr;
+            vbits[r] = 0;
+#pragma unroll
+            for (int j = 0; j < RW; j++) recw[r][j] = 0;
+            if (ent[r] != -2) continue;
+            const uint32_t p = pv[r], v = vv[r];
             const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
             const uint32_t j = v / PART_BLK;
-            uint32_t blk;
-            if (j < have) {
-                blk = cur[p];
-            } else {
-                blk = first[p];
-                if (blk == NO_BLOCK) continue;  // the pool ran out: the launch fails
-                blk += j - have;
+            uint32_t blk = j < have ? cur[p] : first[p];
+            if (blk == NO_BLOCK) {  // the pool ran out: the launch fails
+                ent[r] = -1;
+                continue;
             }
-            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-            write_record(a.rl, b, i, cold_k[r], a.pool + ((uint64_t)blk * PART_BLK + (v % PART_BLK)) * (uint64_t)RW);
+            if (j >= have) blk += j - have;
+            rec[r] = a.pool + ((uint64_t)blk * PART_BLK + (v % PART_BLK)) * (uint64_t)RW;
+        }
+        // COUNT(*): word 0 of every state row
+#pragma unroll
+        for (int r = 0; r < PA_MAX_RPT; r++)
+            if (ent[r] >= 0) atomic_add_u64(hstate + (size_t)ent[r] * S, 1);
+        // value columns: every row's validity and lane loaded together, then the column's program
+        // on the hot rows and the column's field of the records
+        for (int q = 0; q < a.nprog; q++) {
+            const ColProg cp = a.prog[q];
+            const int c = cp.col;
+            const DCol d = b.vals[c];
+            const int o = cp.rec_off;
+            bool vr[PA_MAX_RPT];
+            uint64_t x0[PA_MAX_RPT], x1[PA_MAX_RPT];
+#pragma unroll
+            for (int r = 0; r < PA_MAX_RPT; r++) {
+                vr[r] = false;
+                x0[r] = 0;
+                x1[r] = 0;
+                if (ent[r] == -1) continue;
+                const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+                vr[r] = is_valid(d, i);
+                if (o < 0 || !vr[r]) continue;
+                switch (cp.type) {
+                    case VR_INT32: x0[r] = (uint64_t)(uint32_t)ld_stream((const int32_t*)d.data + i); break;
+                    case VR_INT64:
+                    case VR_FLOAT64: x0[r] = (uint64_t)ld_stream((const int64_t*)d.data + i); break;
+                    case VR_DECIMAL128:
+                        x0[r] = ld_stream((const uint64_t*)d.data + 2 * i);
+                        x1[r] = ld_stream((const uint64_t*)d.data + 2 * i + 1);
+                        break;
+                    default: break;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < PA_MAX_RPT; r++) {
+                if (ent[r] >= 0) {
+                    uint64_t* e = hstate + (size_t)ent[r] * S;
+                    if (cp.flags & CP_GENERIC) {
+                        const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+                        const RegSrc src{vr[r], x0[r], x1[r], (uint64_t)(b.row_base + i), cp.type};
+                        for (int k = 0; k < u.nops; k++)
+                            if (u.ops[k].col == c) apply_one(u, u.ops[k], src, e);
+                    } else {
+                        apply_prog(u, cp, vr[r], x0[r], x1[r], e);
+                    }
+                } else if (ent[r] == -2) {
+                    if (vr[r]) vbits[r] |= 1u << c;
+                    if (o < 0 || !vr[r]) continue;
+                    put_word<RW>(recw[r], o, (uint32_t)x0[r]);
+                    if (cp.type != VR_INT32) put_word<RW>(recw[r], o + 1, (uint32_t)(x0[r] >> 32));
+                    if (cp.type == VR_DECIMAL128) {
+                        put_word<RW>(recw[r], o + 2, (uint32_t)x1[r]);
+                        put_word<RW>(recw[r], o + 3, (uint32_t)(x1[r] >> 32));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PA_MAX_RPT; r++) {
+            if (ent[r] != -2) continue;
+            recw[r][0] = (uint32_t)kk[r];
+            recw[r][1] = (uint32_t)(kk[r] >> 32);
+            put_word<RW>(recw[r], L.valid_off, vbits[r]);
+            if (L.row_off >= 0) {
+                const uint64_t gi = (uint64_t)(b.row_base + row0 + (int64_t)r * PA_BLOCK + threadIdx.x);
+                put_word<RW>(recw[r], L.row_off, (uint32_t)gi);
+                put_word<RW>(recw[r], L.row_off + 1, (uint32_t)(gi >> 32));
+            }
+            uint4* dst = (uint4*)rec[r];
+#pragma unroll
+            for (int q = 0; q < RW / 4; q++)
+                dst[q] = make_uint4(recw[r][4 * q], recw[r][4 * q + 1], recw[r][4 * q + 2], recw[r][4 * q + 3]);
         }
         __syncthreads();
         for (int p = threadIdx.x; p < P; p += blockDim.x) {
@@ -413,6 +614,37 @@ __device__ __forceinline__ int cache_find(uint64_t* ckeys, int E, uint64_t h, ui
     return -1;
 }
 
+// Folds a record held in registers into a cache entry: COUNT(*), then each column's program.
+template <int RW>
+__device__ __forceinline__ void apply_record(const PartArgs& a, const uint32_t (&w)[RW], uint64_t* e) {
+    const UpdateArgs& u = a.u;
+    const RecLayout& L = a.rl;
+    atomic_add_u64(e, 1);
+    const uint32_t vb = get_word<RW>(w, L.valid_off);
+    for (int q = 0; q < a.nprog; q++) {
+        const ColProg cp = a.prog[q];
+        const bool v = (vb >> cp.col) & 1u;
+        uint64_t x0 = 0, x1 = 0;
+        const int o = cp.rec_off;
+        if (o >= 0 && v) {
+            x0 = get_word<RW>(w, o);
+            if (cp.type != VR_INT32) x0 |= (uint64_t)get_word<RW>(w, o + 1) << 32;
+            if (cp.type == VR_DECIMAL128) x1 = (uint64_t)get_word<RW>(w, o + 2) | ((uint64_t)get_word<RW>(w, o + 3) << 32);
+        }
+        if (cp.flags & CP_GENERIC) {
+            const uint64_t gi = L.row_off >= 0
+                                    ? ((uint64_t)get_word<RW>(w, L.row_off) | ((uint64_t)get_word<RW>(w, L.row_off + 1) << 32))
+                                    : 0;
+            const RegSrc src{v, x0, x1, gi, cp.type};
+            for (int k = 0; k < u.nops; k++)
+                if (u.ops[k].col == cp.col) apply_one(u, u.ops[k], src, e);
+        } else {
+            apply_prog(u, cp, v, x0, x1, e);
+        }
+    }
+}
+
+template <int RW>
 __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     extern __shared__ uint64_t lds[];
     const UpdateArgs& u = a.u;
@@ -428,7 +660,6 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = PB_BLOCK / 64;
-    const int RW = a.rl.words;
     uint32_t* rg = u.t.rgroups;
     TableView tv = u.t;
     tv.rgroups = nullptr;  // inserts are counted here, once per merge
@@ -542,29 +773,47 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         blk = a.block_next[blk];
         next_chain();
     };
+    // the record of this lane in block b, loaded into registers ahead of its step (false: none)
+    auto load_block = [&](uint32_t b, uint32_t (&w)[RW]) -> bool {
+        if (b == NO_BLOCK || (uint32_t)lane >= a.block_cnt[b]) return false;
+        const uint4* src = (const uint4*)gptr(a.pool + ((uint64_t)b * PART_BLK + lane) * (uint64_t)RW);
+#pragma unroll
+        for (int q = 0; q < RW / 4; q++) {
+            const uint4 x = src[q];
+            w[4 * q] = x.x;
+            w[4 * q + 1] = x.y;
+            w[4 * q + 2] = x.z;
+            w[4 * q + 3] = x.w;
+        }
+        return true;
+    };
     int step = 0;
+    uint32_t cw[RW];
+    bool have = false;
     if (!stopped) {
         for (; step < committed - 1; step++) advance();  // merged before the table grew
+        have = load_block(blk, cw);
+        advance();
         __syncthreads();
     }
     while (!stopped) {
-        // step `step`: each wave takes one block, lane j its record j
-        const uint32_t* rec = nullptr;
-        if (blk != NO_BLOCK) {
-            if ((uint32_t)lane < a.block_cnt[blk]) rec = a.pool + ((uint64_t)blk * PART_BLK + lane) * (uint64_t)RW;
-            advance();
-        }
+        // step `step`: each wave holds one block, lane j its record j; the next block is loaded
+        // while this one is folded
+        uint32_t nw[RW];
+        const bool more = blk != NO_BLOCK;
+        const bool nhave = load_block(blk, nw);
+        advance();
         if (threadIdx.x == 0) s_flags = 0;
         __syncthreads();
-        uint64_t key = 0, h = 0;
+        const uint64_t key = (uint64_t)cw[0] | ((uint64_t)cw[1] << 32);
+        uint64_t h = 0;
         int e = -1;
-        if (rec) {
-            key = *(const uint64_t*)rec;
+        if (have) {
             h = fmix64(key);
             e = cache_find(ckeys, E, h, key, PB_PROBES);
             if (e < 0) atomicOr(&s_flags, 1);
         }
-        if (blk != NO_BLOCK) atomicOr(&s_flags, 2);
+        if (more) atomicOr(&s_flags, 2);
         __syncthreads();
         const int flags = s_flags;
         if (flags & 1) {
@@ -574,12 +823,15 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
                 break;
             }
             committed = step + 1;
-            if (rec) e = cache_find(ckeys, E, h, key, 0);
+            if (have) e = cache_find(ckeys, E, h, key, 0);
         }
-        if (rec) apply_src(u, RecSrc{rec, a.rl}, cstate + (size_t)e * S);
+        if (have) apply_record<RW>(a, cw, cstate + (size_t)e * S);
         step++;
         if (!(flags & 2)) break;
         __syncthreads();
+        have = nhave;
+#pragma unroll
+        for (int j = 0; j < RW; j++) cw[j] = nw[j];
     }
     if (!stopped) {
         if (flush(false)) committed = -1;

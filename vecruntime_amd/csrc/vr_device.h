@@ -10,7 +10,15 @@ namespace vr {
 // The empty marker of a 64-bit key cell.  A real key equal to it is kept in a reserved row.
 constexpr uint64_t EMPTY_KEY = 0x7FF4DEADBEEF7FF4ull;
 
-__device__ __forceinline__ bool bit_at(const uint8_t* bm, int64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
+// Pointers into device (or host-mapped) memory, never LDS: loads through them may be reordered
+// with LDS atomics.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
+// Bit i of an Arrow bitmap (validity or selection) in device memory.
+__device__ __forceinline__ bool bit_at(const uint8_t* bm, int64_t i) { return (gptr(bm)[i >> 3] >> (i & 7)) & 1; }
 
 __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
     k ^= k >> 33;
@@ -25,7 +33,7 @@ __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
 // the caches (nontemporal hint).
 template <class T>
 __device__ __forceinline__ T ld_stream(const T* p) {
-    return __builtin_nontemporal_load(p);
+    return __builtin_nontemporal_load(gptr(p));
 }
 
 // Coherent (agent-scope) load, for cells another workgroup may have written during this kernel.
