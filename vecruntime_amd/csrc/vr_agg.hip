@@ -673,7 +673,7 @@ struct AggImpl {
 
     // partitioned update: hot-set sampling, record pool and chains, merge resume points
     DevBuf p_cand, p_cand_count, p_hist, p_hot_keys, p_hot_part, p_hot_total;
-    DevBuf p_pool, p_next, p_cnt, p_heads, p_resume;
+    DevBuf p_pool, p_next, p_cnt, p_heads, p_resume, p_tile_batch;
     int part_min_rows = 1 << 22;  // inputs from this many rows take the partitioned update
 
     vr_launch_config cfg{256, 4, 4, -1, 512, 4, 0};
@@ -987,7 +987,7 @@ static int64_t groups_of(const uint64_t* c) {
 // for the size).
 static vr_status grow_table(AggImpl& h, uint64_t want, hipStream_t s, int pbits = -1) {
     if (want > (1ull << 31)) return fail(VR_ERR_UNSUPPORTED, "more than 2^30 groups run on the CPU kernels");
-    if (pbits < 0) pbits = choose_pbits(want);
+    if (pbits < 0) pbits = std::max(choose_pbits(want), std::min(h.pbits, log2_of(want) - 11));  // keep the regions
     DevBuf ncells, nstate, nrg;
     VR_TRY(alloc_table(h, want, pbits, ncells, nstate, nrg, s));
     TableView from = view_of(h);
@@ -1328,6 +1328,8 @@ static int column_programs(const AggImpl& h, const RecLayout& L, ColProg* prog) 
                 default: cp.flags |= CP_GENERIC; break;  // double MIN/MAX: first-seen zero and NaN rows
             }
         }
+        // the lane is read when a record carries it or an aggregate other than a null count uses it
+        if (cp.rec_off >= 0 || (cp.flags & ~CP_NULLS)) cp.width = (int16_t)lane_bytes_of(cp.type);
         prog[n++] = cp;
     }
     return n;
@@ -1336,8 +1338,8 @@ static int column_programs(const AggImpl& h, const RecLayout& L, ColProg* prog) 
 // LDS cache entries of the merge (0: the state rows are too wide for it).
 static int merge_cache_entries(const AggImpl& h) {
     const int64_t budget = h.dev->lds_per_block - 1024;
-    for (int E = 2048; E >= 2 * PB_BLOCK; E >>= 1)
-        if ((int64_t)E * (8 + 8 * h.stride) <= budget) return E;
+    const int E = PB_FE * PB_BLOCK;
+    if ((int64_t)merge_lds_bytes(E, h.stride) <= budget) return E;
     return 0;
 }
 
@@ -1345,7 +1347,14 @@ static bool partitioned_applies(const AggImpl& h, int64_t rows, RecLayout* L, co
     const int32_t mode = (cfg && cfg->partitioned) ? cfg->partitioned : h.cfg.partitioned;
     if (mode < 0 || (mode == 0 && rows < h.part_min_rows)) return false;
     if (h.det || h.proto.ks.mode == KM_WIDE || rows <= 0) return false;
+    // block numbers of the record pool fit a merge list entry; one chain per lane of a merge wave
+    const uint64_t cus = (uint64_t)h.dev->num_cus;
+    if ((uint64_t)(rows + PART_BLK - 1) / PART_BLK + cus * ((1ull << PART_MAX_PBITS) + PA_CHUNK) + 1 >= (1ull << PB_ID_BITS))
+        return false;
+    if (cus > (uint64_t)PB_BLOCK) return false;
     if (!record_layout(h, *L)) return false;
+    ColProg prog[VR_MAX_VALUES];
+    if (column_programs(h, *L, prog) > PART_MAX_PROG) return false;
     return merge_cache_entries(h) > 0;
 }
 
@@ -1360,8 +1369,9 @@ static vr_status allow_lds(K kernel, size_t bytes) {
 static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const RecLayout& L, hipStream_t s,
                                  int* launches) {
     const int S = h.stride;
-    // enough regions for the merge to spread over every CU
-    const int want_pbits = std::max(0, std::min(10, log2_of((uint64_t)(rows >> 17) + 1)));
+    // Regions small enough for the merge cache to hold all the keys a region receives (one merge
+    // of the cache per region at the end), and enough of them to spread the merge over every CU.
+    const int want_pbits = std::max(0, std::min(PART_MAX_PBITS, log2_of((uint64_t)(rows >> 15) + 1)));
     if (h.pbits < want_pbits) VR_TRY(grow_table(h, std::max<uint64_t>(h.slots, 2048ull << want_pbits), s, want_pbits));
     const int P = 1 << h.pbits;
     const int E = merge_cache_entries(h);
@@ -1424,7 +1434,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     // unused rest of each workgroup's last share of the pool
     const uint64_t max_blocks =
         (uint64_t)(rows + PART_BLK - 1) / PART_BLK + (uint64_t)grid_a * P + (uint64_t)grid_a * PA_CHUNK + 1;
-    if (max_blocks >= (uint64_t)NO_BLOCK) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
+    if (max_blocks >= (1ull << PB_ID_BITS)) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
     VR_TRY(h.p_pool.ensure(h.dev, max_blocks * PART_BLK * L.words * sizeof(uint32_t)));
     VR_TRY(h.p_next.ensure(h.dev, max_blocks * sizeof(uint32_t)));
     VR_TRY(h.p_cnt.ensure(h.dev, max_blocks * sizeof(uint32_t)));
@@ -1436,10 +1446,11 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.block_cnt = h.p_cnt.as<uint32_t>();
     pa.heads = h.p_heads.as<uint32_t>();
     pa.resume_step = h.p_resume.as<int32_t>();
+    pa.tile_batch = h.p_tile_batch.as<uint32_t>();
     pa.max_blocks = max_blocks;
     pa.u = a;
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
-    const size_t lds_b = (size_t)E * (8 + 8 * S);
+    const size_t lds_b = merge_lds_bytes(E, S);
     switch (L.words) {
         case 4:
             VR_TRY(allow_lds(part_scatter_kernel<4>, lds_a));
@@ -1519,6 +1530,14 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     }
     VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
     VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs, sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
+    static thread_local std::vector<uint32_t> tile_batch;
+    tile_batch.resize((size_t)tiles);
+    for (int b = 0; b < nb; b++) {
+        const int64_t t1 = b + 1 < nb ? descs[b + 1].tile_begin : tiles;
+        for (int64_t t = descs[b].tile_begin; t < t1; t++) tile_batch[(size_t)t] = (uint32_t)b;
+    }
+    VR_TRY(h.p_tile_batch.ensure(h.dev, sizeof(uint32_t) * (size_t)std::max<int64_t>(tiles, 1)));
+    VR_TRY_HIP(hipMemcpyAsync(h.p_tile_batch.p, tile_batch.data(), sizeof(uint32_t) * (size_t)tiles, hipMemcpyHostToDevice, s));
     a.batches = h.batch_descs.as<BatchDesc>();
     a.num_batches = nb;
     a.one = descs[0];
@@ -2010,7 +2029,7 @@ vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
                        h.batch_descs.bytes + h.rgroups.bytes + h.resume.bytes + h.p_cand.bytes +
                        h.p_cand_count.bytes + h.p_hist.bytes + h.p_hot_keys.bytes + h.p_hot_part.bytes +
                        h.p_hot_total.bytes + h.p_pool.bytes + h.p_next.bytes + h.p_cnt.bytes + h.p_heads.bytes +
-                       h.p_resume.bytes);
+                       h.p_resume.bytes + h.p_tile_batch.bytes);
     return VR_OK;
 }
 

@@ -36,12 +36,18 @@ constexpr int PART_REC_WORDS = 16;  // 32-bit words per record at most (a 64-byt
 constexpr uint32_t NO_BLOCK = 0xFFFFFFFFu;
 constexpr int PA_BLOCK = 1024;       // scatter workgroup
 constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile
+constexpr int PART_MAX_PROG = 4;     // value columns the partitioned update folds (programs)
 constexpr int PA_CHUNK = 256;        // pool blocks a scatter workgroup takes at a time
 constexpr int PB_BLOCK = 1024;       // merge workgroup
+constexpr int PART_MAX_PBITS = 12;   // regions of the partitioned update: at most 2^12
 constexpr int HS_BLOCK = 256;        // sampling workgroup
 constexpr int HS_ROWS = 2048;        // rows per sample
 constexpr int HS_SLOTS = 4096;       // LDS counting slots of a sample
 constexpr int PB_PROBES = 16;        // LDS cache probes before a record waits for the next merge
+constexpr int PB_LIST = 128;         // block list entries per merge wave
+constexpr int PB_FE = 2;             // merge cache entries per thread (the cache holds PB_FE * PB_BLOCK)
+constexpr int PB_LANE_BLOCKS = 8;    // blocks a lane takes from its chain per walk
+constexpr int PB_ID_BITS = 26;       // block number bits of a list entry (the record count above)
 constexpr uint32_t HOT_EMPTY = 0xFFFFFFFFu;
 
 struct RecLayout {
@@ -81,8 +87,14 @@ struct ColProg {
     int16_t min_bit;  // presence bit of MIN/MAX_I64 in the flags word
     int16_t max_bit;
     int16_t rec_off;  // word offset of the lane in the record, -1: not carried
-    int16_t pad[2];
+    int16_t width;    // bytes of the lane (4, 8 or 16); 0: validity only
+    int16_t pad;
 };
+
+// Stand-ins read when a batch has no bitmap, selection or filter, so that every load of a tile is
+// issued whatever the batch holds (no branch between a load and the next).
+__device__ const uint8_t k_all_ones[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF};
+__device__ const uint32_t k_zero_words[4] = {0, 0, 0, 0};
 
 struct PartArgs {
     UpdateArgs u;
@@ -106,6 +118,7 @@ struct PartArgs {
     uint32_t* block_cnt;       // records in the block
     uint32_t* heads;           // [grid_a][nparts] newest block of each chain
     int32_t* resume_step;      // [nparts] merge: see above; -1 when merged
+    const uint32_t* tile_batch;  // scatter: the batch of every tile
     uint64_t max_blocks;
 };
 
@@ -343,35 +356,25 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     const int64_t T = (int64_t)PA_BLOCK * RPT;
     const int pbits = u.t.pbits;
     const RecLayout& L = a.rl;
-    // the current batch's descriptor, copied to LDS when the batch changes: its column pointers are
-    // read there once per column and tile
-    __shared__ BatchDesc lb;
-    // a contiguous range of tiles per workgroup: the batch only moves forward
-    const int64_t ntiles = u.total_tiles - u.tile_lo;
-    const int64_t per_wg = (ntiles + gridDim.x - 1) / gridDim.x;
-    const int64_t t_begin = u.tile_lo + (int64_t)blockIdx.x * per_wg;
-    const int64_t t_end = t_begin + per_wg < u.total_tiles ? t_begin + per_wg : u.total_tiles;
-    int bi = 0;
-    if (u.num_batches > 1 && t_begin < t_end) {
-        int lo = 0, hi = u.num_batches - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (batch_at(u.batches, mid).tile_begin <= t_begin) lo = mid;
-            else hi = mid - 1;
-        }
-        bi = lo;
+    // Tiles are grid-strided, so that the CUs read neighbouring addresses at any time.  The batch
+    // descriptor of a tile is in LDS while the tile runs (its column pointers are read there once
+    // per column and tile); the next tile's is fetched meanwhile into the other buffer.
+    constexpr int DESC_WORDS = (int)(sizeof(BatchDesc) / 8);
+    __shared__ BatchDesc lb[2];
+    int64_t tile = u.tile_lo + blockIdx.x;
+    if (tile < u.total_tiles) {
+        const uint64_t* src = (const uint64_t*)&batch_at(u.batches, a.tile_batch[tile]);
+        for (int w = threadIdx.x; w < DESC_WORDS; w += blockDim.x) ((uint64_t*)&lb[0])[w] = src[w];
     }
-    int have_bi = -1;
-    for (int64_t tile = t_begin; tile < t_end; tile++) {
-        if (u.num_batches > 1)
-            while (bi + 1 < u.num_batches && batch_at(u.batches, bi + 1).tile_begin <= tile) bi++;
-        if (bi != have_bi) {  // uniform: the last tile's readers are past the barrier ending it
-            const uint64_t* src = (const uint64_t*)(u.num_batches > 1 ? &batch_at(u.batches, bi) : &u.one);
-            for (int w = threadIdx.x; w < (int)(sizeof(BatchDesc) / 8); w += blockDim.x) ((uint64_t*)&lb)[w] = src[w];
-            __syncthreads();
-            have_bi = bi;
-        }
-        const BatchDesc& b = lb;
+    __syncthreads();
+    int cur_buf = 0;
+    for (; tile < u.total_tiles; tile += gridDim.x) {
+        const int64_t ntile = tile + gridDim.x;
+        const bool more = ntile < u.total_tiles;
+        uint64_t next_word = 0;
+        if (more && threadIdx.x < DESC_WORDS)
+            next_word = ((const uint64_t*)&batch_at(u.batches, a.tile_batch[ntile]))[threadIdx.x];
+        const BatchDesc& b = lb[cur_buf];
         const int64_t row0 = (tile - b.tile_begin) * T;
         // per row: key, and its fate -- ent >= 0: hot entry; -1: nothing (inactive or folded);
         // -2: a record of region pv at chain position vv
@@ -379,6 +382,49 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         int32_t ent[PA_MAX_RPT];
         uint32_t pv[PA_MAX_RPT], vv[PA_MAX_RPT];
         bool nul[PA_MAX_RPT];
+        // Every load of the tile first, with no branch between them (one round trip per tile): the
+        // row index is clamped into the batch, a lane is read as 32-bit words at i * width (the
+        // second word re-reads the first for a 4-byte lane), bitmaps the batch lacks are read from
+        // k_all_ones.  Validity, selection and the filter are applied afterwards.
+        uint32_t vm[PA_MAX_RPT];
+        uint64_t xa[PA_MAX_RPT][PART_MAX_PROG], xb[PA_MAX_RPT][PART_MAX_PROG];
+        uint32_t vbyte[PA_MAX_RPT][PART_MAX_PROG];
+        uint32_t lo[PA_MAX_RPT][PART_MAX_PROG], hi[PA_MAX_RPT][PART_MAX_PROG];
+        uint32_t w2[PA_MAX_RPT][PART_MAX_PROG], w3[PA_MAX_RPT][PART_MAX_PROG];
+        uint32_t sbyte[PA_MAX_RPT], fbyte[PA_MAX_RPT], kbyte[PA_MAX_RPT];
+        int32_t fval[PA_MAX_RPT];
+        uint32_t klo[PA_MAX_RPT], khi[PA_MAX_RPT];
+        const bool fast_key = u.ks.mode == KM_I64;
+        const bool has_filter = u.filter_col >= 0;
+        const DCol& fcol = b.vals[has_filter ? u.filter_col : 0];
+        const DCol& kcol = b.keys[0];
+#pragma unroll
+        for (int r = 0; r < PA_MAX_RPT; r++) {
+            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+            const int64_t ci = i < b.n ? i : b.n - 1;
+            sbyte[r] = *gptr(b.sel ? b.sel + (ci >> 3) : k_all_ones);
+            fbyte[r] = *gptr(has_filter && fcol.valid ? fcol.valid + (ci >> 3) : k_all_ones);
+            fval[r] = (int32_t)*gptr(has_filter ? (const uint32_t*)fcol.data + ci : k_zero_words);
+            if (fast_key) {
+                kbyte[r] = *gptr(kcol.valid ? kcol.valid + (ci >> 3) : k_all_ones);
+                const uint32_t* kp = (const uint32_t*)kcol.data + 2 * ci;
+                klo[r] = *gptr(kp);
+                khi[r] = *gptr(kp + 1);
+            }
+#pragma unroll
+            for (int q = 0; q < PART_MAX_PROG; q++) {
+                const bool on = q < a.nprog;
+                const ColProg& cp = a.prog[on ? q : 0];
+                const DCol& d = b.vals[cp.col];
+                vbyte[r][q] = *gptr(on && d.valid ? d.valid + (ci >> 3) : k_all_ones);
+                const int wd = on ? cp.width : 0;
+                const uint32_t* vp = wd ? (const uint32_t*)((const uint8_t*)d.data + ci * wd) : k_zero_words;
+                lo[r][q] = *gptr(vp);
+                hi[r][q] = *gptr(vp + (wd >= 8 ? 1 : 0));
+                w2[r][q] = *gptr(vp + (wd >= 16 ? 2 : 0));
+                w3[r][q] = *gptr(vp + (wd >= 16 ? 3 : 0));
+            }
+        }
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++) {
             ent[r] = -1;
@@ -386,17 +432,25 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             nul[r] = false;
             pv[r] = 0;
             vv[r] = 0;
-            if (r >= RPT) continue;
+            vm[r] = 0;
             const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-            bool active = i < b.n;
-            if (active && b.sel) active = bit_at(b.sel, i);
-            if (active && u.filter_col >= 0) {
-                const DCol& f = b.vals[u.filter_col];
-                active = is_valid(f, i) && ld_stream((const int32_t*)f.data + i) < u.filter_threshold;
+            const int bit = (int)(i & 7);
+#pragma unroll
+            for (int q = 0; q < PART_MAX_PROG; q++) {
+                const int wd = q < a.nprog ? a.prog[q].width : 0;
+                xa[r][q] = (uint64_t)lo[r][q] | (wd >= 8 ? (uint64_t)hi[r][q] << 32 : 0ull);
+                xb[r][q] = wd >= 16 ? ((uint64_t)w2[r][q] | ((uint64_t)w3[r][q] << 32)) : 0ull;
+                if (q < a.nprog && ((vbyte[r][q] >> bit) & 1u)) vm[r] |= 1u << q;
             }
-            if (!active) continue;
-            uint64_t slot;
-            nul[r] = !key64_of(u, b, i, &kk[r], &slot);
+            if (r >= RPT || i >= b.n || !((sbyte[r] >> bit) & 1u)) continue;
+            if (has_filter && !(((fbyte[r] >> bit) & 1u) && fval[r] < u.filter_threshold)) continue;
+            if (fast_key) {
+                nul[r] = !((kbyte[r] >> bit) & 1u);
+                kk[r] = (uint64_t)klo[r] | ((uint64_t)khi[r] << 32);
+            } else {
+                uint64_t slot;
+                nul[r] = !key64_of(u, b, i, &kk[r], &slot);
+            }
             ent[r] = -3;  // active, not classified yet
         }
 #pragma unroll
@@ -465,16 +519,13 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             }
         }
         __syncthreads();
-        // record addresses of the cold rows; the records are assembled in registers
+        // record addresses of the cold rows
         uint32_t* rec[PA_MAX_RPT];
-        uint32_t recw[PA_MAX_RPT][RW];
         uint32_t vbits[PA_MAX_RPT];
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++) {
             rec[r] = nullptr;
             vbits[r] = 0;
-#pragma unroll
-            for (int j = 0; j < RW; j++) recw[r][j] = 0;
             if (ent[r] != -2) continue;
             const uint32_t p = pv[r], v = vv[r];
             const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
@@ -491,55 +542,34 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++)
             if (ent[r] >= 0) atomic_add_u64(hstate + (size_t)ent[r] * S, 1);
-        // value columns: every row's validity and lane loaded together, then the column's program
-        // on the hot rows and the column's field of the records
-        for (int q = 0; q < a.nprog; q++) {
+        // each column's program on the hot rows, the column's field of the records
+#pragma unroll
+        for (int q = 0; q < PART_MAX_PROG; q++) {
+            if (q >= a.nprog) break;
             const ColProg cp = a.prog[q];
-            const int c = cp.col;
-            const DCol d = b.vals[c];
             const int o = cp.rec_off;
-            bool vr[PA_MAX_RPT];
-            uint64_t x0[PA_MAX_RPT], x1[PA_MAX_RPT];
 #pragma unroll
             for (int r = 0; r < PA_MAX_RPT; r++) {
-                vr[r] = false;
-                x0[r] = 0;
-                x1[r] = 0;
-                if (ent[r] == -1) continue;
-                const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-                vr[r] = is_valid(d, i);
-                if (o < 0 || !vr[r]) continue;
-                switch (cp.type) {
-                    case VR_INT32: x0[r] = (uint64_t)(uint32_t)ld_stream((const int32_t*)d.data + i); break;
-                    case VR_INT64:
-                    case VR_FLOAT64: x0[r] = (uint64_t)ld_stream((const int64_t*)d.data + i); break;
-                    case VR_DECIMAL128:
-                        x0[r] = ld_stream((const uint64_t*)d.data + 2 * i);
-                        x1[r] = ld_stream((const uint64_t*)d.data + 2 * i + 1);
-                        break;
-                    default: break;
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < PA_MAX_RPT; r++) {
+                const bool v = (vm[r] >> q) & 1u;
                 if (ent[r] >= 0) {
                     uint64_t* e = hstate + (size_t)ent[r] * S;
                     if (cp.flags & CP_GENERIC) {
                         const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-                        const RegSrc src{vr[r], x0[r], x1[r], (uint64_t)(b.row_base + i), cp.type};
+                        const RegSrc src{v, xa[r][q], xb[r][q], (uint64_t)(b.row_base + i), cp.type};
                         for (int k = 0; k < u.nops; k++)
-                            if (u.ops[k].col == c) apply_one(u, u.ops[k], src, e);
+                            if (u.ops[k].col == cp.col) apply_one(u, u.ops[k], src, e);
                     } else {
-                        apply_prog(u, cp, vr[r], x0[r], x1[r], e);
+                        apply_prog(u, cp, v, xa[r][q], xb[r][q], e);
                     }
                 } else if (ent[r] == -2) {
-                    if (vr[r]) vbits[r] |= 1u << c;
-                    if (o < 0 || !vr[r]) continue;
-                    put_word<RW>(recw[r], o, (uint32_t)x0[r]);
-                    if (cp.type != VR_INT32) put_word<RW>(recw[r], o + 1, (uint32_t)(x0[r] >> 32));
-                    if (cp.type == VR_DECIMAL128) {
-                        put_word<RW>(recw[r], o + 2, (uint32_t)x1[r]);
-                        put_word<RW>(recw[r], o + 3, (uint32_t)(x1[r] >> 32));
+                    if (v) vbits[r] |= 1u << cp.col;
+                    if (o < 0 || !v) continue;
+                    uint32_t* dst = rec[r] + o;
+                    if (cp.type == VR_INT32) {
+                        *dst = (uint32_t)xa[r][q];
+                    } else {
+                        *(uint64_t*)dst = xa[r][q];
+                        if (cp.type == VR_DECIMAL128) *(uint64_t*)(dst + 2) = xb[r][q];
                     }
                 }
             }
@@ -547,18 +577,10 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++) {
             if (ent[r] != -2) continue;
-            recw[r][0] = (uint32_t)kk[r];
-            recw[r][1] = (uint32_t)(kk[r] >> 32);
-            put_word<RW>(recw[r], L.valid_off, vbits[r]);
-            if (L.row_off >= 0) {
-                const uint64_t gi = (uint64_t)(b.row_base + row0 + (int64_t)r * PA_BLOCK + threadIdx.x);
-                put_word<RW>(recw[r], L.row_off, (uint32_t)gi);
-                put_word<RW>(recw[r], L.row_off + 1, (uint32_t)(gi >> 32));
-            }
-            uint4* dst = (uint4*)rec[r];
-#pragma unroll
-            for (int q = 0; q < RW / 4; q++)
-                dst[q] = make_uint4(recw[r][4 * q], recw[r][4 * q + 1], recw[r][4 * q + 2], recw[r][4 * q + 3]);
+            *(uint64_t*)rec[r] = kk[r];
+            rec[r][L.valid_off] = vbits[r];
+            if (L.row_off >= 0)
+                *(uint64_t*)(rec[r] + L.row_off) = (uint64_t)(b.row_base + row0 + (int64_t)r * PA_BLOCK + threadIdx.x);
         }
         __syncthreads();
         for (int p = threadIdx.x; p < P; p += blockDim.x) {
@@ -569,7 +591,9 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             if (need && first[p] != NO_BLOCK) cur[p] = first[p] + need - 1;
             start[p] = f;
         }
+        if (more && threadIdx.x < DESC_WORDS) ((uint64_t*)&lb[cur_buf ^ 1])[threadIdx.x] = next_word;
         __syncthreads();
+        cur_buf ^= 1;
     }
     // close the chains (the newest block's record count, the heads) and publish the hot partials
     for (int p = threadIdx.x; p < P; p += blockDim.x) {
@@ -596,6 +620,11 @@ __global__ __launch_bounds__(256) void hot_reduce_kernel(PartArgs a) {
 }
 
 // ------------------------------------------------------------------ phase B: merge
+
+// LDS bytes of the merge: the cache (E keys and state rows) and the waves' block lists.
+__host__ __device__ inline size_t merge_lds_bytes(int E, int stride) {
+    return (size_t)E * (8 + 8 * (size_t)stride) + (size_t)(PB_BLOCK / 64) * PB_LIST * 4;
+}
 
 // Entry of `key` in the LDS cache, claiming an empty one; -1 when neither is found within `limit`
 // probes (limit 0: no limit -- the caller guarantees a free entry).
@@ -644,6 +673,19 @@ __device__ __forceinline__ void apply_record(const PartArgs& a, const uint32_t (
     }
 }
 
+// Exclusive prefix sum of x over the lanes of a wave; *total: the sum over the wave.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t* total) {
+    const int lane = threadIdx.x & 63;
+    uint32_t v = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
+    }
+    *total = __shfl(v, 63, 64);
+    return v - x;
+}
+
 template <int RW>
 __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     extern __shared__ uint64_t lds[];
@@ -654,13 +696,15 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const uint32_t p = blockIdx.x;
     uint64_t* ckeys = lds;
     uint64_t* cstate = ckeys + E;
-    __shared__ int s_flags;  // bit 0: a record found no cache entry, bit 1: a wave has blocks left
-    __shared__ int s_stop;
-    __shared__ unsigned int s_count, s_inserted, s_kept;
+    constexpr int NW = PB_BLOCK / 64;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nwaves = PB_BLOCK / 64;
-    uint32_t* rg = u.t.rgroups;
+    // this wave's list of blocks: (records - 1) << PB_ID_BITS | block
+    uint32_t* wlist = (uint32_t*)(cstate + (size_t)E * S) + wave * PB_LIST;
+    // step flags, three in turn: bit 0: a record found no cache entry, bit 1: a wave has a block
+    // for the next step
+    __shared__ int s_flags[3];
+    __shared__ unsigned int s_count, s_inserted, s_kept, s_rg, s_new;
     TableView tv = u.t;
     tv.rgroups = nullptr;  // inserts are counted here, once per merge
     const uint32_t limit = a.region_limit;
@@ -668,26 +712,25 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     int resume = a.resumed ? a.resume_step[p] : 0;
     if (resume < 0) return;  // merged already
 
-    // Region capacity: can s_count more groups come in?  Counted by every thread before the call;
-    // read after the barrier.  Uniform over the workgroup.
-    auto room_for_count = [&]() -> bool {
-        __syncthreads();
-        if (threadIdx.x == 0) s_stop = rg[p] + s_count > limit;
-        __syncthreads();
-        return !s_stop;
-    };
-    auto add_inserted = [&]() {  // after a barrier that follows the inserts
-        if (threadIdx.x == 0 && s_inserted) {
-            rg[p] += s_inserted;
-            atomic_add_u64(&u.t.counters[C_GROUPS], s_inserted);
-            s_inserted = 0;
-        }
-    };
     if (threadIdx.x == 0) {
         s_count = 0;
         s_inserted = 0;
+        s_kept = 0;
+        s_new = 0;
+        s_rg = u.t.rgroups[p];  // the region is this workgroup's alone: its group count stays here
+        s_flags[0] = s_flags[1] = s_flags[2] = 0;
     }
     __syncthreads();
+    // after a barrier that follows the inserts; read by every thread after the next barrier
+    auto add_inserted = [&]() {
+        if (threadIdx.x == 0) {
+            s_rg += s_inserted;
+            s_new += s_inserted;
+            s_inserted = 0;
+            s_count = 0;
+            s_kept = 0;
+        }
+    };
     bool stopped = false;
     if (resume == 0) {
         // the hot set's partials of this region, straight into the region (one thread per key)
@@ -696,7 +739,8 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             if (key == EMPTY_KEY || a.hot_total[(size_t)k * S] == 0) continue;
             if (region_of_hash(u.t.pbits, fmix64(key)) == p) atomicAdd(&s_count, 1u);
         }
-        if (!room_for_count()) {
+        __syncthreads();
+        if (s_rg + s_count > limit) {
             stopped = true;
         } else {
             for (int k = threadIdx.x; k < a.nhot; k += blockDim.x) {
@@ -717,65 +761,135 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
 
     for (int e = threadIdx.x; e < E; e += blockDim.x) ckeys[e] = EMPTY_KEY;
     for (int w = threadIdx.x; w < E * S; w += blockDim.x) cstate[w] = u.ident[w % S];
+    __syncthreads();
 
     // Merges every cache entry that took rows into the region; entries with two rows or more stay
     // cached (at identity) when `keep`.  Returns false, merging nothing, when the region could not
-    // take every such entry as a new group.
+    // take every such entry as a new group.  Called by every thread.
     auto flush = [&](bool keep) -> bool {
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            s_count = 0;
-            s_kept = 0;
-        }
-        __syncthreads();
+        __syncthreads();  // every fold before the call is done
         unsigned int mine = 0;
         for (int e = threadIdx.x; e < E; e += blockDim.x)
             if (ckeys[e] != EMPTY_KEY && cstate[(size_t)e * S] != 0) mine++;
         if (mine) atomicAdd(&s_count, mine);
-        if (!room_for_count()) return false;
-        uint64_t kept[8];
-        int nkept = 0;
-        for (int e = threadIdx.x; e < E; e += blockDim.x) {
-            const uint64_t key = ckeys[e];
-            if (key == EMPTY_KEY) continue;
-            const uint64_t* l = cstate + (size_t)e * S;
-            const uint64_t rows = l[0];
-            if (rows) {
-                bool ins;
-                const uint64_t slot = find_or_insert64(tv, key, &ins);
-                if (ins) atomicAdd(&s_inserted, 1u);
-                merge_plain(u, l, u.t.state + slot * (uint64_t)S, S);
-            }
-            if (keep && rows >= 2 && nkept < 8 && atomicAdd(&s_kept, 1u) < (unsigned)(E / 4)) kept[nkept++] = key;
+        __syncthreads();
+        if (s_rg + s_count > limit) return false;  // uniform: s_rg and s_count change after the next barrier
+        // this thread's PB_FE entries, side by side: every global access of one is issued with
+        // the other's (the entries' slots, then their state rows)
+        uint64_t fk[PB_FE], slot[PB_FE], rows[PB_FE];
+        const uint64_t* l[PB_FE];
+#pragma unroll
+        for (int j = 0; j < PB_FE; j++) {
+            const int e = threadIdx.x + j * PB_BLOCK;
+            fk[j] = ckeys[e];
+            l[j] = cstate + (size_t)e * S;
+            rows[j] = fk[j] == EMPTY_KEY ? 0 : l[j][0];
+            slot[j] = home_slot(tv, fmix64(fk[j]));
         }
+        uint64_t cell[PB_FE];
+#pragma unroll
+        for (int j = 0; j < PB_FE; j++) cell[j] = tv.cells[slot[j]];
+        unsigned int ins = 0;
+#pragma unroll
+        for (int j = 0; j < PB_FE; j++) {
+            if (!rows[j]) continue;
+            uint64_t probes = 0;
+            for (;; probes++) {  // linear probing inside the region, as find_or_insert64
+                if (cell[j] == fk[j]) break;
+                if (cell[j] == EMPTY_KEY) {
+                    const uint64_t old = atomic_cas_u64(&tv.cells[slot[j]], EMPTY_KEY, fk[j]);
+                    if (old == EMPTY_KEY) {
+                        ins++;
+                        break;
+                    }
+                    if (old == fk[j]) break;
+                }
+                if (probes == tv.rmask) {
+                    slot[j] = region_full(tv);
+                    break;
+                }
+                slot[j] = next_slot(tv, slot[j]);
+                cell[j] = tv.cells[slot[j]];
+            }
+        }
+        if (ins) atomicAdd(&s_inserted, ins);
+        for (int k0 = 0; k0 < S; k0 += 8) {
+            uint64_t y[PB_FE][8];
+#pragma unroll
+            for (int j = 0; j < PB_FE; j++) {
+                const uint64_t* g = u.t.state + slot[j] * (uint64_t)S;
+#pragma unroll
+                for (int q = 0; q < 8; q++) y[j][q] = g[k0 + q < S ? k0 + q : S - 1];  // in the row, unconditional
+            }
+#pragma unroll
+            for (int j = 0; j < PB_FE; j++) {
+                if (!rows[j]) continue;
+                uint64_t* g = u.t.state + slot[j] * (uint64_t)S;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int k = k0 + q;
+                    if (k >= S) break;
+                    const uint64_t x = l[j][k];
+                    if (x != u.ident[k]) g[k] = combine_word(u.comb[k], y[j][q], x);
+                }
+            }
+        }
+        uint64_t kept[PB_FE];
+        int nkept = 0;
+#pragma unroll
+        for (int j = 0; j < PB_FE; j++)
+            if (keep && rows[j] >= 2 && atomicAdd(&s_kept, 1u) < (unsigned)(E / 4)) kept[nkept++] = fk[j];
         __syncthreads();
         add_inserted();
         for (int e = threadIdx.x; e < E; e += blockDim.x) ckeys[e] = EMPTY_KEY;
         for (int w = threadIdx.x; w < E * S; w += blockDim.x) cstate[w] = u.ident[w % S];
         __syncthreads();
         for (int q = 0; q < nkept; q++) (void)cache_find(ckeys, E, fmix64(kept[q]), kept[q], 0);
-        __syncthreads();
         return true;
     };
 
-    // this wave's chains: wave, wave + nwaves, ... < grid_a
-    int chain = wave;
-    uint32_t blk = NO_BLOCK;
-    auto next_chain = [&]() {
-        while (blk == NO_BLOCK && chain < a.grid_a) {
-            blk = a.heads[(size_t)chain * P + p];
-            chain += nwaves;
+    // The blocks of this wave: lanes take the chains wave + NW * lane (< grid_a) and walk them
+    // together, at most `cap` blocks each per walk, into the wave's list (lane-major, so the
+    // order is the same in every launch).  `fetch` hands out the next block of the sequence.
+    const int cpw = (a.grid_a - wave + NW - 1) / NW;  // chains of this wave (<= 64)
+    const int cap = PB_LIST / (cpw > 0 ? cpw : 1) < PB_LANE_BLOCKS ? PB_LIST / (cpw > 0 ? cpw : 1) : PB_LANE_BLOCKS;
+    uint32_t ccur = lane < cpw ? a.heads[(size_t)(wave + NW * lane) * P + p] : NO_BLOCK;
+    int lpos = 0, lend = 0;
+    auto walk = [&]() {  // wave-uniform
+        uint32_t mb[PB_LANE_BLOCKS], mn[PB_LANE_BLOCKS];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int t = 0; t < PB_LANE_BLOCKS; t++) {
+            mb[t] = 0;
+            mn[t] = 0;
+            if (t < cap && ccur != NO_BLOCK) {
+                mb[t] = ccur;
+                mn[t] = a.block_cnt[ccur];
+                ccur = a.block_next[ccur];
+                cnt = t + 1;
+            }
         }
+        uint32_t total;
+        const uint32_t off = wave_prefix(cnt, &total);
+#pragma unroll
+        for (int t = 0; t < PB_LANE_BLOCKS; t++)
+            if (t < (int)cnt) wlist[off + t] = ((mn[t] - 1) << PB_ID_BITS) | mb[t];
+        lpos = 0;
+        lend = (int)total;
     };
-    next_chain();
-    auto advance = [&]() {
-        if (blk == NO_BLOCK) return;
-        blk = a.block_next[blk];
-        next_chain();
+    // the next block of the sequence (NO_BLOCK: none) and its record count
+    auto fetch = [&](uint32_t* n) -> uint32_t {
+        if (lpos == lend && __any(ccur != NO_BLOCK)) walk();
+        if (lpos == lend) {
+            *n = 0;
+            return NO_BLOCK;
+        }
+        const uint32_t x = wlist[lpos++];
+        *n = (x >> PB_ID_BITS) + 1;
+        return x & ((1u << PB_ID_BITS) - 1);
     };
-    // the record of this lane in block b, loaded into registers ahead of its step (false: none)
-    auto load_block = [&](uint32_t b, uint32_t (&w)[RW]) -> bool {
-        if (b == NO_BLOCK || (uint32_t)lane >= a.block_cnt[b]) return false;
+    auto load_rec = [&](uint32_t b, uint32_t n, uint32_t (&w)[RW]) -> bool {
+        if (b == NO_BLOCK || (uint32_t)lane >= n) return false;
         const uint4* src = (const uint4*)gptr(a.pool + ((uint64_t)b * PART_BLK + lane) * (uint64_t)RW);
 #pragma unroll
         for (int q = 0; q < RW / 4; q++) {
@@ -787,35 +901,40 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         }
         return true;
     };
+
+    // Lockstep steps: in step s every wave folds the s-th block of its sequence, lane j record j.
+    // Records are loaded two steps ahead.  One barrier per step.
     int step = 0;
-    uint32_t cw[RW];
-    bool have = false;
+    uint32_t c0[RW], c1[RW], c2[RW];
+    uint32_t b0 = NO_BLOCK, b1 = NO_BLOCK, n0 = 0, n1 = 0;
+    bool h0 = false, h1 = false;
     if (!stopped) {
-        for (; step < committed - 1; step++) advance();  // merged before the table grew
-        have = load_block(blk, cw);
-        advance();
-        __syncthreads();
+        uint32_t nn;
+        for (; step < committed - 1; step++) (void)fetch(&nn);  // merged before the table grew
+        b0 = fetch(&n0);
+        h0 = load_rec(b0, n0, c0);
+        b1 = fetch(&n1);
+        h1 = load_rec(b1, n1, c1);
     }
     while (!stopped) {
-        // step `step`: each wave holds one block, lane j its record j; the next block is loaded
-        // while this one is folded
-        uint32_t nw[RW];
-        const bool more = blk != NO_BLOCK;
-        const bool nhave = load_block(blk, nw);
-        advance();
-        if (threadIdx.x == 0) s_flags = 0;
-        __syncthreads();
-        const uint64_t key = (uint64_t)cw[0] | ((uint64_t)cw[1] << 32);
+        uint32_t n2;
+        const uint32_t b2 = fetch(&n2);
+        const bool h2 = load_rec(b2, n2, c2);
+        const int fb = step % 3;
+        const uint64_t key = (uint64_t)c0[0] | ((uint64_t)c0[1] << 32);
         uint64_t h = 0;
         int e = -1;
-        if (have) {
+        if (h0) {
             h = fmix64(key);
             e = cache_find(ckeys, E, h, key, PB_PROBES);
-            if (e < 0) atomicOr(&s_flags, 1);
+            if (e < 0) atomicOr(&s_flags[fb], 1);
         }
-        if (more) atomicOr(&s_flags, 2);
+        if (lane == 0 && b1 != NO_BLOCK) atomicOr(&s_flags[fb], 2);
         __syncthreads();
-        const int flags = s_flags;
+        // the flags of step - 1 were read by every thread before this barrier; they serve
+        // step + 2, whose writers pass the next barrier after this reset
+        if (threadIdx.x == 0) s_flags[(step + 2) % 3] = 0;
+        const int flags = s_flags[fb];
         if (flags & 1) {
             // the cache is full: merge it (every step before this one), then place this step's rows
             if (!flush(true)) {
@@ -823,21 +942,31 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
                 break;
             }
             committed = step + 1;
-            if (have) e = cache_find(ckeys, E, h, key, 0);
+            if (h0) e = cache_find(ckeys, E, h, key, 0);
         }
-        if (have) apply_record<RW>(a, cw, cstate + (size_t)e * S);
+        if (h0) apply_record<RW>(a, c0, cstate + (size_t)e * S);
         step++;
         if (!(flags & 2)) break;
-        __syncthreads();
-        have = nhave;
+        b0 = b1;
+        n0 = n1;
+        h0 = h1;
+        b1 = b2;
+        n1 = n2;
+        h1 = h2;
 #pragma unroll
-        for (int j = 0; j < RW; j++) cw[j] = nw[j];
+        for (int j = 0; j < RW; j++) {
+            c0[j] = c1[j];
+            c1[j] = c2[j];
+        }
     }
     if (!stopped) {
         if (flush(false)) committed = -1;
         else stopped = true;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        u.t.rgroups[p] = s_rg;
+        if (s_new) atomic_add_u64(&u.t.counters[C_GROUPS], s_new);
         a.resume_step[p] = committed;
         if (stopped) atomicOr((unsigned long long*)&u.t.counters[C_PSTOP], 1ull);
     }
