@@ -27,7 +27,9 @@ BATCH = 65536
 # HBM bytes per launch of the batched update kernel on the 200M-row group-by, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a 1 GiB device copy
 # (profiles/agg_update_groupby200M_pmc_traffic.md).  Measured for the kernel as committed with it.
-PMC_TRAFFIC_BYTES = {200_000_000: 41.34e9}
+# HBM bytes moved by one partitioned update of the group-by (sampling, scatter, hot reduce, merge),
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a device copy of known size
+PMC_TRAFFIC_BYTES = {200_000_000: 16.43e9}
 
 
 def log(*a):
@@ -644,7 +646,8 @@ def main():
     join = None if args.skip_join else run_join(dev, args)
     uneven = None if args.skip_uneven else run_uneven(dev, args)
     serve = None if args.skip_serve else run_serve(dev, args)
-    # roofline of the dominant kernel: algorithmic bytes per launch / its measured duration
+    # roofline of the update: algorithmic bytes (the input columns, read once) / the duration of its
+    # kernels (hot-set sampling, scatter, hot reduce, merge), timed with HIP events on their stream
     bytes_per_row = 8 + 8 + 8 + 4 + 3 / 8.0
     alg_bytes = n * bytes_per_row
     achieved = alg_bytes / (gb["update_kernel_ms"] * 1e-3) / 1e9
@@ -674,7 +677,8 @@ def main():
                    "groups": gb["groups"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get(n),
-                     "kernel": "agg_update_kernel<true>", "kernel_ms_per_step": gb["update_kernel_ms"],
+                     "kernel": "partitioned update: hot_sample_kernel, part_scatter_kernel, hot_reduce_kernel, "
+                               "part_merge_kernel", "kernel_ms_per_step": gb["update_kernel_ms"],
                      "launches_per_step": gb["update_launches_per_step"],
                      "avg_launch_ms": gb["update_kernel_avg_launch_ms"], "algorithmic_bytes_per_step": alg_bytes},
         "cpu_baseline": cpu,

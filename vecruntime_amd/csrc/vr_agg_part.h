@@ -43,7 +43,7 @@ constexpr int PART_MAX_PBITS = 12;   // regions of the partitioned update: at mo
 constexpr int HS_BLOCK = 256;        // sampling workgroup
 constexpr int HS_ROWS = 2048;        // rows per sample
 constexpr int HS_SLOTS = 4096;       // LDS counting slots of a sample
-constexpr int PB_PROBES = 16;        // LDS cache probes before a record waits for the next merge
+constexpr int PB_PROBES = 4;         // LDS cache buckets a record tries before it waits for the next merge
 constexpr int PB_LIST = 128;         // block list entries per merge wave
 constexpr int PB_FE = 2;             // merge cache entries per thread (the cache holds PB_FE * PB_BLOCK)
 constexpr int PB_LANE_BLOCKS = 8;    // blocks a lane takes from its chain per walk
@@ -126,28 +126,30 @@ struct PartArgs {
 // program says.  v: valid; x0/x1: the lane's words (INT32 in the low half of x0).
 __device__ __forceinline__ void apply_prog(const UpdateArgs& u, const ColProg& cp, bool v, uint64_t x0, uint64_t x1,
                                            uint64_t* e) {
+    // Lanes differ only in v and the values: every atomic below is issued by every lane, a null
+    // row with the operation's neutral value, so that the wave never splits (the flags are uniform).
     const uint16_t fl = cp.flags;
-    if ((fl & CP_NULLS) && !v) atomic_add_u64(e + cp.nulls_off, 1);
-    if (!v) return;
+    if (fl & CP_NULLS) atomic_add_u64(e + cp.nulls_off, v ? 0ull : 1ull);
     if (fl & CP_SUM_I64) {
-        const int64_t x = cp.type == VR_INT32 ? (int64_t)(int32_t)(uint32_t)x0 : (int64_t)x0;
+        int64_t x = cp.type == VR_INT32 ? (int64_t)(int32_t)(uint32_t)x0 : (int64_t)x0;
+        x = v ? x : 0;
         atomic_add_u64(e + cp.sum_off, (uint64_t)(uint32_t)x);
-        const uint64_t hi = (uint64_t)(x >> 32);
-        if (hi) atomic_add_u64(e + cp.sum_off + 1, hi);
+        atomic_add_u64(e + cp.sum_off + 1, (uint64_t)(x >> 32));
     }
     if (fl & CP_SUM_D128) {
-        add_if(e + cp.sum_off, x0 & 0xFFFFFFFFull);
-        add_if(e + cp.sum_off + 1, x0 >> 32);
-        add_if(e + cp.sum_off + 2, x1 & 0xFFFFFFFFull);
-        add_if(e + cp.sum_off + 3, (uint64_t)((int64_t)x1 >> 32));
+        const uint64_t a0 = v ? x0 : 0, a1 = v ? x1 : 0;
+        atomic_add_u64(e + cp.sum_off, a0 & 0xFFFFFFFFull);
+        atomic_add_u64(e + cp.sum_off + 1, a0 >> 32);
+        atomic_add_u64(e + cp.sum_off + 2, a1 & 0xFFFFFFFFull);
+        atomic_add_u64(e + cp.sum_off + 3, (uint64_t)((int64_t)a1 >> 32));
     }
-    if (fl & CP_SUM_F64) atomic_add_f64(e + cp.sum_off, __longlong_as_double((long long)x0));
+    if (fl & CP_SUM_F64) atomic_add_f64(e + cp.sum_off, v ? __longlong_as_double((long long)x0) : -0.0);
     if (fl & (CP_MIN_I32 | CP_MAX_I32)) {
         const int64_t x = (int64_t)(int32_t)(uint32_t)x0;
-        if ((fl & CP_MIN_I32) && x < (int64_t)e[cp.min_off]) atomic_min_i64(e + cp.min_off, x);
-        if ((fl & CP_MAX_I32) && x > (int64_t)e[cp.max_off]) atomic_max_i64(e + cp.max_off, x);
+        if (fl & CP_MIN_I32) atomic_min_i64(e + cp.min_off, v ? x : INT64_MAX);
+        if (fl & CP_MAX_I32) atomic_max_i64(e + cp.max_off, v ? x : INT64_MIN);
     }
-    if (fl & (CP_MIN_I64 | CP_MAX_I64)) {
+    if ((fl & (CP_MIN_I64 | CP_MAX_I64)) && v) {
         const uint64_t o = x0 ^ 0x8000000000000000ull;
         uint64_t* f = e + u.flags_off;
         if (fl & CP_MIN_I64) {
@@ -283,21 +285,41 @@ __global__ __launch_bounds__(256) void hot_pick_kernel(const uint64_t* cand, con
 
 // ------------------------------------------------------------------ phase A: scatter
 
-// Number of `key` in the hot set through the LDS index, or HOT_EMPTY.
-__device__ __forceinline__ uint32_t hot_find(const uint32_t* hidx, const uint64_t* hkeys, uint32_t hmask, uint64_t h,
-                                             uint64_t key) {
-    uint32_t pos = (uint32_t)h & hmask;
+// LDS rows of the hot set and of the merge cache: the key, then the state words, padded to an odd
+// number of 8-byte words so that the lanes of a wave updating one word of different rows spread
+// over the LDS banks (a 64-byte row would put them all on two).
+__host__ __device__ inline int lds_row_words(int stride) { return (stride + 1) | 1; }
+
+// The hot set's LDS index: buckets of HB entries, an entry = 16 bits of the key's hash (bits
+// 32-47, not the bucket's) above the key's number in the hot set.  Entries fill each bucket from
+// its start, so a bucket with an empty entry ends the search; one bucket read answers almost
+// every lookup.
+constexpr int HB = 8;
+__device__ __forceinline__ uint32_t hot_tag(uint64_t h) { return (uint32_t)(h >> 16) & 0xFFFF0000u; }
+
+// Number of `key` in the hot set, or HOT_EMPTY.
+__device__ __forceinline__ uint32_t hot_find(const uint32_t* hidx, const uint64_t* hrow, int HW, uint32_t bmask,
+                                             uint64_t h, uint64_t key) {
+    const uint32_t tag = hot_tag(h);
+    uint32_t b = (uint32_t)h & bmask;
     while (true) {
-        uint32_t k = hidx[pos];
-        if (k == HOT_EMPTY) return HOT_EMPTY;
-        if (hkeys[k] == key) return k;
-        pos = (pos + 1) & hmask;
+        const uint4* q = (const uint4*)(hidx + b * HB);
+        const uint4 x = q[0], y = q[1];
+        const uint32_t e[HB] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+        bool full = true;
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+            if (e[j] == HOT_EMPTY) full = false;
+            else if ((e[j] & 0xFFFF0000u) == tag && hrow[(e[j] & 0xFFFFu) * HW] == key) return e[j] & 0xFFFFu;
+        }
+        if (!full) return HOT_EMPTY;
+        b = (b + 1) & bmask;
     }
 }
 
 // LDS bytes of the scatter kernel.
 __host__ __device__ inline size_t scatter_lds_bytes(int nhot, int hot_slots, int stride, int nparts) {
-    return (size_t)nhot * 8 + (size_t)nhot * stride * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 16;
+    return (size_t)nhot * lds_row_words(stride) * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 16;
 }
 
 // Word o of a record held in registers (o uniform): a select per word, no indexed register access.
@@ -319,9 +341,9 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     const int H = a.hot_slots;
     const int K = a.nhot;
     // LDS: hot keys [K], hot states [K][S], hot index [H] (u32), per region: fill, start, cur, first
-    uint64_t* hkeys = lds;
-    uint64_t* hstate = hkeys + K;
-    uint32_t* hidx = (uint32_t*)(hstate + (size_t)K * S);
+    const int HW = lds_row_words(S);
+    uint64_t* hrow = lds;  // row k: key k of the hot set, its state words
+    uint32_t* hidx = (uint32_t*)(hrow + (size_t)K * HW);
     uint32_t* fill = hidx + H;   // positions taken in the region's chain of this workgroup
     uint32_t* start = fill + P;  // `fill` at the start of the tile
     uint32_t* cur = start + P;   // the chain's newest block
@@ -329,8 +351,10 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     // blocks taken from the pool and not handed out yet: [s_next, s_end)
     __shared__ uint32_t s_next, s_end, s_need;
 
-    for (int k = threadIdx.x; k < K; k += blockDim.x) hkeys[k] = a.hot_keys[k];
-    for (int w = threadIdx.x; w < K * S; w += blockDim.x) hstate[w] = u.ident[w % S];
+    for (int w = threadIdx.x; w < K * HW; w += blockDim.x) {
+        const int j = w % HW;
+        hrow[w] = j == 0 ? a.hot_keys[w / HW] : (j <= S ? u.ident[j - 1] : 0);
+    }
     for (int q = threadIdx.x; q < H; q += blockDim.x) hidx[q] = HOT_EMPTY;
     for (int p = threadIdx.x; p < P; p += blockDim.x) {
         fill[p] = 0;
@@ -344,11 +368,17 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         s_need = 0;
     }
     __syncthreads();
+    const uint32_t hbmask = (uint32_t)(H / HB - 1);
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
-        const uint64_t key = hkeys[k];
+        const uint64_t key = hrow[(size_t)k * HW];
         if (key == EMPTY_KEY) continue;
-        uint32_t pos = (uint32_t)fmix64(key) & (uint32_t)(H - 1);
-        while (atomicCAS(&hidx[pos], HOT_EMPTY, (uint32_t)k) != HOT_EMPTY) pos = (pos + 1) & (uint32_t)(H - 1);
+        const uint64_t h = fmix64(key);
+        const uint32_t x = hot_tag(h) | (uint32_t)k;
+        for (uint32_t b = (uint32_t)h & hbmask;; b = (b + 1) & hbmask) {  // K < H: a bucket has room
+            int j = 0;
+            while (j < HB && atomicCAS(&hidx[b * HB + j], HOT_EMPTY, x) != HOT_EMPTY) j++;
+            if (j < HB) break;
+        }
     }
     __syncthreads();
 
@@ -468,7 +498,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             }
             const uint64_t h = fmix64(key);
             if (K) {
-                const uint32_t k = hot_find(hidx, hkeys, (uint32_t)(H - 1), h, key);
+                const uint32_t k = hot_find(hidx, hrow, HW, hbmask, h, key);
                 if (k != HOT_EMPTY) {
                     ent[r] = (int32_t)k;
                     continue;
@@ -541,7 +571,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         // COUNT(*): word 0 of every state row
 #pragma unroll
         for (int r = 0; r < PA_MAX_RPT; r++)
-            if (ent[r] >= 0) atomic_add_u64(hstate + (size_t)ent[r] * S, 1);
+            if (ent[r] >= 0) atomic_add_u64(hrow + (size_t)ent[r] * HW + 1, 1);
         // each column's program on the hot rows, the column's field of the records
 #pragma unroll
         for (int q = 0; q < PART_MAX_PROG; q++) {
@@ -552,7 +582,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             for (int r = 0; r < PA_MAX_RPT; r++) {
                 const bool v = (vm[r] >> q) & 1u;
                 if (ent[r] >= 0) {
-                    uint64_t* e = hstate + (size_t)ent[r] * S;
+                    uint64_t* e = hrow + (size_t)ent[r] * HW + 1;
                     if (cp.flags & CP_GENERIC) {
                         const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
                         const RegSrc src{v, xa[r][q], xb[r][q], (uint64_t)(b.row_base + i), cp.type};
@@ -603,7 +633,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         a.heads[(size_t)blockIdx.x * P + p] = f ? c : NO_BLOCK;
     }
     uint64_t* hp = a.hot_part + (size_t)blockIdx.x * K * S;
-    for (int w = threadIdx.x; w < K * S; w += blockDim.x) hp[w] = hstate[w];
+    for (int w = threadIdx.x; w < K * S; w += blockDim.x) hp[w] = hrow[(size_t)(w / S) * HW + 1 + w % S];
 }
 
 // hot_total[k] = the hot partials of every scatter workgroup combined.
@@ -623,22 +653,36 @@ __global__ __launch_bounds__(256) void hot_reduce_kernel(PartArgs a) {
 
 // LDS bytes of the merge: the cache (E keys and state rows) and the waves' block lists.
 __host__ __device__ inline size_t merge_lds_bytes(int E, int stride) {
-    return (size_t)E * (8 + 8 * (size_t)stride) + (size_t)(PB_BLOCK / 64) * PB_LIST * 4;
+    return (size_t)E * lds_row_words(stride) * 8 + (size_t)(PB_BLOCK / 64) * PB_LIST * 4;
 }
 
 // Entry of `key` in the LDS cache, claiming an empty one; -1 when neither is found within `limit`
-// probes (limit 0: no limit -- the caller guarantees a free entry).
-__device__ __forceinline__ int cache_find(uint64_t* ckeys, int E, uint64_t h, uint64_t key, int limit) {
-    uint32_t pos = (uint32_t)h & (uint32_t)(E - 1);
-    for (int q = 0; limit == 0 || q < limit; q++) {
-        const uint64_t k = ckeys[pos];
-        if (k == key) return (int)pos;
-        if (k == EMPTY_KEY) {
-            const uint64_t old = atomicCAS((unsigned long long*)&ckeys[pos], (unsigned long long)EMPTY_KEY,
-                                           (unsigned long long)key);
-            if (old == EMPTY_KEY || old == key) return (int)pos;
+// buckets (limit 0: no limit -- the caller guarantees a free entry).  The cache is a table of
+// buckets of CB keys, filled from each bucket's start: one bucket read answers most lookups.
+constexpr int CB = 8;
+__device__ __forceinline__ int cache_find(uint64_t* crow, int CW, int E, uint64_t h, uint64_t key, int limit) {
+    const uint32_t bmask = (uint32_t)(E / CB - 1);
+    uint32_t b = (uint32_t)h & bmask;
+    for (int q = 0; limit == 0 || q < limit;) {
+        const uint64_t* p = crow + (size_t)b * CB * CW;
+        uint64_t k[CB];
+#pragma unroll
+        for (int j = 0; j < CB; j++) k[j] = p[(size_t)j * CW];
+        int empty = -1;
+#pragma unroll
+        for (int j = CB - 1; j >= 0; j--) {
+            if (k[j] == key) return (int)(b * CB + j);
+            if (k[j] == EMPTY_KEY) empty = j;
         }
-        pos = (pos + 1) & (uint32_t)(E - 1);
+        if (empty < 0) {
+            b = (b + 1) & bmask;
+            q++;
+            continue;
+        }
+        const uint64_t old = atomicCAS((unsigned long long*)&crow[(size_t)(b * CB + empty) * CW],
+                                       (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+        if (old == EMPTY_KEY || old == key) return (int)(b * CB + empty);
+        // another lane took the entry: read the bucket again
     }
     return -1;
 }
@@ -694,13 +738,13 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const int E = a.cache_e;
     const int P = a.nparts;
     const uint32_t p = blockIdx.x;
-    uint64_t* ckeys = lds;
-    uint64_t* cstate = ckeys + E;
+    const int CW = lds_row_words(S);
+    uint64_t* crow = lds;  // cache entry e: row e (key, state words)
     constexpr int NW = PB_BLOCK / 64;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     // this wave's list of blocks: (records - 1) << PB_ID_BITS | block
-    uint32_t* wlist = (uint32_t*)(cstate + (size_t)E * S) + wave * PB_LIST;
+    uint32_t* wlist = (uint32_t*)(crow + (size_t)E * CW) + wave * PB_LIST;
     // step flags, three in turn: bit 0: a record found no cache entry, bit 1: a wave has a block
     // for the next step
     __shared__ int s_flags[3];
@@ -759,18 +803,23 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     }
     int committed = resume;  // steps before committed - 1 are merged (and the hot partials)
 
-    for (int e = threadIdx.x; e < E; e += blockDim.x) ckeys[e] = EMPTY_KEY;
-    for (int w = threadIdx.x; w < E * S; w += blockDim.x) cstate[w] = u.ident[w % S];
+    auto reset_cache = [&]() {
+        for (int w = threadIdx.x; w < E * CW; w += blockDim.x) {
+            const int j = w % CW;
+            crow[w] = j == 0 ? EMPTY_KEY : (j <= S ? u.ident[j - 1] : 0);
+        }
+    };
+    reset_cache();
     __syncthreads();
 
     // Merges every cache entry that took rows into the region; entries with two rows or more stay
     // cached (at identity) when `keep`.  Returns false, merging nothing, when the region could not
     // take every such entry as a new group.  Called by every thread.
-    auto flush = [&](bool keep) -> bool {
+    auto flush = [&](bool keep) __attribute__((always_inline)) -> bool {
         __syncthreads();  // every fold before the call is done
         unsigned int mine = 0;
         for (int e = threadIdx.x; e < E; e += blockDim.x)
-            if (ckeys[e] != EMPTY_KEY && cstate[(size_t)e * S] != 0) mine++;
+            if (crow[(size_t)e * CW] != EMPTY_KEY && crow[(size_t)e * CW + 1] != 0) mine++;
         if (mine) atomicAdd(&s_count, mine);
         __syncthreads();
         if (s_rg + s_count > limit) return false;  // uniform: s_rg and s_count change after the next barrier
@@ -781,8 +830,8 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
 #pragma unroll
         for (int j = 0; j < PB_FE; j++) {
             const int e = threadIdx.x + j * PB_BLOCK;
-            fk[j] = ckeys[e];
-            l[j] = cstate + (size_t)e * S;
+            fk[j] = crow[(size_t)e * CW];
+            l[j] = crow + (size_t)e * CW + 1;
             rows[j] = fk[j] == EMPTY_KEY ? 0 : l[j][0];
             slot[j] = home_slot(tv, fmix64(fk[j]));
         }
@@ -834,17 +883,16 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
                 }
             }
         }
-        uint64_t kept[PB_FE];
-        int nkept = 0;
+        bool kept[PB_FE];
 #pragma unroll
-        for (int j = 0; j < PB_FE; j++)
-            if (keep && rows[j] >= 2 && atomicAdd(&s_kept, 1u) < (unsigned)(E / 4)) kept[nkept++] = fk[j];
+        for (int j = 0; j < PB_FE; j++) kept[j] = keep && rows[j] >= 2 && atomicAdd(&s_kept, 1u) < (unsigned)(E / 4);
         __syncthreads();
         add_inserted();
-        for (int e = threadIdx.x; e < E; e += blockDim.x) ckeys[e] = EMPTY_KEY;
-        for (int w = threadIdx.x; w < E * S; w += blockDim.x) cstate[w] = u.ident[w % S];
+        reset_cache();
         __syncthreads();
-        for (int q = 0; q < nkept; q++) (void)cache_find(ckeys, E, fmix64(kept[q]), kept[q], 0);
+#pragma unroll
+        for (int j = 0; j < PB_FE; j++)
+            if (kept[j]) (void)cache_find(crow, CW, E, fmix64(fk[j]), fk[j], 0);
         return true;
     };
 
@@ -903,33 +951,30 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     };
 
     // Lockstep steps: in step s every wave folds the s-th block of its sequence, lane j record j.
-    // Records are loaded two steps ahead.  One barrier per step.
+    // Records are loaded two steps ahead into three buffers used in turn; each of the three
+    // unrolled steps names its buffers, so no register copy waits for a load in flight.  One
+    // barrier per step.
     int step = 0;
-    uint32_t c0[RW], c1[RW], c2[RW];
-    uint32_t b0 = NO_BLOCK, b1 = NO_BLOCK, n0 = 0, n1 = 0;
-    bool h0 = false, h1 = false;
-    if (!stopped) {
-        uint32_t nn;
-        for (; step < committed - 1; step++) (void)fetch(&nn);  // merged before the table grew
-        b0 = fetch(&n0);
-        h0 = load_rec(b0, n0, c0);
-        b1 = fetch(&n1);
-        h1 = load_rec(b1, n1, c1);
-    }
-    while (!stopped) {
-        uint32_t n2;
-        const uint32_t b2 = fetch(&n2);
-        const bool h2 = load_rec(b2, n2, c2);
+    uint32_t ra[RW], rb[RW], rc[RW];
+    uint32_t ba = NO_BLOCK, bb = NO_BLOCK, bc = NO_BLOCK;
+    bool ha = false, hb = false, hc = false;
+    // folds buffer X (this step); bY: the next step's block; loads the step after into Z.
+    // Returns 0: go on, 1: no block is left, 2: stopped (the region is full).
+    auto run_step = [&](const uint32_t (&X)[RW], bool hX, uint32_t bY, uint32_t (&Z)[RW], uint32_t& bZ,
+                        bool& hZ) __attribute__((always_inline)) -> int {
+        uint32_t nz;
+        bZ = fetch(&nz);
+        hZ = load_rec(bZ, nz, Z);
         const int fb = step % 3;
-        const uint64_t key = (uint64_t)c0[0] | ((uint64_t)c0[1] << 32);
+        const uint64_t key = (uint64_t)X[0] | ((uint64_t)X[1] << 32);
         uint64_t h = 0;
         int e = -1;
-        if (h0) {
+        if (hX) {
             h = fmix64(key);
-            e = cache_find(ckeys, E, h, key, PB_PROBES);
+            e = cache_find(crow, CW, E, h, key, PB_PROBES);
             if (e < 0) atomicOr(&s_flags[fb], 1);
         }
-        if (lane == 0 && b1 != NO_BLOCK) atomicOr(&s_flags[fb], 2);
+        if (lane == 0 && bY != NO_BLOCK) atomicOr(&s_flags[fb], 2);
         __syncthreads();
         // the flags of step - 1 were read by every thread before this barrier; they serve
         // step + 2, whose writers pass the next barrier after this reset
@@ -937,27 +982,28 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         const int flags = s_flags[fb];
         if (flags & 1) {
             // the cache is full: merge it (every step before this one), then place this step's rows
-            if (!flush(true)) {
-                stopped = true;
-                break;
-            }
+            if (!flush(true)) return 2;
             committed = step + 1;
-            if (h0) e = cache_find(ckeys, E, h, key, 0);
+            if (hX) e = cache_find(crow, CW, E, h, key, 0);
         }
-        if (h0) apply_record<RW>(a, c0, cstate + (size_t)e * S);
+        if (hX) apply_record<RW>(a, X, crow + (size_t)e * CW + 1);
         step++;
-        if (!(flags & 2)) break;
-        b0 = b1;
-        n0 = n1;
-        h0 = h1;
-        b1 = b2;
-        n1 = n2;
-        h1 = h2;
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            c0[j] = c1[j];
-            c1[j] = c2[j];
-        }
+        return (flags & 2) ? 0 : 1;
+    };
+    if (!stopped) {
+        uint32_t n;
+        for (; step < committed - 1; step++) (void)fetch(&n);  // merged before the table grew
+        ba = fetch(&n);
+        ha = load_rec(ba, n, ra);
+        bb = fetch(&n);
+        hb = load_rec(bb, n, rb);
+    }
+    while (!stopped) {
+        int r = run_step(ra, ha, bb, rc, bc, hc);
+        if (!r) r = run_step(rb, hb, bc, ra, ba, ha);
+        if (!r) r = run_step(rc, hc, ba, rb, bb, hb);
+        if (r == 2) stopped = true;
+        if (r) break;
     }
     if (!stopped) {
         if (flush(false)) committed = -1;
