@@ -166,9 +166,11 @@ def run_groupby(dev, args, n):
     torch.cuda.set_stream(stream)
     agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000))
 
+    batch_set = vr.GroupedAggregate.prepare_batches(batches)  # as the plain path's prepare(), once
+
     def step_batched(emit_cols):
         agg.reset()
-        agg.update_batches(batches, stream=stream)
+        agg.update_batches(batch_set, stream=stream)
         agg.emit(0, emit_cols[2], emit_cols[0], emit_cols[1], stream=stream)
 
     prepared = [(vr.GroupedAggregate.prepare(k, v), m) for k, v, m in batches]

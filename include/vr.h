@@ -23,8 +23,10 @@
 #ifndef VR_H
 #define VR_H
 
+#ifndef __HIPCC_RTC__ /* hipRTC provides the fixed-width types itself (vr_types.h) */
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -127,6 +129,7 @@ typedef struct vr_stats {
     int64_t table_grows;
     int64_t cpu_fallbacks_signalled;
     int64_t partitioned_updates;  /* aggregate calls that took the partitioned update */
+    int64_t specialized_updates;  /* of those, calls that ran kernels compiled for their spec */
 } vr_stats;
 vr_status vr_stats_get(vr_device* dev, vr_stats* out);
 
@@ -244,6 +247,9 @@ typedef struct vr_launch_config {
     int32_t partitioned;        /* keys of at most 64 bits: 1 takes the partitioned update (hot-set
                                    scatter, then one merge workgroup per table region) whatever the
                                    input size, -1 never, 0 from 4M rows per call */
+    int32_t specialize;         /* partitioned update: 0 or 1 runs kernels compiled at run time for
+                                   the spec's aggregates (hipRTC, once per spec and device, about
+                                   4 s; the generic kernels if that fails), -1 the generic kernels */
 } vr_launch_config;
 
 /* Sets the handle's default launch shape (what vr_agg_update uses, and what

@@ -73,3 +73,21 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     with pytest.raises(ImportError):
         importlib.import_module("vecruntime_amd.vr")
     sys.modules["vecruntime_amd.vr"] = vrmod
+
+
+def test_prepared_batch_set_points_at_every_column():
+    import numpy as np
+
+    from vecruntime_amd import abi, vr
+    from vecruntime_amd.columns import Column
+    batches = []
+    for b in range(3):
+        n = 5 + b
+        batches.append(([Column.longs(np.arange(n))], [Column.longs(np.arange(n)), Column.doubles(np.ones(n))], n))
+    p = vr.GroupedAggregate.prepare_batches(batches, filter_col=1, threshold=7)
+    bs = p.bs
+    assert bs.num_batches == 3 and bs.filter_col == 1 and bs.filter_threshold == 7
+    assert [bs.lengths[i] for i in range(3)] == [5, 6, 7]
+    for b in range(3):
+        assert bs.keys[b].length == 5 + b and bs.keys[b].type == abi.INT64
+        assert bs.values[2 * b].type == abi.INT64 and bs.values[2 * b + 1].type == abi.FLOAT64

@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 
 FORCE = dict(partitioned=1)
 NEVER = dict(partitioned=-1)
+GENERIC = dict(partitioned=1, specialize=-1)  # the generic kernels, not ones compiled for the spec
 
 
 @pytest.fixture(scope="module")
@@ -38,17 +39,22 @@ def make(dev, spec, mode=FORCE):
 
 
 class Took:
-    """Counts the partitioned updates the block runs (device statistics)."""
+    """Counts the partitioned updates the block runs, and those that ran kernels compiled for their
+    spec (device statistics)."""
 
     def __init__(self, dev):
         self.dev = dev
 
     def __enter__(self):
-        self.before = self.dev.stats()["partitioned_updates"]
+        st = self.dev.stats()
+        self.before = st["partitioned_updates"]
+        self.before_spec = st["specialized_updates"]
         return self
 
     def __exit__(self, *exc):
-        self.count = self.dev.stats()["partitioned_updates"] - self.before
+        st = self.dev.stats()
+        self.count = st["partitioned_updates"] - self.before
+        self.specialized = st["specialized_updates"] - self.before_spec
 
 
 def rows_of(agg, spec):
@@ -138,20 +144,26 @@ def test_zipf_batches_match_oracle_and_atomic_kernel(dev, distinct, s):
     batches, keep = device_batches(*data, 65536)
     gp = make(dev, spec, FORCE)
     ga = make(dev, spec, NEVER)
+    gg = make(dev, spec, GENERIC)
     with Took(dev) as t:
         gp.update_batches(batches, config=abi.launch_config(**FORCE))
-    assert t.count == 1
+    assert t.count == 1 and t.specialized == 1
     with Took(dev) as t:
         ga.update_batches(batches, config=abi.launch_config(**NEVER))
     assert t.count == 0
+    with Took(dev) as t:
+        gg.update_batches(batches, config=abi.launch_config(**GENERIC))
+    assert t.count == 1 and t.specialized == 0
     o = oracle_cfg2(spec, *data)
     assert check(gp, o, spec) < 1e-12
     check(ga, o, spec)
+    check(gg, o, spec)
     exact = [r == "exact" for r in out_roles(spec)]
-    rp, ra = rows_of(gp, spec), rows_of(ga, spec)
-    assert set(rp) == set(ra)
+    rp, ra, rg = rows_of(gp, spec), rows_of(ga, spec), rows_of(gg, spec)
+    assert set(rp) == set(ra) == set(rg)
     for k in rp:
         assert [x for x, e in zip(rp[k], exact) if e] == [x for x, e in zip(ra[k], exact) if e]
+        assert [x for x, e in zip(rp[k], exact) if e] == [x for x, e in zip(rg[k], exact) if e]
 
 
 @pytest.mark.parametrize("n", [1, 3, 63, 64, 65, 127, 129, 1023, 4097, 70_000])

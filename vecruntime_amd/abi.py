@@ -109,6 +109,7 @@ class VrStats(C.Structure):
         ("table_grows", C.c_int64),
         ("cpu_fallbacks_signalled", C.c_int64),
         ("partitioned_updates", C.c_int64),
+        ("specialized_updates", C.c_int64),
     ]
 
 
@@ -132,12 +133,15 @@ class VrLaunchConfig(C.Structure):
         ("lds_entries", C.c_int32),
         ("flush_tiles", C.c_int32),
         ("partitioned", C.c_int32),
+        ("specialize", C.c_int32),
     ]
 
 
-def launch_config(rows_per_thread=0, blocks_per_cu=0, lds=-1, lds_entries=0, flush_tiles=0, partitioned=0):
-    """partitioned: 1 takes the partitioned update at any size, -1 never, 0 from 4M rows per call."""
-    return VrLaunchConfig(0, rows_per_thread, blocks_per_cu, lds, lds_entries, flush_tiles, partitioned)
+def launch_config(rows_per_thread=0, blocks_per_cu=0, lds=-1, lds_entries=0, flush_tiles=0, partitioned=0,
+                  specialize=0):
+    """partitioned: 1 takes the partitioned update at any size, -1 never, 0 from 4M rows per call.
+    specialize: -1 runs the partitioned update's generic kernels instead of ones compiled for the spec."""
+    return VrLaunchConfig(0, rows_per_thread, blocks_per_cu, lds, lds_entries, flush_tiles, partitioned, specialize)
 
 
 class VrJoinSpec(C.Structure):

@@ -11,6 +11,8 @@
 #include <memory>
 
 #include "vr_agg_part.h"
+#include "vr_internal.h"
+#include "vr_jit.h"
 
 namespace vr {
 
@@ -676,7 +678,7 @@ struct AggImpl {
     DevBuf p_pool, p_next, p_cnt, p_heads, p_resume, p_tile_batch;
     int part_min_rows = 1 << 22;  // inputs from this many rows take the partitioned update
 
-    vr_launch_config cfg{256, 4, 4, -1, 512, 4, 0};
+    vr_launch_config cfg{256, 4, 4, -1, 512, 4, 0, 0};
     int lds_budget = 64 * 1024;  // LDS bytes per workgroup for the hot-key cache
 
     // the stream of the last work queued on the handle, and its completion event: work queued on
@@ -1366,8 +1368,15 @@ static vr_status allow_lds(K kernel, size_t bytes) {
 
 // Runs the partitioned update over the tiles [a.tile_lo, a.total_tiles) of the batches in `a`
 // (tiles of PA_BLOCK * a.rows_per_thread rows, columns on the device).
+// Launches a kernel compiled at run time with the partitioned update's arguments.
+static vr_status launch_jit(hipFunction_t f, unsigned grid, unsigned block, size_t lds, hipStream_t s, PartArgs& pa) {
+    void* args[] = {&pa};
+    VR_TRY_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, (unsigned)lds, s, args, nullptr));
+    return VR_OK;
+}
+
 static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const RecLayout& L, hipStream_t s,
-                                 int* launches) {
+                                 int* launches, bool specialize, bool* specialized) {
     const int S = h.stride;
     // Regions small enough for the merge cache to hold all the keys a region receives (one merge
     // of the cache per region at the end), and enough of them to spread the merge over every CU.
@@ -1399,6 +1408,9 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.hot_slots = H;
     pa.grid_a = grid_a;
     pa.cache_e = E;
+
+    // kernels compiled for this spec (the first call per spec compiles them: outside the timed part)
+    const PartJit* jit = specialize ? part_jit(h.dev->ordinal, pa.prog, pa.nprog, L) : nullptr;
 
     VR_TRY(timer_begin(h, s));
     // the hot set, from a sample of the input
@@ -1451,23 +1463,32 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.u = a;
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
     const size_t lds_b = merge_lds_bytes(E, S);
-    switch (L.words) {
-        case 4:
-            VR_TRY(allow_lds(part_scatter_kernel<4>, lds_a));
-            hipLaunchKernelGGL(part_scatter_kernel<4>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
-            break;
-        case 8:
-            VR_TRY(allow_lds(part_scatter_kernel<8>, lds_a));
-            hipLaunchKernelGGL(part_scatter_kernel<8>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
-            break;
-        case 12:
-            VR_TRY(allow_lds(part_scatter_kernel<12>, lds_a));
-            hipLaunchKernelGGL(part_scatter_kernel<12>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
-            break;
-        default:
-            VR_TRY(allow_lds(part_scatter_kernel<16>, lds_a));
-            hipLaunchKernelGGL(part_scatter_kernel<16>, dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
-            break;
+    if (jit && launch_jit(jit->scatter, (unsigned)grid_a, PA_BLOCK, lds_a, s, pa) != VR_OK) {
+        // the module's kernels do not launch here: the generic kernels from now on
+        (void)hipGetLastError();
+        part_jit_disable();
+        jit = nullptr;
+    }
+    *specialized = jit != nullptr;
+    if (!jit) {
+        switch (L.words) {
+            case 4:
+                VR_TRY(allow_lds(part_scatter_kernel<4, RuntimeProgs>, lds_a));
+                hipLaunchKernelGGL((part_scatter_kernel<4, RuntimeProgs>), dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+                break;
+            case 8:
+                VR_TRY(allow_lds(part_scatter_kernel<8, RuntimeProgs>, lds_a));
+                hipLaunchKernelGGL((part_scatter_kernel<8, RuntimeProgs>), dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+                break;
+            case 12:
+                VR_TRY(allow_lds(part_scatter_kernel<12, RuntimeProgs>, lds_a));
+                hipLaunchKernelGGL((part_scatter_kernel<12, RuntimeProgs>), dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+                break;
+            default:
+                VR_TRY(allow_lds(part_scatter_kernel<16, RuntimeProgs>, lds_a));
+                hipLaunchKernelGGL((part_scatter_kernel<16, RuntimeProgs>), dim3(grid_a), dim3(PA_BLOCK), lds_a, s, pa);
+                break;
+        }
     }
     if (K) hipLaunchKernelGGL(hot_reduce_kernel, dim3(grid_for((uint64_t)K * S, 256, cap)), dim3(256), 0, s, pa);
     VR_TRY_HIP(hipGetLastError());
@@ -1478,23 +1499,27 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
         pa.u.t = view_of(h);
         pa.region_limit = (uint32_t)((pa.u.t.rmask + 1) * 3 / 4);
         if (round) VR_TRY(timer_begin(h, s));
-        switch (L.words) {
-            case 4:
-                VR_TRY(allow_lds(part_merge_kernel<4>, lds_b));
-                hipLaunchKernelGGL(part_merge_kernel<4>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
-                break;
-            case 8:
-                VR_TRY(allow_lds(part_merge_kernel<8>, lds_b));
-                hipLaunchKernelGGL(part_merge_kernel<8>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
-                break;
-            case 12:
-                VR_TRY(allow_lds(part_merge_kernel<12>, lds_b));
-                hipLaunchKernelGGL(part_merge_kernel<12>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
-                break;
-            default:
-                VR_TRY(allow_lds(part_merge_kernel<16>, lds_b));
-                hipLaunchKernelGGL(part_merge_kernel<16>, dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
-                break;
+        if (jit) {
+            VR_TRY(launch_jit(jit->merge, (unsigned)P, PB_BLOCK, lds_b, s, pa));
+        } else {
+            switch (L.words) {
+                case 4:
+                    VR_TRY(allow_lds(part_merge_kernel<4, RuntimeProgs>, lds_b));
+                    hipLaunchKernelGGL((part_merge_kernel<4, RuntimeProgs>), dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                    break;
+                case 8:
+                    VR_TRY(allow_lds(part_merge_kernel<8, RuntimeProgs>, lds_b));
+                    hipLaunchKernelGGL((part_merge_kernel<8, RuntimeProgs>), dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                    break;
+                case 12:
+                    VR_TRY(allow_lds(part_merge_kernel<12, RuntimeProgs>, lds_b));
+                    hipLaunchKernelGGL((part_merge_kernel<12, RuntimeProgs>), dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                    break;
+                default:
+                    VR_TRY(allow_lds(part_merge_kernel<16, RuntimeProgs>, lds_b));
+                    hipLaunchKernelGGL((part_merge_kernel<16, RuntimeProgs>), dim3(P), dim3(PB_BLOCK), lds_b, s, pa);
+                    break;
+            }
         }
         VR_TRY(timer_end(h, s));
         VR_TRY_HIP(hipGetLastError());
@@ -1545,7 +1570,9 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     a.total_tiles = tiles;
     a.epoch = ++h.epoch;
     int launches = 0;
-    VR_TRY(run_partitioned(h, a, rows, L, s, &launches));
+    const int32_t spec_mode = (cfg && cfg->specialize) ? cfg->specialize : h.cfg.specialize;
+    bool specialized = false;
+    VR_TRY(run_partitioned(h, a, rows, L, s, &launches, spec_mode >= 0, &specialized));
     if (h.has_f64_minmax) {  // over the final table
         a.t = view_of(h);
         hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid_for(tiles, 1, h.dev->num_cus * 4)), dim3(PA_BLOCK), 0, s, a);
@@ -1554,6 +1581,7 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     }
     VR_TRY_HIP(hipStreamSynchronize(s));  // the descriptors are rewritten by the next call
     h.dev->count(&vr_stats::partitioned_updates, 1);
+    if (specialized) h.dev->count(&vr_stats::specialized_updates, 1);
     h.rows_seen += rows;
     h.order_count = -1;
     h.dev->count(&vr_stats::kernel_launches, launches);
@@ -1888,7 +1916,7 @@ vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg) {
     vr_launch_config& c = agg->h.cfg;
     if (cfg->rows_per_thread < 0 || cfg->rows_per_thread > 64 || cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 32 ||
         cfg->lds_entries < 0 || cfg->lds_entries > 2048 || cfg->flush_tiles < 0 || cfg->partitioned < -1 ||
-        cfg->partitioned > 1)
+        cfg->partitioned > 1 || cfg->specialize < -1 || cfg->specialize > 1)
         return fail(VR_ERR_INVALID_ARGUMENT, "launch config out of range");
     if (cfg->rows_per_thread) c.rows_per_thread = cfg->rows_per_thread;
     if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
@@ -1896,6 +1924,7 @@ vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg) {
     if (cfg->lds_entries) c.lds_entries = cfg->lds_entries;
     if (cfg->flush_tiles) c.flush_tiles = cfg->flush_tiles;
     if (cfg->partitioned) c.partitioned = cfg->partitioned;
+    if (cfg->specialize) c.specialize = cfg->specialize;
     return VR_OK;
 }
 

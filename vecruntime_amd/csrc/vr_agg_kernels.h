@@ -38,7 +38,7 @@
 #pragma once
 
 #include "vr_device.h"
-#include "vr_internal.h"
+#include "vr_types.h"
 
 namespace vr {
 

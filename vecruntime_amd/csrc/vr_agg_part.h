@@ -28,68 +28,9 @@
 #pragma once
 
 #include "vr_agg_kernels.h"
+#include "vr_part_types.h"
 
 namespace vr {
-
-constexpr int PART_BLK = 64;         // records per block: one per lane of a wave
-constexpr int PART_REC_WORDS = 16;  // 32-bit words per record at most (a 64-byte record)
-constexpr uint32_t NO_BLOCK = 0xFFFFFFFFu;
-constexpr int PA_BLOCK = 1024;       // scatter workgroup
-constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile
-constexpr int PART_MAX_PROG = 4;     // value columns the partitioned update folds (programs)
-constexpr int PA_CHUNK = 256;        // pool blocks a scatter workgroup takes at a time
-constexpr int PB_BLOCK = 1024;       // merge workgroup
-constexpr int PART_MAX_PBITS = 12;   // regions of the partitioned update: at most 2^12
-constexpr int HS_BLOCK = 256;        // sampling workgroup
-constexpr int HS_ROWS = 2048;        // rows per sample
-constexpr int HS_SLOTS = 4096;       // LDS counting slots of a sample
-constexpr int PB_PROBES = 4;         // LDS cache buckets a record tries before it waits for the next merge
-constexpr int PB_LIST = 128;         // block list entries per merge wave
-constexpr int PB_FE = 2;             // merge cache entries per thread (the cache holds PB_FE * PB_BLOCK)
-constexpr int PB_LANE_BLOCKS = 8;    // blocks a lane takes from its chain per walk
-constexpr int PB_ID_BITS = 26;       // block number bits of a list entry (the record count above)
-constexpr uint32_t HOT_EMPTY = 0xFFFFFFFFu;
-
-struct RecLayout {
-    int32_t words;                    // 32-bit words per record, a multiple of 4
-    int32_t nvals;                    // value columns
-    uint32_t vmask;                   // value columns whose validity the aggregates read
-    int32_t row_off;                  // word offset of the global row index, -1: not carried
-    int32_t valid_off;                // word offset of the validity bits (bit c: value column c)
-    int32_t pad;
-    int32_t val_off[VR_MAX_VALUES];   // word offset of each value lane carried, -1: not carried
-    int32_t val_type[VR_MAX_VALUES];
-};
-
-// The aggregates of one value column, compiled by the host: which state words a row's value
-// updates (offsets into the state row; -1 when absent).  Columns with a double MIN/MAX (first-seen
-// zero and NaN rows) are folded op by op instead (CP_GENERIC).
-enum ColProgFlags : uint16_t {
-    CP_NULLS = 1,
-    CP_SUM_I64 = 2,
-    CP_SUM_D128 = 4,
-    CP_SUM_F64 = 8,
-    CP_MIN_I32 = 16,
-    CP_MAX_I32 = 32,
-    CP_MIN_I64 = 64,
-    CP_MAX_I64 = 128,
-    CP_GENERIC = 256
-};
-
-struct ColProg {
-    uint16_t flags;
-    int16_t col;
-    int16_t type;
-    int16_t nulls_off;
-    int16_t sum_off;
-    int16_t min_off;
-    int16_t max_off;
-    int16_t min_bit;  // presence bit of MIN/MAX_I64 in the flags word
-    int16_t max_bit;
-    int16_t rec_off;  // word offset of the lane in the record, -1: not carried
-    int16_t width;    // bytes of the lane (4, 8 or 16); 0: validity only
-    int16_t pad;
-};
 
 // Stand-ins read when a batch has no bitmap, selection or filter, so that every load of a tile is
 // issued whatever the batch holds (no branch between a load and the next).
@@ -120,6 +61,17 @@ struct PartArgs {
     int32_t* resume_step;      // [nparts] merge: see above; -1 when merged
     const uint32_t* tile_batch;  // scatter: the batch of every tile
     uint64_t max_blocks;
+};
+
+// Where the kernels read the column programs and the record layout.  RuntimeProgs reads them from
+// the launch arguments (one build serves every spec); a policy with the same members whose values
+// are compile-time constants -- generated per spec and compiled at run time -- lets the compiler
+// fold the programs into the per-row code (no per-row flag tests or register selects).
+struct RuntimeProgs {
+    static __device__ __forceinline__ int nprog(const PartArgs& a) { return a.nprog; }
+    static __device__ __forceinline__ const ColProg& prog(const PartArgs& a, int q) { return a.prog[q]; }
+    static __device__ __forceinline__ int valid_off(const PartArgs& a) { return a.rl.valid_off; }
+    static __device__ __forceinline__ int row_off(const PartArgs& a) { return a.rl.row_off; }
 };
 
 // Folds one value of a column into the state words at `e` (an LDS entry: LDS atomics), as its
@@ -330,9 +282,8 @@ __device__ __forceinline__ void put_word(uint32_t (&w)[RW], int o, uint32_t x) {
         if (j == o) w[j] = x;
 }
 
-// RW: 32-bit words per record (the layout's `words`), so that a record is assembled in registers
-// and stored whole (16-byte stores): a record's line then leaves L2 in full sectors.
-template <int RW>
+// RW: 32-bit words per record (the layout's `words`).  PS: where the column programs come from.
+template <int RW, class PS>
 __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     extern __shared__ uint64_t lds[];
     const UpdateArgs& u = a.u;
@@ -385,7 +336,6 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     const int RPT = u.rows_per_thread;
     const int64_t T = (int64_t)PA_BLOCK * RPT;
     const int pbits = u.t.pbits;
-    const RecLayout& L = a.rl;
     // Tiles are grid-strided, so that the CUs read neighbouring addresses at any time.  The batch
     // descriptor of a tile is in LDS while the tile runs (its column pointers are read there once
     // per column and tile); the next tile's is fetched meanwhile into the other buffer.
@@ -443,8 +393,8 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             }
 #pragma unroll
             for (int q = 0; q < PART_MAX_PROG; q++) {
-                const bool on = q < a.nprog;
-                const ColProg& cp = a.prog[on ? q : 0];
+                const bool on = q < PS::nprog(a);
+                const ColProg& cp = PS::prog(a, on ? q : 0);
                 const DCol& d = b.vals[cp.col];
                 vbyte[r][q] = *gptr(on && d.valid ? d.valid + (ci >> 3) : k_all_ones);
                 const int wd = on ? cp.width : 0;
@@ -467,10 +417,10 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             const int bit = (int)(i & 7);
 #pragma unroll
             for (int q = 0; q < PART_MAX_PROG; q++) {
-                const int wd = q < a.nprog ? a.prog[q].width : 0;
+                const int wd = q < PS::nprog(a) ? PS::prog(a, q).width : 0;
                 xa[r][q] = (uint64_t)lo[r][q] | (wd >= 8 ? (uint64_t)hi[r][q] << 32 : 0ull);
                 xb[r][q] = wd >= 16 ? ((uint64_t)w2[r][q] | ((uint64_t)w3[r][q] << 32)) : 0ull;
-                if (q < a.nprog && ((vbyte[r][q] >> bit) & 1u)) vm[r] |= 1u << q;
+                if (q < PS::nprog(a) && ((vbyte[r][q] >> bit) & 1u)) vm[r] |= 1u << q;
             }
             if (r >= RPT || i >= b.n || !((sbyte[r] >> bit) & 1u)) continue;
             if (has_filter && !(((fbyte[r] >> bit) & 1u) && fval[r] < u.filter_threshold)) continue;
@@ -575,8 +525,8 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         // each column's program on the hot rows, the column's field of the records
 #pragma unroll
         for (int q = 0; q < PART_MAX_PROG; q++) {
-            if (q >= a.nprog) break;
-            const ColProg cp = a.prog[q];
+            if (q >= PS::nprog(a)) break;
+            const ColProg cp = PS::prog(a, q);
             const int o = cp.rec_off;
 #pragma unroll
             for (int r = 0; r < PA_MAX_RPT; r++) {
@@ -608,9 +558,9 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         for (int r = 0; r < PA_MAX_RPT; r++) {
             if (ent[r] != -2) continue;
             *(uint64_t*)rec[r] = kk[r];
-            rec[r][L.valid_off] = vbits[r];
-            if (L.row_off >= 0)
-                *(uint64_t*)(rec[r] + L.row_off) = (uint64_t)(b.row_base + row0 + (int64_t)r * PA_BLOCK + threadIdx.x);
+            rec[r][PS::valid_off(a)] = vbits[r];
+            if (PS::row_off(a) >= 0)
+                *(uint64_t*)(rec[r] + PS::row_off(a)) = (uint64_t)(b.row_base + row0 + (int64_t)r * PA_BLOCK + threadIdx.x);
         }
         __syncthreads();
         for (int p = threadIdx.x; p < P; p += blockDim.x) {
@@ -688,14 +638,13 @@ __device__ __forceinline__ int cache_find(uint64_t* crow, int CW, int E, uint64_
 }
 
 // Folds a record held in registers into a cache entry: COUNT(*), then each column's program.
-template <int RW>
+template <int RW, class PS>
 __device__ __forceinline__ void apply_record(const PartArgs& a, const uint32_t (&w)[RW], uint64_t* e) {
     const UpdateArgs& u = a.u;
-    const RecLayout& L = a.rl;
     atomic_add_u64(e, 1);
-    const uint32_t vb = get_word<RW>(w, L.valid_off);
-    for (int q = 0; q < a.nprog; q++) {
-        const ColProg cp = a.prog[q];
+    const uint32_t vb = get_word<RW>(w, PS::valid_off(a));
+    for (int q = 0; q < PS::nprog(a); q++) {
+        const ColProg cp = PS::prog(a, q);
         const bool v = (vb >> cp.col) & 1u;
         uint64_t x0 = 0, x1 = 0;
         const int o = cp.rec_off;
@@ -705,8 +654,8 @@ __device__ __forceinline__ void apply_record(const PartArgs& a, const uint32_t (
             if (cp.type == VR_DECIMAL128) x1 = (uint64_t)get_word<RW>(w, o + 2) | ((uint64_t)get_word<RW>(w, o + 3) << 32);
         }
         if (cp.flags & CP_GENERIC) {
-            const uint64_t gi = L.row_off >= 0
-                                    ? ((uint64_t)get_word<RW>(w, L.row_off) | ((uint64_t)get_word<RW>(w, L.row_off + 1) << 32))
+            const int ro = PS::row_off(a);
+            const uint64_t gi = ro >= 0 ? ((uint64_t)get_word<RW>(w, ro) | ((uint64_t)get_word<RW>(w, ro + 1) << 32))
                                     : 0;
             const RegSrc src{v, x0, x1, gi, cp.type};
             for (int k = 0; k < u.nops; k++)
@@ -730,7 +679,7 @@ __device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t* total) {
     return v - x;
 }
 
-template <int RW>
+template <int RW, class PS>
 __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     extern __shared__ uint64_t lds[];
     const UpdateArgs& u = a.u;
@@ -986,7 +935,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             committed = step + 1;
             if (hX) e = cache_find(crow, CW, E, h, key, 0);
         }
-        if (hX) apply_record<RW>(a, X, crow + (size_t)e * CW + 1);
+        if (hX) apply_record<RW, PS>(a, X, crow + (size_t)e * CW + 1);
         step++;
         return (flags & 2) ? 0 : 1;
     };

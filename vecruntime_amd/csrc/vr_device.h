@@ -2,8 +2,7 @@
 // multi-limb atomics and 128/192-bit decimal arithmetic.
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include "vr_types.h"
 
 namespace vr {
 

@@ -11,12 +11,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/vr.h"
-
-#define VR_MAX_KEYS 8
-#define VR_MAX_VALUES 16
-#define VR_MAX_OPS 24
-#define VR_MAX_AGGS 24
+#include "vr_types.h"
 
 namespace vr {
 
@@ -154,16 +149,6 @@ inline size_t lane_bytes(int32_t type, int64_t n) {
     return 0;
 }
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// ------------------------------------------------------------------ device-side column view
-
-struct DCol {
-    const uint8_t* valid;  // nullptr: no nulls
-    const void* data;
-    const int32_t* remap;  // dictionary-encoded UTF8: input code -> table dictionary id
-    int32_t type;
-    int32_t remap_len;     // entries of `remap` (codes outside it read as null)
-};
 
 // Host-side string dictionary of one UTF8 key column (the table's own ids, first-use order), with
 // the device copy of the last input dictionary's code -> id map (GroupKeyTable.toIds,

@@ -125,6 +125,14 @@ class Device:
         return {f: getattr(s, f) for f, _ in abi.VrStats._fields_}
 
 
+class PreparedBatches:
+    """A vr_batch_set and the ctypes/numpy arrays it points to (kept alive with it)."""
+
+    def __init__(self, bs, keep):
+        self.bs = bs
+        self.keep = keep
+
+
 class GroupedAggregate:
     """Group-by aggregation handle: GroupKeyTable.assign + GroupedAccumulators updates per batch,
     writeKeys + the aggregate buffers (or results) at emission."""
@@ -166,17 +174,25 @@ class GroupedAggregate:
         check(_lib.vr_agg_update_filtered(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), filter_col, threshold,
                                           _stream(stream)))
 
-    def update_batches(self, batches, filter_col=-1, threshold=0, config=None, stream=None):
-        """batches: list of (keys, values, n).  One persistent launch when every column is
-        device resident."""
+    @staticmethod
+    def prepare_batches(batches, filter_col=-1, threshold=0):
+        """The vr_batch_set of `batches` (list of (keys, values, n)) and the arrays it points to,
+        built once for repeated vr_agg_update_batches calls over the same columns."""
         ka = c_array([c for b in batches for c in b[0]])
         va = c_array([c for b in batches for c in b[1]])
         ln = np.asarray([b[2] for b in batches], np.int64)
         bs = abi.VrBatchSet(len(batches), C.cast(ka, C.POINTER(abi.VrColumn)), C.cast(va, C.POINTER(abi.VrColumn)),
                             ln.ctypes.data_as(C.POINTER(C.c_int64)), filter_col, threshold)
+        return PreparedBatches(bs, (ka, va, ln, batches))
+
+    def update_batches(self, batches, filter_col=-1, threshold=0, config=None, stream=None):
+        """batches: list of (keys, values, n), or a prepare_batches() result (which carries its own
+        filter).  One persistent launch when every column is device resident."""
+        if not isinstance(batches, PreparedBatches):
+            batches = self.prepare_batches(batches, filter_col, threshold)
         cfg = config if config is not None else abi.launch_config()
-        check(_lib.vr_agg_update_batches(self.h, C.byref(bs), C.byref(cfg), _stream(stream)))
-        self._keep = [ka, va, ln, batches]
+        check(_lib.vr_agg_update_batches(self.h, C.byref(batches.bs), C.byref(cfg), _stream(stream)))
+        self._keep = [batches]
 
     def set_timing(self, on):
         """Brackets every update-kernel launch with HIP events (a measurement aid)."""
