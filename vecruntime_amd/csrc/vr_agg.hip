@@ -675,7 +675,7 @@ struct AggImpl {
 
     // partitioned update: hot-set sampling, record pool and chains, merge resume points
     DevBuf p_cand, p_cand_count, p_hist, p_hot_keys, p_hot_part, p_hot_total;
-    DevBuf p_pool, p_next, p_cnt, p_heads, p_resume, p_tile_batch;
+    DevBuf p_pool, p_next, p_cnt, p_heads, p_resume, p_tile_batch, p_pending;
     int part_min_rows = 1 << 22;  // inputs from this many rows take the partitioned update
 
     vr_launch_config cfg{256, 4, 4, -1, 512, 4, 0, 0};
@@ -1452,6 +1452,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     VR_TRY(h.p_cnt.ensure(h.dev, max_blocks * sizeof(uint32_t)));
     VR_TRY(h.p_heads.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
     VR_TRY(h.p_resume.ensure(h.dev, (size_t)P * sizeof(int32_t)));
+    VR_TRY(h.p_pending.ensure(h.dev, 2 * max_blocks * sizeof(uint64_t)));
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>() + C_POOL, 2ull, 0ull);
     pa.pool = h.p_pool.as<uint32_t>();
     pa.block_next = h.p_next.as<uint32_t>();
@@ -1459,6 +1460,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.heads = h.p_heads.as<uint32_t>();
     pa.resume_step = h.p_resume.as<int32_t>();
     pa.tile_batch = h.p_tile_batch.as<uint32_t>();
+    pa.pending = h.p_pending.as<uint64_t>();
     pa.max_blocks = max_blocks;
     pa.u = a;
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
@@ -2058,7 +2060,7 @@ vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
                        h.batch_descs.bytes + h.rgroups.bytes + h.resume.bytes + h.p_cand.bytes +
                        h.p_cand_count.bytes + h.p_hist.bytes + h.p_hot_keys.bytes + h.p_hot_part.bytes +
                        h.p_hot_total.bytes + h.p_pool.bytes + h.p_next.bytes + h.p_cnt.bytes + h.p_heads.bytes +
-                       h.p_resume.bytes + h.p_tile_batch.bytes);
+                       h.p_resume.bytes + h.p_tile_batch.bytes + h.p_pending.bytes);
     return VR_OK;
 }
 

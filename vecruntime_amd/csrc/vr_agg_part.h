@@ -15,16 +15,18 @@
 //                chains heads[w][r] of all scatter workgroups w.
 //   B (merge)    One workgroup per region owns that region's slots and state rows.  It first
 //                merges the hot set's partials (reduced over the scatter workgroups) of its region,
-//                then its waves walk the chains in lockstep steps of one block each, folding
-//                records into an LDS cache of groups with LDS atomics.  When a step's record finds
-//                no room in the cache, the cache is merged into the region -- new slots claimed by
-//                CAS, state rows by plain read-modify-write, since no other workgroup touches the
-//                region -- keeping the entries that took two rows or more.
+//                then each of its waves walks its share of the chains on its own (no barrier),
+//                folding records into an LDS cache of groups with LDS atomics, and at the end the
+//                cache is merged into the region -- new slots claimed by CAS, state rows by plain
+//                read-modify-write, since no other workgroup touches the region.  A record whose
+//                key finds the cache full around it is marked in a pending map and folded by a
+//                further pass after that merge.
 //
-// Growth.  A merge first checks that the region can take every cached group as a new one; when
-// it cannot, the workgroup stops and leaves in resume_step the first step (+1; 0: the hot
-// partials too) not merged yet.  The table grows (regions double, their number is kept) and the
-// stopped regions resume there.
+// Growth.  Before a merge adds groups to its region the workgroup checks that the region can
+// take every cached group as a new one; when it cannot, the workgroup stops and leaves in
+// resume_step the pass to redo (0: the hot partials too).  The pending maps are used in turn, so
+// the stopped pass's input is intact; the table grows (regions double, their number is kept) and
+// the stopped regions resume there.
 #pragma once
 
 #include "vr_agg_kernels.h"
@@ -60,6 +62,7 @@ struct PartArgs {
     uint32_t* heads;           // [grid_a][nparts] newest block of each chain
     int32_t* resume_step;      // [nparts] merge: see above; -1 when merged
     const uint32_t* tile_batch;  // scatter: the batch of every tile
+    uint64_t* pending;         // merge: 2 x max_blocks words, the records of a block pending in a pass
     uint64_t max_blocks;
 };
 
@@ -694,10 +697,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const int wave = threadIdx.x >> 6;
     // this wave's list of blocks: (records - 1) << PB_ID_BITS | block
     uint32_t* wlist = (uint32_t*)(crow + (size_t)E * CW) + wave * PB_LIST;
-    // step flags, three in turn: bit 0: a record found no cache entry, bit 1: a wave has a block
-    // for the next step
-    __shared__ int s_flags[3];
-    __shared__ unsigned int s_count, s_inserted, s_kept, s_rg, s_new;
+    __shared__ unsigned int s_count, s_inserted, s_kept, s_rg, s_new, s_more;
     TableView tv = u.t;
     tv.rgroups = nullptr;  // inserts are counted here, once per merge
     const uint32_t limit = a.region_limit;
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         s_kept = 0;
         s_new = 0;
         s_rg = u.t.rgroups[p];  // the region is this workgroup's alone: its group count stays here
-        s_flags[0] = s_flags[1] = s_flags[2] = 0;
+        s_more = 0;
     }
     __syncthreads();
     // after a barrier that follows the inserts; read by every thread after the next barrier
@@ -722,6 +722,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             s_inserted = 0;
             s_count = 0;
             s_kept = 0;
+            s_more = 0;
         }
     };
     bool stopped = false;
@@ -750,7 +751,6 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             resume = 1;
         }
     }
-    int committed = resume;  // steps before committed - 1 are merged (and the hot partials)
 
     auto reset_cache = [&]() {
         for (int w = threadIdx.x; w < E * CW; w += blockDim.x) {
@@ -764,14 +764,17 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     // Merges every cache entry that took rows into the region; entries with two rows or more stay
     // cached (at identity) when `keep`.  Returns false, merging nothing, when the region could not
     // take every such entry as a new group.  Called by every thread.
-    auto flush = [&](bool keep) __attribute__((always_inline)) -> bool {
+    // *more: whether a record of the pass found no cache entry (read here by every thread).
+    auto flush = [&](bool keep, bool* more) __attribute__((always_inline)) -> bool {
         __syncthreads();  // every fold before the call is done
         unsigned int mine = 0;
         for (int e = threadIdx.x; e < E; e += blockDim.x)
             if (crow[(size_t)e * CW] != EMPTY_KEY && crow[(size_t)e * CW + 1] != 0) mine++;
         if (mine) atomicAdd(&s_count, mine);
         __syncthreads();
-        if (s_rg + s_count > limit) return false;  // uniform: s_rg and s_count change after the next barrier
+        *more = s_more != 0;
+        // uniform: s_rg, s_count and s_more change after the next barrier
+        if (s_rg + s_count > limit) return false;
         // this thread's PB_FE entries, side by side: every global access of one is issued with
         // the other's (the entries' slots, then their state rows)
         uint64_t fk[PB_FE], slot[PB_FE], rows[PB_FE];
@@ -850,7 +853,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     // order is the same in every launch).  `fetch` hands out the next block of the sequence.
     const int cpw = (a.grid_a - wave + NW - 1) / NW;  // chains of this wave (<= 64)
     const int cap = PB_LIST / (cpw > 0 ? cpw : 1) < PB_LANE_BLOCKS ? PB_LIST / (cpw > 0 ? cpw : 1) : PB_LANE_BLOCKS;
-    uint32_t ccur = lane < cpw ? a.heads[(size_t)(wave + NW * lane) * P + p] : NO_BLOCK;
+    uint32_t ccur = NO_BLOCK;
     int lpos = 0, lend = 0;
     auto walk = [&]() {  // wave-uniform
         uint32_t mb[PB_LANE_BLOCKS], mn[PB_LANE_BLOCKS];
@@ -885,84 +888,79 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         *n = (x >> PB_ID_BITS) + 1;
         return x & ((1u << PB_ID_BITS) - 1);
     };
-    auto load_rec = [&](uint32_t b, uint32_t n, uint32_t (&w)[RW]) -> bool {
-        if (b == NO_BLOCK || (uint32_t)lane >= n) return false;
-        const uint4* src = (const uint4*)gptr(a.pool + ((uint64_t)b * PART_BLK + lane) * (uint64_t)RW);
-#pragma unroll
-        for (int q = 0; q < RW / 4; q++) {
-            const uint4 x = src[q];
-            w[4 * q] = x.x;
-            w[4 * q + 1] = x.y;
-            w[4 * q + 2] = x.z;
-            w[4 * q + 3] = x.w;
-        }
-        return true;
-    };
 
-    // Lockstep steps: in step s every wave folds the s-th block of its sequence, lane j record j.
-    // Records are loaded two steps ahead into three buffers used in turn; each of the three
-    // unrolled steps names its buffers, so no register copy waits for a load in flight.  One
-    // barrier per step.
-    int step = 0;
+    // Passes.  Pass 0 folds every record; a record that finds no cache entry (the cache is full
+    // around its key) is marked in its block's word of the other pending map, and pass k >= 1
+    // folds the records marked by pass k - 1.  Each pass ends with one merge of the cache into
+    // the region.  The two maps are used in turn, so a pass whose merge finds the region short
+    // stops with its input intact and resumes there after the table grows.  Within a pass the
+    // waves run on their own (no barrier): each loads the records of its blocks two blocks
+    // ahead, into three buffers used in turn (each unrolled step names its buffers, so no
+    // register copy waits for a load in flight).
+    int pass = resume - 1;
     uint32_t ra[RW], rb[RW], rc[RW];
-    uint32_t ba = NO_BLOCK, bb = NO_BLOCK, bc = NO_BLOCK;
-    bool ha = false, hb = false, hc = false;
-    // folds buffer X (this step); bY: the next step's block; loads the step after into Z.
-    // Returns 0: go on, 1: no block is left, 2: stopped (the region is full).
-    auto run_step = [&](const uint32_t (&X)[RW], bool hX, uint32_t bY, uint32_t (&Z)[RW], uint32_t& bZ,
-                        bool& hZ) __attribute__((always_inline)) -> int {
-        uint32_t nz;
-        bZ = fetch(&nz);
-        hZ = load_rec(bZ, nz, Z);
-        const int fb = step % 3;
-        const uint64_t key = (uint64_t)X[0] | ((uint64_t)X[1] << 32);
-        uint64_t h = 0;
-        int e = -1;
-        if (hX) {
-            h = fmix64(key);
-            e = cache_find(crow, CW, E, h, key, PB_PROBES);
-            if (e < 0) atomicOr(&s_flags[fb], 1);
-        }
-        if (lane == 0 && bY != NO_BLOCK) atomicOr(&s_flags[fb], 2);
-        __syncthreads();
-        // the flags of step - 1 were read by every thread before this barrier; they serve
-        // step + 2, whose writers pass the next barrier after this reset
-        if (threadIdx.x == 0) s_flags[(step + 2) % 3] = 0;
-        const int flags = s_flags[fb];
-        if (flags & 1) {
-            // the cache is full: merge it (every step before this one), then place this step's rows
-            if (!flush(true)) return 2;
-            committed = step + 1;
-            if (hX) e = cache_find(crow, CW, E, h, key, 0);
-        }
-        if (hX) apply_record<RW, PS>(a, X, crow + (size_t)e * CW + 1);
-        step++;
-        return (flags & 2) ? 0 : 1;
-    };
-    if (!stopped) {
-        uint32_t n;
-        for (; step < committed - 1; step++) (void)fetch(&n);  // merged before the table grew
-        ba = fetch(&n);
-        ha = load_rec(ba, n, ra);
-        bb = fetch(&n);
-        hb = load_rec(bb, n, rb);
-    }
     while (!stopped) {
-        int r = run_step(ra, ha, bb, rc, bc, hc);
-        if (!r) r = run_step(rb, hb, bc, ra, ba, ha);
-        if (!r) r = run_step(rc, hc, ba, rb, bb, hb);
-        if (r == 2) stopped = true;
-        if (r) break;
-    }
-    if (!stopped) {
-        if (flush(false)) committed = -1;
-        else stopped = true;
+        const uint64_t* rd = pass ? a.pending + (size_t)(pass & 1) * a.max_blocks : nullptr;
+        uint64_t* wr = a.pending + (size_t)((pass + 1) & 1) * a.max_blocks;
+        ccur = lane < cpw ? a.heads[(size_t)(wave + NW * lane) * P + p] : NO_BLOCK;
+        lpos = lend = 0;
+        // this lane's record of block b (n records); true when it is pending in this pass
+        auto load = [&](uint32_t b, uint32_t n, uint32_t (&w)[RW]) -> bool {
+            if (b == NO_BLOCK || (uint32_t)lane >= n) return false;
+            const uint4* src = (const uint4*)gptr(a.pool + ((uint64_t)b * PART_BLK + lane) * (uint64_t)RW);
+#pragma unroll
+            for (int q = 0; q < RW / 4; q++) {
+                const uint4 x = src[q];
+                w[4 * q] = x.x;
+                w[4 * q + 1] = x.y;
+                w[4 * q + 2] = x.z;
+                w[4 * q + 3] = x.w;
+            }
+            return rd == nullptr || ((*gptr(rd + b) >> lane) & 1ull);
+        };
+        // folds buffer X (block bX) and marks its records that found no entry; loads the block
+        // after next into Z.  False when the wave has no block after X.
+        auto run_step = [&](const uint32_t (&X)[RW], bool hX, uint32_t bX, uint32_t bY, uint32_t (&Z)[RW],
+                            uint32_t& bZ, bool& hZ) __attribute__((always_inline)) -> bool {
+            uint32_t nz;
+            bZ = fetch(&nz);
+            hZ = load(bZ, nz, Z);
+            int e = -1;
+            if (hX) {
+                const uint64_t key = (uint64_t)X[0] | ((uint64_t)X[1] << 32);
+                e = cache_find(crow, CW, E, fmix64(key), key, PB_PROBES);
+                if (e >= 0) apply_record<RW, PS>(a, X, crow + (size_t)e * CW + 1);
+            }
+            const uint64_t missed = __ballot(hX && e < 0);
+            if (bX != NO_BLOCK && lane == 0) {
+                wr[bX] = missed;
+                if (missed) s_more = 1;
+            }
+            return bY != NO_BLOCK;
+        };
+        uint32_t ba, bb, bc = NO_BLOCK, n;
+        bool ha, hb, hc = false;
+        ba = fetch(&n);
+        ha = load(ba, n, ra);
+        bb = fetch(&n);
+        hb = load(bb, n, rb);
+        while (run_step(ra, ha, ba, bb, rc, bc, hc) && run_step(rb, hb, bb, bc, ra, ba, ha) &&
+               run_step(rc, hc, bc, ba, rb, bb, hb)) {
+        }
+        bool more;
+        if (!flush(false, &more)) {
+            stopped = true;  // this pass resumes after the table grows
+            break;
+        }
+        if (!more) break;
+        pass++;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         u.t.rgroups[p] = s_rg;
         if (s_new) atomic_add_u64(&u.t.counters[C_GROUPS], s_new);
-        a.resume_step[p] = committed;
+        // -1: merged; 0: the hot partials too; k + 1: pass k
+        a.resume_step[p] = stopped ? (resume == 0 ? 0 : pass + 1) : -1;
         if (stopped) atomicOr((unsigned long long*)&u.t.counters[C_PSTOP], 1ull);
     }
 }
