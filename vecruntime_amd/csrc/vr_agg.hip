@@ -1376,7 +1376,7 @@ static vr_status launch_jit(hipFunction_t f, unsigned grid, unsigned block, size
 }
 
 static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const RecLayout& L, hipStream_t s,
-                                 int* launches, bool specialize, bool* specialized) {
+                                 int* launches, const PartJit* jit, bool* specialized) {
     const int S = h.stride;
     // Regions small enough for the merge cache to hold all the keys a region receives (one merge
     // of the cache per region at the end), and enough of them to spread the merge over every CU.
@@ -1408,9 +1408,6 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.hot_slots = H;
     pa.grid_a = grid_a;
     pa.cache_e = E;
-
-    // kernels compiled for this spec (the first call per spec compiles them: outside the timed part)
-    const PartJit* jit = specialize ? part_jit(h.dev->ordinal, pa.prog, pa.nprog, L) : nullptr;
 
     VR_TRY(timer_begin(h, s));
     // the hot set, from a sample of the input
@@ -1466,10 +1463,11 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
     const size_t lds_b = merge_lds_bytes(E, S);
     if (jit && launch_jit(jit->scatter, (unsigned)grid_a, PA_BLOCK, lds_a, s, pa) != VR_OK) {
-        // the module's kernels do not launch here: the generic kernels from now on
+        // the module's kernels do not launch here: the generic kernels from the next call on (this
+        // call's tiles are cut for the compiled kernel's rows per thread)
         (void)hipGetLastError();
         part_jit_disable();
-        jit = nullptr;
+        return fail(VR_ERR_DEVICE, "the partitioned update's compiled kernels failed to launch");
     }
     *specialized = jit != nullptr;
     if (!jit) {
@@ -1545,8 +1543,16 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     UpdateArgs a = h.proto;
     a.filter_col = filter_col;
     a.filter_threshold = filter_threshold;
+    // kernels compiled for this spec (the first call per spec compiles them: outside the timed part)
+    const int32_t spec_mode = (cfg && cfg->specialize) ? cfg->specialize : h.cfg.specialize;
+    const PartJit* jit = nullptr;
+    if (spec_mode >= 0) {
+        ColProg prog[VR_MAX_VALUES];
+        const int np = column_programs(h, L, prog);
+        jit = part_jit(h.dev->ordinal, prog, np, L);
+    }
     int rpt = (cfg && cfg->rows_per_thread) ? cfg->rows_per_thread : h.cfg.rows_per_thread;
-    a.rows_per_thread = std::max(1, std::min(PA_MAX_RPT, rpt));
+    a.rows_per_thread = std::max(1, std::min(jit ? PA_JIT_RPT : PA_MAX_RPT, rpt));
     const int64_t T = (int64_t)PA_BLOCK * a.rows_per_thread;
     int64_t tiles = 0, seen = 0;
     for (int b = 0; b < nb; b++) {
@@ -1572,9 +1578,8 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     a.total_tiles = tiles;
     a.epoch = ++h.epoch;
     int launches = 0;
-    const int32_t spec_mode = (cfg && cfg->specialize) ? cfg->specialize : h.cfg.specialize;
     bool specialized = false;
-    VR_TRY(run_partitioned(h, a, rows, L, s, &launches, spec_mode >= 0, &specialized));
+    VR_TRY(run_partitioned(h, a, rows, L, s, &launches, jit, &specialized));
     if (h.has_f64_minmax) {  // over the final table
         a.t = view_of(h);
         hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid_for(tiles, 1, h.dev->num_cus * 4)), dim3(PA_BLOCK), 0, s, a);

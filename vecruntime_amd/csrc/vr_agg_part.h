@@ -71,6 +71,7 @@ struct PartArgs {
 // are compile-time constants -- generated per spec and compiled at run time -- lets the compiler
 // fold the programs into the per-row code (no per-row flag tests or register selects).
 struct RuntimeProgs {
+    static constexpr int max_rpt = PA_MAX_RPT;
     static __device__ __forceinline__ int nprog(const PartArgs& a) { return a.nprog; }
     static __device__ __forceinline__ const ColProg& prog(const PartArgs& a, int q) { return a.prog[q]; }
     static __device__ __forceinline__ int valid_off(const PartArgs& a) { return a.rl.valid_off; }
@@ -288,6 +289,7 @@ __device__ __forceinline__ void put_word(uint32_t (&w)[RW], int o, uint32_t x) {
 // RW: 32-bit words per record (the layout's `words`).  PS: where the column programs come from.
 template <int RW, class PS>
 __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
+    constexpr int MR = PS::max_rpt;  // rows per thread per tile at most (registers per row)
     extern __shared__ uint64_t lds[];
     const UpdateArgs& u = a.u;
     const int S = u.t.stride;
@@ -361,28 +363,28 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         const int64_t row0 = (tile - b.tile_begin) * T;
         // per row: key, and its fate -- ent >= 0: hot entry; -1: nothing (inactive or folded);
         // -2: a record of region pv at chain position vv
-        uint64_t kk[PA_MAX_RPT];
-        int32_t ent[PA_MAX_RPT];
-        uint32_t pv[PA_MAX_RPT], vv[PA_MAX_RPT];
-        bool nul[PA_MAX_RPT];
+        uint64_t kk[MR];
+        int32_t ent[MR];
+        uint32_t pv[MR], vv[MR];
+        bool nul[MR];
         // Every load of the tile first, with no branch between them (one round trip per tile): the
         // row index is clamped into the batch, a lane is read as 32-bit words at i * width (the
         // second word re-reads the first for a 4-byte lane), bitmaps the batch lacks are read from
         // k_all_ones.  Validity, selection and the filter are applied afterwards.
-        uint32_t vm[PA_MAX_RPT];
-        uint64_t xa[PA_MAX_RPT][PART_MAX_PROG], xb[PA_MAX_RPT][PART_MAX_PROG];
-        uint32_t vbyte[PA_MAX_RPT][PART_MAX_PROG];
-        uint32_t lo[PA_MAX_RPT][PART_MAX_PROG], hi[PA_MAX_RPT][PART_MAX_PROG];
-        uint32_t w2[PA_MAX_RPT][PART_MAX_PROG], w3[PA_MAX_RPT][PART_MAX_PROG];
-        uint32_t sbyte[PA_MAX_RPT], fbyte[PA_MAX_RPT], kbyte[PA_MAX_RPT];
-        int32_t fval[PA_MAX_RPT];
-        uint32_t klo[PA_MAX_RPT], khi[PA_MAX_RPT];
+        uint32_t vm[MR];
+        uint64_t xa[MR][PART_MAX_PROG], xb[MR][PART_MAX_PROG];
+        uint32_t vbyte[MR][PART_MAX_PROG];
+        uint32_t lo[MR][PART_MAX_PROG], hi[MR][PART_MAX_PROG];
+        uint32_t w2[MR][PART_MAX_PROG], w3[MR][PART_MAX_PROG];
+        uint32_t sbyte[MR], fbyte[MR], kbyte[MR];
+        int32_t fval[MR];
+        uint32_t klo[MR], khi[MR];
         const bool fast_key = u.ks.mode == KM_I64;
         const bool has_filter = u.filter_col >= 0;
         const DCol& fcol = b.vals[has_filter ? u.filter_col : 0];
         const DCol& kcol = b.keys[0];
 #pragma unroll
-        for (int r = 0; r < PA_MAX_RPT; r++) {
+        for (int r = 0; r < MR; r++) {
             const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
             const int64_t ci = i < b.n ? i : b.n - 1;
             sbyte[r] = *gptr(b.sel ? b.sel + (ci >> 3) : k_all_ones);
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             }
         }
 #pragma unroll
-        for (int r = 0; r < PA_MAX_RPT; r++) {
+        for (int r = 0; r < MR; r++) {
             ent[r] = -1;
             kk[r] = 0;
             nul[r] = false;
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             ent[r] = -3;  // active, not classified yet
         }
 #pragma unroll
-        for (int r = 0; r < PA_MAX_RPT; r++) {
+        for (int r = 0; r < MR; r++) {
             if (ent[r] != -3) continue;
             const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
             const uint64_t key = kk[r];
@@ -503,10 +505,10 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         }
         __syncthreads();
         // record addresses of the cold rows
-        uint32_t* rec[PA_MAX_RPT];
-        uint32_t vbits[PA_MAX_RPT];
+        uint32_t* rec[MR];
+        uint32_t vbits[MR];
 #pragma unroll
-        for (int r = 0; r < PA_MAX_RPT; r++) {
+        for (int r = 0; r < MR; r++) {
             rec[r] = nullptr;
             vbits[r] = 0;
             if (ent[r] != -2) continue;
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         }
         // COUNT(*): word 0 of every state row
 #pragma unroll
-        for (int r = 0; r < PA_MAX_RPT; r++)
+        for (int r = 0; r < MR; r++)
             if (ent[r] >= 0) atomic_add_u64(hrow + (size_t)ent[r] * HW + 1, 1);
         // each column's program on the hot rows, the column's field of the records
 #pragma unroll
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             const ColProg cp = PS::prog(a, q);
             const int o = cp.rec_off;
 #pragma unroll
-            for (int r = 0; r < PA_MAX_RPT; r++) {
+            for (int r = 0; r < MR; r++) {
                 const bool v = (vm[r] >> q) & 1u;
                 if (ent[r] >= 0) {
                     uint64_t* e = hrow + (size_t)ent[r] * HW + 1;
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             }
         }
 #pragma unroll
-        for (int r = 0; r < PA_MAX_RPT; r++) {
+        for (int r = 0; r < MR; r++) {
             if (ent[r] != -2) continue;
             *(uint64_t*)rec[r] = kk[r];
             rec[r][PS::valid_off(a)] = vbits[r];

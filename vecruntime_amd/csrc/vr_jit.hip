@@ -42,6 +42,7 @@ std::string jit_source(const ColProg* prog, int nprog, const RecLayout& L) {
         s += "    " + prog_literal(q < nprog ? prog[q] : none) + (q + 1 < PART_MAX_PROG ? ",\n" : "};\n");
     }
     s += "struct JitProgs {\n";
+    s += "    static constexpr int max_rpt = PA_JIT_RPT;\n";
     s += "    static __device__ __forceinline__ int nprog(const PartArgs&) { return " + std::to_string(nprog) + "; }\n";
     s += "    static __device__ __forceinline__ const ColProg& prog(const PartArgs&, int q) { return kJitProg[q]; }\n";
     s += "    static __device__ __forceinline__ int valid_off(const PartArgs&) { return " + std::to_string(L.valid_off) +

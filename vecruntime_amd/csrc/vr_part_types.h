@@ -10,7 +10,8 @@ constexpr int PART_BLK = 64;         // records per block: one per lane of a wav
 constexpr int PART_REC_WORDS = 16;  // 32-bit words per record at most (a 64-byte record)
 constexpr uint32_t NO_BLOCK = 0xFFFFFFFFu;
 constexpr int PA_BLOCK = 1024;       // scatter workgroup
-constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile
+constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile (generic kernel)
+constexpr int PA_JIT_RPT = 4;        // the same, kernels compiled for their spec (fewer registers per row)
 constexpr int PART_MAX_PROG = 4;     // value columns the partitioned update folds (programs)
 constexpr int PA_CHUNK = 256;        // pool blocks a scatter workgroup takes at a time
 constexpr int PB_BLOCK = 1024;       // merge workgroup
