@@ -345,3 +345,19 @@ def test_few_groups_all_hot(dev):
     o = OracleAgg(spec)
     o.update([Column.longs(keys)], [Column.longs(i64)], n)
     check(g, o, spec)
+
+
+def test_large_regions_claim_by_cas(dev):
+    """A table far larger than the input (4M-group hint: 8M slots in few regions, each above the
+    merge's LDS claim bits) makes the merge claim its new slots with device CAS."""
+    rng = np.random.default_rng(131)
+    n = 600_000
+    data = cfg2_data(rng, n, 200_000, 0.8)
+    spec = cfg2_spec(capacity_hint=4_000_000)
+    batches, keep = device_batches(*data, 65536)
+    g = make(dev, spec, FORCE)
+    with Took(dev) as t:
+        g.update_batches(batches, config=abi.launch_config(**FORCE))
+    assert t.count == 1
+    o = oracle_cfg2(spec, *data)
+    check(g, o, spec)

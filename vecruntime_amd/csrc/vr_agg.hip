@@ -1339,7 +1339,7 @@ static int column_programs(const AggImpl& h, const RecLayout& L, ColProg* prog) 
 
 // LDS cache entries of the merge (0: the state rows are too wide for it).
 static int merge_cache_entries(const AggImpl& h) {
-    const int64_t budget = h.dev->lds_per_block - 1024;
+    const int64_t budget = h.dev->lds_per_block - 2048;  // the kernel's static LDS (counters, claim bits)
     const int E = PB_FE * PB_BLOCK;
     if ((int64_t)merge_lds_bytes(E, h.stride) <= budget) return E;
     return 0;

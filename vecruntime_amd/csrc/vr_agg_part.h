@@ -699,7 +699,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const int wave = threadIdx.x >> 6;
     // this wave's list of blocks: (records - 1) << PB_ID_BITS | block
     uint32_t* wlist = (uint32_t*)(crow + (size_t)E * CW) + wave * PB_LIST;
-    __shared__ unsigned int s_count, s_inserted, s_kept, s_rg, s_new, s_more;
+    __shared__ unsigned int s_count, s_inserted, s_rg, s_new, s_more;
     TableView tv = u.t;
     tv.rgroups = nullptr;  // inserts are counted here, once per merge
     const uint32_t limit = a.region_limit;
@@ -710,7 +710,6 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     if (threadIdx.x == 0) {
         s_count = 0;
         s_inserted = 0;
-        s_kept = 0;
         s_new = 0;
         s_rg = u.t.rgroups[p];  // the region is this workgroup's alone: its group count stays here
         s_more = 0;
@@ -723,7 +722,6 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             s_new += s_inserted;
             s_inserted = 0;
             s_count = 0;
-            s_kept = 0;
             s_more = 0;
         }
     };
@@ -763,12 +761,19 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     reset_cache();
     __syncthreads();
 
-    // Merges every cache entry that took rows into the region; entries with two rows or more stay
-    // cached (at identity) when `keep`.  Returns false, merging nothing, when the region could not
-    // take every such entry as a new group.  Called by every thread.
+    // The new slots this merge claims: the region is this workgroup's alone, so an empty cell it
+    // reads can only be taken by another of its threads, and a bit per slot in LDS settles which
+    // (no device atomic).  A later pass of the workgroup reads the cells after barriers, which
+    // order its own stores.  Regions of more than PB_CLAIM_BITS slots claim by CAS instead.
+    __shared__ uint32_t s_claim[PB_CLAIM_BITS / 32];
+    const bool lds_claim = tv.rmask < (uint64_t)PB_CLAIM_BITS;
+
+    // Merges every cache entry that took rows into the region.  Returns false, merging nothing,
+    // when the region could not take every such entry as a new group.  Called by every thread.
     // *more: whether a record of the pass found no cache entry (read here by every thread).
-    auto flush = [&](bool keep, bool* more) __attribute__((always_inline)) -> bool {
+    auto flush = [&](bool* more) __attribute__((always_inline)) -> bool {
         __syncthreads();  // every fold before the call is done
+        for (int w = threadIdx.x; w < PB_CLAIM_BITS / 32; w += blockDim.x) s_claim[w] = 0;
         unsigned int mine = 0;
         for (int e = threadIdx.x; e < E; e += blockDim.x)
             if (crow[(size_t)e * CW] != EMPTY_KEY && crow[(size_t)e * CW + 1] != 0) mine++;
@@ -793,19 +798,29 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
 #pragma unroll
         for (int j = 0; j < PB_FE; j++) cell[j] = tv.cells[slot[j]];
         unsigned int ins = 0;
+        bool fresh[PB_FE];  // a new group: its state row is at identity, no need to read it
 #pragma unroll
         for (int j = 0; j < PB_FE; j++) {
+            fresh[j] = false;
             if (!rows[j]) continue;
-            uint64_t probes = 0;
-            for (;; probes++) {  // linear probing inside the region, as find_or_insert64
+            for (uint64_t probes = 0;; probes++) {  // linear probing inside the region, as find_or_insert64
                 if (cell[j] == fk[j]) break;
                 if (cell[j] == EMPTY_KEY) {
-                    const uint64_t old = atomic_cas_u64(&tv.cells[slot[j]], EMPTY_KEY, fk[j]);
-                    if (old == EMPTY_KEY) {
+                    bool won;
+                    if (lds_claim) {
+                        const uint32_t r = (uint32_t)(slot[j] & tv.rmask), bit = 1u << (r & 31);
+                        won = (atomicOr(&s_claim[r >> 5], bit) & bit) == 0;
+                        if (won) tv.cells[slot[j]] = fk[j];
+                    } else {
+                        const uint64_t old = atomic_cas_u64(&tv.cells[slot[j]], EMPTY_KEY, fk[j]);
+                        won = old == EMPTY_KEY;
+                        if (old == fk[j]) break;
+                    }
+                    if (won) {
                         ins++;
+                        fresh[j] = true;
                         break;
                     }
-                    if (old == fk[j]) break;
                 }
                 if (probes == tv.rmask) {
                     slot[j] = region_full(tv);
@@ -822,7 +837,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             for (int j = 0; j < PB_FE; j++) {
                 const uint64_t* g = u.t.state + slot[j] * (uint64_t)S;
 #pragma unroll
-                for (int q = 0; q < 8; q++) y[j][q] = g[k0 + q < S ? k0 + q : S - 1];  // in the row, unconditional
+                for (int q = 0; q < 8; q++) y[j][q] = fresh[j] ? 0 : g[k0 + q < S ? k0 + q : S - 1];
             }
 #pragma unroll
             for (int j = 0; j < PB_FE; j++) {
@@ -833,20 +848,15 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
                     const int k = k0 + q;
                     if (k >= S) break;
                     const uint64_t x = l[j][k];
-                    if (x != u.ident[k]) g[k] = combine_word(u.comb[k], y[j][q], x);
+                    if (fresh[j]) g[k] = x;  // identity combined with x
+                    else if (x != u.ident[k]) g[k] = combine_word(u.comb[k], y[j][q], x);
                 }
             }
         }
-        bool kept[PB_FE];
-#pragma unroll
-        for (int j = 0; j < PB_FE; j++) kept[j] = keep && rows[j] >= 2 && atomicAdd(&s_kept, 1u) < (unsigned)(E / 4);
         __syncthreads();
         add_inserted();
         reset_cache();
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PB_FE; j++)
-            if (kept[j]) (void)cache_find(crow, CW, E, fmix64(fk[j]), fk[j], 0);
         return true;
     };
 
@@ -930,7 +940,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             int e = -1;
             if (hX) {
                 const uint64_t key = (uint64_t)X[0] | ((uint64_t)X[1] << 32);
-                e = cache_find(crow, CW, E, fmix64(key), key, PB_PROBES);
+                e = cache_find(crow, CW, E, fmix64(key), key, E / CB);  // every bucket if need be
                 if (e >= 0) apply_record<RW, PS>(a, X, crow + (size_t)e * CW + 1);
             }
             const uint64_t missed = __ballot(hX && e < 0);
@@ -950,7 +960,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
                run_step(rc, hc, bc, ba, rb, bb, hb)) {
         }
         bool more;
-        if (!flush(false, &more)) {
+        if (!flush(&more)) {
             stopped = true;  // this pass resumes after the table grows
             break;
         }
