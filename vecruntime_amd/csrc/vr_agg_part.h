@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void hot_reduce_kernel(PartArgs a) {
 
 // LDS bytes of the merge: the cache (E keys and state rows) and the waves' block lists.
 __host__ __device__ inline size_t merge_lds_bytes(int E, int stride) {
-    return (size_t)E * lds_row_words(stride) * 8 + (size_t)(PB_BLOCK / 64) * PB_LIST * 4;
+    return (size_t)E * lds_row_words(stride) * 8 + (size_t)PB_GLIST * 4;
 }
 
 // Entry of `key` in the LDS cache, claiming an empty one; -1 when neither is found within `limit`
@@ -671,19 +671,6 @@ __device__ __forceinline__ void apply_record(const PartArgs& a, const uint32_t (
     }
 }
 
-// Exclusive prefix sum of x over the lanes of a wave; *total: the sum over the wave.
-__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t* total) {
-    const int lane = threadIdx.x & 63;
-    uint32_t v = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(v, d, 64);
-        if (lane >= d) v += y;
-    }
-    *total = __shfl(v, 63, 64);
-    return v - x;
-}
-
 template <int RW, class PS>
 __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     extern __shared__ uint64_t lds[];
@@ -694,11 +681,9 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const uint32_t p = blockIdx.x;
     const int CW = lds_row_words(S);
     uint64_t* crow = lds;  // cache entry e: row e (key, state words)
-    constexpr int NW = PB_BLOCK / 64;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    // this wave's list of blocks: (records - 1) << PB_ID_BITS | block
-    uint32_t* wlist = (uint32_t*)(crow + (size_t)E * CW) + wave * PB_LIST;
+    // the workgroup's list of blocks: (records - 1) << PB_ID_BITS | block
+    uint32_t* glist = (uint32_t*)(crow + (size_t)E * CW);
     __shared__ unsigned int s_count, s_inserted, s_rg, s_new, s_more;
     TableView tv = u.t;
     tv.rgroups = nullptr;  // inserts are counted here, once per merge
@@ -860,62 +845,56 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         return true;
     };
 
-    // The blocks of this wave: lanes take the chains wave + NW * lane (< grid_a) and walk them
-    // together, at most `cap` blocks each per walk, into the wave's list (lane-major, so the
-    // order is the same in every launch).  `fetch` hands out the next block of the sequence.
-    const int cpw = (a.grid_a - wave + NW - 1) / NW;  // chains of this wave (<= 64)
-    const int cap = PB_LIST / (cpw > 0 ? cpw : 1) < PB_LANE_BLOCKS ? PB_LIST / (cpw > 0 ? cpw : 1) : PB_LANE_BLOCKS;
+    // The region's blocks, handed out to the waves as they ask (the waves' work evens out whatever
+    // the chains' lengths).  A round: the threads owning a chain (thread c: chain c < grid_a) walk
+    // it, at most `cap` blocks each, into the workgroup's block list; after a barrier the waves
+    // take blocks from the list with an LDS counter; rounds repeat until every chain is walked.
+    __shared__ unsigned int s_gcount, s_gnext, s_walk;
+    const int cap = PB_GLIST / a.grid_a < PB_LANE_BLOCKS ? PB_GLIST / a.grid_a : PB_LANE_BLOCKS;
     uint32_t ccur = NO_BLOCK;
-    int lpos = 0, lend = 0;
-    auto walk = [&]() {  // wave-uniform
-        uint32_t mb[PB_LANE_BLOCKS], mn[PB_LANE_BLOCKS];
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int t = 0; t < PB_LANE_BLOCKS; t++) {
-            mb[t] = 0;
-            mn[t] = 0;
-            if (t < cap && ccur != NO_BLOCK) {
-                mb[t] = ccur;
-                mn[t] = a.block_cnt[ccur];
-                ccur = a.block_next[ccur];
-                cnt = t + 1;
-            }
+    auto walk = [&]() {
+        for (int t = 0; t < cap && ccur != NO_BLOCK; t++) {
+            const uint32_t b = ccur;
+            const uint32_t n = a.block_cnt[b];
+            ccur = a.block_next[b];
+            glist[atomicAdd(&s_gcount, 1u)] = ((n - 1) << PB_ID_BITS) | b;
         }
-        uint32_t total;
-        const uint32_t off = wave_prefix(cnt, &total);
-#pragma unroll
-        for (int t = 0; t < PB_LANE_BLOCKS; t++)
-            if (t < (int)cnt) wlist[off + t] = ((mn[t] - 1) << PB_ID_BITS) | mb[t];
-        lpos = 0;
-        lend = (int)total;
+        if (ccur != NO_BLOCK) s_walk = 1;
     };
-    // the next block of the sequence (NO_BLOCK: none) and its record count
+    // the next block of the list for this wave (NO_BLOCK: none) and its record count
     auto fetch = [&](uint32_t* n) -> uint32_t {
-        if (lpos == lend && __any(ccur != NO_BLOCK)) walk();
-        if (lpos == lend) {
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(&s_gnext, 1u);
+        i = __shfl(i, 0, 64);
+        if (i >= s_gcount) {
             *n = 0;
             return NO_BLOCK;
         }
-        const uint32_t x = wlist[lpos++];
+        const uint32_t x = glist[i];
         *n = (x >> PB_ID_BITS) + 1;
         return x & ((1u << PB_ID_BITS) - 1);
     };
+    if (threadIdx.x == 0) {
+        s_gcount = 0;
+        s_gnext = 0;
+        s_walk = 0;
+    }
+    __syncthreads();
 
-    // Passes.  Pass 0 folds every record; a record that finds no cache entry (the cache is full
-    // around its key) is marked in its block's word of the other pending map, and pass k >= 1
-    // folds the records marked by pass k - 1.  Each pass ends with one merge of the cache into
-    // the region.  The two maps are used in turn, so a pass whose merge finds the region short
-    // stops with its input intact and resumes there after the table grows.  Within a pass the
-    // waves run on their own (no barrier): each loads the records of its blocks two blocks
-    // ahead, into three buffers used in turn (each unrolled step names its buffers, so no
-    // register copy waits for a load in flight).
+    // Passes.  Pass 0 folds every record; a record that finds no cache entry (the cache is full)
+    // is marked in its block's word of the other pending map, and pass k >= 1 folds the records
+    // marked by pass k - 1.  Each pass ends with one merge of the cache into the region.  The two
+    // maps are used in turn, so a pass whose merge finds the region short stops with its input
+    // intact and resumes there after the table grows.  Within a round the waves run on their own
+    // (no barrier): each loads the records of its blocks two blocks ahead, into three buffers
+    // used in turn (each unrolled step names its buffers, so no register copy waits for a load in
+    // flight).
     int pass = resume - 1;
     uint32_t ra[RW], rb[RW], rc[RW];
     while (!stopped) {
         const uint64_t* rd = pass ? a.pending + (size_t)(pass & 1) * a.max_blocks : nullptr;
         uint64_t* wr = a.pending + (size_t)((pass + 1) & 1) * a.max_blocks;
-        ccur = lane < cpw ? a.heads[(size_t)(wave + NW * lane) * P + p] : NO_BLOCK;
-        lpos = lend = 0;
+        ccur = threadIdx.x < a.grid_a ? a.heads[(size_t)threadIdx.x * P + p] : NO_BLOCK;
         // this lane's record of block b (n records); true when it is pending in this pass
         auto load = [&](uint32_t b, uint32_t n, uint32_t (&w)[RW]) -> bool {
             if (b == NO_BLOCK || (uint32_t)lane >= n) return false;
@@ -950,14 +929,29 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             }
             return bY != NO_BLOCK;
         };
-        uint32_t ba, bb, bc = NO_BLOCK, n;
-        bool ha, hb, hc = false;
-        ba = fetch(&n);
-        ha = load(ba, n, ra);
-        bb = fetch(&n);
-        hb = load(bb, n, rb);
-        while (run_step(ra, ha, ba, bb, rc, bc, hc) && run_step(rb, hb, bb, bc, ra, ba, ha) &&
-               run_step(rc, hc, bc, ba, rb, bb, hb)) {
+        while (true) {  // rounds
+            walk();
+            __syncthreads();
+            uint32_t ba, bb, bc = NO_BLOCK, n;
+            bool ha, hb, hc = false;
+            ba = fetch(&n);
+            ha = load(ba, n, ra);
+            bb = fetch(&n);
+            hb = load(bb, n, rb);
+            if (ba != NO_BLOCK)
+                while (run_step(ra, ha, ba, bb, rc, bc, hc) && run_step(rb, hb, bb, bc, ra, ba, ha) &&
+                       run_step(rc, hc, bc, ba, rb, bb, hb)) {
+                }
+            __syncthreads();
+            const bool again = s_walk != 0;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                s_gcount = 0;
+                s_gnext = 0;
+                s_walk = 0;
+            }
+            __syncthreads();
+            if (!again) break;
         }
         bool more;
         if (!flush(&more)) {

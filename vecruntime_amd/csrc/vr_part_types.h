@@ -19,10 +19,10 @@ constexpr int PART_MAX_PBITS = 12;   // regions of the partitioned update: at mo
 constexpr int HS_BLOCK = 256;        // sampling workgroup
 constexpr int HS_ROWS = 2048;        // rows per sample
 constexpr int HS_SLOTS = 4096;       // LDS counting slots of a sample
-constexpr int PB_LIST = 128;         // block list entries per merge wave
+constexpr int PB_GLIST = 2048;       // block list entries of a merge workgroup
 constexpr int PB_CLAIM_BITS = 8192;  // merge: regions of at most this many slots claim new slots in LDS
 constexpr int PB_FE = 2;             // merge cache entries per thread (the cache holds PB_FE * PB_BLOCK)
-constexpr int PB_LANE_BLOCKS = 8;    // blocks a lane takes from its chain per walk
+constexpr int PB_LANE_BLOCKS = 8;    // blocks a thread takes from its chain per walk
 constexpr int PB_ID_BITS = 26;       // block number bits of a list entry (the record count above)
 constexpr uint32_t HOT_EMPTY = 0xFFFFFFFFu;
 
