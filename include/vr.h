@@ -178,7 +178,8 @@ typedef struct vr_agg_desc {
 } vr_agg_desc;
 
 typedef struct vr_agg_spec {
-    int32_t num_keys;
+    int32_t num_keys;             /* 0: ungrouped aggregation, one group (one output row even for no
+                                     input: AggKernels' reductions); keys are then passed as NULL */
     const int32_t* key_types;     /* vr_type per key; UTF8 keys must be dictionary encoded */
     const uint8_t* key_nullable;  /* may be NULL: every key nullable */
     int32_t num_values;

@@ -634,6 +634,9 @@ struct vro_agg {
     std::vector<Acc> accs;
     std::vector<int32_t> valueTypes;
     std::vector<int32_t> ids;
+    // No grouping key: the ungrouped reductions of AggKernels (AggKernels.java:133-621) fold every
+    // selected row into one group, and the operator emits that one row even for an empty input.
+    bool ungrouped = false;
 };
 
 struct vro_join {
@@ -651,12 +654,14 @@ const char* vro_last_error(void) { return g_error.c_str(); }
 int vro_agg_create(const vr_agg_spec* spec, vro_agg** out) {
     if (!spec || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     auto* h = new vro_agg();
+    h->ungrouped = spec->num_keys == 0;
     h->table.reset(new GroupKeyTable(std::vector<int32_t>(spec->key_types, spec->key_types + spec->num_keys)));
     h->valueTypes.assign(spec->value_types, spec->value_types + spec->num_values);
     for (int a = 0; a < spec->num_aggs; a++) {
         Acc acc;
         acc.d = spec->aggs[a];
         acc.inType = acc.d.input >= 0 ? h->valueTypes[acc.d.input] : -1;
+        if (h->ungrouped) acc.ensure(1);
         h->accs.push_back(std::move(acc));
     }
     *out = h;
@@ -668,7 +673,12 @@ int vro_agg_create(const vr_agg_spec* spec, vro_agg** out) {
 int vro_agg_update(vro_agg* h, const vr_column* keys, const vr_column* values, int64_t n,
                    const uint8_t* selection, int32_t* out_ids) {
     h->ids.resize(n);
-    int64_t groups = h->table->assign(keys, n, h->ids.data(), selection);
+    int64_t groups = 1;
+    if (h->ungrouped) {
+        for (int64_t i = 0; i < n; i++) h->ids[i] = (!selection || ((selection[i >> 3] >> (i & 7)) & 1)) ? 0 : -1;
+    } else {
+        groups = h->table->assign(keys, n, h->ids.data(), selection);
+    }
     for (auto& a : h->accs) {
         a.ensure(groups);
         int st = a.update(values, h->ids.data(), n);
@@ -678,7 +688,7 @@ int vro_agg_update(vro_agg* h, const vr_column* keys, const vr_column* values, i
     return VR_OK;
 }
 
-int64_t vro_agg_num_groups(vro_agg* h) { return h->table->size(); }
+int64_t vro_agg_num_groups(vro_agg* h) { return h->ungrouped ? 1 : h->table->size(); }
 
 int vro_agg_output_layout(vro_agg* h, int32_t* types, int32_t max) {
     int k = 0;

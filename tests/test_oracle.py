@@ -374,3 +374,65 @@ def test_planted_mistake_dropped_carry_is_caught():
         l0, h0 = nocarry.get(k, (0, 0))
         nocarry[k] = ((l0 + lo) & ((1 << 64) - 1), h0 + hi)  # carry dropped
     assert any(((h << 64) | l) != exact[k] for k, (l, h) in nocarry.items())
+
+
+UNGROUPED_AGGS = [abi.agg_desc(abi.COUNT_STAR), abi.agg_desc(abi.COUNT, 0), abi.agg_desc(abi.SUM_LONG, 0),
+                  abi.agg_desc(abi.SUM_DOUBLE, 1), abi.agg_desc(abi.AVG_DOUBLE, 1), abi.agg_desc(abi.MIN, 2),
+                  abi.agg_desc(abi.MAX, 2), abi.agg_desc(abi.SUM_WIDE, 0, buffer_precision=22, buffer_scale=2)]
+
+
+def ungrouped_input(rng, n):
+    v = rng.integers(-10 ** 9, 10 ** 9, n)
+    d = rng.standard_normal(n)
+    m = rng.integers(-2 ** 31, 2 ** 31, n).astype(np.int32)
+    nulls = [rng.random(n) < 0.1 for _ in range(3)]
+    return [Column.longs(v, nulls=nulls[0]), Column.doubles(d, nulls=nulls[1]), Column.ints(m, nulls=nulls[2])], (v, d, m, nulls)
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 1000, 4097])
+def test_ungrouped_equals_one_constant_key_group(n):
+    """AggKernels' ungrouped reductions (AggKernels.java:133-621) are the grouped accumulators of a
+    single group: the oracle without keys must emit the one row the grouped oracle gives for a
+    constant key, with selection, over several batches."""
+    rng = np.random.default_rng(n + 5)
+    types = [abi.INT64, abi.FLOAT64, abi.INT32]
+    ug = OracleAgg(abi.AggSpec([], types, UNGROUPED_AGGS))
+    gr = OracleAgg(abi.AggSpec([abi.INT64], types, UNGROUPED_AGGS, key_nullable=[False]))
+    for part in range(3):
+        cols, _ = ungrouped_input(rng, n)
+        sel = pack_bits(rng.random(n) < 0.8)
+        ids = ug.update([], cols, n, sel)
+        assert set(ids.tolist()) <= {0, -1}
+        gr.update([Column.longs(np.full(n, 42))], cols, n, sel)
+    assert ug.num_groups() == 1 and gr.num_groups() == 1
+    _, uo = ug.emit()
+    _, go = gr.emit()
+    for a, b in zip(uo, go):
+        assert a.values(1) == b.values(1)
+
+
+def test_ungrouped_closed_forms_and_empty_input():
+    """COUNT(*) = the selected rows, COUNT(x) = the non-null ones, SUM = the Python integer total,
+    MIN/MAX = numpy's over the non-null values; with no input the one row is COUNT 0 and nulls
+    (Spark's ungrouped aggregate over an empty relation)."""
+    rng = np.random.default_rng(77)
+    n = 3000
+    cols, (v, d, m, nulls) = ungrouped_input(rng, n)
+    spec = abi.AggSpec([], [abi.INT64, abi.FLOAT64, abi.INT32], UNGROUPED_AGGS)
+    ug = OracleAgg(spec)
+    ug.update([], cols, n)
+    _, out = ug.emit()
+    # columns: COUNT(*), COUNT, SUM, SUM double, AVG (sum, count), MIN, MAX, wide SUM (sum, isEmpty)
+    by = dict(zip(range(len(out)), out))
+    assert by[0].values(1) == [n]
+    assert by[1].values(1) == [int((~nulls[0]).sum())]
+    assert by[2].values(1) == [sum(int(x) for x in v[~nulls[0]])]
+    assert by[5].values(1) == [int((~nulls[1]).sum())]
+    assert by[6].values(1) == [int(m[~nulls[2]].min())] and by[7].values(1) == [int(m[~nulls[2]].max())]
+    assert math.isclose(by[3].values(1)[0], math.fsum(d[~nulls[1]]), rel_tol=1e-12)
+    empty = OracleAgg(spec)
+    assert empty.num_groups() == 1
+    _, eo = empty.emit()
+    assert eo[0].values(1) == [0] and eo[1].values(1) == [0]
+    assert eo[2].values(1) == [None] and eo[3].values(1) == [None]
+    assert eo[6].values(1) == [None] and eo[7].values(1) == [None]

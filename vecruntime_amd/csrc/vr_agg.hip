@@ -211,6 +211,14 @@ __global__ void reset_rows_kernel(const uint64_t* order, int64_t m, uint64_t* ce
     }
 }
 
+// Inserts `key` into a table of 64-bit keys (its state row is at identity already).
+__global__ void insert_key_kernel(TableView t, uint64_t key) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    bool ins;
+    (void)find_or_insert64(t, key, &ins);
+    if (ins) atomic_add_u64(&t.counters[C_GROUPS], 1);
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -676,6 +684,12 @@ struct AggImpl {
     // partitioned update: hot-set sampling, record pool and chains, merge resume points
     DevBuf p_cand, p_cand_count, p_hist, p_hot_keys, p_hot_part, p_hot_total;
     DevBuf p_pool, p_next, p_cnt, p_heads, p_resume, p_tile_batch, p_pending;
+    // Ungrouped aggregation (no grouping key: AggKernels' reductions): one group, keyed by a
+    // constant INT64 column of zeros the library supplies, present from creation on so that an
+    // empty input still emits its one row.
+    bool ungrouped = false;
+    DevBuf zero_keys;
+    int64_t zero_rows = 0;
     int part_min_rows = 1 << 22;  // inputs from this many rows take the partitioned update
 
     vr_launch_config cfg{256, 4, 4, -1, 512, 4, 0, 0};
@@ -1033,6 +1047,32 @@ static vr_status init_table(AggImpl& h, int64_t capacity_hint) {
     VR_TRY_HIP(hipGetLastError());
     VR_TRY_HIP(hipStreamSynchronize(s));
     h.slots = want;
+    return VR_OK;
+}
+
+// The ungrouped handle's one group (after creation and after each reset).
+static vr_status insert_single_group(AggImpl& h, hipStream_t s) {
+    hipLaunchKernelGGL(insert_key_kernel, dim3(1), dim3(64), 0, s, view_of(h), 0ull);
+    VR_TRY_HIP(hipGetLastError());
+    h.groups_upper = 1;
+    return VR_OK;
+}
+
+// The ungrouped handle's key column for n rows: zeros on the device.
+static vr_status zero_key_column(AggImpl& h, int64_t n, hipStream_t s, vr_column* col) {
+    if (h.zero_rows < n) {
+        const int64_t want = std::max<int64_t>(n, 65536);
+        VR_TRY(h.zero_keys.ensure(h.dev, (size_t)want * sizeof(int64_t)));
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for((uint64_t)want, 256, h.dev->num_cus * 8)), dim3(256), 0, s,
+                           h.zero_keys.as<uint64_t>(), (uint64_t)want, 0ull);
+        VR_TRY_HIP(hipGetLastError());
+        h.zero_rows = want;
+    }
+    std::memset(col, 0, sizeof(*col));
+    col->type = VR_INT64;
+    col->mem = VR_MEM_DEVICE;
+    col->length = n;
+    col->data = h.zero_keys.p;
     return VR_OK;
 }
 
@@ -1749,10 +1789,17 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     AggImpl& h = a->h;
     h.dev = &dev->d;
     VR_TRY_HIP(hipSetDevice(h.dev->ordinal));
-    h.key_types.assign(spec->key_types, spec->key_types + spec->num_keys);
-    h.key_nullable.assign(spec->num_keys, 1);
-    if (spec->key_nullable)
-        for (int c = 0; c < spec->num_keys; c++) h.key_nullable[c] = spec->key_nullable[c] ? 1 : 0;
+    if (spec->num_keys < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative key count");
+    h.ungrouped = spec->num_keys == 0;
+    if (h.ungrouped) {  // one group: a constant key the library supplies
+        h.key_types.assign(1, VR_INT64);
+        h.key_nullable.assign(1, 0);
+    } else {
+        h.key_types.assign(spec->key_types, spec->key_types + spec->num_keys);
+        h.key_nullable.assign(spec->num_keys, 1);
+        if (spec->key_nullable)
+            for (int c = 0; c < spec->num_keys; c++) h.key_nullable[c] = spec->key_nullable[c] ? 1 : 0;
+    }
     if (spec->num_values > 0) h.value_types.assign(spec->value_types, spec->value_types + spec->num_values);
     if (spec->num_aggs > 0) h.aggs.assign(spec->aggs, spec->aggs + spec->num_aggs);
     h.det = spec->deterministic != 0;
@@ -1761,20 +1808,29 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     if (spec->max_load_percent) h.max_load = spec->max_load_percent / 100.0;
     VR_TRY(compile_spec(h));
     h.dicts.resize(h.key_types.size());
-    VR_TRY(init_table(h, spec->capacity_hint));
+    VR_TRY(init_table(h, h.ungrouped ? 1 : spec->capacity_hint));
+    if (h.ungrouped) {
+        VR_TRY(insert_single_group(h, h.dev->stream));
+        VR_TRY_HIP(hipStreamSynchronize(h.dev->stream));
+    }
     *out = a.release();
     return VR_OK;
 }
 
 vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* values, int64_t n,
                         const uint8_t* selection, void* stream) {
-    if (!agg || (!keys && !agg->h.key_types.empty())) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (!agg || (!keys && !agg->h.ungrouped)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
     AggImpl& h = agg->h;
     if (n == 0) return VR_OK;
     if ((uint64_t)n >= (1ull << 32) && h.det) return fail(VR_ERR_UNSUPPORTED, "deterministic batches are below 2^32 rows");
     hipStream_t s = h.dev->pick(stream);
     VR_TRY(enter(h, s));
+    vr_column zk;
+    if (h.ungrouped) {
+        VR_TRY(zero_key_column(h, n, s, &zk));
+        keys = &zk;
+    }
     BatchDesc b;
     StagePlan plan;
     std::vector<std::vector<int32_t>> ids;
@@ -1812,6 +1868,19 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     if (bs->filter_col >= nv || (bs->filter_col >= 0 && h.value_types[bs->filter_col] != VR_INT32))
         return fail(VR_ERR_INVALID_ARGUMENT, "the filter column must be an INT32 value column");
     hipStream_t s = h.dev->pick(stream);
+    vr_batch_set with_keys;
+    std::vector<vr_column> zkeys;
+    if (h.ungrouped) {  // the constant key column of every batch
+        int64_t longest = 0;
+        for (int b = 0; b < nb; b++) longest = std::max(longest, bs->lengths[b]);
+        vr_column zk;
+        VR_TRY(zero_key_column(h, longest, s, &zk));
+        zkeys.assign(nb, zk);
+        for (int b = 0; b < nb; b++) zkeys[b].length = bs->lengths[b];
+        with_keys = *bs;
+        with_keys.keys = zkeys.data();
+        bs = &with_keys;
+    }
     // One persistent launch needs device-resident columns and one dictionary per UTF8 key; anything
     // else (and the deterministic mode, which orders each batch) takes the per-batch path.
     bool fused = !h.det;
@@ -1961,10 +2030,11 @@ vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys
     if (from < 0 || to < from || to > h.order_count) return fail(VR_ERR_INVALID_ARGUMENT, "emission range outside the groups");
     const int64_t m = to - from;
     if (m == 0) return VR_OK;
-    const int nk = (int)h.key_types.size(), no = (int)h.out_types.size();
+    const int nk = h.ungrouped ? 0 : (int)h.key_types.size(), no = (int)h.out_types.size();
     EmitArgs a{};
     a.t = view_of(h);
     a.ks = h.proto.ks;
+    if (h.ungrouped) a.ks.nkeys = 0;  // no key columns out
     a.order = h.order.as<uint64_t>();
     a.from = from;
     a.m = m;
@@ -2089,8 +2159,9 @@ vr_status vr_agg_reset(vr_agg* agg) {
         VR_TRY(alloc_table(h, h.slots, h.pbits, h.cells, h.state, h.rgroups, s));
     }
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
-    VR_TRY_HIP(hipStreamSynchronize(s));
     h.groups_upper = 0;
+    if (h.ungrouped) VR_TRY(insert_single_group(h, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
     h.order_count = -1;
     return leave(h, s);
 }
