@@ -385,7 +385,9 @@ struct Acc {
     std::vector<uint8_t> overflowed, any, sumNull, merged;
     std::vector<int64_t> lbest;
     std::vector<double> dbest;
+    std::vector<int64_t> wraps;  // exact ungrouped final merge: multiples of 2^128 beyond lo/hi
     bool checkedOverflow = false;
+    bool ungrouped = false;      // the handle has no grouping key
 
     void ensure(int64_t g) {
         if ((int64_t)count.size() >= g) return;
@@ -400,6 +402,7 @@ struct Acc {
         merged.resize(g, 0);
         lbest.resize(g, 0);
         dbest.resize(g, 0.0);
+        wraps.resize(g, 0);
     }
 
     // WideLongSum.add / addWide (GroupedAccumulators.java:749-824)
@@ -411,6 +414,23 @@ struct Acc {
         int64_t hsum = (int64_t)((uint64_t)h + (uint64_t)xh + (uint64_t)carry);
         if (((h ^ hsum) & (xh ^ hsum)) < 0) overflowed[g] = 1;
         lo[g] = sum;
+        hi[g] = hsum;
+    }
+
+    // WideDecimalAvgMergeAgg.State.add (VectorAggregates.scala:580-592): a signed 128-bit overflow
+    // marks the group overflowed, except in an ungrouped final, which goes on exactly (the total
+    // is then wraps * 2^128 + the 128 bits).
+    void mergeWide(int32_t g, int64_t xh, int64_t xl, bool exact) {
+        if (!exact) {
+            addWide(g, xh, xl);
+            return;
+        }
+        int64_t h = hi[g];
+        uint64_t l = (uint64_t)lo[g] + (uint64_t)xl;
+        int64_t carry = l < (uint64_t)lo[g] ? 1 : 0;
+        int64_t hsum = (int64_t)((uint64_t)h + (uint64_t)xh + (uint64_t)carry);
+        if (((h ^ hsum) & (xh ^ hsum)) < 0) wraps[g] += h < 0 ? -1 : 1;
+        lo[g] = (int64_t)l;
         hi[g] = hsum;
     }
 
@@ -429,7 +449,50 @@ struct Acc {
                 case VR_AGG_COUNT:
                     if (valid(*v, i)) count[g]++;
                     break;
+                // Merge modes (VectorAggregates.compileMerge, VectorAggregates.scala:1141-1250).
+                case VR_AGG_MERGE_COUNT:  // CountMergeAgg: a LongSum over the partial counts (:1042-1062)
+                    if (valid(*v, i)) count[g] = (int64_t)((uint64_t)count[g] + (uint64_t)readLong(*v, i));
+                    break;
+                case VR_AGG_MERGE_AVG_DOUBLE: {  // AverageMergeAgg: DoubleSum(strict) + LongSum (:1065-1090)
+                    if (valid(*v, i)) dsum[g] += ((const double*)v->data)[i];
+                    const vr_column& c = values[d.input2];
+                    if (valid(c, i)) count[g] = (int64_t)((uint64_t)count[g] + (uint64_t)readLong(c, i));
+                    break;
+                }
+                case VR_AGG_MERGE_SUM_WIDE: {  // WideDecimalSumMergeAgg.State.merge (:688-712)
+                    const vr_column& e = values[d.input2];
+                    if (!valid(e, i) || bit((const uint8_t*)e.data, i)) break;  // isEmpty null or true
+                    count[g]++;                                                 // nonEmpty
+                    if (!valid(*v, i)) sumNull[g] = 1;
+                    else if (!sumNull[g] && !overflowed[g]) {
+                        if (v->type == VR_DECIMAL128) {
+                            const int64_t* p = (const int64_t*)v->data;
+                            addWide(g, p[2 * i + 1], p[2 * i]);
+                        } else {
+                            int64_t x = readLong(*v, i);
+                            addWide(g, x >> 63, x);
+                        }
+                    }
+                    break;
+                }
+                case VR_AGG_MERGE_AVG_WIDE: {  // WideDecimalAvgMergeAgg.State.merge (:562-578)
+                    const vr_column& c = values[d.input2];
+                    if (valid(c, i)) count[g] = (int64_t)((uint64_t)count[g] + (uint64_t)readLong(c, i));
+                    const bool exact = ungrouped && (d.flags & VR_AGG_RESULT);
+                    if (!valid(*v, i)) sumNull[g] = 1;
+                    else if (!sumNull[g] && !overflowed[g]) {
+                        if (v->type == VR_DECIMAL128) {
+                            const int64_t* p = (const int64_t*)v->data;
+                            mergeWide(g, p[2 * i + 1], p[2 * i], exact);
+                        } else {
+                            int64_t x = readLong(*v, i);
+                            mergeWide(g, x >> 63, x, exact);
+                        }
+                    }
+                    break;
+                }
                 case VR_AGG_SUM_LONG:
+                case VR_AGG_MERGE_SUM_LONG:  // SumLongAgg over the sum buffer (:1187-1188)
                     if (!valid(*v, i)) break;
                     if (d.flags & VR_AGG_CHECKED) {
                         int64_t r;
@@ -442,6 +505,7 @@ struct Acc {
                     count[g]++;
                     break;
                 case VR_AGG_SUM_DOUBLE:
+                case VR_AGG_MERGE_SUM_DOUBLE:  // SumDoubleAgg over the sum buffer (:1186)
                 case VR_AGG_AVG_DOUBLE:
                     if (!valid(*v, i)) break;
                     dsum[g] += ((const double*)v->data)[i];  // strict: one chain, row order
@@ -460,9 +524,11 @@ struct Acc {
                     count[g]++;
                     break;
                 case VR_AGG_MIN:
-                case VR_AGG_MAX: {
+                case VR_AGG_MAX:
+                case VR_AGG_MERGE_MIN:  // MinMaxAgg over the min / max buffer (:1209-1210)
+                case VR_AGG_MERGE_MAX: {
                     if (!valid(*v, i)) break;
-                    bool isMin = d.kind == VR_AGG_MIN;
+                    bool isMin = d.kind == VR_AGG_MIN || d.kind == VR_AGG_MERGE_MIN;
                     if (v->type == VR_FLOAT64) {
                         double x = ((const double*)v->data)[i];
                         if (!any[g]) {
@@ -556,8 +622,65 @@ int emit_one(const Acc& a, int64_t g, vr_out_column* out, int64_t r) {
                 put_i64(out[1], r, true, a.count[g]);
             }
             break;
-        case VR_AGG_SUM_WIDE:
-        case VR_AGG_MERGE_SUM_WIDE: {
+        case VR_AGG_MERGE_SUM_WIDE: {  // WideDecimalSumMergeAgg.State.value / write (VectorAggregates.scala:719-762)
+            i128 t = make128(a.hi[g], a.lo[g]);
+            const bool nonEmpty = a.count[g] > 0;
+            const bool poisoned = a.sumNull[g] || a.overflowed[g];
+            const bool past = past_precision(t, d.buffer_precision);
+            if (result) {
+                if (!nonEmpty) put_d128(out[0], r, false, 0);
+                else if (poisoned) {
+                    if (ansi) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "overflow in sum of decimals");
+                    put_d128(out[0], r, false, 0);
+                } else if (past) {
+                    if (ansi) return fail(VR_ERR_DECIMAL_OVERFLOW, "decimal sum past its precision");
+                    put_d128(out[0], r, false, 0);
+                } else put_d128(out[0], r, true, t);
+            } else {
+                if (!nonEmpty) put_d128(out[0], r, true, 0);  // zero while empty
+                else put_d128(out[0], r, !poisoned && !past, t);
+                set_bit(out[1].validity, r, true);
+                set_bit((uint8_t*)out[1].data, r, !nonEmpty);  // isEmpty
+            }
+            break;
+        }
+        case VR_AGG_MERGE_AVG_WIDE: {  // WideDecimalAvgMergeAgg.State.sum / value (:596-609)
+            i128 t = make128(a.hi[g], a.lo[g]);
+            const bool poisoned = a.sumNull[g] || a.overflowed[g];
+            const bool exact = a.ungrouped && result;  // the exact ungrouped final
+            if (!result) {
+                put_d128(out[0], r, !poisoned && !past_precision(t, d.buffer_precision), t);
+                put_i64(out[1], r, true, a.count[g]);
+                break;
+            }
+            bool wide = d.result_precision > 18;
+            auto put_null = [&]() {
+                if (wide) put_d128(out[0], r, false, 0);
+                else put_i64(out[0], r, false, 0);
+            };
+            if (a.count[g] == 0) {  // If(count = 0, null, ...)
+                put_null();
+                break;
+            }
+            if (poisoned || (!exact && past_precision(t, d.buffer_precision))) {  // a null sum
+                if (ansi) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "decimal average over an overflowed sum");
+                put_null();
+                break;
+            }
+            if (exact && a.wraps[g] != 0)
+                return fail(VR_ERR_UNSUPPORTED, "ungrouped decimal average over a total beyond 128 bits");
+            i128 q;
+            bool fits = decimal_avg(t, a.count[g], d.buffer_scale, d.result_scale, &q);
+            if (!fits || past_precision(q, d.result_precision)) {
+                if (ansi) return fail(VR_ERR_DECIMAL_OVERFLOW, "decimal average past its precision");
+                put_null();
+                break;
+            }
+            if (wide) put_d128(out[0], r, true, q);
+            else put_i64(out[0], r, true, (int64_t)q);
+            break;
+        }
+        case VR_AGG_SUM_WIDE: {
             i128 t = make128(a.hi[g], a.lo[g]);
             bool over = a.overflowed[g] || a.sumNull[g] || past_precision(t, d.buffer_precision);
             if (result) {
@@ -574,10 +697,13 @@ int emit_one(const Acc& a, int64_t g, vr_out_column* out, int64_t r) {
             }
             break;
         }
-        case VR_AGG_AVG_WIDE:
-        case VR_AGG_MERGE_AVG_WIDE: {
+        case VR_AGG_AVG_WIDE: {
             i128 t = make128(a.hi[g], a.lo[g]);
-            bool over = a.overflowed[g] || a.sumNull[g] || past_precision(t, d.buffer_precision);
+            // An ungrouped result divides the total unchecked against the buffer precision, and a
+            // total past 128 bits is past any precision (WideDecimalAvgAgg.sumValue,
+            // VectorAggregates.scala:455-467; Escalation.total, :283-284).
+            const bool ungroupedResult = a.ungrouped && result;
+            bool over = a.overflowed[g] || a.sumNull[g] || (!ungroupedResult && past_precision(t, d.buffer_precision));
             if (!result) {
                 if (a.count[g] == 0 && !a.sumNull[g]) put_d128(out[0], r, true, 0);
                 else put_d128(out[0], r, !over, t);
@@ -594,6 +720,7 @@ int emit_one(const Acc& a, int64_t g, vr_out_column* out, int64_t r) {
                 break;
             }
             if (over) {  // the buffer could not hold the sum
+                if (ansi && ungroupedResult) return fail(VR_ERR_DECIMAL_OVERFLOW, "decimal average past its precision");
                 if (ansi) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "decimal average over an overflowed sum");
                 put_null();
                 break;
@@ -661,6 +788,7 @@ int vro_agg_create(const vr_agg_spec* spec, vro_agg** out) {
         Acc acc;
         acc.d = spec->aggs[a];
         acc.inType = acc.d.input >= 0 ? h->valueTypes[acc.d.input] : -1;
+        acc.ungrouped = h->ungrouped;
         if (h->ungrouped) acc.ensure(1);
         h->accs.push_back(std::move(acc));
     }

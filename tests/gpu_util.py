@@ -15,11 +15,11 @@ def out_roles(spec):
     roles = []
     for a in spec.aggs:
         result = bool(a.flags & abi.RESULT)
-        if a.kind == abi.SUM_DOUBLE:
+        if a.kind in (abi.SUM_DOUBLE, abi.MERGE_SUM_DOUBLE):
             roles.append("fsum")
-        elif a.kind == abi.AVG_DOUBLE:
+        elif a.kind in (abi.AVG_DOUBLE, abi.MERGE_AVG_DOUBLE):
             roles += ["fsum"] if result else ["fsum", "exact"]
-        elif a.kind in (abi.SUM_WIDE, abi.AVG_WIDE):
+        elif a.kind in (abi.SUM_WIDE, abi.AVG_WIDE, abi.MERGE_SUM_WIDE, abi.MERGE_AVG_WIDE):
             roles += ["exact"] if result else ["exact", "exact"]
         else:
             roles.append("exact")

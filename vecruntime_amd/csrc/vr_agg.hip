@@ -387,7 +387,12 @@ struct EmitAgg {
     int32_t kind, flags, op_code, off;
     int32_t nulls_off, in_type, out0, bp;
     int32_t bs, rp, rs, bit;
+    int32_t off2;    // a merge's count buffer total (SUM_I64 words), -1: none
+    int32_t ne_off;  // a merged decimal sum's non-empty buffer rows, -1: none
 };
+
+// EmitAgg.flags beyond vr_agg_flags: the handle has no grouping key.
+enum { EF_UNGROUPED = 1 << 16 };
 
 struct EmitArgs {
     TableView t;
@@ -539,7 +544,7 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
         const bool result = e.flags & VR_AGG_RESULT;
         const bool ansi = e.flags & VR_AGG_CHECKED;
         const uint64_t* s = g + e.off;
-        const uint64_t cnt = rows - ((in && e.nulls_off >= 0) ? g[e.nulls_off] : 0);  // non-null rows
+        uint64_t cnt = rows - ((in && e.nulls_off >= 0) ? g[e.nulls_off] : 0);  // non-null rows
         switch (e.kind) {
             case VR_AGG_COUNT_STAR: put_i64(o0, o, in, (int64_t)rows, in); break;
             case VR_AGG_COUNT: put_i64(o0, o, in, (int64_t)cnt, in); break;
@@ -564,12 +569,97 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
                 }
                 break;
             }
+            case VR_AGG_MERGE_COUNT: {
+                uint64_t lo = 0, hi = 0;
+                bool fits = true;
+                if (in) sum_total(s, false, &lo, &hi, &fits);
+                put_i64(o0, o, in, (int64_t)lo, in);  // LongSum: wraps like Java's long adds
+                break;
+            }
+            case VR_AGG_MERGE_AVG_DOUBLE: {
+                double sum = in ? __longlong_as_double((long long)s[0]) : 0.0;
+                uint64_t c = 0, chi = 0;
+                bool cf = true;
+                if (in) sum_total(g + e.off2, false, &c, &chi, &cf);
+                if (result) {
+                    double q = c ? sum / (double)(int64_t)c : 0.0;
+                    put_i64(o0, o, c != 0, __double_as_longlong(q), in);
+                } else {
+                    put_i64(o0, o, true, __double_as_longlong(sum), in);
+                    put_i64(o1, o, true, (int64_t)c, in);
+                }
+                break;
+            }
+            case VR_AGG_MERGE_SUM_WIDE: {
+                uint64_t lo = 0, hi = 0;
+                bool fits = true;
+                if (in) sum_total(s, e.op_code == OP_SUM_D128, &lo, &hi, &fits);
+                const bool nonempty = in && g[e.ne_off] != 0;
+                const bool poisoned = !fits || (in && g[e.nulls_off] != 0);
+                const bool past = past_precision(lo, hi, e.bp);
+                if (result) {
+                    // If(isEmpty, null, CheckOverflowInSum(sum)) over the merged buffer
+                    if (nonempty && ansi) err |= poisoned ? ERR_SUM_OF_OVERFLOW : (past ? ERR_DECIMAL_OVERFLOW : 0);
+                    put_d128(o0, o, nonempty && !poisoned && !past, lo, hi, in);
+                } else {
+                    // zero while empty, null once poisoned or past the buffer precision
+                    put_d128(o0, o, !nonempty || (!poisoned && !past), nonempty ? lo : 0, nonempty ? hi : 0, in);
+                    put_bit(o1.valid, o, true, in);
+                    put_bit((uint8_t*)o1.data, o, !nonempty, in);
+                }
+                break;
+            }
             case VR_AGG_SUM_WIDE:
-            case VR_AGG_AVG_WIDE: {
+            case VR_AGG_AVG_WIDE:
+            case VR_AGG_MERGE_AVG_WIDE: {
                 uint64_t lo = 0, hi = 0;
                 bool fits = true;
                 if (in) sum_total(s, e.op_code == OP_SUM_D128, &lo, &hi, &fits);
                 bool over = !fits || past_precision(lo, hi, e.bp);
+                if (e.kind == VR_AGG_MERGE_AVG_WIDE) {
+                    // The count is the partial counts' total, and a null partial sum poisons the group.
+                    uint64_t chi = 0;
+                    bool cf = true;
+                    cnt = 0;
+                    if (in) sum_total(g + e.off2, false, &cnt, &chi, &cf);
+                    const bool poisoned = in && g[e.nulls_off] != 0;
+                    if (!result) {
+                        put_d128(o0, o, !poisoned && !over, lo, hi, in);
+                        put_i64(o1, o, true, (int64_t)cnt, in);
+                        break;
+                    }
+                    if (poisoned) {
+                        if (cnt && ansi) err |= ERR_AVG_OF_OVERFLOW;
+                        if (e.rp > 18) put_d128(o0, o, false, 0, 0, in);
+                        else put_i64(o0, o, false, 0, in);
+                        break;
+                    }
+                }
+                if (result && (e.flags & EF_UNGROUPED) && e.kind != VR_AGG_SUM_WIDE) {
+                    // An ungrouped final average divides the total as it is: Spark's generated code
+                    // keeps it in a local that nothing checks against the buffer precision
+                    // (WideDecimalAvgAgg.sumValue, VectorAggregates.scala:455-467, and the merge's
+                    // exact ungrouped final, :583-604).  A total past 128 bits is past any result
+                    // precision in update mode; the merge's exact total beyond 128 bits is left to
+                    // the CPU kernels.
+                    bool ok = cnt > 0;
+                    uint64_t qlo = 0, qhi = 0;
+                    if (ok && !fits) {
+                        if (e.kind == VR_AGG_MERGE_AVG_WIDE) err |= ERR_WIDE_TOTAL;
+                        else if (ansi) err |= ERR_DECIMAL_OVERFLOW;
+                        ok = false;
+                    }
+                    if (ok) {
+                        bool q_fits = decimal_avg(lo, hi, cnt, e.rs - e.bs, &qlo, &qhi);
+                        if (!q_fits || past_precision(qlo, qhi, e.rp)) {
+                            if (ansi) err |= ERR_DECIMAL_OVERFLOW;
+                            ok = false;
+                        }
+                    }
+                    if (e.rp > 18) put_d128(o0, o, ok, qlo, qhi, in);
+                    else put_i64(o0, o, ok, (int64_t)qlo, in);
+                    break;
+                }
                 if (e.kind == VR_AGG_SUM_WIDE) {
                     if (result) {
                         if (cnt && over && ansi) err |= ERR_DECIMAL_OVERFLOW;
@@ -768,13 +858,14 @@ static vr_status compile_spec(AggImpl& h) {
     int off = 0;
     std::vector<uint64_t> ident;
     std::vector<uint8_t> comb;  // how two partial states combine, word by word
-    auto add_op = [&](int32_t code, int32_t col, int nwords) -> int {
+    auto add_op = [&](int32_t code, int32_t col, int nwords, int32_t gate = -1) -> int {
         for (size_t k = 0; k < ops.size(); k++)
-            if (ops[k].code == code && ops[k].col == col) return (int)k;
+            if (ops[k].code == code && ops[k].col == col && ops[k].gate == gate) return (int)k;
         Op op{};
         op.code = code;
         op.col = col;
         op.off = off;
+        op.gate = (int16_t)gate;
         ops.push_back(op);
         for (int i = 0; i < nwords; i++) {
             ident.push_back(0);
@@ -827,18 +918,45 @@ static vr_status compile_spec(AggImpl& h) {
         e.rp = d.result_precision;
         e.rs = d.result_scale;
         e.out0 = (int)h.out_types.size();
-        e.nulls_off = -1;
-        int32_t in_type = -1;
+        e.nulls_off = e.off2 = e.ne_off = -1;
+        if (h.ungrouped) e.flags |= EF_UNGROUPED;
+        int32_t in_type = -1, in2_type = -1;
         if (d.kind != VR_AGG_COUNT_STAR) {
             if (d.input < 0 || d.input >= (int)h.value_types.size())
                 return fail(VR_ERR_INVALID_ARGUMENT, "aggregate input column out of range");
             in_type = h.value_types[d.input];
         }
+        const bool two = d.kind == VR_AGG_MERGE_SUM_WIDE || d.kind == VR_AGG_MERGE_AVG_WIDE || d.kind == VR_AGG_MERGE_AVG_DOUBLE;
+        if (two) {
+            if (d.input2 < 0 || d.input2 >= (int)h.value_types.size())
+                return fail(VR_ERR_INVALID_ARGUMENT, "second buffer column of a merge out of range");
+            in2_type = h.value_types[d.input2];
+            if (in2_type != (d.kind == VR_AGG_MERGE_SUM_WIDE ? VR_BOOL : VR_INT64))
+                return fail(VR_ERR_INVALID_ARGUMENT, d.kind == VR_AGG_MERGE_SUM_WIDE ? "isEmpty buffer is not a boolean lane"
+                                                                                      : "count buffer is not a bigint lane");
+        }
+        // Merge modes whose state and result are the update mode's over the buffer column
+        // (VectorAggregates.compileMerge: SumLongAgg, SumDoubleAgg and MinMaxAgg over the buffer,
+        // VectorAggregates.scala:1183-1200,1209-1210).
+        if (d.kind == VR_AGG_MERGE_SUM_LONG) {
+            if (in_type != VR_INT64) return fail(VR_ERR_INVALID_ARGUMENT, "sum buffer is not a bigint lane");
+            e.kind = VR_AGG_SUM_LONG;
+        } else if (d.kind == VR_AGG_MERGE_SUM_DOUBLE) {
+            e.kind = VR_AGG_SUM_DOUBLE;
+        } else if (d.kind == VR_AGG_MERGE_MIN || d.kind == VR_AGG_MERGE_MAX) {
+            e.kind = d.kind == VR_AGG_MERGE_MIN ? VR_AGG_MIN : VR_AGG_MAX;
+        }
         e.in_type = in_type;
         bool result = d.flags & VR_AGG_RESULT;
         int k = 0;
         auto nulls = [&]() { e.nulls_off = ops[add_op(OP_NULLS, d.input, 1)].off; };
-        switch (d.kind) {
+        auto wide_sum = [&](int32_t gate) -> bool {
+            if (in_type == VR_INT64) k = add_op(OP_SUM_I64, d.input, 2, gate);
+            else if (in_type == VR_DECIMAL128) k = add_op(OP_SUM_D128, d.input, 4, gate);
+            else return false;
+            return true;
+        };
+        switch (e.kind) {
             case VR_AGG_COUNT_STAR: h.out_types.push_back(VR_INT64); break;
             case VR_AGG_COUNT:
                 nulls();
@@ -856,7 +974,7 @@ static vr_status compile_spec(AggImpl& h) {
                 k = add_op(OP_SUM_F64, d.input, 1);
                 nulls();
                 h.out_types.push_back(VR_FLOAT64);
-                if (d.kind == VR_AGG_AVG_DOUBLE && !result) h.out_types.push_back(VR_INT64);
+                if (e.kind == VR_AGG_AVG_DOUBLE && !result) h.out_types.push_back(VR_INT64);
                 break;
             case VR_AGG_SUM_WIDE:
             case VR_AGG_AVG_WIDE:
@@ -864,7 +982,7 @@ static vr_status compile_spec(AggImpl& h) {
                 else if (in_type == VR_DECIMAL128) k = add_op(OP_SUM_D128, d.input, 4);
                 else return fail(VR_ERR_UNSUPPORTED, "decimal sum over a non-integral lane");
                 nulls();
-                if (d.kind == VR_AGG_SUM_WIDE) {
+                if (e.kind == VR_AGG_SUM_WIDE) {
                     h.out_types.push_back(VR_DECIMAL128);
                     if (!result) h.out_types.push_back(VR_BOOL);
                 } else if (result) {
@@ -877,7 +995,7 @@ static vr_status compile_spec(AggImpl& h) {
                 break;
             case VR_AGG_MIN:
             case VR_AGG_MAX: {
-                bool mn = d.kind == VR_AGG_MIN;
+                bool mn = e.kind == VR_AGG_MIN;
                 if (in_type == VR_INT32) k = add_op(mn ? OP_MIN_I32 : OP_MAX_I32, d.input, 1);
                 else if (in_type == VR_INT64) {
                     size_t before = ops.size();
@@ -895,6 +1013,46 @@ static vr_status compile_spec(AggImpl& h) {
                 h.out_types.push_back(in_type);
                 break;
             }
+            // COUNT's merge: a LongSum over the partial counts, never null (CountMergeAgg,
+            // VectorAggregates.scala:1042-1062).
+            case VR_AGG_MERGE_COUNT:
+                if (in_type != VR_INT64) return fail(VR_ERR_INVALID_ARGUMENT, "count buffer is not a bigint lane");
+                k = add_op(OP_SUM_I64, d.input, 2);
+                h.out_types.push_back(VR_INT64);
+                break;
+            // AVG's merge: the sums (DoubleSum, strict in deterministic mode) and the counts added
+            // component-wise (AverageMergeAgg, VectorAggregates.scala:1065-1090).
+            case VR_AGG_MERGE_AVG_DOUBLE:
+                if (in_type != VR_FLOAT64) return fail(VR_ERR_INVALID_ARGUMENT, "avg sum buffer is not a double lane");
+                k = add_op(OP_SUM_F64, d.input, 1);
+                e.off2 = ops[add_op(OP_SUM_I64, d.input2, 2)].off;
+                h.out_types.push_back(VR_FLOAT64);
+                if (!result) h.out_types.push_back(VR_INT64);
+                break;
+            // The wide decimal sum's merge: buffer rows whose isEmpty is false (and non-null) mark the
+            // group non-empty; a null sum on such a row poisons it; the others add exactly
+            // (WideDecimalSumMergeAgg.State.merge, VectorAggregates.scala:688-712).
+            case VR_AGG_MERGE_SUM_WIDE:
+                if (!wide_sum(d.input2)) return fail(VR_ERR_INVALID_ARGUMENT, "decimal sum buffer is not a decimal lane");
+                e.ne_off = ops[add_op(OP_ROWS, -1, 1, d.input2)].off;
+                e.nulls_off = ops[add_op(OP_NULLS, d.input, 1, d.input2)].off;
+                h.out_types.push_back(VR_DECIMAL128);
+                if (!result) h.out_types.push_back(VR_BOOL);
+                break;
+            // The wide decimal average's merge: counts added, a null sum poisons the group, the sums
+            // add exactly (WideDecimalAvgMergeAgg.State.merge, VectorAggregates.scala:562-578).
+            case VR_AGG_MERGE_AVG_WIDE:
+                if (!wide_sum(-1)) return fail(VR_ERR_INVALID_ARGUMENT, "decimal avg buffer is not a decimal lane");
+                nulls();
+                e.off2 = ops[add_op(OP_SUM_I64, d.input2, 2)].off;
+                if (result) {
+                    if (d.result_scale < d.buffer_scale) return fail(VR_ERR_UNSUPPORTED, "result scale below the buffer scale");
+                    h.out_types.push_back(d.result_precision > 18 ? VR_DECIMAL128 : VR_INT64);
+                } else {
+                    h.out_types.push_back(VR_DECIMAL128);
+                    h.out_types.push_back(VR_INT64);
+                }
+                break;
             default:
                 return fail(VR_ERR_UNSUPPORTED, "aggregate kind " + std::to_string(d.kind) + " runs on the CPU kernels");
         }
@@ -1309,6 +1467,7 @@ static bool record_layout(const AggImpl& h, RecLayout& L) {
     for (int k = 0; k < h.proto.nops; k++) {
         const Op& op = h.proto.ops[k];
         if (op.code == OP_FIRST_ROW) return false;
+        if (op.gate >= 0) return false;  // records carry no gate column
         if (op.col < 0) continue;
         L.vmask |= 1u << op.col;
         if (op.code == OP_NULLS) continue;
@@ -2098,6 +2257,9 @@ vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys
         VR_TRY_HIP(hipStreamSynchronize(s));
         if (e & ERR_LONG_OVERFLOW) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "long overflow");
         if (e & ERR_AVG_OF_OVERFLOW) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "decimal average over an overflowed sum");
+        if (e & ERR_SUM_OF_OVERFLOW) return fail(VR_ERR_ARITHMETIC_OVERFLOW, "overflow in sum of decimals");
+        if (e & ERR_WIDE_TOTAL)
+            return fail(VR_ERR_UNSUPPORTED, "ungrouped decimal average over a total beyond 128 bits runs on the CPU kernels");
         return fail(VR_ERR_DECIMAL_OVERFLOW, "decimal value past its precision");
     }
     return VR_OK;

@@ -62,16 +62,25 @@ enum OpCode : int32_t {
 // counters[] words
 enum { C_GROUPS = 0, C_RESERVED = 1, C_NAN = 2, C_ERROR = 3, C_OUT = 4, C_STOP = 5, C_POOL = 6, C_PSTOP = 7, C_WORDS = 8 };
 enum { RES_SENTINEL = 1, RES_NULL = 2 };
-enum { ERR_LONG_OVERFLOW = 1, ERR_DECIMAL_OVERFLOW = 2, ERR_AVG_OF_OVERFLOW = 4, ERR_TABLE_FULL = 8 };
+enum {
+    ERR_LONG_OVERFLOW = 1,
+    ERR_DECIMAL_OVERFLOW = 2,
+    ERR_AVG_OF_OVERFLOW = 4,
+    ERR_TABLE_FULL = 8,
+    ERR_SUM_OF_OVERFLOW = 16,  // a merged decimal sum that a partial left null or that overflowed
+    ERR_WIDE_TOTAL = 32        // an ungrouped final average whose exact total needs more than 128 bits
+};
 
 // How two partial states of a group combine, word by word.
 enum CombCode : uint8_t { CB_ADD = 0, CB_FADD = 1, CB_SMIN = 2, CB_SMAX = 3, CB_UMIN = 4, CB_UMAX = 5, CB_OR = 6 };
 
 struct Op {
     int32_t code;
-    int32_t col;  // value column, -1 for ROWS / FIRST_ROW
-    int32_t off;  // first state word
-    int32_t bit;  // presence bit (MIN/MAX_I64), else 0
+    int32_t col;   // value column, -1 for ROWS / FIRST_ROW
+    int32_t off;   // first state word
+    int16_t bit;   // presence bit (MIN/MAX_I64), else 0
+    int16_t gate;  // BOOL value column: the op folds only rows where it is non-null and false
+                   // (a merged decimal sum's isEmpty flag); -1: every row
 };
 
 struct KeySpec {
@@ -421,6 +430,9 @@ struct ColSrc {
         *hi = ld_stream(p + 1);
     }
     __device__ __forceinline__ uint64_t row() const { return (uint64_t)(b.row_base + i); }
+    __device__ __forceinline__ bool gate_open(int c) const {
+        return is_valid(b.vals[c], i) && !bit_at((const uint8_t*)b.vals[c].data, i);
+    }
 };
 
 // A value already in registers: validity, the lane's 64-bit word(s) (a DECIMAL128 in x0/x1, an
@@ -438,12 +450,15 @@ struct RegSrc {
         *hi = x1;
     }
     __device__ __forceinline__ uint64_t row() const { return gi; }
+    // Gated operations never reach a register source: specs that have one take the per-tile path.
+    __device__ __forceinline__ bool gate_open(int) const { return true; }
 };
 
 // Folds one operation of one row into the state words at `row` (an LDS entry or a global row)
 // with atomics.
 template <class Src>
 __device__ __forceinline__ void apply_one(const UpdateArgs& a, const Op& op, const Src& src, uint64_t* row) {
+    if (op.gate >= 0 && !src.gate_open(op.gate)) return;
     {
         uint64_t* w = row + op.off;
         switch (op.code) {

@@ -67,7 +67,12 @@ static int64_t estimate_agg(const QueryCopy& c) {
     for (int32_t t : c.key_types) words += key_lane_words(t);
     int width = words > 2 || c.key_types.size() > 2 ? words + 1 : 1;
     int stride = 2;
-    for (auto& a : c.aggs) stride += a.kind == VR_AGG_MIN || a.kind == VR_AGG_MAX ? 4 : 3;
+    for (auto& a : c.aggs) {  // state words per aggregate, at most
+        const bool wide = a.kind == VR_AGG_SUM_WIDE || a.kind == VR_AGG_AVG_WIDE || a.kind == VR_AGG_MERGE_SUM_WIDE ||
+                          a.kind == VR_AGG_MERGE_AVG_WIDE;
+        const bool minmax = a.kind == VR_AGG_MIN || a.kind == VR_AGG_MAX || a.kind == VR_AGG_MERGE_MIN || a.kind == VR_AGG_MERGE_MAX;
+        stride += wide ? 7 : minmax ? 4 : 3;
+    }
     int64_t out_row = 8 * (int64_t)c.key_types.size() + 24 * (int64_t)c.aggs.size() + 8;
     return (int64_t)slots * 8 * (width + stride) + g * out_row + (64ll << 20);
 }
