@@ -130,6 +130,7 @@ typedef struct vr_stats {
     int64_t cpu_fallbacks_signalled;
     int64_t partitioned_updates;  /* aggregate calls that took the partitioned update */
     int64_t specialized_updates;  /* of those, calls that ran kernels compiled for their spec */
+    int64_t dense_join_builds;    /* join builds that took the direct key index */
 } vr_stats;
 vr_status vr_stats_get(vr_device* dev, vr_stats* out);
 
@@ -281,6 +282,11 @@ typedef struct vr_join_spec {
     const int32_t* column_types;
     const int32_t* key_columns;    /* index into the build columns of each key */
     int64_t capacity_hint;         /* expected build rows */
+    /* 0: a direct index over the key range when the build allows it -- one INT32/INT64 key whose
+     * non-null values span at most 2^22 and at most 10x the distinct keys (DenseKeyIndex.tryBuild,
+     * kernels/src/main/java/io/vecruntime/kernels/DenseKeyIndex.java:70-110); -1: always the hash
+     * table.  The probe's results are the same either way. */
+    int32_t dense_index;
 } vr_join_spec;
 
 typedef struct vr_join vr_join;

@@ -110,6 +110,7 @@ class VrStats(C.Structure):
         ("cpu_fallbacks_signalled", C.c_int64),
         ("partitioned_updates", C.c_int64),
         ("specialized_updates", C.c_int64),
+        ("dense_join_builds", C.c_int64),
     ]
 
 
@@ -152,6 +153,7 @@ class VrJoinSpec(C.Structure):
         ("column_types", C.POINTER(C.c_int32)),
         ("key_columns", C.POINTER(C.c_int32)),
         ("capacity_hint", C.c_int64),
+        ("dense_index", C.c_int32),
     ]
 
 
@@ -294,7 +296,7 @@ class AggSpec:
 class JoinSpec:
     """Owns a VrJoinSpec and the arrays it points at."""
 
-    def __init__(self, key_types, column_types, key_columns, capacity_hint=0):
+    def __init__(self, key_types, column_types, key_columns, capacity_hint=0, dense_index=0):
         self.key_types = list(key_types)
         self.column_types = list(column_types)
         self.key_columns = list(key_columns)
@@ -303,4 +305,4 @@ class JoinSpec:
         self._kc = int_array(self.key_columns)
         self.c = VrJoinSpec(len(self.key_types), C.cast(self._kt, C.POINTER(C.c_int32)),
                             len(self.column_types), C.cast(self._ct, C.POINTER(C.c_int32)),
-                            C.cast(self._kc, C.POINTER(C.c_int32)), int(capacity_hint))
+                            C.cast(self._kc, C.POINTER(C.c_int32)), int(capacity_hint), int(dense_index))

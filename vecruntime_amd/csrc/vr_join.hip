@@ -128,8 +128,28 @@ struct OutCol {
     int32_t pad;
 };
 
+// The direct index over a single key's range [min, min + range): the key's value word, 0 when no
+// build row has it (DenseKeyIndex.slots, kernels/src/main/java/io/vecruntime/kernels/DenseKeyIndex.java:44-110).
+struct DenseIndex {
+    const uint64_t* slots;  // nullptr: probe the hash table
+    int64_t min;
+    uint64_t range;
+};
+
+// The key's signed value (INT32 keys are read zero-extended into the 64-bit key).
+__device__ __forceinline__ int64_t key_value(int32_t type, uint64_t key) {
+    return type == VR_INT32 ? (int64_t)(int32_t)(uint32_t)key : (int64_t)key;
+}
+
+// DenseKeyIndex.lookup (:131-176): an offset outside the range, compared unsigned, is a miss.
+__device__ __forceinline__ uint64_t dense_lookup(const DenseIndex& d, int32_t type, uint64_t key) {
+    const uint64_t off = (uint64_t)key_value(type, key) - (uint64_t)d.min;
+    return off < d.range ? d.slots[off] : 0;
+}
+
 struct ProbeArgs {
     JoinTable t;
+    DenseIndex dense;
     DCol keys[2];
     int32_t nk;
     int32_t join_type;
@@ -187,7 +207,8 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
         if (i < a.n && (!a.sel || bit_at(a.sel, i))) {
             uint64_t key;
             bool nonnull = key64_of(a.keys, a.nk, i, &key);
-            uint64_t v = (nonnull && a.build_rows) ? lookup(a.t, key) : 0;
+            uint64_t v = 0;
+            if (nonnull && a.build_rows) v = a.dense.slots ? dense_lookup(a.dense, a.keys[0].type, key) : lookup(a.t, key);
             uint32_t matches = (uint32_t)(v >> 32);
             pl = (uint32_t)v;
             if (a.join_type == VR_JOIN_INNER) {
@@ -269,6 +290,40 @@ __global__ __launch_bounds__(256) void cell_stats_kernel(const uint32_t* cell_co
     }
 }
 
+// stats[3] = smallest, stats[4] = largest key of the occupied cells (signed values).
+__global__ void key_range_kernel(const uint64_t* buckets, uint64_t nbuckets, int32_t type, uint64_t* stats) {
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nbuckets * BUCKET; c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = buckets[(c / BUCKET) * 16 + (c % BUCKET)];
+        if (key == EMPTY_KEY) continue;
+        const int64_t v = key_value(type, key);
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+    if (lo <= hi) {
+        atomic_min_i64(&stats[3], lo);
+        atomic_max_i64(&stats[4], hi);
+    }
+}
+__global__ void range_init_kernel(uint64_t* stats) {
+    if (threadIdx.x == 0) {
+        stats[3] = (uint64_t)INT64_MAX;
+        stats[4] = (uint64_t)INT64_MIN;
+    }
+}
+// slots[key - min] = the cell's value word, for every occupied cell.
+__global__ void dense_fill_kernel(const uint64_t* buckets, uint64_t nbuckets, int32_t type, int64_t min, uint64_t* slots) {
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nbuckets * BUCKET; c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t* b = buckets + (c / BUCKET) * 16;
+        const uint64_t key = b[c % BUCKET];
+        if (key == EMPTY_KEY) continue;
+        slots[(uint64_t)key_value(type, key) - (uint64_t)min] = b[BUCKET + (c % BUCKET)];
+    }
+}
+__global__ void store_u64_kernel(uint64_t* p, uint64_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
 __global__ void iota_kernel(uint32_t* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
 }
@@ -299,6 +354,11 @@ struct JoinImpl {
     uint64_t nbuckets = 0;
     uint64_t sentinel_value = 0;
     int64_t distinct = 0, max_chain = 0;
+    // the direct key index (DenseKeyIndex), when the build allows it
+    int32_t dense_mode = 0;  // vr_join_spec.dense_index
+    DevBuf dense;
+    int64_t dense_min = 0;
+    uint64_t dense_range = 0;
     // probe scratch
     DevBuf tile_state, probe_stage, out_stage;
     Staging staging;
@@ -343,6 +403,43 @@ struct vr_join {
     JoinImpl j;
 };
 
+// The largest key range indexed directly, relative to the distinct keys and in slots
+// (DenseKeyIndex.DENSE_FACTOR, MAX_RANGE: DenseKeyIndex.java:36-43).
+constexpr int64_t DENSE_FACTOR = 10;
+constexpr uint64_t DENSE_MAX_RANGE = 1ull << 22;
+
+// DenseKeyIndex.tryBuild (DenseKeyIndex.java:70-110) over the finished table: the key range of the
+// non-null build keys [lo, hi] (the empty marker's key, held beside the table, included), and when
+// it spans at most DENSE_MAX_RANGE slots and DENSE_FACTOR times the distinct keys, one value word
+// per key of the range.
+static vr_status build_dense(JoinImpl& j, bool candidate, int64_t lo, int64_t hi, bool marker, hipStream_t s) {
+    j.dense.reset();
+    j.dense_range = 0;
+    if (!candidate) return VR_OK;
+    if (marker && j.key_types[0] == VR_INT64) {
+        lo = std::min(lo, (int64_t)EMPTY_KEY);
+        hi = std::max(hi, (int64_t)EMPTY_KEY);
+    }
+    if (lo > hi) return VR_OK;
+    const uint64_t span = (uint64_t)hi - (uint64_t)lo;  // exact: hi >= lo
+    if (span >= DENSE_MAX_RANGE || span + 1 > (uint64_t)(DENSE_FACTOR * j.distinct)) return VR_OK;
+    const uint64_t range = span + 1;
+    VR_TRY(j.dense.ensure(j.dev, range * sizeof(uint64_t)));
+    const int cap = j.dev->num_cus * 16;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(range, 256, cap)), dim3(256), 0, s, j.dense.as<uint64_t>(), range);
+    hipLaunchKernelGGL(dense_fill_kernel, dim3(grid_of(j.nbuckets * BUCKET, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(),
+                       j.nbuckets, j.key_types[0], lo, j.dense.as<uint64_t>());
+    if (marker && j.key_types[0] == VR_INT64)
+        hipLaunchKernelGGL(store_u64_kernel, dim3(1), dim3(64), 0, s, j.dense.as<uint64_t>() + (EMPTY_KEY - (uint64_t)lo),
+                           j.sentinel_value);
+    VR_TRY_HIP(hipGetLastError());
+    j.dense_min = lo;
+    j.dense_range = range;
+    j.dev->count(&vr_stats::dense_join_builds, 1);
+    j.dev->count(&vr_stats::kernel_launches, marker ? 3 : 2);
+    return VR_OK;
+}
+
 extern "C" {
 
 vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out) {
@@ -365,6 +462,8 @@ vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out
     }
     for (int32_t t : j.col_types)
         if (t == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "UTF8 payloads run on the CPU kernels");
+    if (spec->dense_index != 0 && spec->dense_index != -1) return fail(VR_ERR_INVALID_ARGUMENT, "dense_index is 0 or -1");
+    j.dense_mode = spec->dense_index;
     j.cols.resize(spec->num_columns);
     VR_TRY_HIP(hipSetDevice(j.dev->ordinal));
     if (spec->capacity_hint > 0)
@@ -478,8 +577,15 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     hipLaunchKernelGGL(cell_stats_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint32_t>(),
                        ncells, j.small.as<uint64_t>());
     VR_TRY_HIP(hipGetLastError());
+    const bool dense_candidate = j.dense_mode == 0 && j.key_types.size() == 1 && n > 0;
+    if (dense_candidate) {
+        hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>());
+        hipLaunchKernelGGL(key_range_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(), nb,
+                           j.key_types[0], j.small.as<uint64_t>());
+        VR_TRY_HIP(hipGetLastError());
+    }
     // the key equal to the empty marker: its rows follow every cell's rows in range_rows
-    uint64_t small[3] = {0, 0, 0};
+    uint64_t small[5] = {0, 0, 0, 0, 0};
     uint32_t last[2] = {0, 0};
     VR_TRY_HIP(hipMemcpyAsync(small, j.small.p, sizeof(small), hipMemcpyDeviceToHost, s));
     VR_TRY_HIP(hipMemcpyAsync(&last[0], j.cell_start.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
@@ -501,6 +607,7 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     }
     j.distinct = (int64_t)small[1] + (sentinel_rows ? 1 : 0);
     j.max_chain = std::max<int64_t>((int64_t)small[2], (int64_t)sentinel_rows);
+    VR_TRY(build_dense(j, dense_candidate, (int64_t)small[3], (int64_t)small[4], sentinel_rows != 0, s));
     j.row_cell.reset();
     j.sorted_cells.reset();
     j.rows_in.reset();
@@ -536,6 +643,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     hipStream_t s = j.dev->pick(stream);
     ProbeArgs a{};
     a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets - 1, j.range_rows.as<uint32_t>(), j.sentinel_value};
+    a.dense = DenseIndex{j.dense_range ? j.dense.as<uint64_t>() : nullptr, j.dense_min, j.dense_range};
     a.nk = (int)j.key_types.size();
     a.join_type = join_type;
     a.drop_null_keys = drop_null_keys;
@@ -692,7 +800,7 @@ vr_status vr_join_memory(vr_join* h, int64_t* bytes) {
     if (!h || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     JoinImpl& j = h->j;
     int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.tile_state.bytes +
-                          j.out_stage.bytes);
+                          j.out_stage.bytes + j.dense.bytes);
     for (auto& c : j.cols) b += (int64_t)(c.data.bytes + c.valid.bytes);
     *bytes = b;
     return VR_OK;
