@@ -273,7 +273,15 @@ vr_status vr_agg_update_filtered(vr_agg* agg, const vr_column* keys, const vr_co
 
 /* ------------------------------------------------------------------ join */
 
-enum vr_join_type { VR_JOIN_INNER = 0, VR_JOIN_LEFT_SEMI = 1, VR_JOIN_LEFT_ANTI = 2 };
+enum vr_join_type {
+    VR_JOIN_INNER = 0,
+    VR_JOIN_LEFT_SEMI = 1,
+    VR_JOIN_LEFT_ANTI = 2,
+    /* The streamed side preserved (LeftOuter built on the right, RightOuter built on the left,
+     * FullOuter): the inner pairs, and (row, -1) with the build columns null for every selected
+     * streamed row that pairs with nothing (VectorHashJoinExec.scala:999-1009,1160-1175). */
+    VR_JOIN_OUTER = 3
+};
 
 typedef struct vr_join_spec {
     int32_t num_keys;
@@ -287,6 +295,9 @@ typedef struct vr_join_spec {
      * kernels/src/main/java/io/vecruntime/kernels/DenseKeyIndex.java:70-110); -1: always the hash
      * table.  The probe's results are the same either way. */
     int32_t dense_index;
+    /* 1: remember every build row an INNER or OUTER probe pairs, for vr_join_unmatched_build (an
+     * outer join that preserves the build side, VectorHashJoinExec.scala:1004-1013). */
+    int32_t track_matched;
 } vr_join_spec;
 
 typedef struct vr_join vr_join;
@@ -325,6 +336,10 @@ vr_status vr_join_probe(vr_join* join, const vr_column* columns, int32_t num_col
                         const int32_t* key_columns, int64_t n, const uint8_t* selection,
                         int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
                         void* stream);
+/* The build rows no probe has paired since the build, as (-1, row) pairs in build order: probe_idx -1, build_idx the row, the streamed columns null (of the
+ * types `columns` gives) and the build columns materialized (emitUnmatchedBuild,
+ * VectorHashJoinExec.scala:1457-1484).  Needs track_matched; `out->probe_out_columns` is ignored. */
+vr_status vr_join_unmatched_build(vr_join* join, vr_probe_output* out, void* stream);
 vr_status vr_join_memory(vr_join* join, int64_t* bytes);
 vr_status vr_join_destroy(vr_join* join);
 

@@ -772,6 +772,8 @@ struct vro_join {
     int64_t numRows = 0;
     std::vector<int32_t> keyCols;
     std::vector<vr_column> cols;  // borrowed build columns
+    bool track = false;           // buildMatched (VectorHashJoinExec.scala:1012)
+    std::vector<uint8_t> matched;
 };
 
 extern "C" {
@@ -892,6 +894,8 @@ int vro_join_build(const vr_join_spec* spec, const vr_column* columns, int64_t n
     j->keyCols.assign(spec->key_columns, spec->key_columns + spec->num_keys);
     j->cols.assign(columns, columns + spec->num_columns);
     j->numRows = n;
+    j->track = spec->track_matched != 0;
+    j->matched.assign(n, 0);
     std::vector<vr_column> keys;
     std::vector<uint8_t> sel((n + 7) / 8, 0xFF);
     for (int32_t c : j->keyCols) keys.push_back(columns[c]);
@@ -914,6 +918,22 @@ int vro_join_build(const vr_join_spec* spec, const vr_column* columns, int64_t n
 }
 
 int64_t vro_join_distinct_keys(vro_join* j) { return j->table->size(); }
+
+// A null row of an output column (the padded side of an outer join's pair).
+static void null_row(vr_out_column& dst, int64_t r) {
+    set_bit(dst.validity, r, false);
+    switch (dst.type) {
+        case VR_INT32: ((int32_t*)dst.data)[r] = 0; break;
+        case VR_BOOL: set_bit((uint8_t*)dst.data, r, false); break;
+        case VR_INT64:
+        case VR_FLOAT64: ((int64_t*)dst.data)[r] = 0; break;
+        case VR_DECIMAL128:
+            ((int64_t*)dst.data)[2 * r] = 0;
+            ((int64_t*)dst.data)[2 * r + 1] = 0;
+            break;
+        default: break;
+    }
+}
 
 static void copy_row(const vr_column& src, int64_t i, vr_out_column& dst, int64_t r) {
     set_bit(dst.validity, r, valid(src, i));
@@ -951,23 +971,34 @@ int vro_join_probe(vro_join* j, const vr_column* columns, int32_t num_columns, c
     if (j->numRows > 0 && n > 0) j->table->lookup(keys.data(), n, ids.data(), cand.data());
     bool dropNull = drop_null_keys && j->numRows > 0;
     int64_t rows = 0;
+    const bool pairs = join_type == VR_JOIN_INNER || join_type == VR_JOIN_OUTER;
     auto emit = [&](int64_t i, int64_t b) {
         if (rows < out->capacity) {
             if (out->probe_idx) out->probe_idx[rows] = (int32_t)i;
-            if (out->build_idx && b >= 0) out->build_idx[rows] = (int32_t)b;
+            if (out->build_idx && pairs) out->build_idx[rows] = (int32_t)b;
             for (int c = 0; c < out->num_probe_out; c++) copy_row(columns[out->probe_out_columns[c]], i, out->columns[c], rows);
-            if (b >= 0)
-                for (int c = 0; c < out->num_build_out; c++)
-                    copy_row(j->cols[out->build_out_columns[c]], b, out->columns[out->num_probe_out + c], rows);
+            for (int c = 0; c < out->num_build_out; c++) {
+                vr_out_column& o = out->columns[out->num_probe_out + c];
+                if (b >= 0) copy_row(j->cols[out->build_out_columns[c]], b, o, rows);
+                else null_row(o, rows);
+            }
         }
         rows++;
     };
     for (int64_t i = 0; i < n; i++) {
         bool sel = !selection || bit(selection, i);
         if (!sel) continue;
-        if (join_type == VR_JOIN_INNER) {
-            if (ids[i] < 0) continue;
-            for (int32_t r = j->head[ids[i]]; r >= 0; r = j->next[r]) emit(i, r);
+        if (pairs) {
+            // emitMatches / the outer path (VectorHashJoinExec.scala:1160-1175): the row's build rows
+            // in build order, each marked matched; a row without one is padded when preserved
+            if (ids[i] < 0) {
+                if (join_type == VR_JOIN_OUTER) emit(i, -1);
+                continue;
+            }
+            for (int32_t r = j->head[ids[i]]; r >= 0; r = j->next[r]) {
+                emit(i, r);
+                if (j->track) j->matched[r] = 1;
+            }
         } else {
             bool matched = ids[i] >= 0 || (dropNull && nullKey[i]);
             if (matched == (join_type == VR_JOIN_LEFT_SEMI)) emit(i, -1);
@@ -975,6 +1006,27 @@ int vro_join_probe(vro_join* j, const vr_column* columns, int32_t num_columns, c
     }
     out->rows = rows;
     if (rows > out->capacity) return fail(VR_ERR_CAPACITY, "probe output needs " + std::to_string(rows) + " rows");
+    return VR_OK;
+}
+
+// emitUnmatchedBuild (VectorHashJoinExec.scala:1457-1484): (-1, r) for every build row no probe
+// paired, in build order, the streamed side null.
+int vro_join_unmatched_build(vro_join* j, vr_probe_output* out) {
+    if (!j->track) return fail(VR_ERR_STATE, "the join does not track matched build rows");
+    int64_t rows = 0;
+    for (int64_t r = 0; r < j->numRows; r++) {
+        if (j->matched[r]) continue;
+        if (rows < out->capacity) {
+            if (out->probe_idx) out->probe_idx[rows] = -1;
+            if (out->build_idx) out->build_idx[rows] = (int32_t)r;
+            for (int c = 0; c < out->num_probe_out; c++) null_row(out->columns[c], rows);
+            for (int c = 0; c < out->num_build_out; c++)
+                copy_row(j->cols[out->build_out_columns[c]], r, out->columns[out->num_probe_out + c], rows);
+        }
+        rows++;
+    }
+    out->rows = rows;
+    if (rows > out->capacity) return fail(VR_ERR_CAPACITY, "unmatched build rows need " + std::to_string(rows) + " rows");
     return VR_OK;
 }
 

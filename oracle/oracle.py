@@ -47,6 +47,7 @@ def lib():
         L.vro_join_distinct_keys.restype = C.c_int64
         L.vro_join_probe.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int32, C.c_int32,
                                      C.POINTER(abi.VrProbeOutput)]
+        L.vro_join_unmatched_build.argtypes = [P, C.POINTER(abi.VrProbeOutput)]
         L.vro_join_destroy.argtypes = [P]
         L.vro_agg_partial_threads.argtypes = [C.POINTER(abi.VrAggSpec), P, P, P, C.c_int32, C.c_int32,
                                               C.POINTER(C.c_double)]
@@ -120,6 +121,7 @@ class OracleJoin:
 
     def __init__(self, spec, columns, n):
         self.spec = spec
+        self.n = n
         self.columns = columns  # borrowed by the oracle: keep alive
         self._ca = c_array(columns)
         self.h = C.c_void_p()
@@ -141,7 +143,7 @@ class OracleJoin:
         ca = c_array(columns)
         kc = abi.int_array(key_columns)
         pidx = np.zeros(capacity, np.int32)
-        bidx = np.zeros(capacity, np.int32) if join_type == abi.INNER else None
+        bidx = np.zeros(capacity, np.int32) if join_type in (abi.INNER, abi.OUTER) else None
         types = [columns[c].type for c in probe_out] + [self.spec.column_types[c] for c in build_out]
         outs = [OutColumn(t, capacity) for t in types]
         oa = c_array(outs, abi.VrOutColumn)
@@ -155,6 +157,22 @@ class OracleJoin:
                                     _ptr(selection), join_type, 1 if drop_null_keys else 0, C.byref(out)))
         r = out.rows
         return pidx[:r], (bidx[:r] if bidx is not None else None), outs, r
+
+    def unmatched_build(self, probe_types=(), build_out=(), capacity=None):
+        """The build rows no probe paired: (probe_idx (-1), build_idx, output columns, rows)."""
+        if capacity is None:
+            capacity = max(1, self.n)
+        pidx = np.zeros(capacity, np.int32)
+        bidx = np.zeros(capacity, np.int32)
+        types = list(probe_types) + [self.spec.column_types[c] for c in build_out]
+        outs = [OutColumn(t, capacity) for t in types]
+        oa = c_array(outs, abi.VrOutColumn)
+        bo = abi.int_array(list(build_out))
+        out = abi.VrProbeOutput(abi.MEM_HOST, capacity, pidx.ctypes.data, bidx.ctypes.data, len(probe_types), None,
+                                len(build_out), C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        _check(lib().vro_join_unmatched_build(self.h, C.byref(out)))
+        r = out.rows
+        return pidx[:r], bidx[:r], outs, r
 
 
 def partial_threads(spec, batch_keys, batch_values, lengths, threads):

@@ -44,7 +44,7 @@ CHECKED = 1
 RESULT = 2
 
 # vr_join_type
-INNER, LEFT_SEMI, LEFT_ANTI = 0, 1, 2
+INNER, LEFT_SEMI, LEFT_ANTI, OUTER = 0, 1, 2, 3
 
 
 class VrColumn(C.Structure):
@@ -154,6 +154,7 @@ class VrJoinSpec(C.Structure):
         ("key_columns", C.POINTER(C.c_int32)),
         ("capacity_hint", C.c_int64),
         ("dense_index", C.c_int32),
+        ("track_matched", C.c_int32),
     ]
 
 
@@ -296,7 +297,7 @@ class AggSpec:
 class JoinSpec:
     """Owns a VrJoinSpec and the arrays it points at."""
 
-    def __init__(self, key_types, column_types, key_columns, capacity_hint=0, dense_index=0):
+    def __init__(self, key_types, column_types, key_columns, capacity_hint=0, dense_index=0, track_matched=False):
         self.key_types = list(key_types)
         self.column_types = list(column_types)
         self.key_columns = list(key_columns)
@@ -305,4 +306,5 @@ class JoinSpec:
         self._kc = int_array(self.key_columns)
         self.c = VrJoinSpec(len(self.key_types), C.cast(self._kt, C.POINTER(C.c_int32)),
                             len(self.column_types), C.cast(self._ct, C.POINTER(C.c_int32)),
-                            C.cast(self._kc, C.POINTER(C.c_int32)), int(capacity_hint), int(dense_index))
+                            C.cast(self._kc, C.POINTER(C.c_int32)), int(capacity_hint), int(dense_index),
+                            1 if track_matched else 0)

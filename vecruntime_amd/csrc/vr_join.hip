@@ -167,9 +167,23 @@ struct ProbeArgs {
     uint64_t* tile_state;   // per tile: (status << 62) | value, zeroed before the launch
     uint64_t* tile_counter; // [0] next tile id, [1] total rows out
     int64_t ntiles;
+    uint32_t* matched;      // build rows paired so far (one bit each), nullptr: not tracked
 };
 
 __device__ __forceinline__ void copy_value(const DCol& src, int64_t i, const OutCol& dst, int64_t o) {
+    if (i < 0) {  // the padded side of an outer pair: null (validity words start cleared), data zero
+        switch (dst.type) {
+            case VR_INT32: ((int32_t*)dst.data)[o] = 0; break;
+            case VR_INT64:
+            case VR_FLOAT64: ((int64_t*)dst.data)[o] = 0; break;
+            case VR_DECIMAL128:
+                ((int64_t*)dst.data)[2 * o] = 0;
+                ((int64_t*)dst.data)[2 * o + 1] = 0;
+                break;
+            default: break;
+        }
+        return;
+    }
     if (dst.valid && (!src.valid || bit_at(src.valid, i))) atomicOr((unsigned int*)dst.valid + (o >> 5), 1u << (o & 31));
     switch (src.type) {
         case VR_INT32:
@@ -199,11 +213,12 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     const int64_t tile = tile_id_sh;
     const int64_t row0 = tile * PROBE_BLOCK * PROBE_ITEMS;
     uint32_t cnt[PROBE_ITEMS], off[PROBE_ITEMS];
-    uint32_t payload[PROBE_ITEMS];
+    uint32_t payload[PROBE_ITEMS], found[PROBE_ITEMS];
+    const bool pairs = a.join_type == VR_JOIN_INNER || a.join_type == VR_JOIN_OUTER;
     for (int r = 0; r < PROBE_ITEMS; r++) {
         int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
         uint32_t c = 0;
-        uint32_t pl = 0;
+        uint32_t pl = 0, fd = 0;
         if (i < a.n && (!a.sel || bit_at(a.sel, i))) {
             uint64_t key;
             bool nonnull = key64_of(a.keys, a.nk, i, &key);
@@ -211,8 +226,9 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
             if (nonnull && a.build_rows) v = a.dense.slots ? dense_lookup(a.dense, a.keys[0].type, key) : lookup(a.t, key);
             uint32_t matches = (uint32_t)(v >> 32);
             pl = (uint32_t)v;
-            if (a.join_type == VR_JOIN_INNER) {
-                c = matches;
+            fd = matches;
+            if (pairs) {
+                c = matches ? matches : (a.join_type == VR_JOIN_OUTER ? 1u : 0u);  // (row, -1) when preserved
             } else {
                 bool matched = matches > 0 || (a.drop_null_keys && a.build_rows && !nonnull);
                 c = (matched == (a.join_type == VR_JOIN_LEFT_SEMI)) ? 1u : 0u;
@@ -220,6 +236,7 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
         }
         cnt[r] = c;
         payload[r] = pl;
+        found[r] = fd;
         uint32_t total;
         Scan(scan_tmp).ExclusiveSum(c, off[r], total);
         if (threadIdx.x == 0) sub_total[r] = total;
@@ -254,7 +271,12 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
         for (uint32_t m = 0; m < cnt[r]; m++, o++) {
             if ((int64_t)o >= a.capacity) break;
             int64_t brow = -1;
-            if (a.join_type == VR_JOIN_INNER) brow = cnt[r] == 1 ? (int64_t)payload[r] : (int64_t)a.t.range_rows[payload[r] + m];
+            if (pairs && found[r]) brow = found[r] == 1 ? (int64_t)payload[r] : (int64_t)a.t.range_rows[payload[r] + m];
+            if (a.matched && brow >= 0) {  // buildMatched(r) = true; bits only ever get set
+                uint32_t* w = a.matched + (brow >> 5);
+                const uint32_t b = 1u << (brow & 31);
+                if (!(*w & b)) atomicOr(w, b);
+            }
             if (a.probe_idx) a.probe_idx[o] = (int32_t)i;
             if (a.build_idx) a.build_idx[o] = (int32_t)brow;
             for (int c = 0; c < a.nprobe_out; c++) copy_value(a.probe_src[c], i, a.outs[c], (int64_t)o);
@@ -332,6 +354,36 @@ __global__ void sentinel_rows_kernel(const uint32_t* row_cell, int64_t n, uint32
         if (row_cell[i] == 0xFFFFFFFEu) out[atomic_add_u64(count, 1)] = (uint32_t)i;
 }
 
+// hipcub selection of the build rows without their matched bit.
+struct UnmatchedRow {
+    const uint32_t* matched;
+    __host__ __device__ __forceinline__ bool operator()(const int32_t& r) const {
+        return !((matched[r >> 5] >> (r & 31)) & 1u);
+    }
+};
+
+struct UnmatchedArgs {
+    const int32_t* rows;  // the unmatched build rows, in build order
+    int64_t m;
+    int32_t* probe_idx;
+    int32_t* build_idx;
+    int32_t nprobe_out, nbuild_out;
+    DCol build_src[8];
+    OutCol outs[16];
+};
+
+// emitUnmatchedBuild's rows (VectorHashJoinExec.scala:1457-1484): (-1, r), the streamed columns
+// null, the build columns gathered.
+__global__ void unmatched_rows_kernel(UnmatchedArgs a) {
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < a.m; o += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = a.rows[o];
+        if (a.probe_idx) a.probe_idx[o] = -1;
+        if (a.build_idx) a.build_idx[o] = (int32_t)r;
+        for (int c = 0; c < a.nprobe_out; c++) copy_value(DCol{}, -1, a.outs[c], o);
+        for (int c = 0; c < a.nbuild_out; c++) copy_value(a.build_src[c], r, a.outs[a.nprobe_out + c], o);
+    }
+}
+
 static int grid_of(uint64_t work, int block, int cap) {
     uint64_t g = (work + block - 1) / block;
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cap));
@@ -354,6 +406,9 @@ struct JoinImpl {
     uint64_t nbuckets = 0;
     uint64_t sentinel_value = 0;
     int64_t distinct = 0, max_chain = 0;
+    // build rows paired by probes (buildMatched), when the spec asks for them
+    bool track = false;
+    DevBuf matched, unmatched;
     // the direct key index (DenseKeyIndex), when the build allows it
     int32_t dense_mode = 0;  // vr_join_spec.dense_index
     DevBuf dense;
@@ -464,6 +519,7 @@ vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out
         if (t == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "UTF8 payloads run on the CPU kernels");
     if (spec->dense_index != 0 && spec->dense_index != -1) return fail(VR_ERR_INVALID_ARGUMENT, "dense_index is 0 or -1");
     j.dense_mode = spec->dense_index;
+    j.track = spec->track_matched != 0;
     j.cols.resize(spec->num_columns);
     VR_TRY_HIP(hipSetDevice(j.dev->ordinal));
     if (spec->capacity_hint > 0)
@@ -608,6 +664,12 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     j.distinct = (int64_t)small[1] + (sentinel_rows ? 1 : 0);
     j.max_chain = std::max<int64_t>((int64_t)small[2], (int64_t)sentinel_rows);
     VR_TRY(build_dense(j, dense_candidate, (int64_t)small[3], (int64_t)small[4], sentinel_rows != 0, s));
+    if (j.track) {
+        const uint64_t words = ((uint64_t)std::max<int64_t>(n, 1) + 63) / 64;
+        VR_TRY(j.matched.ensure(j.dev, words * 8));
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(words, 256, cap)), dim3(256), 0, s, j.matched.as<uint64_t>(), words);
+        VR_TRY_HIP(hipGetLastError());
+    }
     j.row_cell.reset();
     j.sorted_cells.reset();
     j.rows_in.reset();
@@ -626,16 +688,97 @@ vr_status vr_join_info(vr_join* h, int64_t* build_rows, int64_t* distinct_keys, 
     return VR_OK;
 }
 
+// Where a call's outputs go: the caller's device buffers, or a device staging area copied back to
+// the caller's host buffers after the kernel.
+struct OutPlan {
+    bool host = false;
+    uint8_t* ob = nullptr;
+    size_t pidx_off = 0, bidx_off = 0;
+    std::vector<size_t> voff, doff;
+};
+
+// Points the kernel's pair arrays and output columns (of `types`) at their buffers and clears the
+// validity words (bits are OR-ed in).
+static vr_status plan_outputs(JoinImpl& j, const vr_probe_output* out, const std::vector<int32_t>& types, bool want_build_idx,
+                              OutCol* outs, int32_t** pidx, int32_t** bidx, hipStream_t s, OutPlan& P) {
+    const int nout = (int)types.size();
+    const int64_t cap = out->capacity;
+    size_t need = 0;
+    auto place = [&](size_t bytes) {
+        need = align_up(need, 256);
+        size_t o = need;
+        need += bytes;
+        return o;
+    };
+    P.host = out->mem == VR_MEM_HOST;
+    P.voff.assign(nout, 0);
+    P.doff.assign(nout, 0);
+    if (P.host) {
+        if (out->probe_idx) P.pidx_off = place((size_t)cap * 4);
+        if (out->build_idx && want_build_idx) P.bidx_off = place((size_t)cap * 4);
+    }
+    for (int q = 0; q < nout; q++) {
+        const vr_out_column& oc = out->columns[q];
+        if (oc.type != types[q]) return fail(VR_ERR_INVALID_ARGUMENT, "output column lane differs from its source");
+        if (P.host) {
+            if (oc.validity) P.voff[q] = place(align_up(bitmap_bytes(cap), 4));
+            P.doff[q] = place(lane_bytes(types[q], cap));
+        }
+    }
+    if (need) {
+        VR_TRY(j.out_stage.ensure(j.dev, need));
+        P.ob = j.out_stage.as<uint8_t>();
+    }
+    *pidx = P.host ? (out->probe_idx ? (int32_t*)(P.ob + P.pidx_off) : nullptr) : out->probe_idx;
+    *bidx = want_build_idx ? (P.host ? (out->build_idx ? (int32_t*)(P.ob + P.bidx_off) : nullptr) : out->build_idx) : nullptr;
+    for (int q = 0; q < nout; q++) {
+        const vr_out_column& oc = out->columns[q];
+        OutCol& d = outs[q];
+        d.type = oc.type;
+        if (P.host) {
+            d.valid = oc.validity ? P.ob + P.voff[q] : nullptr;
+            d.data = P.ob + P.doff[q];
+        } else {
+            d.valid = oc.validity;
+            d.data = oc.data;
+        }
+        if (d.valid) {
+            uint64_t words = (bitmap_bytes(cap) + 7) / 8;  // output bitmaps are padded to 8 bytes
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(words, 256, 1024)), dim3(256), 0, s, (uint64_t*)d.valid, words);
+        }
+    }
+    return VR_OK;
+}
+
+// Copies `written` rows of staged outputs back to the caller's host buffers.
+static vr_status copy_back(const vr_probe_output* out, const OutPlan& P, int64_t written, bool build_idx) {
+    if (!P.host || written <= 0) return VR_OK;
+    if (out->probe_idx) VR_TRY_HIP(hipMemcpy(out->probe_idx, P.ob + P.pidx_off, (size_t)written * 4, hipMemcpyDeviceToHost));
+    if (out->build_idx && build_idx)
+        VR_TRY_HIP(hipMemcpy(out->build_idx, P.ob + P.bidx_off, (size_t)written * 4, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < P.doff.size(); q++) {
+        const vr_out_column& oc = out->columns[q];
+        if (oc.validity) VR_TRY_HIP(hipMemcpy(oc.validity, P.ob + P.voff[q], bitmap_bytes(written), hipMemcpyDeviceToHost));
+        VR_TRY_HIP(hipMemcpy(oc.data, P.ob + P.doff[q], lane_bytes(oc.type, written), hipMemcpyDeviceToHost));
+    }
+    return VR_OK;
+}
+
 vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_columns, const int32_t* key_columns, int64_t n,
                         const uint8_t* selection, int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
                         void* stream) {
     if (!h || !out || (!columns && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     JoinImpl& j = h->j;
     if (!j.finished) return fail(VR_ERR_STATE, "probe before vr_join_build_finish");
-    if (join_type < VR_JOIN_INNER || join_type > VR_JOIN_LEFT_ANTI) return fail(VR_ERR_INVALID_ARGUMENT, "join type");
-    if (join_type != VR_JOIN_INNER && out->num_build_out) return fail(VR_ERR_INVALID_ARGUMENT, "semi/anti joins output streamed columns only");
+    if (join_type < VR_JOIN_INNER || join_type > VR_JOIN_OUTER) return fail(VR_ERR_INVALID_ARGUMENT, "join type");
+    const bool pairs = join_type == VR_JOIN_INNER || join_type == VR_JOIN_OUTER;
+    if (!pairs && out->num_build_out) return fail(VR_ERR_INVALID_ARGUMENT, "semi/anti joins output streamed columns only");
     if (out->num_probe_out + out->num_build_out > 16 || out->num_probe_out > 8 || out->num_build_out > 8)
         return fail(VR_ERR_UNSUPPORTED, "at most 8 + 8 output columns");
+    if (join_type == VR_JOIN_OUTER)
+        for (int q = 0; q < out->num_build_out; q++)
+            if (!out->columns[out->num_probe_out + q].validity)
+                return fail(VR_ERR_INVALID_ARGUMENT, "an outer join's build columns need validity for the padded rows");
     out->rows = 0;
     if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
     if (n == 0) return VR_OK;
@@ -644,6 +787,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     ProbeArgs a{};
     a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets - 1, j.range_rows.as<uint32_t>(), j.sentinel_value};
     a.dense = DenseIndex{j.dense_range ? j.dense.as<uint64_t>() : nullptr, j.dense_min, j.dense_range};
+    a.matched = (pairs && j.track) ? j.matched.as<uint32_t>() : nullptr;
     a.nk = (int)j.key_types.size();
     a.join_type = join_type;
     a.drop_null_keys = drop_null_keys;
@@ -673,6 +817,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     }
     a.nprobe_out = out->num_probe_out;
     a.nbuild_out = out->num_build_out;
+    std::vector<int32_t> types;
     for (int q = 0; q < a.nprobe_out; q++) {
         int pc = out->probe_out_columns[q];
         if (pc < 0 || pc >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "probe output column out of range");
@@ -681,6 +826,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         a.probe_src[q].type = c.type;
         want(c.validity, bitmap_bytes(n), (const void**)&a.probe_src[q].valid, c.mem);
         want(c.data, lane_bytes(c.type, n), &a.probe_src[q].data, c.mem);
+        types.push_back(c.type);
     }
     for (int q = 0; q < a.nbuild_out; q++) {
         int bc = out->build_out_columns[q];
@@ -688,6 +834,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         a.build_src[q].type = j.col_types[bc];
         a.build_src[q].data = j.cols[bc].data.p;
         a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
+        types.push_back(j.col_types[bc]);
     }
     a.sel = selection;
     if (selection && mem == VR_MEM_HOST) want(selection, bitmap_bytes(n), (const void**)&a.sel, VR_MEM_HOST);
@@ -706,62 +853,9 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         }
         VR_TRY(j.staging.upload(slot, total, s));
     }
-    // outputs: device buffers directly, host buffers through a staging area
-    const int nout = a.nprobe_out + a.nbuild_out;
-    int64_t cap = out->capacity;
-    struct Back {
-        void* host;
-        size_t off;
-        int32_t type;
-        bool bits;
-    };
-    std::vector<Back> back;
-    size_t need = 0;
-    auto place = [&](size_t bytes) {
-        need = align_up(need, 256);
-        size_t o = need;
-        need += bytes;
-        return o;
-    };
-    size_t pidx_off = 0, bidx_off = 0;
-    bool host_out = out->mem == VR_MEM_HOST;
-    if (host_out) {
-        if (out->probe_idx) pidx_off = place((size_t)cap * 4);
-        if (out->build_idx) bidx_off = place((size_t)cap * 4);
-    }
-    std::vector<size_t> voff(nout), doff(nout);
-    for (int q = 0; q < nout; q++) {
-        const vr_out_column& oc = out->columns[q];
-        int32_t t = q < a.nprobe_out ? a.probe_src[q].type : a.build_src[q - a.nprobe_out].type;
-        if (oc.type != t) return fail(VR_ERR_INVALID_ARGUMENT, "output column lane differs from its source");
-        if (host_out) {
-            if (oc.validity) voff[q] = place(align_up(bitmap_bytes(cap), 4));
-            doff[q] = place(lane_bytes(t, cap));
-        }
-    }
-    uint8_t* ob = nullptr;
-    if (need) {
-        VR_TRY(j.out_stage.ensure(j.dev, need));
-        ob = j.out_stage.as<uint8_t>();
-    }
-    a.probe_idx = host_out ? (out->probe_idx ? (int32_t*)(ob + pidx_off) : nullptr) : out->probe_idx;
-    a.build_idx = join_type == VR_JOIN_INNER ? (host_out ? (out->build_idx ? (int32_t*)(ob + bidx_off) : nullptr) : out->build_idx) : nullptr;
-    for (int q = 0; q < nout; q++) {
-        const vr_out_column& oc = out->columns[q];
-        OutCol& d = a.outs[q];
-        d.type = oc.type;
-        if (host_out) {
-            d.valid = oc.validity ? ob + voff[q] : nullptr;
-            d.data = ob + doff[q];
-        } else {
-            d.valid = oc.validity;
-            d.data = oc.data;
-        }
-        if (d.valid) {  // bits are OR-ed in: clear the words first
-            uint64_t words = (bitmap_bytes(cap) + 7) / 8;  // output bitmaps are padded to 8 bytes
-            hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(words, 256, 1024)), dim3(256), 0, s, (uint64_t*)d.valid, words);
-        }
-    }
+    OutPlan P;
+    VR_TRY(plan_outputs(j, out, types, pairs, a.outs, &a.probe_idx, &a.build_idx, s, P));
+    const int64_t cap = out->capacity;
     // tiles and their look-back state
     const int64_t T = (int64_t)PROBE_BLOCK * PROBE_ITEMS;
     a.ntiles = (n + T - 1) / T;
@@ -777,16 +871,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     VR_TRY_HIP(hipStreamSynchronize(s));
     if (slot >= 0) VR_TRY(j.staging.done(slot, s));
     int64_t written = std::min<int64_t>((int64_t)rows, cap);
-    if (host_out && written > 0) {
-        if (out->probe_idx) VR_TRY_HIP(hipMemcpy(out->probe_idx, ob + pidx_off, (size_t)written * 4, hipMemcpyDeviceToHost));
-        if (out->build_idx && join_type == VR_JOIN_INNER)
-            VR_TRY_HIP(hipMemcpy(out->build_idx, ob + bidx_off, (size_t)written * 4, hipMemcpyDeviceToHost));
-        for (int q = 0; q < nout; q++) {
-            const vr_out_column& oc = out->columns[q];
-            if (oc.validity) VR_TRY_HIP(hipMemcpy(oc.validity, ob + voff[q], bitmap_bytes(written), hipMemcpyDeviceToHost));
-            VR_TRY_HIP(hipMemcpy(oc.data, ob + doff[q], lane_bytes(oc.type, written), hipMemcpyDeviceToHost));
-        }
-    }
+    VR_TRY(copy_back(out, P, written, pairs));
     out->rows = (int64_t)rows;
     j.dev->count(&vr_stats::calls, 1);
     j.dev->count(&vr_stats::input_rows, n);
@@ -796,11 +881,68 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     return VR_OK;
 }
 
+vr_status vr_join_unmatched_build(vr_join* h, vr_probe_output* out, void* stream) {
+    if (!h || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    if (!j.finished) return fail(VR_ERR_STATE, "unmatched rows before vr_join_build_finish");
+    if (!j.track) return fail(VR_ERR_STATE, "the join does not track matched build rows (track_matched)");
+    if (out->num_probe_out + out->num_build_out > 16 || out->num_probe_out > 8 || out->num_build_out > 8)
+        return fail(VR_ERR_UNSUPPORTED, "at most 8 + 8 output columns");
+    for (int q = 0; q < out->num_probe_out; q++)
+        if (!out->columns[q].validity) return fail(VR_ERR_INVALID_ARGUMENT, "the null streamed columns need validity");
+    out->rows = 0;
+    hipStream_t s = j.dev->pick(stream);
+    const int64_t n = j.rows;
+    // the unmatched rows in build order
+    VR_TRY(j.unmatched.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4 + 8));
+    int32_t* list = j.unmatched.as<int32_t>();
+    int64_t* count = (int64_t*)(j.unmatched.as<uint8_t>() + (size_t)std::max<int64_t>(n, 1) * 4);
+    int64_t found = 0;
+    if (n > 0) {
+        size_t tmp = 0;
+        hipcub::CountingInputIterator<int32_t> rows_in(0);
+        const UnmatchedRow pick{j.matched.as<uint32_t>()};
+        VR_TRY_HIP(hipcub::DeviceSelect::If(nullptr, tmp, rows_in, list, count, (int)n, pick, s));
+        VR_TRY(j.cub_tmp.ensure(j.dev, tmp));
+        VR_TRY_HIP(hipcub::DeviceSelect::If(j.cub_tmp.p, tmp, rows_in, list, count, (int)n, pick, s));
+        VR_TRY_HIP(hipMemcpyAsync(&found, count, 8, hipMemcpyDeviceToHost, s));
+        VR_TRY_HIP(hipStreamSynchronize(s));
+    }
+    out->rows = found;
+    if (found > out->capacity) return fail(VR_ERR_CAPACITY, "unmatched build rows need " + std::to_string(found) + " rows");
+    if (found == 0) return VR_OK;
+    UnmatchedArgs a{};
+    a.rows = list;
+    a.m = found;
+    a.nprobe_out = out->num_probe_out;
+    a.nbuild_out = out->num_build_out;
+    std::vector<int32_t> types;
+    for (int q = 0; q < a.nprobe_out; q++) types.push_back(out->columns[q].type);
+    for (int q = 0; q < a.nbuild_out; q++) {
+        int bc = out->build_out_columns[q];
+        if (bc < 0 || bc >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "build output column out of range");
+        a.build_src[q].type = j.col_types[bc];
+        a.build_src[q].data = j.cols[bc].data.p;
+        a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
+        types.push_back(j.col_types[bc]);
+    }
+    OutPlan P;
+    VR_TRY(plan_outputs(j, out, types, true, a.outs, &a.probe_idx, &a.build_idx, s, P));
+    hipLaunchKernelGGL(unmatched_rows_kernel, dim3(grid_of((uint64_t)found, 256, j.dev->num_cus * 16)), dim3(256), 0, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    VR_TRY(copy_back(out, P, found, true));
+    j.dev->count(&vr_stats::calls, 1);
+    j.dev->count(&vr_stats::output_rows, found);
+    j.dev->count(&vr_stats::kernel_launches, 3);
+    return VR_OK;
+}
+
 vr_status vr_join_memory(vr_join* h, int64_t* bytes) {
     if (!h || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     JoinImpl& j = h->j;
     int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.tile_state.bytes +
-                          j.out_stage.bytes + j.dense.bytes);
+                          j.out_stage.bytes + j.dense.bytes + j.matched.bytes + j.unmatched.bytes);
     for (auto& c : j.cols) b += (int64_t)(c.data.bytes + c.valid.bytes);
     *bytes = b;
     return VR_OK;

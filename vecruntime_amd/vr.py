@@ -61,6 +61,7 @@ _sig("vr_join_build_finish", [_P, _P])
 _sig("vr_join_info", [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
 _sig("vr_join_probe", [_P, _P, C.c_int32, _P, C.c_int64, _P, C.c_int32, C.c_int32,
                        C.POINTER(abi.VrProbeOutput), _P])
+_sig("vr_join_unmatched_build", [_P, C.POINTER(abi.VrProbeOutput), _P])
 _sig("vr_join_memory", [_P, C.POINTER(C.c_int64)])
 _sig("vr_join_destroy", [_P])
 
@@ -294,7 +295,7 @@ class BuildTable:
         if mem == abi.MEM_HOST:
             if want_pairs and probe_idx is None:
                 probe_idx = np.zeros(capacity, np.int32)
-            if want_pairs and build_idx is None and join_type == abi.INNER:
+            if want_pairs and build_idx is None and join_type in (abi.INNER, abi.OUTER):
                 build_idx = np.zeros(capacity, np.int32)
             if outputs is None:
                 types = [columns[c].type for c in probe_out] + [self.spec.column_types[c] for c in build_out]
@@ -302,7 +303,8 @@ class BuildTable:
         outputs = outputs or []
         oa = c_array(outputs, abi.VrOutColumn)
         po, bo = abi.int_array(list(probe_out)), abi.int_array(list(build_out))
-        out = abi.VrProbeOutput(mem, int(capacity), _ptr(probe_idx), _ptr(build_idx) if join_type == abi.INNER else None,
+        pairs = join_type in (abi.INNER, abi.OUTER)
+        out = abi.VrProbeOutput(mem, int(capacity), _ptr(probe_idx), _ptr(build_idx) if pairs else None,
                                 len(probe_out), C.cast(po, C.POINTER(C.c_int32)), len(build_out),
                                 C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
         st = _lib.vr_join_probe(self.h, C.cast(ca, _P), len(columns), C.cast(kc, _P), int(n), _ptr(selection),
@@ -313,6 +315,22 @@ class BuildTable:
             return (probe_idx[:r] if probe_idx is not None else None,
                     build_idx[:r] if build_idx is not None else None, outputs, r)
         return probe_idx, build_idx, outputs, r
+
+    def unmatched_build(self, probe_types=(), build_out=(), capacity=None, stream=None):
+        """vr_join_unmatched_build into host buffers: (probe_idx (-1), build_idx, output columns, rows)."""
+        if capacity is None:
+            capacity = max(1, self.info()[0])
+        pidx = np.zeros(capacity, np.int32)
+        bidx = np.zeros(capacity, np.int32)
+        types = list(probe_types) + [self.spec.column_types[c] for c in build_out]
+        outputs = [OutColumn(t, capacity) for t in types]
+        oa = c_array(outputs, abi.VrOutColumn)
+        bo = abi.int_array(list(build_out))
+        out = abi.VrProbeOutput(abi.MEM_HOST, int(capacity), _ptr(pidx), _ptr(bidx), len(probe_types), None,
+                                len(build_out), C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        check(_lib.vr_join_unmatched_build(self.h, C.byref(out), _stream(stream)))
+        r = out.rows
+        return pidx[:r], bidx[:r], outputs, r
 
 
 def agg_key_dictionary(handle, c):
