@@ -361,3 +361,39 @@ def test_large_regions_claim_by_cas(dev):
     assert t.count == 1
     o = oracle_cfg2(spec, *data)
     check(g, o, spec)
+
+
+@pytest.mark.parametrize("mode", [FORCE, GENERIC])
+def test_uniform_keys_fill_the_record_pool(dev, mode):
+    """Every row a record (uniform keys, no hot set to speak of): 85M rows over 10M groups in
+    65,536-row device batches, the shape whose record pool once ran out.  Same groups and states
+    as the per-tile kernel, compared column by column after sorting by key."""
+    import torch
+    from vecruntime_amd import vr
+    n, G, bs = 85_000_000, 10_000_000, 65536
+    print("uniform record pool: %s" % mode, flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(12)
+    keys = torch.randint(0, G, (n,), dtype=torch.int64, device="cuda", generator=g) * 2654435761
+    vals = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
+    spec = abi.AggSpec([abi.INT64], [abi.INT32],
+                       [abi.agg_desc(abi.COUNT_STAR), abi.agg_desc(abi.SUM_LONG, 0), abi.agg_desc(abi.MIN, 0),
+                        abi.agg_desc(abi.MAX, 0)], key_nullable=[False], capacity_hint=G)
+    batches = [([vr.device_column(abi.INT64, keys[s:s + bs])], [vr.device_column(abi.INT32, vals[s:s + bs])],
+                min(bs, n - s)) for s in range(0, n, bs)]
+    out = []
+    for m in (mode, NEVER):
+        a = vr.GroupedAggregate(dev, spec)
+        with Took(dev) as t:
+            a.update_batches(batches, config=abi.launch_config(**m))
+        assert t.count == (1 if m is not NEVER else 0)
+        k, o = a.emit()
+        m_ = a.num_groups()
+        kk = k[0].data[:m_].copy()
+        order = np.argsort(kk, kind="stable")
+        out.append([kk[order]] + [c.data[:m_][order].copy() for c in o])
+        a.close()
+    assert len(out[0][0]) == len(out[1][0])
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+    assert int(out[0][1].sum()) == n

@@ -1544,14 +1544,25 @@ static int merge_cache_entries(const AggImpl& h) {
     return 0;
 }
 
+// Record blocks one partitioned update over `rows` rows may take from the pool: a block per
+// PART_BLK records (every active row a record) plus the open last block of each of the grid * P
+// chains; each refill of a workgroup's share leaves under PA_GAP blocks unused (the region whose
+// blocks straddle the end of the old share), and a share is refilled only after at least
+// PA_CHUNK - PA_GAP of its blocks went to records; the last share of each workgroup may go unused.
+static uint64_t pool_blocks(int64_t rows, uint64_t grid, uint64_t P) {
+    const uint64_t used = (uint64_t)(rows + PART_BLK - 1) / PART_BLK + grid * P;
+    const uint64_t refills = used / (uint64_t)(PA_CHUNK - PA_GAP) + grid;
+    return used + refills * (uint64_t)PA_GAP + grid * (uint64_t)PA_CHUNK + 1;
+}
+static_assert(PA_CHUNK > 2 * PA_GAP, "a share outlasts a tile's straddling gap");
+
 static bool partitioned_applies(const AggImpl& h, int64_t rows, RecLayout* L, const vr_launch_config* cfg) {
     const int32_t mode = (cfg && cfg->partitioned) ? cfg->partitioned : h.cfg.partitioned;
     if (mode < 0 || (mode == 0 && rows < h.part_min_rows)) return false;
     if (h.det || h.proto.ks.mode == KM_WIDE || rows <= 0) return false;
     // block numbers of the record pool fit a merge list entry; one chain per lane of a merge wave
     const uint64_t cus = (uint64_t)h.dev->num_cus;
-    if ((uint64_t)(rows + PART_BLK - 1) / PART_BLK + cus * ((1ull << PART_MAX_PBITS) + PA_CHUNK) + 1 >= (1ull << PB_ID_BITS))
-        return false;
+    if (pool_blocks(rows, cus, 1ull << PART_MAX_PBITS) >= (1ull << PB_ID_BITS)) return false;
     if (cus > (uint64_t)PB_BLOCK) return false;
     if (!record_layout(h, *L)) return false;
     ColProg prog[VR_MAX_VALUES];
@@ -1638,10 +1649,8 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
         pa.hot_part = h.p_hot_part.as<uint64_t>();
         pa.hot_total = h.p_hot_total.as<uint64_t>();
     }
-    // the record pool: every active row could be a record, plus one open block per chain and the
-    // unused rest of each workgroup's last share of the pool
-    const uint64_t max_blocks =
-        (uint64_t)(rows + PART_BLK - 1) / PART_BLK + (uint64_t)grid_a * P + (uint64_t)grid_a * PA_CHUNK + 1;
+    // the record pool
+    const uint64_t max_blocks = pool_blocks(rows, (uint64_t)grid_a, (uint64_t)P);
     if (max_blocks >= (1ull << PB_ID_BITS)) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
     VR_TRY(h.p_pool.ensure(h.dev, max_blocks * PART_BLK * L.words * sizeof(uint32_t)));
     VR_TRY(h.p_next.ensure(h.dev, max_blocks * sizeof(uint32_t)));

@@ -304,8 +304,9 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
     uint32_t* start = fill + P;  // `fill` at the start of the tile
     uint32_t* cur = start + P;   // the chain's newest block
     uint32_t* first = cur + P;   // first block taken for the tile's records
-    // blocks taken from the pool and not handed out yet: [s_next, s_end)
-    __shared__ uint32_t s_next, s_end, s_need;
+    // blocks taken from the pool and not handed out yet: [s_next, s_end); a tile needing more takes
+    // a new range [s_alt, s_alt_end) for the regions the rest cannot hold, which then replaces it
+    __shared__ uint32_t s_next, s_end, s_need, s_alt, s_alt_end, s_fail;
 
     for (int w = threadIdx.x; w < K * HW; w += blockDim.x) {
         const int j = w % HW;
@@ -322,6 +323,9 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         s_next = 0;
         s_end = 0;
         s_need = 0;
+        s_alt = 0;
+        s_alt_end = 0;
+        s_fail = 0;
     }
     __syncthreads();
     const uint32_t hbmask = (uint32_t)(H / HB - 1);
@@ -466,7 +470,8 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         }
         __syncthreads();
         // blocks for the positions [start, fill) the tile took in each chain, from the
-        // workgroup's share of the pool (refilled with one device atomic per PA_CHUNK blocks)
+        // workgroup's share of the pool (refilled with one device atomic per PA_CHUNK blocks; the
+        // one region whose blocks straddle the end of the old share leaves under PA_GAP unused)
         for (int p = threadIdx.x; p < P; p += blockDim.x) {
             const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - (start[p] + PART_BLK - 1) / PART_BLK;
             if (need) atomicAdd(&s_need, need);
@@ -474,15 +479,15 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         __syncthreads();
         if (threadIdx.x == 0) {
             const uint32_t need = s_need;
-            if (s_end - s_next < need) {
+            if (!s_fail && s_end - s_next < need) {
                 const uint32_t take = need > (uint32_t)PA_CHUNK ? need : (uint32_t)PA_CHUNK;
                 const uint64_t b0 = atomic_add_u64(&u.t.counters[C_POOL], take);
                 if (b0 + take > a.max_blocks) {
                     atomicOr((unsigned long long*)&u.t.counters[C_ERROR], (unsigned long long)ERR_TABLE_FULL);
-                    s_next = s_end = 0;  // nothing handed out: the launch fails
+                    s_fail = 1;  // nothing handed out from here on: the launch fails
                 } else {
-                    s_next = (uint32_t)b0;
-                    s_end = (uint32_t)(b0 + take);
+                    s_alt = (uint32_t)b0;
+                    s_alt_end = (uint32_t)(b0 + take);
                 }
             }
             s_need = 0;
@@ -492,11 +497,13 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
             const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - have;
             if (need == 0) continue;
-            if (s_end == 0) {
+            if (s_fail) {
                 first[p] = NO_BLOCK;
                 continue;
             }
-            const uint32_t b0 = atomicAdd(&s_next, need);
+            uint32_t b0 = atomicAdd(&s_next, need);
+            // the rest of the old share cannot hold it: the new range (which holds the whole tile)
+            if (b0 + need > s_end) b0 = atomicAdd(&s_alt, need);
             first[p] = b0;
             for (uint32_t j = 0; j < need; j++) {
                 a.block_next[b0 + j] = j ? b0 + j - 1 : cur[p];
@@ -577,6 +584,11 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             start[p] = f;
         }
         if (more && threadIdx.x < DESC_WORDS) ((uint64_t*)&lb[cur_buf ^ 1])[threadIdx.x] = next_word;
+        if (threadIdx.x == 0 && s_alt_end) {  // the new range is the share from here on
+            s_next = s_alt;
+            s_end = s_alt_end;
+            s_alt = s_alt_end = 0;
+        }
         __syncthreads();
         cur_buf ^= 1;
     }

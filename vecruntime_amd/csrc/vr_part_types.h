@@ -13,7 +13,8 @@ constexpr int PA_BLOCK = 1024;       // scatter workgroup
 constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile (generic kernel)
 constexpr int PA_JIT_RPT = 4;        // the same, kernels compiled for their spec (fewer registers per row)
 constexpr int PART_MAX_PROG = 4;     // value columns the partitioned update folds (programs)
-constexpr int PA_CHUNK = 256;        // pool blocks a scatter workgroup takes at a time
+constexpr int PA_CHUNK = 1024;       // pool blocks a scatter workgroup takes at a time
+constexpr int PA_GAP = PA_BLOCK * PA_JIT_RPT / PART_BLK + 1;  // most blocks one region takes in one tile
 constexpr int PB_BLOCK = 1024;       // merge workgroup
 constexpr int PART_MAX_PBITS = 12;   // regions of the partitioned update: at most 2^12
 constexpr int HS_BLOCK = 256;        // sampling workgroup
