@@ -336,12 +336,51 @@ vr_status vr_join_probe(vr_join* join, const vr_column* columns, int32_t num_col
                         const int32_t* key_columns, int64_t n, const uint8_t* selection,
                         int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
                         void* stream);
-/* The build rows no probe has paired since the build, as (-1, row) pairs in build order: probe_idx -1, build_idx the row, the streamed columns null (of the
- * types `columns` gives) and the build columns materialized (emitUnmatchedBuild,
- * VectorHashJoinExec.scala:1457-1484).  Needs track_matched; `out->probe_out_columns` is ignored. */
+/* The build rows no probe has paired since the build, as (-1, row) pairs in build order:
+ * probe_idx -1, build_idx the row, the streamed columns null (of the types `columns` gives) and the
+ * build columns materialized (emitUnmatchedBuild, VectorHashJoinExec.scala:1457-1484).  Needs
+ * track_matched; `out->probe_out_columns` is ignored. */
 vr_status vr_join_unmatched_build(vr_join* join, vr_probe_output* out, void* stream);
 vr_status vr_join_memory(vr_join* join, int64_t* bytes);
 vr_status vr_join_destroy(vr_join* join);
+
+/* ------------------------------------------------------------------ filters (selection producers) */
+
+/* Comparison operators (CompareOp, kernels/src/main/java/io/vecruntime/kernels/CompareOp.java). */
+enum vr_compare_op { VR_CMP_EQ = 0, VR_CMP_NE = 1, VR_CMP_LT = 2, VR_CMP_LE = 3, VR_CMP_GT = 4, VR_CMP_GE = 5 };
+
+/* A comparison's literal: `i64` for INT32 (narrowed as Java's intValue) and INT64 lanes, `f64` for
+ * FLOAT64, the unscaled value's two limbs for DECIMAL128 (already at the column's scale). */
+typedef struct vr_scalar {
+    int64_t i64;
+    double f64;
+    int64_t lo, hi;
+} vr_scalar;
+
+/* The selection of `col <op> scalar` over rows [0, n) (CompareKernels.compareScalar,
+ * kernels/src/main/java/io/vecruntime/kernels/CompareKernels.java:57-160, turned into a selection by
+ * VectorExpr.selection, spark/src/main/scala/io/vecruntime/spark/expr/VectorExpr.scala:438-443):
+ * bit i of `out` = row i selected by `selection` (NULL: every row) AND non-null AND the comparison
+ * true; FLOAT64 compares in Spark's NaN-safe order (NaN equals NaN and is above everything, -0.0
+ * equals 0.0).  `out` holds ceil(n/8) bytes padded to 8; `count` (may be NULL) receives its set
+ * bits.  INT32 / INT64 / FLOAT64 / DECIMAL128 lanes; the column and `out` in device memory. */
+vr_status vr_filter_compare(vr_device* dev, const vr_column* col, int32_t op, const vr_scalar* scalar,
+                            const uint8_t* selection, int64_t n, uint8_t* out, int64_t* count, void* stream);
+
+/* Bytes of the plain UTF8 values of the rows set in `selection` (CompactKernels.selectedUtf8Bytes,
+ * CompactKernels.java:379-408): what a compacted UTF8 column's `data` must hold. */
+vr_status vr_compact_utf8_bytes(vr_device* dev, const vr_column* col, int64_t n, const uint8_t* selection,
+                                int64_t* bytes, void* stream);
+
+/* Compacts `num_columns` columns of n rows to the rows set in `selection`, in row order
+ * (CompactKernels.compactFixed / compactUtf8, CompactKernels.java:108-471; the compacting branch of
+ * VectorFilterExec.process, spark/.../sql/vecruntime/VectorFilterExec.scala:80-115): out[c] gets the
+ * values (fixed-width lanes, BOOL bits, the int32 codes of a dictionary-encoded UTF8 column, or a
+ * plain UTF8 column's offsets and bytes) and, when it has a validity buffer, the validity bits.
+ * `rows` (may be NULL) receives the selected count; every out[c] needs that capacity.  Columns and
+ * outputs in device memory. */
+vr_status vr_compact(vr_device* dev, const vr_column* columns, int32_t num_columns, int64_t n,
+                     const uint8_t* selection, vr_out_column* out, int64_t* rows, void* stream);
 
 #ifdef __cplusplus
 }

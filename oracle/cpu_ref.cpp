@@ -1032,6 +1032,98 @@ int vro_join_unmatched_build(vro_join* j, vr_probe_output* out) {
 
 void vro_join_destroy(vro_join* j) { delete j; }
 
+// ------------------------------------------------------------------ filters
+
+// CompareKernels.compareScalar (CompareKernels.java:57-160) turned into a selection by
+// VectorExpr.selection (VectorExpr.scala:438-443): selected AND non-null AND the comparison, doubles
+// in CompareOp.nanSafeCompare's order (CompareOp.java:68-82), decimals by Decimal128.compare
+// (signed high limb, then unsigned low limb, Decimal128.java:145).
+int vro_filter_compare(const vr_column* col, int32_t op, const vr_scalar* sc, const uint8_t* selection, int64_t n,
+                       uint8_t* out, int64_t* count) {
+    int64_t c = 0;
+    std::memset(out, 0, (size_t)((n + 63) / 64 * 8));
+    for (int64_t i = 0; i < n; i++) {
+        if ((selection && !bit(selection, i)) || !valid(*col, i)) continue;
+        int r = 0;
+        switch (col->type) {
+            case VR_INT32: {
+                int32_t x = ((const int32_t*)col->data)[i], y = (int32_t)(uint32_t)(uint64_t)sc->i64;
+                r = (x > y) - (x < y);
+                break;
+            }
+            case VR_INT64: {
+                int64_t x = ((const int64_t*)col->data)[i], y = sc->i64;
+                r = (x > y) - (x < y);
+                break;
+            }
+            case VR_FLOAT64: r = nan_safe_compare(((const double*)col->data)[i], sc->f64); break;
+            case VR_DECIMAL128: {
+                const int64_t* p = (const int64_t*)col->data + 2 * i;
+                if (p[1] != sc->hi) r = p[1] > sc->hi ? 1 : -1;
+                else r = ((uint64_t)p[0] > (uint64_t)sc->lo) - ((uint64_t)p[0] < (uint64_t)sc->lo);
+                break;
+            }
+            default: return fail(VR_ERR_UNSUPPORTED, "compare lane");
+        }
+        bool t = false;
+        switch (op) {
+            case VR_CMP_EQ: t = r == 0; break;
+            case VR_CMP_NE: t = r != 0; break;
+            case VR_CMP_LT: t = r < 0; break;
+            case VR_CMP_LE: t = r <= 0; break;
+            case VR_CMP_GT: t = r > 0; break;
+            case VR_CMP_GE: t = r >= 0; break;
+            default: return fail(VR_ERR_INVALID_ARGUMENT, "comparison operator");
+        }
+        if (t) {
+            set_bit(out, i, true);
+            c++;
+        }
+    }
+    if (count) *count = c;
+    return VR_OK;
+}
+
+// CompactKernels.compactFixed / compactUtf8 (CompactKernels.java:108-471): the selected rows of each
+// column in row order, values and validity; a dictionary-encoded column compacts its int32 codes.
+int vro_compact(const vr_column* cols, int32_t ncols, int64_t n, const uint8_t* selection, vr_out_column* out,
+                int64_t* rows) {
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++) m += !selection || bit(selection, i);
+    if (rows) *rows = m;
+    for (int c = 0; c < ncols; c++) {
+        const vr_column& in = cols[c];
+        vr_out_column& o = out[c];
+        if (o.capacity < m) return fail(VR_ERR_CAPACITY, "compacted columns need " + std::to_string(m) + " rows");
+        const bool dict = in.type == VR_UTF8 && in.dictionary;
+        int64_t r = 0;
+        int32_t pos = 0;
+        if (in.type == VR_UTF8 && !dict) o.offsets[0] = 0;
+        for (int64_t i = 0; i < n; i++) {
+            if (selection && !bit(selection, i)) continue;
+            if (o.validity) set_bit(o.validity, r, valid(in, i));
+            switch (dict ? VR_INT32 : in.type) {
+                case VR_INT32: ((int32_t*)o.data)[r] = ((const int32_t*)in.data)[i]; break;
+                case VR_INT64:
+                case VR_FLOAT64: ((int64_t*)o.data)[r] = ((const int64_t*)in.data)[i]; break;
+                case VR_DECIMAL128:
+                    ((int64_t*)o.data)[2 * r] = ((const int64_t*)in.data)[2 * i];
+                    ((int64_t*)o.data)[2 * r + 1] = ((const int64_t*)in.data)[2 * i + 1];
+                    break;
+                case VR_BOOL: set_bit((uint8_t*)o.data, r, bit((const uint8_t*)in.data, i)); break;
+                default: {  // plain UTF8
+                    const int32_t a = in.offsets[i], len = in.offsets[i + 1] - a;
+                    std::memcpy((uint8_t*)o.data + pos, (const uint8_t*)in.data + a, (size_t)len);
+                    pos += len;
+                    o.offsets[r + 1] = pos;
+                }
+            }
+            r++;
+        }
+    }
+    return VR_OK;
+}
+
 // ------------------------------------------------------------------ CPU baseline driver
 
 // Aggregates `num_batches` batches split round-robin over `threads` independent tables, as

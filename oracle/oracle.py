@@ -48,6 +48,9 @@ def lib():
         L.vro_join_probe.argtypes = [P, P, C.c_int32, P, C.c_int64, P, C.c_int32, C.c_int32,
                                      C.POINTER(abi.VrProbeOutput)]
         L.vro_join_unmatched_build.argtypes = [P, C.POINTER(abi.VrProbeOutput)]
+        L.vro_filter_compare.argtypes = [C.POINTER(abi.VrColumn), C.c_int32, C.POINTER(abi.VrScalar), P, C.c_int64, P,
+                                         C.POINTER(C.c_int64)]
+        L.vro_compact.argtypes = [P, C.c_int32, C.c_int64, P, P, C.POINTER(C.c_int64)]
         L.vro_join_destroy.argtypes = [P]
         L.vro_agg_partial_threads.argtypes = [C.POINTER(abi.VrAggSpec), P, P, P, C.c_int32, C.c_int32,
                                               C.POINTER(C.c_double)]
@@ -186,3 +189,31 @@ def partial_threads(spec, batch_keys, batch_values, lengths, threads):
     g = lib().vro_agg_partial_threads(C.byref(spec.c), C.cast(ka, C.c_void_p), C.cast(va, C.c_void_p),
                                       ln.ctypes.data, nb, threads, C.byref(secs))
     return g, secs.value
+
+
+def filter_compare(col, op, value, n, selection=None):
+    """The selection of `col <op> value` over rows [0, n): (bitmap bytes padded to 8, count)."""
+    out = np.zeros(max(8, ((n + 63) // 64) * 8), np.uint8)
+    sc = abi.scalar(value, col.type)
+    cnt = C.c_int64()
+    _check(lib().vro_filter_compare(C.byref(col.c()), int(op), C.byref(sc), _ptr(selection), int(n), out.ctypes.data,
+                                    C.byref(cnt)))
+    return out, cnt.value
+
+
+def compact(columns, n, selection, types, capacity, utf8_bytes=0):
+    """The selected rows of each column: (OutColumns, rows).  `types`: the output lane per column."""
+    outs = []
+    for t in types:
+        if t != abi.UTF8:
+            oc = OutColumn(t, capacity)
+        else:
+            oc = OutColumn(t, capacity, data=np.zeros(max(1, utf8_bytes), np.uint8))
+            oc.offsets = np.zeros(capacity + 1, np.int32)
+            oc._c = abi.VrOutColumn(t, abi.MEM_HOST, capacity, _ptr(oc.validity), _ptr(oc.data), _ptr(oc.offsets))
+        outs.append(oc)
+    ca, oa = c_array(columns), c_array(outs, abi.VrOutColumn)
+    rows = C.c_int64()
+    _check(lib().vro_compact(C.cast(ca, C.c_void_p), len(columns), int(n), _ptr(selection), C.cast(oa, C.c_void_p),
+                             C.byref(rows)))
+    return outs, rows.value

@@ -45,6 +45,7 @@ RESULT = 2
 
 # vr_join_type
 INNER, LEFT_SEMI, LEFT_ANTI, OUTER = 0, 1, 2, 3
+CMP_EQ, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE = 0, 1, 2, 3, 4, 5
 
 
 class VrColumn(C.Structure):
@@ -143,6 +144,23 @@ def launch_config(rows_per_thread=0, blocks_per_cu=0, lds=-1, lds_entries=0, flu
     """partitioned: 1 takes the partitioned update at any size, -1 never, 0 from 4M rows per call.
     specialize: -1 runs the partitioned update's generic kernels instead of ones compiled for the spec."""
     return VrLaunchConfig(0, rows_per_thread, blocks_per_cu, lds, lds_entries, flush_tiles, partitioned, specialize)
+
+
+class VrScalar(C.Structure):
+    _fields_ = [("i64", C.c_int64), ("f64", C.c_double), ("lo", C.c_int64), ("hi", C.c_int64)]
+
+
+def scalar(value, type):
+    """A comparison literal for a lane of `type`: ints for INT32/INT64, float for FLOAT64, the
+    unscaled Python int for DECIMAL128."""
+    if type == DECIMAL128:
+        x = int(value) & ((1 << 128) - 1)
+        lo, hi = x & ((1 << 64) - 1), x >> 64
+        return VrScalar(0, 0.0, lo - (1 << 64) if lo >= 1 << 63 else lo, hi - (1 << 64) if hi >= 1 << 63 else hi)
+    if type == FLOAT64:
+        return VrScalar(0, float(value), 0, 0)
+    x = int(value) & ((1 << 64) - 1)
+    return VrScalar(x - (1 << 64) if x >= 1 << 63 else x, 0.0, 0, 0)
 
 
 class VrJoinSpec(C.Structure):

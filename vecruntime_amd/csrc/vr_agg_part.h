@@ -493,6 +493,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             s_need = 0;
         }
         __syncthreads();
+        const bool refilled = s_alt_end != 0;  // else the share holds the whole tile
         for (int p = threadIdx.x; p < P; p += blockDim.x) {
             const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
             const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - have;
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             }
             uint32_t b0 = atomicAdd(&s_next, need);
             // the rest of the old share cannot hold it: the new range (which holds the whole tile)
-            if (b0 + need > s_end) b0 = atomicAdd(&s_alt, need);
+            if (refilled && b0 + need > s_end) b0 = atomicAdd(&s_alt, need);
             first[p] = b0;
             for (uint32_t j = 0; j < need; j++) {
                 a.block_next[b0 + j] = j ? b0 + j - 1 : cur[p];

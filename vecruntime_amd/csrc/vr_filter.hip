@@ -1,0 +1,377 @@
+// Selection producers and compaction on gfx950: vr_filter_compare, vr_compact_utf8_bytes and
+// vr_compact of include/vr.h.
+//
+// Replaces the filter operator's predicate-to-selection step and its compacting branch
+// (spark/src/main/scala/org/apache/spark/sql/vecruntime/VectorFilterExec.scala:80-115) for a
+// column-versus-literal comparison: CompareKernels.compareScalar
+// (kernels/src/main/java/io/vecruntime/kernels/CompareKernels.java:57-160) + VectorExpr.selection
+// (spark/src/main/scala/io/vecruntime/spark/expr/VectorExpr.scala:438-443), then
+// CompactKernels.compactFixed / compactUtf8 (CompactKernels.java:108-471).
+//
+// Every kernel is a streaming pass bound by HBM: one wave per 64-row word (the selection, validity
+// and BOOL bitmaps are 64-bit words, built with a ballot), coalesced lane reads, and compacted
+// output positions from an exclusive scan of the words' popcounts.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "vr_device.h"
+#include "vr_internal.h"
+
+namespace vr {
+namespace {
+
+constexpr int FILTER_BLOCK = 256;  // four waves, one 64-row word each per step
+
+__device__ __forceinline__ uint64_t word_at(const uint8_t* bm, int64_t w, int64_t n) {
+    // the bits of rows [64w, min(64w + 64, n)) of an LSB-first bitmap; a null bitmap is all ones
+    const int64_t base = w * 64;
+    const int64_t left = n - base;
+    const uint64_t live = left >= 64 ? ~0ull : ((1ull << left) - 1);
+    if (!bm) return live;
+    uint64_t x = 0;
+    const int64_t bytes = (left + 7) / 8 < 8 ? (left + 7) / 8 : 8;
+    for (int64_t b = 0; b < bytes; b++) x |= (uint64_t)bm[w * 8 + b] << (8 * b);
+    return x & live;
+}
+
+// Three-way comparison of row i against the literal (CompareOp.nanSafeCompare for doubles,
+// Decimal128.compare for the two-limb lane: signed high limbs, then unsigned low limbs).
+__device__ __forceinline__ int compare_row(int32_t type, const void* data, int64_t i, const vr_scalar& s) {
+    switch (type) {
+        case VR_INT32: {
+            const int32_t x = ((const int32_t*)data)[i], y = (int32_t)s.i64;
+            return (x > y) - (x < y);
+        }
+        case VR_INT64: {
+            const int64_t x = ((const int64_t*)data)[i], y = s.i64;
+            return (x > y) - (x < y);
+        }
+        case VR_FLOAT64: {
+            const double x = ((const double*)data)[i], y = s.f64;
+            const bool xn = x != x, yn = y != y;
+            if ((xn && yn) || x == y) return 0;
+            if (xn) return 1;
+            if (yn) return -1;
+            return x > y ? 1 : -1;
+        }
+        default: {  // VR_DECIMAL128
+            const int64_t* p = (const int64_t*)data + 2 * i;
+            const int64_t xh = p[1], yh = s.hi;
+            if (xh != yh) return xh > yh ? 1 : -1;
+            const uint64_t xl = (uint64_t)p[0], yl = (uint64_t)s.lo;
+            return (xl > yl) - (xl < yl);
+        }
+    }
+}
+
+__device__ __forceinline__ bool holds(int op, int c) {
+    switch (op) {
+        case VR_CMP_EQ: return c == 0;
+        case VR_CMP_NE: return c != 0;
+        case VR_CMP_LT: return c < 0;
+        case VR_CMP_LE: return c <= 0;
+        case VR_CMP_GT: return c > 0;
+        default: return c >= 0;
+    }
+}
+
+struct CompareArgs {
+    const void* data;
+    const uint8_t* valid;
+    const uint8_t* sel;
+    int64_t n;
+    int32_t type, op;
+    vr_scalar s;
+    uint64_t* out;
+    unsigned long long* count;
+};
+
+__global__ __launch_bounds__(FILTER_BLOCK) void compare_kernel(CompareArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t words = (a.n + 63) / 64;
+    const int64_t waves = (int64_t)gridDim.x * (FILTER_BLOCK / 64);
+    unsigned long long total = 0;
+    for (int64_t w = (int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64; w < words; w += waves) {
+        const uint64_t live = word_at(a.sel, w, a.n) & word_at(a.valid, w, a.n);
+        const int64_t i = w * 64 + lane;
+        const bool on = ((live >> lane) & 1ull) && holds(a.op, compare_row(a.type, a.data, i, a.s));
+        const uint64_t bits = __ballot(on);
+        if (lane == 0) {
+            a.out[w] = bits;
+            total += (unsigned long long)__popcll(bits);
+        }
+    }
+    if (lane == 0 && total) atomicAdd(a.count, total);
+}
+
+// Selected rows per 64-row word (the exclusive scan of these places each word's rows).
+__global__ void word_counts_kernel(const uint8_t* sel, int64_t n, uint32_t* counts) {
+    const int64_t words = (n + 63) / 64;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (int64_t)gridDim.x * blockDim.x)
+        counts[w] = (uint32_t)__popcll(word_at(sel, w, n));
+}
+
+struct CompactCol {
+    int32_t kind;        // 0: 4-byte lane, 1: 8-byte, 2: 16-byte, 3: BOOL bits, 4: plain UTF8
+    const void* data;
+    const uint8_t* valid;  // nullptr: no validity to carry
+    const int32_t* offsets;
+    void* out_data;
+    uint64_t* out_valid;   // nullptr: not written
+    int32_t* out_offsets;
+    int32_t* lens;         // plain UTF8: the selected values' lengths, in compacted order
+};
+
+// ORs `bits` (the low `k` bits significant) into a bitmap at bit position `at`.
+__device__ __forceinline__ void or_bits(uint64_t* bm, uint64_t at, uint64_t bits, int k) {
+    if (!k) return;
+    const uint64_t q = at >> 6;
+    const int sh = (int)(at & 63);
+    atomicOr((unsigned long long*)&bm[q], (unsigned long long)(bits << sh));
+    if (sh && k > 64 - sh) atomicOr((unsigned long long*)&bm[q + 1], (unsigned long long)(bits >> (64 - sh)));
+}
+
+// The bits of the selected lanes, packed in lane order (a compress of `v` by `sel` over the wave).
+__device__ __forceinline__ uint64_t packed_bits(bool selected, bool v, uint64_t sel_word, int lane) {
+    const uint64_t below = sel_word & ((1ull << lane) - 1);
+    uint64_t mine = (selected && v) ? (1ull << __popcll(below)) : 0ull;
+    for (int d = 1; d < 64; d <<= 1) mine |= __shfl_xor(mine, d, 64);
+    return mine;
+}
+
+__global__ __launch_bounds__(FILTER_BLOCK) void compact_kernel(CompactCol c, const uint8_t* sel, int64_t n,
+                                                            const uint32_t* word_off) {
+    const int lane = threadIdx.x & 63;
+    const int64_t words = (n + 63) / 64;
+    const int64_t waves = (int64_t)gridDim.x * (FILTER_BLOCK / 64);
+    for (int64_t w = (int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64; w < words; w += waves) {
+        const uint64_t sw = word_at(sel, w, n);
+        if (!sw) continue;  // uniform over the wave
+        const int64_t i = w * 64 + lane;
+        const bool selected = (sw >> lane) & 1ull;
+        const uint64_t o = (uint64_t)word_off[w] + (uint64_t)__popcll(sw & ((1ull << lane) - 1));
+        const int k = __popcll(sw);
+        if (selected) {
+            switch (c.kind) {
+                case 0: ((int32_t*)c.out_data)[o] = ((const int32_t*)c.data)[i]; break;
+                case 1: ((int64_t*)c.out_data)[o] = ((const int64_t*)c.data)[i]; break;
+                case 2:
+                    ((int64_t*)c.out_data)[2 * o] = ((const int64_t*)c.data)[2 * i];
+                    ((int64_t*)c.out_data)[2 * o + 1] = ((const int64_t*)c.data)[2 * i + 1];
+                    break;
+                case 4: c.lens[o] = c.offsets[i + 1] - c.offsets[i]; break;  // lengths, scanned into offsets
+                default: break;
+            }
+        }
+        if (c.kind == 3) {
+            const bool v = selected && (((const uint8_t*)c.data)[i >> 3] >> (i & 7)) & 1u;
+            const uint64_t bits = packed_bits(selected, v, sw, lane);
+            if (lane == 0) or_bits((uint64_t*)c.out_data, word_off[w], bits, k);
+        }
+        if (c.out_valid) {
+            const bool v = selected && (!c.valid || ((c.valid[i >> 3] >> (i & 7)) & 1u));
+            const uint64_t bits = packed_bits(selected, v, sw, lane);
+            if (lane == 0) or_bits(c.out_valid, word_off[w], bits, k);
+        }
+    }
+}
+
+// Copies the bytes of the selected plain UTF8 values to their compacted offsets.
+__global__ void utf8_bytes_kernel(CompactCol c, const uint8_t* sel, int64_t n, const uint32_t* word_off) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w = i >> 6;
+        const uint64_t sw = word_at(sel, w, n);
+        const int lane = (int)(i & 63);
+        if (!((sw >> lane) & 1ull)) continue;
+        const uint64_t o = (uint64_t)word_off[w] + (uint64_t)__popcll(sw & ((1ull << lane) - 1));
+        const int32_t src = c.offsets[i], len = c.offsets[i + 1] - src, dst = c.out_offsets[o];
+        const uint8_t* s = (const uint8_t*)c.data + src;
+        uint8_t* d = (uint8_t*)c.out_data + dst;
+        for (int32_t b = 0; b < len; b++) d[b] = s[b];
+    }
+}
+
+__global__ void selected_bytes_kernel(const int32_t* offsets, const uint8_t* sel, int64_t n, unsigned long long* total) {
+    unsigned long long t = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (!sel || ((sel[i >> 3] >> (i & 7)) & 1u)) t += (unsigned long long)(offsets[i + 1] - offsets[i]);
+    if (t) atomicAdd(total, t);
+}
+
+__global__ void store_i32_kernel(int32_t* p, int32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+__global__ void zero_words_kernel(uint64_t* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int grid_of(uint64_t work, int block, int cap) {
+    const uint64_t g = (work + block - 1) / block;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cap));
+}
+
+vr_status check_device(const vr_column& c) {
+    if (c.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "filters take device-resident columns");
+    return VR_OK;
+}
+
+}  // namespace
+}  // namespace vr
+
+using namespace vr;
+
+extern "C" {
+
+vr_status vr_filter_compare(vr_device* dev, const vr_column* col, int32_t op, const vr_scalar* scalar,
+                            const uint8_t* selection, int64_t n, uint8_t* out, int64_t* count, void* stream) {
+    if (!dev || !col || !scalar || (!out && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (op < VR_CMP_EQ || op > VR_CMP_GE) return fail(VR_ERR_INVALID_ARGUMENT, "comparison operator");
+    if (n < 0 || n > col->length) return fail(VR_ERR_INVALID_ARGUMENT, "row count outside the column");
+    if (col->type != VR_INT32 && col->type != VR_INT64 && col->type != VR_FLOAT64 && col->type != VR_DECIMAL128)
+        return fail(VR_ERR_UNSUPPORTED, "comparisons over this lane run on the CPU kernels");
+    VR_TRY(check_device(*col));
+    if (count) *count = 0;
+    if (n == 0) return VR_OK;
+    Device& d = dev->d;
+    hipStream_t s = d.pick(stream);
+    DevBuf cnt;
+    VR_TRY(cnt.ensure(&d, 8));
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, cnt.as<uint64_t>(), 1ull);
+    CompareArgs a{col->data, col->validity, selection, n, col->type, op, *scalar, (uint64_t*)out,
+                  cnt.as<unsigned long long>()};
+    const uint64_t words = (uint64_t)(n + 63) / 64;
+    hipLaunchKernelGGL(compare_kernel, dim3(grid_of(words * 64, FILTER_BLOCK, d.num_cus * 16)), dim3(FILTER_BLOCK), 0, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t c = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&c, cnt.p, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    if (count) *count = (int64_t)c;
+    d.count(&vr_stats::calls, 1);
+    d.count(&vr_stats::input_rows, n);
+    d.count(&vr_stats::kernel_launches, 2);
+    return VR_OK;
+}
+
+vr_status vr_compact_utf8_bytes(vr_device* dev, const vr_column* col, int64_t n, const uint8_t* selection,
+                                int64_t* bytes, void* stream) {
+    if (!dev || !col || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (col->type != VR_UTF8 || col->dictionary || !col->offsets)
+        return fail(VR_ERR_INVALID_ARGUMENT, "expected a plain UTF8 column");
+    if (n < 0 || n > col->length) return fail(VR_ERR_INVALID_ARGUMENT, "row count outside the column");
+    VR_TRY(check_device(*col));
+    *bytes = 0;
+    if (n == 0) return VR_OK;
+    Device& d = dev->d;
+    hipStream_t s = d.pick(stream);
+    DevBuf tot;
+    VR_TRY(tot.ensure(&d, 8));
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, tot.as<uint64_t>(), 1ull);
+    hipLaunchKernelGGL(selected_bytes_kernel, dim3(grid_of((uint64_t)n, 256, d.num_cus * 16)), dim3(256), 0, s, col->offsets,
+                       selection, n, tot.as<unsigned long long>());
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t b = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&b, tot.p, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    *bytes = (int64_t)b;
+    return VR_OK;
+}
+
+vr_status vr_compact(vr_device* dev, const vr_column* columns, int32_t num_columns, int64_t n, const uint8_t* selection,
+                     vr_out_column* out, int64_t* rows, void* stream) {
+    if (!dev || (num_columns && (!columns || !out)) || num_columns < 0) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < 0 || n >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "compaction of batches past 2^31 rows");
+    if (rows) *rows = 0;
+    std::vector<CompactCol> cols(num_columns);
+    for (int c = 0; c < num_columns; c++) {  // every column checked before anything is queued
+        const vr_column& in = columns[c];
+        const vr_out_column& o = out[c];
+        VR_TRY(check_device(in));
+        if (o.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "compaction writes device-resident outputs");
+        if (in.length < n) return fail(VR_ERR_INVALID_ARGUMENT, "column shorter than the batch");
+        const bool dict = in.type == VR_UTF8 && in.dictionary;
+        CompactCol& cc = cols[c];
+        cc.data = in.data;
+        cc.valid = in.validity;
+        cc.out_data = o.data;
+        cc.out_valid = (uint64_t*)o.validity;
+        switch (dict ? VR_INT32 : in.type) {
+            case VR_INT32: cc.kind = 0; break;
+            case VR_INT64:
+            case VR_FLOAT64: cc.kind = 1; break;
+            case VR_DECIMAL128: cc.kind = 2; break;
+            case VR_BOOL: cc.kind = 3; break;
+            default: cc.kind = 4; break;  // plain UTF8
+        }
+        if (o.type != (dict ? VR_INT32 : in.type))
+            return fail(VR_ERR_INVALID_ARGUMENT, "output lane differs from the column (a dictionary column compacts its int32 codes)");
+        if (cc.kind == 4) {
+            if (!in.offsets || !o.offsets) return fail(VR_ERR_INVALID_ARGUMENT, "a UTF8 compaction needs offsets");
+            cc.offsets = in.offsets;
+            cc.out_offsets = o.offsets;
+        }
+    }
+    Device& d = dev->d;
+    hipStream_t s = d.pick(stream);
+    const int cap = d.num_cus * 16;
+    // the compacted position of each word's first selected row; every temporary lives until the
+    // final synchronization
+    const uint64_t words = (uint64_t)(n + 63) / 64;
+    DevBuf counts, offs, tmp;
+    std::vector<DevBuf> keep;
+    VR_TRY(counts.ensure(&d, (words + 1) * 4));
+    VR_TRY(offs.ensure(&d, (words + 1) * 4));
+    uint32_t last[2] = {0, 0};
+    if (words) {
+        hipLaunchKernelGGL(word_counts_kernel, dim3(grid_of(words, 256, cap)), dim3(256), 0, s, selection, n, counts.as<uint32_t>());
+        size_t tb = 0;
+        VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts.as<uint32_t>(), offs.as<uint32_t>(), (int)words, s));
+        VR_TRY(tmp.ensure(&d, tb));
+        VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, counts.as<uint32_t>(), offs.as<uint32_t>(), (int)words, s));
+        VR_TRY_HIP(hipMemcpyAsync(&last[0], offs.as<uint32_t>() + words - 1, 4, hipMemcpyDeviceToHost, s));
+        VR_TRY_HIP(hipMemcpyAsync(&last[1], counts.as<uint32_t>() + words - 1, 4, hipMemcpyDeviceToHost, s));
+    }
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    const int64_t m = (int64_t)last[0] + last[1];
+    if (rows) *rows = m;
+    for (int c = 0; c < num_columns; c++)
+        if (out[c].capacity < m) return fail(VR_ERR_CAPACITY, "compacted columns need " + std::to_string(m) + " rows");
+    const uint64_t mw = (uint64_t)(m + 63) / 64;
+    for (int c = 0; c < num_columns; c++) {
+        CompactCol& cc = cols[c];
+        // bits are OR-ed in: clear the words the compacted rows cover
+        if (cc.out_valid && mw)
+            hipLaunchKernelGGL(zero_words_kernel, dim3(grid_of(mw, 256, cap)), dim3(256), 0, s, cc.out_valid, mw);
+        if (cc.kind == 3 && mw)
+            hipLaunchKernelGGL(zero_words_kernel, dim3(grid_of(mw, 256, cap)), dim3(256), 0, s, (uint64_t*)cc.out_data, mw);
+        if (cc.kind == 4) {
+            hipLaunchKernelGGL(store_i32_kernel, dim3(1), dim3(64), 0, s, cc.out_offsets, 0);
+            keep.emplace_back();
+            VR_TRY(keep.back().ensure(&d, (size_t)std::max<int64_t>(m, 1) * 4));
+            cc.lens = keep.back().as<int32_t>();
+        }
+        if (words)
+            hipLaunchKernelGGL(compact_kernel, dim3(grid_of(words * 64, FILTER_BLOCK, cap)), dim3(FILTER_BLOCK), 0, s, cc, selection, n,
+                               offs.as<uint32_t>());
+        if (cc.kind == 4 && m) {
+            // lengths -> offsets[1..m] (running ends), then the bytes
+            size_t tb = 0;
+            VR_TRY_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cc.lens, cc.out_offsets + 1, (int)m, s));
+            keep.emplace_back();
+            VR_TRY(keep.back().ensure(&d, tb));
+            VR_TRY_HIP(hipcub::DeviceScan::InclusiveSum(keep.back().p, tb, cc.lens, cc.out_offsets + 1, (int)m, s));
+            hipLaunchKernelGGL(utf8_bytes_kernel, dim3(grid_of((uint64_t)n, 256, cap)), dim3(256), 0, s, cc, selection, n,
+                               offs.as<uint32_t>());
+        }
+        VR_TRY_HIP(hipGetLastError());
+    }
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    d.count(&vr_stats::calls, 1);
+    d.count(&vr_stats::input_rows, n);
+    d.count(&vr_stats::output_rows, m);
+    return VR_OK;
+}
+
+}  // extern "C"

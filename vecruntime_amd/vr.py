@@ -63,6 +63,10 @@ _sig("vr_join_probe", [_P, _P, C.c_int32, _P, C.c_int64, _P, C.c_int32, C.c_int3
                        C.POINTER(abi.VrProbeOutput), _P])
 _sig("vr_join_unmatched_build", [_P, C.POINTER(abi.VrProbeOutput), _P])
 _sig("vr_join_memory", [_P, C.POINTER(C.c_int64)])
+_sig("vr_filter_compare", [_P, C.POINTER(abi.VrColumn), C.c_int32, C.POINTER(abi.VrScalar), _P, C.c_int64, _P,
+                           C.POINTER(C.c_int64), _P])
+_sig("vr_compact_utf8_bytes", [_P, C.POINTER(abi.VrColumn), C.c_int64, _P, C.POINTER(C.c_int64), _P])
+_sig("vr_compact", [_P, _P, C.c_int32, C.c_int64, _P, _P, C.POINTER(C.c_int64), _P])
 _sig("vr_join_destroy", [_P])
 
 
@@ -361,3 +365,31 @@ def device_out_column(type, capacity, data, validity=None):
     oc.data = data
     oc._c = abi.VrOutColumn(type, abi.MEM_DEVICE, int(capacity), _ptr(validity), _ptr(data), None)
     return oc
+
+
+def filter_compare(device, col, op, value, n, selection=None, out=None, stream=None):
+    """vr_filter_compare over a device column: (selection bitmap as a uint8 device tensor padded to
+    8 bytes, selected rows)."""
+    import torch
+    if out is None:
+        out = torch.zeros(((n + 63) // 64) * 8, dtype=torch.uint8, device="cuda")
+    sc = abi.scalar(value, col.type)
+    cnt = C.c_int64()
+    check(_lib.vr_filter_compare(device.h, C.byref(col.c()), int(op), C.byref(sc), _ptr(selection), int(n), _ptr(out),
+                                 C.byref(cnt), _stream(stream)))
+    return out, cnt.value
+
+
+def compact_utf8_bytes(device, col, n, selection, stream=None):
+    b = C.c_int64()
+    check(_lib.vr_compact_utf8_bytes(device.h, C.byref(col.c()), int(n), _ptr(selection), C.byref(b), _stream(stream)))
+    return b.value
+
+
+def compact(device, columns, n, selection, outputs, stream=None):
+    """vr_compact of device columns into device output columns (device_out_column); returns the rows."""
+    ca, oa = c_array(columns), c_array(outputs, abi.VrOutColumn)
+    rows = C.c_int64()
+    check(_lib.vr_compact(device.h, C.cast(ca, _P), len(columns), int(n), _ptr(selection), C.cast(oa, _P), C.byref(rows),
+                          _stream(stream)))
+    return rows.value
