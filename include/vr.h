@@ -315,6 +315,23 @@ vr_status vr_join_info(vr_join* join, int64_t* build_rows, int64_t* distinct_key
  * reference's probeIdx/buildIdx) or materialized columns (probe_out_columns of the streamed batch
  * then build_out_columns of the build side, in that order), or both.  For semi/anti: `pairs`'
  * probe_idx lists the kept streamed rows and/or the streamed columns are compacted. */
+/* Comparison operators (CompareOp, kernels/src/main/java/io/vecruntime/kernels/CompareOp.java). */
+enum vr_compare_op { VR_CMP_EQ = 0, VR_CMP_NE = 1, VR_CMP_LT = 2, VR_CMP_LE = 3, VR_CMP_GT = 4, VR_CMP_GE = 5 };
+
+/* A residual condition on a join's candidate pairs, evaluated in the probe (the fused
+ * PairPredicate, VectorHashJoinExec.scala:1225-1334): `left + left_offset <op> right +
+ * right_offset` over two INT32 or two INT64 columns, each a streamed column (side 0: an index into
+ * the probe's `columns`) or a build column (side 1).  The offsets are added in the lane's width,
+ * wrapping; a pair with either operand null fails.  Inner: the passing pairs; semi / anti: rows with
+ * a passing pair / without one; outer: the passing pairs, or (row, -1) when none passes
+ * (emitOuterConditional, :1415-1450).  vr_compare_op values for `op`. */
+typedef struct vr_join_residual {
+    int32_t op;
+    int32_t left_side, left_column;
+    int32_t right_side, right_column;
+    int64_t left_offset, right_offset;
+} vr_join_residual;
+
 typedef struct vr_probe_output {
     int32_t mem;                   /* vr_mem of every output buffer */
     int64_t capacity;              /* rows the outputs hold */
@@ -326,6 +343,7 @@ typedef struct vr_probe_output {
     const int32_t* build_out_columns;
     vr_out_column* columns;        /* num_probe_out + num_build_out columns */
     int64_t rows;                  /* out: rows produced (or needed, with VR_ERR_CAPACITY) */
+    const struct vr_join_residual* residual;  /* NULL: pairs by key equality only */
 } vr_probe_output;
 
 /* Probes one streamed batch (`columns`: num_columns streamed columns; `key_columns`: the key
@@ -345,9 +363,6 @@ vr_status vr_join_memory(vr_join* join, int64_t* bytes);
 vr_status vr_join_destroy(vr_join* join);
 
 /* ------------------------------------------------------------------ filters (selection producers) */
-
-/* Comparison operators (CompareOp, kernels/src/main/java/io/vecruntime/kernels/CompareOp.java). */
-enum vr_compare_op { VR_CMP_EQ = 0, VR_CMP_NE = 1, VR_CMP_LT = 2, VR_CMP_LE = 3, VR_CMP_GT = 4, VR_CMP_GE = 5 };
 
 /* A comparison's literal: `i64` for INT32 (narrowed as Java's intValue) and INT64 lanes, `f64` for
  * FLOAT64, the unscaled value's two limbs for DECIMAL128 (already at the column's scale). */

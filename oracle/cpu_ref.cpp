@@ -972,6 +972,33 @@ int vro_join_probe(vro_join* j, const vr_column* columns, int32_t num_columns, c
     bool dropNull = drop_null_keys && j->numRows > 0;
     int64_t rows = 0;
     const bool pairs = join_type == VR_JOIN_INNER || join_type == VR_JOIN_OUTER;
+    // PairPredicate.test (VectorHashJoinExec.scala:1287-1303): both operands non-null, each plus its
+    // offset in the lane's width (wrapping), compared
+    const vr_join_residual* res = out->residual;
+    auto operand = [&](int32_t side, int32_t col, int64_t off, int64_t i, int64_t r, int64_t* v) -> bool {
+        const vr_column& c = side == 0 ? columns[col] : j->cols[col];
+        const int64_t row = side == 0 ? i : r;
+        if (!valid(c, row)) return false;
+        if (c.type == VR_INT32) *v = (int32_t)((uint32_t)((const int32_t*)c.data)[row] + (uint32_t)(int32_t)off);
+        else *v = (int64_t)((uint64_t)((const int64_t*)c.data)[row] + (uint64_t)off);
+        return true;
+    };
+    auto passes = [&](int64_t i, int64_t r) -> bool {
+        if (!res) return true;
+        int64_t x, y;
+        if (!operand(res->left_side, res->left_column, res->left_offset, i, r, &x) ||
+            !operand(res->right_side, res->right_column, res->right_offset, i, r, &y))
+            return false;
+        const int c = (x > y) - (x < y);
+        switch (res->op) {
+            case VR_CMP_EQ: return c == 0;
+            case VR_CMP_NE: return c != 0;
+            case VR_CMP_LT: return c < 0;
+            case VR_CMP_LE: return c <= 0;
+            case VR_CMP_GT: return c > 0;
+            default: return c >= 0;
+        }
+    };
     auto emit = [&](int64_t i, int64_t b) {
         if (rows < out->capacity) {
             if (out->probe_idx) out->probe_idx[rows] = (int32_t)i;
@@ -991,16 +1018,21 @@ int vro_join_probe(vro_join* j, const vr_column* columns, int32_t num_columns, c
         if (pairs) {
             // emitMatches / the outer path (VectorHashJoinExec.scala:1160-1175): the row's build rows
             // in build order, each marked matched; a row without one is padded when preserved
-            if (ids[i] < 0) {
-                if (join_type == VR_JOIN_OUTER) emit(i, -1);
-                continue;
-            }
-            for (int32_t r = j->head[ids[i]]; r >= 0; r = j->next[r]) {
-                emit(i, r);
-                if (j->track) j->matched[r] = 1;
-            }
+            bool any = false;
+            if (ids[i] >= 0)
+                for (int32_t r = j->head[ids[i]]; r >= 0; r = j->next[r]) {
+                    if (!passes(i, r)) continue;
+                    emit(i, r);
+                    any = true;
+                    if (j->track) j->matched[r] = 1;
+                }
+            // emitOuterConditional (:1415-1450): a row none of whose pairs passes is padded
+            if (!any && join_type == VR_JOIN_OUTER) emit(i, -1);
         } else {
-            bool matched = ids[i] >= 0 || (dropNull && nullKey[i]);
+            bool hit = false;
+            if (ids[i] >= 0)
+                for (int32_t r = j->head[ids[i]]; r >= 0 && !hit; r = j->next[r]) hit = passes(i, r);
+            bool matched = hit || (dropNull && nullKey[i]);
             if (matched == (join_type == VR_JOIN_LEFT_SEMI)) emit(i, -1);
         }
     }

@@ -139,7 +139,7 @@ class OracleJoin:
         return lib().vro_join_distinct_keys(self.h)
 
     def probe(self, columns, key_columns, n, join_type, selection=None, drop_null_keys=False,
-              probe_out=(), build_out=(), capacity=None):
+              probe_out=(), build_out=(), capacity=None, residual=None):
         """Returns (probe_idx, build_idx or None, output columns, rows)."""
         if capacity is None:
             capacity = max(1, n) * 4
@@ -156,6 +156,8 @@ class OracleJoin:
                                 len(probe_out), C.cast(po, C.POINTER(C.c_int32)),
                                 len(build_out), C.cast(bo, C.POINTER(C.c_int32)),
                                 C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        if residual is not None:
+            out.residual = C.addressof(residual)
         _check(lib().vro_join_probe(self.h, C.cast(ca, C.c_void_p), len(columns), C.cast(kc, C.c_void_p), n,
                                     _ptr(selection), join_type, 1 if drop_null_keys else 0, C.byref(out)))
         r = out.rows

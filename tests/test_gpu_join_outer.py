@@ -102,3 +102,46 @@ def test_outer_over_an_empty_build_and_untracked_handle(dev):
         bt.unmatched_build(capacity=10)
     assert e.value.status == abi.VR_ERR_STATE
     bt.close()
+
+
+@pytest.mark.parametrize("dense", [0, -1])
+@pytest.mark.parametrize("lane", [abi.INT32, abi.INT64])
+def test_residual_matches_oracle(dev, dense, lane):
+    """A residual `left + lo <op> right + ro` over streamed and build columns (every side pairing,
+    every operator, offsets that wrap the lane) on every join type, then the build rows no passing
+    pair touched."""
+    print("residual: dense=%d lane=%d" % (dense, lane), flush=True)
+    rng = np.random.default_rng(50 + dense + lane)
+    nb, n = 6000, 20000
+    top = 2 ** 31 - 1 if lane == abi.INT32 else 2 ** 63 - 1
+    mk = Column.ints if lane == abi.INT32 else Column.longs
+
+    def vals(m):
+        v = rng.integers(-50, 50, m)
+        v[rng.random(m) < 0.05] = top  # + offset wraps
+        return v
+    bkeys = rng.integers(0, 2000, nb)
+    bcols = [Column.longs(bkeys, nulls=rng.random(nb) < 0.02), mk(vals(nb), nulls=rng.random(nb) < 0.1),
+             Column.ints(np.arange(nb, dtype=np.int32))]
+    spec = abi.JoinSpec([abi.INT64], [abi.INT64, lane, abi.INT32], [0], dense_index=dense, track_matched=True)
+    bt, oj = build_both(dev, spec, bcols, nb)
+    ops = [abi.CMP_EQ, abi.CMP_NE, abi.CMP_LT, abi.CMP_LE, abi.CMP_GT, abi.CMP_GE]
+    sides = [((0, 1), (1, 1)), ((1, 1), (0, 1)), ((0, 1), (0, 1)), ((1, 1), (1, 1))]
+    for t in range(8):
+        pk = np.where(rng.random(n) < 0.7, rng.choice(bkeys, n), rng.integers(2000, 4000, n))
+        pcols = [Column.longs(pk, nulls=rng.random(n) < 0.03), mk(vals(n), nulls=rng.random(n) < 0.1)]
+        (ls, rs) = sides[t % 4]
+        res = abi.residual(ops[t % 6], ls, rs, int(rng.choice([0, 1, -7])), int(rng.choice([0, 3])))
+        for jt in (abi.INNER, abi.OUTER, abi.LEFT_SEMI, abi.LEFT_ANTI):
+            bo = [2] if jt in (abi.INNER, abi.OUTER) else []
+            cap = n * 16
+            d = bt.probe(pcols, [0], n, jt, probe_out=[1], build_out=bo, capacity=cap, residual=res)
+            o = oj.probe(pcols, [0], n, jt, probe_out=[1], build_out=bo, capacity=cap, residual=res)
+            assert d[3] == o[3], (t, jt)
+            assert d[0].tolist() == o[0].tolist(), (t, jt)
+            if jt in (abi.INNER, abi.OUTER):
+                assert d[1].tolist() == o[1].tolist(), (t, jt)
+            for a, c in zip(d[2], o[2]):
+                assert a.values(d[3]) == c.values(o[3])
+    same(bt.unmatched_build(build_out=[2], capacity=nb), oj.unmatched_build(build_out=[2], capacity=nb))
+    bt.close()

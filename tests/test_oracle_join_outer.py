@@ -63,3 +63,60 @@ def test_outer_and_unmatched_build_exhaustive():
                         assert outs[1].values(r) == [j + 100 for _, j in want_un]
                         cases += 1
     assert cases > 2000
+
+
+def wrap32(x):
+    return ((x + 2 ** 31) % 2 ** 32) - 2 ** 31
+
+
+def test_residual_against_nested_loop():
+    """Pairs by key equality and a residual `left + lo <op> right + ro` (PairPredicate.test,
+    VectorHashJoinExec.scala:1287-1303: both operands non-null, the offset added in the lane's width,
+    wrapping) for every join type, against a nested loop."""
+    rng = np.random.default_rng(19)
+    ops = [abi.CMP_EQ, abi.CMP_NE, abi.CMP_LT, abi.CMP_LE, abi.CMP_GT, abi.CMP_GE]
+    cases = 0
+    for trial in range(300):
+        nb, npr = int(rng.integers(0, 7)), int(rng.integers(0, 7))
+        bkeys = [None if rng.random() < 0.15 else int(rng.integers(0, 3)) for _ in range(nb)]
+        pkeys = [None if rng.random() < 0.15 else int(rng.integers(0, 3)) for _ in range(npr)]
+        edge = [2 ** 31 - 1, -2 ** 31, 0, 5, -5]
+        bval = [None if rng.random() < 0.15 else edge[int(rng.integers(0, 5))] for _ in range(nb)]
+        pval = [None if rng.random() < 0.15 else edge[int(rng.integers(0, 5))] for _ in range(npr)]
+        op = ops[trial % 6]
+        lo, ro = int(rng.choice([0, 1, -1])), int(rng.choice([0, 2, -3]))
+        sides = [(0, 1), (1, 1), (0, 0), (1, 0)][trial % 4]  # (left side, right side)
+        spec = abi.JoinSpec([abi.INT64], [abi.INT64, abi.INT32], [0], track_matched=True)
+        bcols = [col(bkeys), Column.ints([0 if v is None else v for v in bval], nulls=[v is None for v in bval])]
+        pcols = [col(pkeys), Column.ints([0 if v is None else v for v in pval], nulls=[v is None for v in pval])]
+        oj = OracleJoin(spec, bcols, nb)
+        res = abi.residual(op, (sides[0], 1), (sides[1], 1), lo, ro)
+
+        def val(side, i, r):
+            return pval[i] if side == 0 else bval[r]
+
+        def passes(i, r):
+            x, y = val(sides[0], i, r), val(sides[1], i, r)
+            if x is None or y is None:
+                return False
+            x, y = wrap32(x + lo), wrap32(y + ro)
+            return [x == y, x != y, x < y, x <= y, x > y, x >= y][op]
+
+        hits = {i: [r for r in range(nb) if pkeys[i] is not None and bkeys[r] == pkeys[i] and passes(i, r)]
+                for i in range(npr)}
+        for jt in (abi.INNER, abi.OUTER, abi.LEFT_SEMI, abi.LEFT_ANTI):
+            p, b, _, r = oj.probe(pcols, [0], npr, jt, capacity=max(1, npr * max(1, nb)), residual=res)
+            if jt == abi.INNER:
+                want = [(i, x) for i in range(npr) for x in hits[i]]
+            elif jt == abi.OUTER:
+                want = [(i, x) for i in range(npr) for x in (hits[i] or [-1])]
+            else:
+                keep = [i for i in range(npr) if bool(hits[i]) == (jt == abi.LEFT_SEMI)]
+                assert p.tolist() == keep
+                continue
+            assert list(zip(p.tolist(), b.tolist())) == want, (trial, jt)
+        paired = {x for i in range(npr) for x in hits[i]}
+        p, b, _, r = oj.unmatched_build(capacity=max(1, nb))
+        assert b.tolist() == [x for x in range(nb) if x not in paired]
+        cases += 1
+    assert cases == 300

@@ -188,7 +188,22 @@ class VrProbeOutput(C.Structure):
         ("build_out_columns", C.POINTER(C.c_int32)),
         ("columns", C.POINTER(VrOutColumn)),
         ("rows", C.c_int64),
+        ("residual", C.c_void_p),
     ]
+
+
+class VrJoinResidual(C.Structure):
+    _fields_ = [("op", C.c_int32), ("left_side", C.c_int32), ("left_column", C.c_int32), ("right_side", C.c_int32),
+                ("right_column", C.c_int32), ("left_offset", C.c_int64), ("right_offset", C.c_int64)]
+
+
+PROBE_SIDE, BUILD_SIDE = 0, 1
+
+
+def residual(op, left, right, left_offset=0, right_offset=0):
+    """left / right: (side, column) -- PROBE_SIDE with a streamed column index, BUILD_SIDE with a
+    build column index."""
+    return VrJoinResidual(op, left[0], left[1], right[0], right[1], left_offset, right_offset)
 
 
 # vr_query_kind (include/vr_serve.h)

@@ -168,7 +168,42 @@ struct ProbeArgs {
     uint64_t* tile_counter; // [0] next tile id, [1] total rows out
     int64_t ntiles;
     uint32_t* matched;      // build rows paired so far (one bit each), nullptr: not tracked
+    // residual condition on the pairs (res_on 0: none): left + loff <op> right + roff, each operand
+    // a streamed column (indexed by the streamed row) or a build column (by the build row)
+    int32_t res_on, res_op;
+    int32_t res_lb, res_rb;
+    DCol res_l, res_r;
+    int64_t res_loff, res_roff;
 };
+
+// One operand of the residual plus its offset in the lane's width (wrapping); false when null.
+__device__ __forceinline__ bool res_operand(const DCol& c, int64_t row, int64_t off, int64_t* v) {
+    if (c.valid && !bit_at(c.valid, row)) return false;
+    if (c.type == VR_INT32) *v = (int32_t)((uint32_t)((const int32_t*)c.data)[row] + (uint32_t)(int32_t)off);
+    else *v = (int64_t)((uint64_t)((const int64_t*)c.data)[row] + (uint64_t)off);
+    return true;
+}
+
+// PairPredicate.test (VectorHashJoinExec.scala:1287-1303) for streamed row i and build row b.
+__device__ __forceinline__ bool res_passes(const ProbeArgs& a, int64_t i, int64_t b) {
+    int64_t x, y;
+    if (!res_operand(a.res_l, a.res_lb ? b : i, a.res_loff, &x) || !res_operand(a.res_r, a.res_rb ? b : i, a.res_roff, &y))
+        return false;
+    const int c = (x > y) - (x < y);
+    switch (a.res_op) {
+        case VR_CMP_EQ: return c == 0;
+        case VR_CMP_NE: return c != 0;
+        case VR_CMP_LT: return c < 0;
+        case VR_CMP_LE: return c <= 0;
+        case VR_CMP_GT: return c > 0;
+        default: return c >= 0;
+    }
+}
+
+// Candidate k of a key's value word: the single row, or the k-th row of its run in build order.
+__device__ __forceinline__ int64_t candidate(const ProbeArgs& a, uint32_t matches, uint32_t payload, uint32_t k) {
+    return matches == 1 ? (int64_t)payload : (int64_t)a.t.range_rows[payload + k];
+}
 
 __device__ __forceinline__ void copy_value(const DCol& src, int64_t i, const OutCol& dst, int64_t o) {
     if (i < 0) {  // the padded side of an outer pair: null (validity words start cleared), data zero
@@ -213,12 +248,12 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     const int64_t tile = tile_id_sh;
     const int64_t row0 = tile * PROBE_BLOCK * PROBE_ITEMS;
     uint32_t cnt[PROBE_ITEMS], off[PROBE_ITEMS];
-    uint32_t payload[PROBE_ITEMS], found[PROBE_ITEMS];
+    uint32_t payload[PROBE_ITEMS], found[PROBE_ITEMS], npass[PROBE_ITEMS];
     const bool pairs = a.join_type == VR_JOIN_INNER || a.join_type == VR_JOIN_OUTER;
     for (int r = 0; r < PROBE_ITEMS; r++) {
         int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
         uint32_t c = 0;
-        uint32_t pl = 0, fd = 0;
+        uint32_t pl = 0, fd = 0, np = 0;
         if (i < a.n && (!a.sel || bit_at(a.sel, i))) {
             uint64_t key;
             bool nonnull = key64_of(a.keys, a.nk, i, &key);
@@ -227,16 +262,22 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
             uint32_t matches = (uint32_t)(v >> 32);
             pl = (uint32_t)v;
             fd = matches;
+            np = matches;
+            if (a.res_on && matches) {  // the candidates that pass the residual
+                np = 0;
+                for (uint32_t k = 0; k < matches; k++) np += res_passes(a, i, candidate(a, matches, pl, k)) ? 1u : 0u;
+            }
             if (pairs) {
-                c = matches ? matches : (a.join_type == VR_JOIN_OUTER ? 1u : 0u);  // (row, -1) when preserved
+                c = np ? np : (a.join_type == VR_JOIN_OUTER ? 1u : 0u);  // (row, -1) when preserved
             } else {
-                bool matched = matches > 0 || (a.drop_null_keys && a.build_rows && !nonnull);
+                bool matched = np > 0 || (a.drop_null_keys && a.build_rows && !nonnull);
                 c = (matched == (a.join_type == VR_JOIN_LEFT_SEMI)) ? 1u : 0u;
             }
         }
         cnt[r] = c;
         payload[r] = pl;
         found[r] = fd;
+        npass[r] = np;
         uint32_t total;
         Scan(scan_tmp).ExclusiveSum(c, off[r], total);
         if (threadIdx.x == 0) sub_total[r] = total;
@@ -268,10 +309,16 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     for (int r = 0; r < PROBE_ITEMS; r++) {
         int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
         uint64_t o = base + off[r];
+        uint32_t k = 0;  // the next candidate of the row's key
         for (uint32_t m = 0; m < cnt[r]; m++, o++) {
             if ((int64_t)o >= a.capacity) break;
             int64_t brow = -1;
-            if (pairs && found[r]) brow = found[r] == 1 ? (int64_t)payload[r] : (int64_t)a.t.range_rows[payload[r] + m];
+            if (pairs && npass[r]) {  // the next passing candidate; a row with none is padded
+                while (k < found[r]) {
+                    brow = candidate(a, found[r], payload[r], k++);
+                    if (!a.res_on || res_passes(a, i, brow)) break;
+                }
+            }
             if (a.matched && brow >= 0) {  // buildMatched(r) = true; bits only ever get set
                 uint32_t* w = a.matched + (brow >> 5);
                 const uint32_t b = 1u << (brow & 31);
@@ -835,6 +882,39 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         a.build_src[q].data = j.cols[bc].data.p;
         a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
         types.push_back(j.col_types[bc]);
+    }
+    if (const vr_join_residual* res = out->residual) {
+        if (res->op < VR_CMP_EQ || res->op > VR_CMP_GE) return fail(VR_ERR_INVALID_ARGUMENT, "residual operator");
+        if (drop_null_keys) return fail(VR_ERR_UNSUPPORTED, "a null-aware anti join with a residual runs on the CPU kernels");
+        int32_t types[2];
+        DCol* dst[2] = {&a.res_l, &a.res_r};
+        const int32_t side[2] = {res->left_side, res->right_side}, col[2] = {res->left_column, res->right_column};
+        for (int q = 0; q < 2; q++) {
+            if (side[q] == 0) {
+                if (col[q] < 0 || col[q] >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "residual column out of range");
+                const vr_column& c = columns[col[q]];
+                types[q] = c.type;
+                dst[q]->type = c.type;
+                want(c.validity, bitmap_bytes(n), (const void**)&dst[q]->valid, c.mem);
+                want(c.data, lane_bytes(c.type, n), &dst[q]->data, c.mem);
+            } else if (side[q] == 1) {
+                if (col[q] < 0 || col[q] >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "residual column out of range");
+                types[q] = j.col_types[col[q]];
+                dst[q]->type = types[q];
+                dst[q]->data = j.cols[col[q]].data.p;
+                dst[q]->valid = j.cols[col[q]].has_valid ? j.cols[col[q]].valid.as<uint8_t>() : nullptr;
+            } else {
+                return fail(VR_ERR_INVALID_ARGUMENT, "residual side is 0 (streamed) or 1 (build)");
+            }
+        }
+        if (types[0] != types[1] || (types[0] != VR_INT32 && types[0] != VR_INT64))
+            return fail(VR_ERR_UNSUPPORTED, "residuals over other than two INT32 or two INT64 lanes run on the CPU kernels");
+        a.res_on = 1;
+        a.res_op = res->op;
+        a.res_lb = side[0];
+        a.res_rb = side[1];
+        a.res_loff = res->left_offset;
+        a.res_roff = res->right_offset;
     }
     a.sel = selection;
     if (selection && mem == VR_MEM_HOST) want(selection, bitmap_bytes(n), (const void**)&a.sel, VR_MEM_HOST);

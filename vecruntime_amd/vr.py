@@ -288,7 +288,7 @@ class BuildTable:
 
     def probe(self, columns, key_columns, n, join_type, selection=None, drop_null_keys=False, probe_out=(),
               build_out=(), capacity=None, outputs=None, probe_idx=None, build_idx=None, mem=abi.MEM_HOST,
-              stream=None, want_pairs=True):
+              stream=None, want_pairs=True, residual=None):
         """Returns (probe_idx, build_idx or None, output columns, rows).  Host outputs are
         allocated here unless given; device outputs (mem=MEM_DEVICE) must be given."""
         if capacity is None:
@@ -311,6 +311,8 @@ class BuildTable:
         out = abi.VrProbeOutput(mem, int(capacity), _ptr(probe_idx), _ptr(build_idx) if pairs else None,
                                 len(probe_out), C.cast(po, C.POINTER(C.c_int32)), len(build_out),
                                 C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        if residual is not None:
+            out.residual = C.addressof(residual)
         st = _lib.vr_join_probe(self.h, C.cast(ca, _P), len(columns), C.cast(kc, _P), int(n), _ptr(selection),
                                 join_type, 1 if drop_null_keys else 0, C.byref(out), _stream(stream))
         check(st)
