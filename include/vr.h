@@ -397,6 +397,18 @@ vr_status vr_compact_utf8_bytes(vr_device* dev, const vr_column* col, int64_t n,
 vr_status vr_compact(vr_device* dev, const vr_column* columns, int32_t num_columns, int64_t n,
                      const uint8_t* selection, vr_out_column* out, int64_t* rows, void* stream);
 
+/* Gathers rows of `col` by an index array (GatherKernels.gatherFixed / gatherValidity / gatherUtf8,
+ * kernels/src/main/java/io/vecruntime/kernels/GatherKernels.java:42-308; ArrowOutput.gather, the
+ * join's materialization of probeIdx / buildIdx): out row k = row idx[k], and an index of -1 is a
+ * null (an outer join's padding) whose fixed-width value is zero.  Validity is set unless the row
+ * is null or padded; a plain UTF8 column gets the bytes of its valid, real rows only (size `data`
+ * with vr_gather_utf8_bytes).  Fixed-width, BOOL and plain UTF8 lanes; `idx` (int32), the column
+ * and `out` in device memory; `out->capacity` at least m. */
+vr_status vr_gather_utf8_bytes(vr_device* dev, const vr_column* col, const int32_t* idx, int64_t m,
+                               int64_t* bytes, void* stream);
+vr_status vr_gather(vr_device* dev, const vr_column* col, const int32_t* idx, int64_t m, vr_out_column* out,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

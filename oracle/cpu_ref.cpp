@@ -1116,6 +1116,39 @@ int vro_filter_compare(const vr_column* col, int32_t op, const vr_scalar* sc, co
     return VR_OK;
 }
 
+// GatherKernels.gatherFixed / gatherBits / gatherValidity / gatherUtf8 (GatherKernels.java:99-308):
+// out row k = row idx[k]; -1 is a null whose fixed-width value (and BOOL bit) is zero; validity set
+// unless null or padded; a UTF8 row's bytes only when valid and real.
+int vro_gather(const vr_column* col, const int32_t* idx, int64_t m, vr_out_column* out) {
+    int32_t pos = 0;
+    if (col->type == VR_UTF8) out->offsets[0] = 0;
+    for (int64_t o = 0; o < m; o++) {
+        const int32_t i = idx[o];
+        const bool real = i >= 0, ok = real && valid(*col, i);
+        if (out->validity) set_bit(out->validity, o, ok);
+        switch (col->type) {
+            case VR_INT32: ((int32_t*)out->data)[o] = real ? ((const int32_t*)col->data)[i] : 0; break;
+            case VR_INT64:
+            case VR_FLOAT64: ((int64_t*)out->data)[o] = real ? ((const int64_t*)col->data)[i] : 0; break;
+            case VR_DECIMAL128:
+                ((int64_t*)out->data)[2 * o] = real ? ((const int64_t*)col->data)[2 * i] : 0;
+                ((int64_t*)out->data)[2 * o + 1] = real ? ((const int64_t*)col->data)[2 * i + 1] : 0;
+                break;
+            case VR_BOOL: set_bit((uint8_t*)out->data, o, real && bit((const uint8_t*)col->data, i)); break;
+            case VR_UTF8:
+                if (ok) {
+                    const int32_t a = col->offsets[i], len = col->offsets[i + 1] - a;
+                    std::memcpy((uint8_t*)out->data + pos, (const uint8_t*)col->data + a, (size_t)len);
+                    pos += len;
+                }
+                out->offsets[o + 1] = pos;
+                break;
+            default: return fail(VR_ERR_UNSUPPORTED, "gather lane");
+        }
+    }
+    return VR_OK;
+}
+
 // CompactKernels.compactFixed / compactUtf8 (CompactKernels.java:108-471): the selected rows of each
 // column in row order, values and validity; a dictionary-encoded column compacts its int32 codes.
 int vro_compact(const vr_column* cols, int32_t ncols, int64_t n, const uint8_t* selection, vr_out_column* out,

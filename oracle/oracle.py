@@ -51,6 +51,7 @@ def lib():
         L.vro_filter_compare.argtypes = [C.POINTER(abi.VrColumn), C.c_int32, C.POINTER(abi.VrScalar), P, C.c_int64, P,
                                          C.POINTER(C.c_int64)]
         L.vro_compact.argtypes = [P, C.c_int32, C.c_int64, P, P, C.POINTER(C.c_int64)]
+        L.vro_gather.argtypes = [C.POINTER(abi.VrColumn), P, C.c_int64, C.POINTER(abi.VrOutColumn)]
         L.vro_join_destroy.argtypes = [P]
         L.vro_agg_partial_threads.argtypes = [C.POINTER(abi.VrAggSpec), P, P, P, C.c_int32, C.c_int32,
                                               C.POINTER(C.c_double)]
@@ -219,3 +220,17 @@ def compact(columns, n, selection, types, capacity, utf8_bytes=0):
     _check(lib().vro_compact(C.cast(ca, C.c_void_p), len(columns), int(n), _ptr(selection), C.cast(oa, C.c_void_p),
                              C.byref(rows)))
     return outs, rows.value
+
+
+def gather(col, idx, utf8_bytes=0):
+    """Rows idx[k] of `col` (-1: null): an OutColumn of len(idx) rows."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    m = len(idx)
+    if col.type == abi.UTF8:
+        oc = OutColumn(abi.UTF8, max(1, m), data=np.zeros(max(1, utf8_bytes), np.uint8))
+        oc.offsets = np.zeros(max(1, m) + 1, np.int32)
+        oc._c = abi.VrOutColumn(abi.UTF8, abi.MEM_HOST, max(1, m), _ptr(oc.validity), _ptr(oc.data), _ptr(oc.offsets))
+    else:
+        oc = OutColumn(col.type, max(1, m))
+    _check(lib().vro_gather(C.byref(col.c()), idx.ctypes.data, m, C.byref(oc.c())))
+    return oc

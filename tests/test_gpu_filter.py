@@ -163,3 +163,28 @@ def test_filter_then_compact_and_capacity(dev):
     with pytest.raises(vr.VrError) as e:
         vr.compact(dev, [vr.device_column(abi.INT64, a)], n, sel, small)
     assert e.value.status == abi.VR_ERR_CAPACITY
+
+
+def test_gather_matches_oracle(dev):
+    """vr_gather of every lane by an index array with -1 padding (a join's probeIdx / buildIdx)."""
+    import torch
+    from vecruntime_amd import vr
+    rng = np.random.default_rng(33)
+    for n, m in ((1, 1), (100, 65), (5000, 200_003)):
+        print("gather n %d m %d" % (n, m), flush=True)
+        nulls = rng.random(n) < 0.2
+        idx = rng.integers(-1, n, m).astype(np.int32)
+        strs = ["".join(chr(97 + int(c)) for c in rng.integers(0, 26, int(k))) for k in rng.integers(0, 9, n)]
+        cols = [Column.ints(rng.integers(-9, 9, n), nulls=nulls), Column.longs(rng.integers(-10 ** 15, 10 ** 15, n), nulls=nulls),
+                Column.doubles(rng.standard_normal(n), nulls=nulls),
+                Column.decimals([int(x) * 10 ** 25 - 3 for x in rng.integers(-10 ** 6, 10 ** 6, n)], nulls=nulls),
+                Column.bools(rng.random(n) < 0.5, nulls=nulls), utf8_column(strs, nulls)]
+        keep = []
+        didx = torch.from_numpy(idx).cuda()
+        for col in cols:
+            dcol = on_device(col, keep)
+            nbytes = vr.gather_utf8_bytes(dev, dcol, didx, m) if col.type == abi.UTF8 else 0
+            want = oracle.gather(col, idx, utf8_bytes=nbytes)
+            out = DevOut(col.type, m, nbytes)
+            vr.gather(dev, dcol, didx, m, out)
+            assert as_bytes(out, col.type, m) == as_bytes(want, col.type, m), (col.type, n, m)

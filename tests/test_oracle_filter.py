@@ -150,3 +150,36 @@ def test_compact_every_lane():
                 assert [v for v in o.values(m) if v is not None] == [dec[i] for i in idx if not nulls[c][i]]
             else:
                 assert o.data[:m].tolist() == [src[i] for i in idx]
+
+
+def test_gather_every_lane():
+    """vro_gather against the rules of GatherKernels (GatherKernels.java:99-308): row idx[k], -1 a
+    null whose fixed value and BOOL bit are zero, validity unless null or padded, UTF8 bytes of the
+    valid real rows only."""
+    rng = np.random.default_rng(12)
+    n, m = 200, 333
+    nulls = rng.random(n) < 0.2
+    idx = rng.integers(-1, n, m).astype(np.int32)
+    idx[idx < 0] = -1
+    strs = ["".join(chr(97 + int(c)) for c in rng.integers(0, 26, int(k))) for k in rng.integers(0, 7, n)]
+    i32, i64, f64 = rng.integers(-9, 9, n), rng.integers(-10 ** 15, 10 ** 15, n), rng.standard_normal(n)
+    dec = [int(x) * 10 ** 25 - 3 for x in rng.integers(-10 ** 6, 10 ** 6, n)]
+    bools = rng.random(n) < 0.5
+    cases = [(Column.ints(i32, nulls=nulls), i32), (Column.longs(i64, nulls=nulls), i64),
+             (Column.doubles(f64, nulls=nulls), f64), (Column.decimals(dec, nulls=nulls), dec),
+             (Column.bools(bools, nulls=nulls), bools), (utf8_column(strs, nulls), strs)]
+    for col, src in cases:
+        nbytes = sum(len(strs[i]) for i in idx if i >= 0 and not nulls[i]) if col.type == abi.UTF8 else 0
+        o = oracle.gather(col, idx, utf8_bytes=nbytes)
+        assert o.valid(m).tolist() == [bool(i >= 0 and not nulls[i]) for i in idx]
+        if col.type == abi.UTF8:
+            got = [bytes(o.data[o.offsets[k]:o.offsets[k + 1]]).decode() for k in range(m)]
+            assert got == [strs[i] if i >= 0 and not nulls[i] else "" for i in idx]
+        elif col.type == abi.BOOL:
+            assert unpack_bits(o.data, m).tolist() == [bool(i >= 0 and bools[i]) for i in idx]
+        elif col.type == abi.DECIMAL128:
+            raw = o.data[: 2 * m].reshape(m, 2)
+            want = [(dec[i] if i >= 0 else 0) for i in idx]
+            assert [(int(lo) & (2 ** 64 - 1)) | (int(hi) << 64) for lo, hi in raw] == want
+        else:
+            assert o.data[:m].tolist() == [src[i] if i >= 0 else 0 for i in idx]

@@ -1,5 +1,5 @@
-// Selection producers and compaction on gfx950: vr_filter_compare, vr_compact_utf8_bytes and
-// vr_compact of include/vr.h.
+// Selection producers, compaction and gathers on gfx950: vr_filter_compare, vr_compact_utf8_bytes,
+// vr_compact, vr_gather_utf8_bytes and vr_gather of include/vr.h.
 //
 // Replaces the filter operator's predicate-to-selection step and its compacting branch
 // (spark/src/main/scala/org/apache/spark/sql/vecruntime/VectorFilterExec.scala:80-115) for a
@@ -200,6 +200,79 @@ __global__ void selected_bytes_kernel(const int32_t* offsets, const uint8_t* sel
     if (t) atomicAdd(total, t);
 }
 
+struct GatherArgs {
+    int32_t kind;  // 0: 4-byte lane, 1: 8-byte, 2: 16-byte, 3: BOOL bits, 4: plain UTF8
+    const void* data;
+    const uint8_t* valid;
+    const int32_t* offsets;
+    const int32_t* idx;
+    int64_t m;
+    void* out_data;
+    uint64_t* out_valid;
+    int32_t* out_offsets;
+    int32_t* lens;
+};
+
+__device__ __forceinline__ bool bit_of(const uint8_t* bm, int64_t i) { return (bm[i >> 3] >> (i & 7)) & 1u; }
+
+// GatherKernels.gatherFixed / gatherBits / gatherValidity (GatherKernels.java:99-248): a wave per
+// 64 output rows, so bitmaps are written as whole words.
+__global__ __launch_bounds__(FILTER_BLOCK) void gather_kernel(GatherArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t words = (a.m + 63) / 64;
+    const int64_t waves = (int64_t)gridDim.x * (FILTER_BLOCK / 64);
+    for (int64_t w = (int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64; w < words; w += waves) {
+        const int64_t o = w * 64 + lane;
+        const bool live = o < a.m;
+        const int64_t i = live ? a.idx[o] : -1;
+        const bool real = i >= 0;
+        const bool valid = real && (!a.valid || bit_of(a.valid, i));
+        if (live) {
+            switch (a.kind) {
+                case 0: ((int32_t*)a.out_data)[o] = real ? ((const int32_t*)a.data)[i] : 0; break;
+                case 1: ((int64_t*)a.out_data)[o] = real ? ((const int64_t*)a.data)[i] : 0; break;
+                case 2:
+                    ((int64_t*)a.out_data)[2 * o] = real ? ((const int64_t*)a.data)[2 * i] : 0;
+                    ((int64_t*)a.out_data)[2 * o + 1] = real ? ((const int64_t*)a.data)[2 * i + 1] : 0;
+                    break;
+                case 4: a.lens[o] = valid ? a.offsets[i + 1] - a.offsets[i] : 0; break;
+                default: break;
+            }
+        }
+        if (a.kind == 3) {
+            const uint64_t bits = __ballot(real && bit_of((const uint8_t*)a.data, i));
+            if (lane == 0) ((uint64_t*)a.out_data)[w] = bits;
+        }
+        if (a.out_valid) {
+            const uint64_t bits = __ballot(valid);
+            if (lane == 0) a.out_valid[w] = bits;
+        }
+    }
+}
+
+// GatherKernels.gatherUtf8 (:274-308): the bytes of the valid, real rows at their offsets.
+__global__ void gather_utf8_kernel(GatherArgs a) {
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < a.m; o += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = a.idx[o];
+        if (i < 0 || (a.valid && !bit_of(a.valid, i))) continue;
+        const int32_t src = a.offsets[i], len = a.offsets[i + 1] - src, dst = a.out_offsets[o];
+        const uint8_t* s = (const uint8_t*)a.data + src;
+        uint8_t* d = (uint8_t*)a.out_data + dst;
+        for (int32_t b = 0; b < len; b++) d[b] = s[b];
+    }
+}
+
+// GatherKernels.gatherUtf8Bytes (:253-268).
+__global__ void gather_bytes_kernel(const int32_t* offsets, const uint8_t* valid, const int32_t* idx, int64_t m,
+                                    unsigned long long* total) {
+    unsigned long long t = 0;
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < m; o += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx[o];
+        if (i >= 0 && (!valid || bit_of(valid, i))) t += (unsigned long long)(offsets[i + 1] - offsets[i]);
+    }
+    if (t) atomicAdd(total, t);
+}
+
 __global__ void store_i32_kernel(int32_t* p, int32_t v) {
     if (threadIdx.x == 0) *p = v;
 }
@@ -370,6 +443,81 @@ vr_status vr_compact(vr_device* dev, const vr_column* columns, int32_t num_colum
     VR_TRY_HIP(hipStreamSynchronize(s));
     d.count(&vr_stats::calls, 1);
     d.count(&vr_stats::input_rows, n);
+    d.count(&vr_stats::output_rows, m);
+    return VR_OK;
+}
+
+vr_status vr_gather_utf8_bytes(vr_device* dev, const vr_column* col, const int32_t* idx, int64_t m, int64_t* bytes,
+                               void* stream) {
+    if (!dev || !col || !bytes || (!idx && m)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (col->type != VR_UTF8 || col->dictionary || !col->offsets) return fail(VR_ERR_INVALID_ARGUMENT, "expected a plain UTF8 column");
+    VR_TRY(check_device(*col));
+    *bytes = 0;
+    if (m <= 0) return VR_OK;
+    Device& d = dev->d;
+    hipStream_t s = d.pick(stream);
+    DevBuf tot;
+    VR_TRY(tot.ensure(&d, 8));
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, tot.as<uint64_t>(), 1ull);
+    hipLaunchKernelGGL(gather_bytes_kernel, dim3(grid_of((uint64_t)m, 256, d.num_cus * 16)), dim3(256), 0, s, col->offsets,
+                       col->validity, idx, m, tot.as<unsigned long long>());
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t b = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&b, tot.p, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    *bytes = (int64_t)b;
+    return VR_OK;
+}
+
+vr_status vr_gather(vr_device* dev, const vr_column* col, const int32_t* idx, int64_t m, vr_out_column* out, void* stream) {
+    if (!dev || !col || !out || (!idx && m)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    if (m < 0 || m >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "gathers of past 2^31 rows");
+    VR_TRY(check_device(*col));
+    if (out->mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "gathers write device-resident outputs");
+    if (out->capacity < m) return fail(VR_ERR_CAPACITY, "the gather needs " + std::to_string(m) + " rows");
+    if (col->type == VR_UTF8 && col->dictionary) return fail(VR_ERR_UNSUPPORTED, "gather a dictionary column's int32 codes");
+    if (out->type != col->type) return fail(VR_ERR_INVALID_ARGUMENT, "output lane differs from the column");
+    GatherArgs a{};
+    switch (col->type) {
+        case VR_INT32: a.kind = 0; break;
+        case VR_INT64:
+        case VR_FLOAT64: a.kind = 1; break;
+        case VR_DECIMAL128: a.kind = 2; break;
+        case VR_BOOL: a.kind = 3; break;
+        default:
+            a.kind = 4;
+            if (!col->offsets || !out->offsets) return fail(VR_ERR_INVALID_ARGUMENT, "a UTF8 gather needs offsets");
+    }
+    a.data = col->data;
+    a.valid = col->validity;
+    a.offsets = col->offsets;
+    a.idx = idx;
+    a.m = m;
+    a.out_data = out->data;
+    a.out_valid = (uint64_t*)out->validity;
+    a.out_offsets = out->offsets;
+    Device& d = dev->d;
+    hipStream_t s = d.pick(stream);
+    const int cap = d.num_cus * 16;
+    DevBuf lens, tmp;
+    if (a.kind == 4) {
+        VR_TRY(lens.ensure(&d, (size_t)std::max<int64_t>(m, 1) * 4));
+        a.lens = lens.as<int32_t>();
+        hipLaunchKernelGGL(store_i32_kernel, dim3(1), dim3(64), 0, s, out->offsets, 0);
+    }
+    const uint64_t words = (uint64_t)(m + 63) / 64;
+    if (words)
+        hipLaunchKernelGGL(gather_kernel, dim3(grid_of(words * 64, FILTER_BLOCK, cap)), dim3(FILTER_BLOCK), 0, s, a);
+    if (a.kind == 4 && m) {
+        size_t tb = 0;
+        VR_TRY_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, a.lens, out->offsets + 1, (int)m, s));
+        VR_TRY(tmp.ensure(&d, tb));
+        VR_TRY_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, a.lens, out->offsets + 1, (int)m, s));
+        hipLaunchKernelGGL(gather_utf8_kernel, dim3(grid_of((uint64_t)m, 256, cap)), dim3(256), 0, s, a);
+    }
+    VR_TRY_HIP(hipGetLastError());
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    d.count(&vr_stats::calls, 1);
     d.count(&vr_stats::output_rows, m);
     return VR_OK;
 }

@@ -67,6 +67,8 @@ _sig("vr_filter_compare", [_P, C.POINTER(abi.VrColumn), C.c_int32, C.POINTER(abi
                            C.POINTER(C.c_int64), _P])
 _sig("vr_compact_utf8_bytes", [_P, C.POINTER(abi.VrColumn), C.c_int64, _P, C.POINTER(C.c_int64), _P])
 _sig("vr_compact", [_P, _P, C.c_int32, C.c_int64, _P, _P, C.POINTER(C.c_int64), _P])
+_sig("vr_gather_utf8_bytes", [_P, C.POINTER(abi.VrColumn), _P, C.c_int64, C.POINTER(C.c_int64), _P])
+_sig("vr_gather", [_P, C.POINTER(abi.VrColumn), _P, C.c_int64, C.POINTER(abi.VrOutColumn), _P])
 _sig("vr_join_destroy", [_P])
 
 
@@ -395,3 +397,14 @@ def compact(device, columns, n, selection, outputs, stream=None):
     check(_lib.vr_compact(device.h, C.cast(ca, _P), len(columns), int(n), _ptr(selection), C.cast(oa, _P), C.byref(rows),
                           _stream(stream)))
     return rows.value
+
+
+def gather_utf8_bytes(device, col, idx, m, stream=None):
+    b = C.c_int64()
+    check(_lib.vr_gather_utf8_bytes(device.h, C.byref(col.c()), _ptr(idx), int(m), C.byref(b), _stream(stream)))
+    return b.value
+
+
+def gather(device, col, idx, m, out, stream=None):
+    """vr_gather of a device column by a device int32 index tensor into a device output column."""
+    check(_lib.vr_gather(device.h, C.byref(col.c()), _ptr(idx), int(m), C.byref(out.c()), _stream(stream)))
