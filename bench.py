@@ -436,6 +436,56 @@ def run_join(dev, args):
     return res
 
 
+def run_filters(dev, args):
+    """The filter operator's device kernels on 200M-row columns: an int32 < literal selection
+    (vr_filter_compare), its compaction of three columns (int64, float64, int32; about 30% pass,
+    vr_compact) and a gather of 40M rows of an int64 column by a random index array (vr_gather).
+    Kernel time from HIP events on the stream the calls run on; bytes are the columns read whole
+    plus what is written."""
+    import torch
+    from vecruntime_amd import abi, vr
+    n = args.filter_rows
+    g = torch.Generator(device="cuda")
+    g.manual_seed(107)
+    f = torch.randint(0, 1000, (n,), dtype=torch.int32, device="cuda", generator=g)
+    a = torch.randint(-10 ** 12, 10 ** 12, (n,), dtype=torch.int64, device="cuda", generator=g)
+    b = torch.rand(n, dtype=torch.float64, device="cuda", generator=g)
+    stream = torch.cuda.current_stream()
+    fcol = vr.device_column(abi.INT32, f)
+    cols = [vr.device_column(abi.INT64, a), vr.device_column(abi.FLOAT64, b), fcol]
+    sel = torch.zeros(((n + 63) // 64) * 8, dtype=torch.uint8, device="cuda")
+
+    def timed(fn, reps=5):
+        fn()  # warm-up (and the output sizes)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, r
+
+    ms_cmp, (_, m) = timed(lambda: vr.filter_compare(dev, fcol, abi.CMP_LT, 300, n, out=sel, stream=stream))
+    outs = [vr.device_out_column(t, m, torch.empty(m * w, dtype=torch.uint8, device="cuda"))
+            for t, w in ((abi.INT64, 8), (abi.FLOAT64, 8), (abi.INT32, 4))]
+    ms_cpt, rows = timed(lambda: vr.compact(dev, cols, n, sel, outs, stream=stream))
+    assert rows == m
+    gm = 40_000_000
+    idx = torch.randint(0, 10_000_000, (gm,), dtype=torch.int32, device="cuda", generator=g)
+    gout = vr.device_out_column(abi.INT64, gm, torch.empty(gm * 8, dtype=torch.uint8, device="cuda"))
+    src = vr.device_column(abi.INT64, a[:10_000_000])
+    ms_gat, _ = timed(lambda: vr.gather(dev, src, idx, gm, gout, stream=stream))
+    res = {}
+    for name, ms, nbytes in (("compare", ms_cmp, n * 4 + n / 8), ("compact", ms_cpt, n * 20 + n / 8 + m * 20),
+                             ("gather", ms_gat, gm * (4 + 8 + 8))):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+        log("filters %s: %.3f ms, %.0f GB/s (%.1f%% of %d)" % (name, ms, gbs, 100 * gbs / HBM_PEAK_GBS, HBM_PEAK_GBS))
+    res.update({"rows": n, "selected": m, "gather_rows": gm})
+    del f, a, b, sel, outs, idx, gout
+    return res
+
+
 SERVE_VALUE_TYPES = ("int32", "int64", "decimal(12,2)", "float64")
 
 
@@ -618,6 +668,8 @@ def main():
     ap.add_argument("--uneven-steps", type=int, default=2)
     ap.add_argument("--tune-budget", type=float, default=60.0)
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-filters", action="store_true")
+    ap.add_argument("--filter-rows", type=int, default=200_000_000)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-workers", type=int, default=4)
@@ -648,6 +700,7 @@ def main():
     join = None if args.skip_join else run_join(dev, args)
     uneven = None if args.skip_uneven else run_uneven(dev, args)
     serve = None if args.skip_serve else run_serve(dev, args)
+    filters = None if args.skip_filters else run_filters(dev, args)
     # roofline of the update: algorithmic bytes (the input columns, read once) / the duration of its
     # kernels (hot-set sampling, scatter, hot reduce, merge), timed with HIP events on their stream
     bytes_per_row = 8 + 8 + 8 + 4 + 3 / 8.0
@@ -689,6 +742,7 @@ def main():
         "join": join,
         "batched_uneven": uneven,
         "serving": serve,
+        "filters": filters,
     }
     dev.close()
     print(json.dumps(line), flush=True)

@@ -22,7 +22,9 @@
 namespace vr {
 namespace {
 
-constexpr int FILTER_BLOCK = 256;  // four waves, one 64-row word each per step
+constexpr int FILTER_BLOCK = 256;  // four waves
+
+__device__ __forceinline__ bool bit_of(const uint8_t* bm, int64_t i) { return (bm[i >> 3] >> (i & 7)) & 1u; }
 
 __device__ __forceinline__ uint64_t word_at(const uint8_t* bm, int64_t w, int64_t n) {
     // the bits of rows [64w, min(64w + 64, n)) of an LSB-first bitmap; a null bitmap is all ones
@@ -30,6 +32,9 @@ __device__ __forceinline__ uint64_t word_at(const uint8_t* bm, int64_t w, int64_
     const int64_t left = n - base;
     const uint64_t live = left >= 64 ? ~0ull : ((1ull << left) - 1);
     if (!bm) return live;
+    // a whole word of an 8-byte-aligned bitmap in one load; the last, partial word byte by byte
+    // (input bitmaps hold ceil(n/8) bytes, nothing past them)
+    if (left >= 64 && ((uintptr_t)bm & 7) == 0) return ((const uint64_t*)bm)[w];
     uint64_t x = 0;
     const int64_t bytes = (left + 7) / 8 < 8 ? (left + 7) / 8 : 8;
     for (int64_t b = 0; b < bytes; b++) x |= (uint64_t)bm[w * 8 + b] << (8 * b);
@@ -85,25 +90,105 @@ struct CompareArgs {
     int32_t type, op;
     vr_scalar s;
     uint64_t* out;
-    unsigned long long* count;
+    unsigned long long* partial;  // selected rows per workgroup (summed by sum_partials_kernel)
 };
 
+constexpr int FU = 4;    // 64-row words a compaction or gather wave has in flight per step
+constexpr int FUC = 16;  // the same for a comparison (its wave holds only the values)
+
+// The lane's value as a comparable pair (hi, lo): integers and doubles use one word.
+template <int32_t TYPE>
+struct Lane;
+template <>
+struct Lane<VR_INT32> {
+    typedef int32_t T;
+    __device__ static T load(const void* d, int64_t i) { return ((const int32_t*)d)[i]; }
+    __device__ static int cmp(T x, const vr_scalar& s) {
+        const int32_t y = (int32_t)s.i64;
+        return (x > y) - (x < y);
+    }
+};
+template <>
+struct Lane<VR_INT64> {
+    typedef int64_t T;
+    __device__ static T load(const void* d, int64_t i) { return ((const int64_t*)d)[i]; }
+    __device__ static int cmp(T x, const vr_scalar& s) { return (x > s.i64) - (x < s.i64); }
+};
+template <>
+struct Lane<VR_FLOAT64> {
+    typedef double T;
+    __device__ static T load(const void* d, int64_t i) { return ((const double*)d)[i]; }
+    __device__ static int cmp(T x, const vr_scalar& s) {
+        const double y = s.f64;
+        const bool xn = x != x, yn = y != y;
+        if ((xn && yn) || x == y) return 0;
+        if (xn) return 1;
+        if (yn) return -1;
+        return x > y ? 1 : -1;
+    }
+};
+template <>
+struct Lane<VR_DECIMAL128> {
+    struct T {
+        int64_t lo, hi;
+    };
+    __device__ static T load(const void* d, int64_t i) {
+        const int64_t* p = (const int64_t*)d + 2 * i;
+        return T{p[0], p[1]};
+    }
+    __device__ static int cmp(T x, const vr_scalar& s) {
+        if (x.hi != s.hi) return x.hi > s.hi ? 1 : -1;
+        return ((uint64_t)x.lo > (uint64_t)s.lo) - ((uint64_t)x.lo < (uint64_t)s.lo);
+    }
+};
+
+// A wave takes FUC consecutive words per step: the values of all of them are loaded before any
+// comparison, so each wave has FUC coalesced reads in flight.
+template <int32_t TYPE>
 __global__ __launch_bounds__(FILTER_BLOCK) void compare_kernel(CompareArgs a) {
+    typedef Lane<TYPE> L;
     const int lane = threadIdx.x & 63;
     const int64_t words = (a.n + 63) / 64;
     const int64_t waves = (int64_t)gridDim.x * (FILTER_BLOCK / 64);
     unsigned long long total = 0;
-    for (int64_t w = (int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64; w < words; w += waves) {
-        const uint64_t live = word_at(a.sel, w, a.n) & word_at(a.valid, w, a.n);
-        const int64_t i = w * 64 + lane;
-        const bool on = ((live >> lane) & 1ull) && holds(a.op, compare_row(a.type, a.data, i, a.s));
-        const uint64_t bits = __ballot(on);
-        if (lane == 0) {
-            a.out[w] = bits;
-            total += (unsigned long long)__popcll(bits);
+    for (int64_t w0 = ((int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64) * FUC; w0 < words; w0 += waves * FUC) {
+        typename L::T x[FUC];
+#pragma unroll
+        for (int u = 0; u < FUC; u++) {
+            const int64_t i = (w0 + u) * 64 + lane;
+            if (i < a.n) x[u] = L::load(a.data, i);
+        }
+#pragma unroll
+        for (int u = 0; u < FUC; u++) {
+            const int64_t w = w0 + u;
+            if (w >= words) break;  // uniform over the wave
+            const uint64_t live = word_at(a.sel, w, a.n) & word_at(a.valid, w, a.n);
+            const bool on = ((live >> lane) & 1ull) && holds(a.op, L::cmp(x[u], a.s));
+            const uint64_t bits = __ballot(on);
+            if (lane == 0) {
+                a.out[w] = bits;
+                total += (unsigned long long)__popcll(bits);
+            }
         }
     }
-    if (lane == 0 && total) atomicAdd(a.count, total);
+    // the workgroup's count, summed by one later workgroup (no contended atomics)
+    __shared__ unsigned long long wave_total[FILTER_BLOCK / 64];
+    if (lane == 0) wave_total[threadIdx.x / 64] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int q = 0; q < FILTER_BLOCK / 64; q++) t += wave_total[q];
+        a.partial[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const unsigned long long* partial, int n, unsigned long long* out) {
+    typedef hipcub::BlockReduce<unsigned long long, 256> Reduce;
+    __shared__ typename Reduce::TempStorage tmp;
+    unsigned long long t = 0;
+    for (int q = threadIdx.x; q < n; q += 256) t += partial[q];
+    const unsigned long long sum = Reduce(tmp).Sum(t);
+    if (threadIdx.x == 0) *out = sum;
 }
 
 // Selected rows per 64-row word (the exclusive scan of these places each word's rows).
@@ -141,39 +226,67 @@ __device__ __forceinline__ uint64_t packed_bits(bool selected, bool v, uint64_t 
     return mine;
 }
 
+// A wave takes FU consecutive words per step: their selection words, positions and the selected
+// lanes' values are all loaded before anything is stored.
 __global__ __launch_bounds__(FILTER_BLOCK) void compact_kernel(CompactCol c, const uint8_t* sel, int64_t n,
                                                             const uint32_t* word_off) {
     const int lane = threadIdx.x & 63;
     const int64_t words = (n + 63) / 64;
     const int64_t waves = (int64_t)gridDim.x * (FILTER_BLOCK / 64);
-    for (int64_t w = (int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64; w < words; w += waves) {
-        const uint64_t sw = word_at(sel, w, n);
-        if (!sw) continue;  // uniform over the wave
-        const int64_t i = w * 64 + lane;
-        const bool selected = (sw >> lane) & 1ull;
-        const uint64_t o = (uint64_t)word_off[w] + (uint64_t)__popcll(sw & ((1ull << lane) - 1));
-        const int k = __popcll(sw);
-        if (selected) {
-            switch (c.kind) {
-                case 0: ((int32_t*)c.out_data)[o] = ((const int32_t*)c.data)[i]; break;
-                case 1: ((int64_t*)c.out_data)[o] = ((const int64_t*)c.data)[i]; break;
-                case 2:
-                    ((int64_t*)c.out_data)[2 * o] = ((const int64_t*)c.data)[2 * i];
-                    ((int64_t*)c.out_data)[2 * o + 1] = ((const int64_t*)c.data)[2 * i + 1];
-                    break;
-                case 4: c.lens[o] = c.offsets[i + 1] - c.offsets[i]; break;  // lengths, scanned into offsets
-                default: break;
+    for (int64_t w0 = ((int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64) * FU; w0 < words; w0 += waves * FU) {
+        uint64_t sw[FU], base[FU], x0[FU], x1[FU];
+        bool vb[FU];
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            const int64_t w = w0 + u;
+            sw[u] = w < words ? word_at(sel, w, n) : 0ull;
+            base[u] = w < words ? word_off[w] : 0;
+            const int64_t i = w * 64 + lane;
+            const bool selected = (sw[u] >> lane) & 1ull;
+            x0[u] = x1[u] = 0;
+            vb[u] = false;
+            if (selected) {
+                switch (c.kind) {
+                    case 0: x0[u] = (uint32_t)((const int32_t*)c.data)[i]; break;
+                    case 1: x0[u] = ((const uint64_t*)c.data)[i]; break;
+                    case 2:
+                        x0[u] = ((const uint64_t*)c.data)[2 * i];
+                        x1[u] = ((const uint64_t*)c.data)[2 * i + 1];
+                        break;
+                    case 3: vb[u] = bit_of((const uint8_t*)c.data, i); break;
+                    case 4: x0[u] = (uint32_t)(c.offsets[i + 1] - c.offsets[i]); break;
+                    default: break;
+                }
             }
         }
-        if (c.kind == 3) {
-            const bool v = selected && (((const uint8_t*)c.data)[i >> 3] >> (i & 7)) & 1u;
-            const uint64_t bits = packed_bits(selected, v, sw, lane);
-            if (lane == 0) or_bits((uint64_t*)c.out_data, word_off[w], bits, k);
-        }
-        if (c.out_valid) {
-            const bool v = selected && (!c.valid || ((c.valid[i >> 3] >> (i & 7)) & 1u));
-            const uint64_t bits = packed_bits(selected, v, sw, lane);
-            if (lane == 0) or_bits(c.out_valid, word_off[w], bits, k);
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            if (!sw[u]) continue;  // uniform over the wave
+            const int64_t i = (w0 + u) * 64 + lane;
+            const bool selected = (sw[u] >> lane) & 1ull;
+            const uint64_t o = base[u] + (uint64_t)__popcll(sw[u] & ((1ull << lane) - 1));
+            const int k = __popcll(sw[u]);
+            if (selected) {
+                switch (c.kind) {
+                    case 0: ((uint32_t*)c.out_data)[o] = (uint32_t)x0[u]; break;
+                    case 1: ((uint64_t*)c.out_data)[o] = x0[u]; break;
+                    case 2:
+                        ((uint64_t*)c.out_data)[2 * o] = x0[u];
+                        ((uint64_t*)c.out_data)[2 * o + 1] = x1[u];
+                        break;
+                    case 4: c.lens[o] = (int32_t)x0[u]; break;  // lengths, scanned into offsets
+                    default: break;
+                }
+            }
+            if (c.kind == 3) {
+                const uint64_t bits = packed_bits(selected, vb[u], sw[u], lane);
+                if (lane == 0) or_bits((uint64_t*)c.out_data, base[u], bits, k);
+            }
+            if (c.out_valid) {
+                const bool v = selected && (!c.valid || bit_of(c.valid, i));
+                const uint64_t bits = packed_bits(selected, v, sw[u], lane);
+                if (lane == 0) or_bits(c.out_valid, base[u], bits, k);
+            }
         }
     }
 }
@@ -213,39 +326,67 @@ struct GatherArgs {
     int32_t* lens;
 };
 
-__device__ __forceinline__ bool bit_of(const uint8_t* bm, int64_t i) { return (bm[i >> 3] >> (i & 7)) & 1u; }
-
 // GatherKernels.gatherFixed / gatherBits / gatherValidity (GatherKernels.java:99-248): a wave per
 // 64 output rows, so bitmaps are written as whole words.
 __global__ __launch_bounds__(FILTER_BLOCK) void gather_kernel(GatherArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t words = (a.m + 63) / 64;
     const int64_t waves = (int64_t)gridDim.x * (FILTER_BLOCK / 64);
-    for (int64_t w = (int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64; w < words; w += waves) {
-        const int64_t o = w * 64 + lane;
-        const bool live = o < a.m;
-        const int64_t i = live ? a.idx[o] : -1;
-        const bool real = i >= 0;
-        const bool valid = real && (!a.valid || bit_of(a.valid, i));
-        if (live) {
-            switch (a.kind) {
-                case 0: ((int32_t*)a.out_data)[o] = real ? ((const int32_t*)a.data)[i] : 0; break;
-                case 1: ((int64_t*)a.out_data)[o] = real ? ((const int64_t*)a.data)[i] : 0; break;
-                case 2:
-                    ((int64_t*)a.out_data)[2 * o] = real ? ((const int64_t*)a.data)[2 * i] : 0;
-                    ((int64_t*)a.out_data)[2 * o + 1] = real ? ((const int64_t*)a.data)[2 * i + 1] : 0;
-                    break;
-                case 4: a.lens[o] = valid ? a.offsets[i + 1] - a.offsets[i] : 0; break;
-                default: break;
+    for (int64_t w0 = ((int64_t)blockIdx.x * (FILTER_BLOCK / 64) + threadIdx.x / 64) * FU; w0 < words; w0 += waves * FU) {
+        int64_t idx[FU];
+        uint64_t x0[FU], x1[FU];
+        bool valid[FU], bitv[FU];
+#pragma unroll
+        for (int u = 0; u < FU; u++) {  // the indices of FU words, then their rows' values
+            const int64_t o = (w0 + u) * 64 + lane;
+            idx[u] = o < a.m ? a.idx[o] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            const int64_t i = idx[u];
+            const bool real = i >= 0;
+            valid[u] = real && (!a.valid || bit_of(a.valid, i));
+            x0[u] = x1[u] = 0;
+            bitv[u] = false;
+            if (real) {
+                switch (a.kind) {
+                    case 0: x0[u] = (uint32_t)((const int32_t*)a.data)[i]; break;
+                    case 1: x0[u] = ((const uint64_t*)a.data)[i]; break;
+                    case 2:
+                        x0[u] = ((const uint64_t*)a.data)[2 * i];
+                        x1[u] = ((const uint64_t*)a.data)[2 * i + 1];
+                        break;
+                    case 3: bitv[u] = bit_of((const uint8_t*)a.data, i); break;
+                    case 4: x0[u] = valid[u] ? (uint32_t)(a.offsets[i + 1] - a.offsets[i]) : 0u; break;
+                    default: break;
+                }
             }
         }
-        if (a.kind == 3) {
-            const uint64_t bits = __ballot(real && bit_of((const uint8_t*)a.data, i));
-            if (lane == 0) ((uint64_t*)a.out_data)[w] = bits;
-        }
-        if (a.out_valid) {
-            const uint64_t bits = __ballot(valid);
-            if (lane == 0) a.out_valid[w] = bits;
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            const int64_t w = w0 + u;
+            if (w >= words) break;  // uniform over the wave
+            const int64_t o = w * 64 + lane;
+            if (o < a.m) {
+                switch (a.kind) {
+                    case 0: ((uint32_t*)a.out_data)[o] = (uint32_t)x0[u]; break;
+                    case 1: ((uint64_t*)a.out_data)[o] = x0[u]; break;
+                    case 2:
+                        ((uint64_t*)a.out_data)[2 * o] = x0[u];
+                        ((uint64_t*)a.out_data)[2 * o + 1] = x1[u];
+                        break;
+                    case 4: a.lens[o] = (int32_t)x0[u]; break;
+                    default: break;
+                }
+            }
+            if (a.kind == 3) {
+                const uint64_t bits = __ballot(bitv[u]);
+                if (lane == 0) ((uint64_t*)a.out_data)[w] = bits;
+            }
+            if (a.out_valid) {
+                const uint64_t bits = __ballot(valid[u]);
+                if (lane == 0) a.out_valid[w] = bits;
+            }
         }
     }
 }
@@ -310,13 +451,20 @@ vr_status vr_filter_compare(vr_device* dev, const vr_column* col, int32_t op, co
     if (n == 0) return VR_OK;
     Device& d = dev->d;
     hipStream_t s = d.pick(stream);
-    DevBuf cnt;
-    VR_TRY(cnt.ensure(&d, 8));
-    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, cnt.as<uint64_t>(), 1ull);
-    CompareArgs a{col->data, col->validity, selection, n, col->type, op, *scalar, (uint64_t*)out,
-                  cnt.as<unsigned long long>()};
     const uint64_t words = (uint64_t)(n + 63) / 64;
-    hipLaunchKernelGGL(compare_kernel, dim3(grid_of(words * 64, FILTER_BLOCK, d.num_cus * 16)), dim3(FILTER_BLOCK), 0, s, a);
+    const dim3 grid(grid_of((words + FUC - 1) / FUC * 64, FILTER_BLOCK, d.num_cus * 32));
+    DevBuf cnt;
+    VR_TRY(cnt.ensure(&d, ((size_t)grid.x + 1) * 8));
+    CompareArgs a{col->data, col->validity, selection, n, col->type, op, *scalar, (uint64_t*)out,
+                  cnt.as<unsigned long long>() + 1};
+    switch (col->type) {
+        case VR_INT32: hipLaunchKernelGGL(compare_kernel<VR_INT32>, grid, dim3(FILTER_BLOCK), 0, s, a); break;
+        case VR_INT64: hipLaunchKernelGGL(compare_kernel<VR_INT64>, grid, dim3(FILTER_BLOCK), 0, s, a); break;
+        case VR_FLOAT64: hipLaunchKernelGGL(compare_kernel<VR_FLOAT64>, grid, dim3(FILTER_BLOCK), 0, s, a); break;
+        default: hipLaunchKernelGGL(compare_kernel<VR_DECIMAL128>, grid, dim3(FILTER_BLOCK), 0, s, a); break;
+    }
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, cnt.as<unsigned long long>() + 1, (int)grid.x,
+                       cnt.as<unsigned long long>());
     VR_TRY_HIP(hipGetLastError());
     uint64_t c = 0;
     VR_TRY_HIP(hipMemcpyAsync(&c, cnt.p, 8, hipMemcpyDeviceToHost, s));
@@ -426,8 +574,8 @@ vr_status vr_compact(vr_device* dev, const vr_column* columns, int32_t num_colum
             cc.lens = keep.back().as<int32_t>();
         }
         if (words)
-            hipLaunchKernelGGL(compact_kernel, dim3(grid_of(words * 64, FILTER_BLOCK, cap)), dim3(FILTER_BLOCK), 0, s, cc, selection, n,
-                               offs.as<uint32_t>());
+            hipLaunchKernelGGL(compact_kernel, dim3(grid_of((words + FU - 1) / FU * 64, FILTER_BLOCK, 2 * cap)), dim3(FILTER_BLOCK), 0,
+                               s, cc, selection, n, offs.as<uint32_t>());
         if (cc.kind == 4 && m) {
             // lengths -> offsets[1..m] (running ends), then the bytes
             size_t tb = 0;
@@ -507,7 +655,7 @@ vr_status vr_gather(vr_device* dev, const vr_column* col, const int32_t* idx, in
     }
     const uint64_t words = (uint64_t)(m + 63) / 64;
     if (words)
-        hipLaunchKernelGGL(gather_kernel, dim3(grid_of(words * 64, FILTER_BLOCK, cap)), dim3(FILTER_BLOCK), 0, s, a);
+        hipLaunchKernelGGL(gather_kernel, dim3(grid_of((words + FU - 1) / FU * 64, FILTER_BLOCK, 2 * cap)), dim3(FILTER_BLOCK), 0, s, a);
     if (a.kind == 4 && m) {
         size_t tb = 0;
         VR_TRY_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, a.lens, out->offsets + 1, (int)m, s));
