@@ -29,7 +29,7 @@ BATCH = 65536
 # (profiles/agg_update_groupby200M_pmc_traffic.md).  Measured for the kernel as committed with it.
 # HBM bytes moved by one partitioned update of the group-by (sampling, scatter, hot reduce, merge),
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a device copy of known size
-PMC_TRAFFIC_BYTES = {200_000_000: 12.08e9}
+PMC_TRAFFIC_BYTES = {200_000_000: 12.10e9}
 
 
 def log(*a):
