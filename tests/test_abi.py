@@ -42,7 +42,8 @@ def test_struct_layouts_match_c():
                "vr_launch_config": abi.VrLaunchConfig, "vr_join_spec": abi.VrJoinSpec,
                "vr_probe_output": abi.VrProbeOutput, "vr_server_config": abi.VrServerConfig,
                "vr_query": abi.VrQuery, "vr_query_result": abi.VrQueryResult,
-               "vr_server_stats": abi.VrServerStats}
+               "vr_server_stats": abi.VrServerStats, "vr_scalar": abi.VrScalar,
+               "vr_join_residual": abi.VrJoinResidual}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vr.h"', '#include "vr_serve.h"',
              "int main(void) {"]
     for cname, py in structs.items():
