@@ -486,6 +486,42 @@ def run_filters(dev, args):
     return res
 
 
+def run_shuffle(dev, args):
+    """The shuffle writer's map side on the device over 200M rows: Spark's hash partition ids of an
+    int64 key into 200 partitions (vr_hash_partition_ids: 8 bytes read and 4 written per row), then
+    the rows grouped by partition with their destinations (vr_partition_rows: the ids read, rows
+    and destinations written, 12 bytes per row at the least).  HIP events on the calls' stream."""
+    import torch
+    from vecruntime_amd import abi, vr
+    n, parts = args.filter_rows, 200
+    g = torch.Generator(device="cuda")
+    g.manual_seed(109)
+    a = torch.randint(-2 ** 62, 2 ** 62, (n,), dtype=torch.int64, device="cuda", generator=g)
+    stream = torch.cuda.current_stream()
+    key = [vr.device_column(abi.INT64, a)]
+
+    def timed(fn, reps=5):
+        r = fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, r
+
+    ms_hash, (ids, _) = timed(lambda: vr.hash_partition_ids(dev, key, [abi.HASH_LONG], n, parts, stream=stream))
+    ms_group, (rows, dest, starts) = timed(lambda: vr.partition_rows(dev, ids, n, parts, stream=stream))
+    assert int(starts[-1].item()) == n
+    res = {"rows": n, "partitions": parts}
+    for name, ms, nbytes in (("hash_ids", ms_hash, n * 12), ("group_rows", ms_group, n * 12)):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "rows_per_s": n / (ms * 1e-3), "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+        log("shuffle %s: %.3f ms, %.0f GB/s (%.1f%% of %d)" % (name, ms, gbs, 100 * gbs / HBM_PEAK_GBS, HBM_PEAK_GBS))
+    del a, ids, rows, dest, starts
+    return res
+
+
 SERVE_VALUE_TYPES = ("int32", "int64", "decimal(12,2)", "float64")
 
 
@@ -701,6 +737,7 @@ def main():
     uneven = None if args.skip_uneven else run_uneven(dev, args)
     serve = None if args.skip_serve else run_serve(dev, args)
     filters = None if args.skip_filters else run_filters(dev, args)
+    shuffle = None if args.skip_filters else run_shuffle(dev, args)
     # roofline of the update: algorithmic bytes (the input columns, read once) / the duration of its
     # kernels (hot-set sampling, scatter, hot reduce, merge), timed with HIP events on their stream
     bytes_per_row = 8 + 8 + 8 + 4 + 3 / 8.0
@@ -743,6 +780,7 @@ def main():
         "batched_uneven": uneven,
         "serving": serve,
         "filters": filters,
+        "shuffle": shuffle,
     }
     dev.close()
     print(json.dumps(line), flush=True)

@@ -409,6 +409,52 @@ vr_status vr_gather_utf8_bytes(vr_device* dev, const vr_column* col, const int32
 vr_status vr_gather(vr_device* dev, const vr_column* col, const int32_t* idx, int64_t m, vr_out_column* out,
                     void* stream);
 
+/* ------------------------------------------------------------------ shuffle partitioning */
+
+/* How Spark hashes a key column (PartitionKernels.KeyKind,
+ * kernels/src/main/java/io/vecruntime/kernels/PartitionKernels.java:55-86); the lane alone does not
+ * say (a long or a decimal, a float or a double). */
+enum vr_hash_kind {
+    VR_HASH_INT = 0,        /* IntegerType, DateType on an INT32 lane: hashInt */
+    VR_HASH_LONG = 1,       /* LongType, timestamps, decimals of at most 18 digits on an INT64 lane: hashLong */
+    VR_HASH_DOUBLE = 2,     /* DoubleType: hashLong of the bits, -0.0 as 0.0, one NaN */
+    VR_HASH_BOOL = 3,       /* BooleanType on a BOOL lane: hashInt(1 / 0) */
+    VR_HASH_UTF8 = 4,       /* StringType on a plain UTF8 lane: hashUnsafeBytes (Spark's byte-wise tail) */
+    VR_HASH_DECIMAL128 = 5, /* a decimal above 18 digits: hashUnsafeBytes of the unscaled value's
+                               minimal big-endian two's-complement bytes */
+    VR_HASH_FLOAT = 6       /* FloatType on a FLOAT64 lane: hashInt of the float's bits, both zeros 0,
+                               one NaN */
+};
+
+/* Spark's HashPartitioning ids, Pmod(Murmur3Hash(keys, seed), num_partitions) with Spark's seed 42
+ * (PartitionKernels.hashPartitionIds, kernels/src/main/java/io/vecruntime/kernels/PartitionKernels.java:
+ * 375-401; VectorShuffleManager.partitionIds, shuffle/src/main/scala/org/apache/spark/sql/vecruntime/
+ * shuffle/VectorShuffleManager.scala:332-350; another seed for the buckets of an aggregate spill,
+ * AggregateSpill.scala:106): every key mixes into the running hash in key order, a null key leaving
+ * it unchanged (PartitionKernels.mixColumn, :199-250).  `ids` gets each row's partition, `hashes`
+ * (may be NULL) its hash.  Key data, validity and offsets, `ids` and `hashes` in device memory; the
+ * dictionary of a dictionary-encoded UTF8 key may be in host or device memory.  A code outside its
+ * dictionary fails with VR_ERR_INVALID_ARGUMENT.  Synchronous with respect to `stream`. */
+vr_status vr_hash_partition_ids(vr_device* dev, const vr_column* keys, const int32_t* kinds, int32_t num_keys,
+                                int64_t n, int32_t num_partitions, int32_t seed, int32_t* ids, int32_t* hashes,
+                                void* stream);
+
+/* Spark's RoundRobinPartitioning ids continuing from `start` (PartitionKernels.roundRobinIds,
+ * PartitionKernels.java:411-422): ids[i] = (start + i) mod num_partitions; `next` (may be NULL) gets
+ * the start of the following batch.  `ids` in device memory. */
+vr_status vr_round_robin_ids(vr_device* dev, int64_t n, int32_t num_partitions, int32_t start, int32_t* ids,
+                             int32_t* next, void* stream);
+
+/* The rows grouped by partition, row order kept within each -- a stable counting sort
+ * (PartitionKernels.partitionOrder / partitionDestinations, PartitionKernels.java:437-482, as the
+ * shuffle writer uses them, shuffle/src/main/scala/io/vecruntime/shuffle/PartitionedIpcWriter.scala:
+ * 1003,1150-1154).  `rows` (may be NULL, n int32) lists partition 0's rows, then partition 1's, ...;
+ * `dest` (may be NULL, n int32) the slot each row takes in that order; `starts` (num_partitions + 1
+ * int64) where each partition begins, starts[num_partitions] = n.  An id outside [0, num_partitions)
+ * fails with VR_ERR_INVALID_ARGUMENT.  Device memory; synchronous with respect to `stream`. */
+vr_status vr_partition_rows(vr_device* dev, const int32_t* ids, int64_t n, int32_t num_partitions, int32_t* rows,
+                            int32_t* dest, int64_t* starts, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,9 @@
 //   * BuildTable.build's head/next chains in build order and the probe's pair order, semi/anti
 //     selection and null-key rules (spark/.../sql/vecruntime/VectorHashJoinExec.scala:478-516,
 //     622-633,994,1038-1182).
+//   * shuffle partitioning: Spark's Pmod(Murmur3Hash(keys, seed), n) with its Murmur3_x86_32
+//     routines and per-type rules, round-robin ids and the stable grouping by partition
+//     (kernels/src/main/java/io/vecruntime/kernels/PartitionKernels.java:89-482).
 // The decimal average's final division follows Spark's DecimalDivideWithOverflowCheck
 // (org.apache.spark:spark-sql_2.13:4.1.3, not vendored in the reference): the exact quotient
 // rounded half away from zero at the result scale; its double rounding through Spark's 38-digit
@@ -1216,6 +1219,149 @@ int64_t vro_agg_partial_threads(const vr_agg_spec* spec, const vr_column* keys, 
     int64_t total = 0;
     for (auto g : groups) total += g;
     return total;
+}
+
+}  // extern "C"
+
+// ---- shuffle partitioning (PartitionKernels.java)
+
+// Murmur3_x86_32 as Spark has it (PartitionKernels.mixK1 / mixH1 / fmix, :89-110).
+static uint32_t mm_k1(uint32_t k) {
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    return k * 0x1b873593u;
+}
+static uint32_t mm_h1(uint32_t h, uint32_t k) {
+    h ^= k;
+    h = (h << 13) | (h >> 19);
+    return h * 5u + 0xe6546b64u;
+}
+static uint32_t mm_fmix(uint32_t h, uint32_t len) {
+    h ^= len;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+static uint32_t mm_int(uint32_t v, uint32_t seed) { return mm_fmix(mm_h1(seed, mm_k1(v)), 4); }  // hashInt :113-117
+static uint32_t mm_long(uint64_t v, uint32_t seed) {  // hashLong :120-128: low word, then high word
+    return mm_fmix(mm_h1(mm_h1(seed, mm_k1((uint32_t)v)), mm_k1((uint32_t)(v >> 32))), 8);
+}
+// hashUnsafeBytes (:135-150): little-endian words, then the tail a signed byte at a time.
+static uint32_t mm_bytes(const uint8_t* p, int64_t len, uint32_t seed) {
+    uint32_t h = seed;
+    const int64_t aligned = len - len % 4;
+    for (int64_t i = 0; i < aligned; i += 4) {
+        uint32_t w;
+        std::memcpy(&w, p + i, 4);
+        h = mm_h1(h, mm_k1(w));
+    }
+    for (int64_t i = aligned; i < len; i++) h = mm_h1(h, mm_k1((uint32_t)(int32_t)(int8_t)p[i]));
+    return mm_fmix(h, (uint32_t)len);
+}
+
+extern "C" {
+
+// hashPartitionIds (:389-401) with mixColumn's per-kind rules (:199-250, doubleBits / floatBits
+// :160-177, the decimal as BigInteger.toByteArray of the unscaled value).
+int vro_hash_partition_ids(const vr_column* keys, const int32_t* kinds, int32_t nkeys, int64_t n, int32_t parts,
+                           int32_t seed, int32_t* ids, int32_t* hashes) {
+    if (parts < 1) return fail(VR_ERR_INVALID_ARGUMENT, "partition count below 1");
+    std::vector<uint32_t> h((size_t)n, (uint32_t)seed);
+    for (int c = 0; c < nkeys; c++) {
+        const vr_column& k = keys[c];
+        for (int64_t i = 0; i < n; i++) {
+            if (!valid(k, i)) continue;  // a null key leaves the hash as it was
+            uint32_t& x = h[(size_t)i];
+            switch (kinds[c]) {
+                case VR_HASH_INT: x = mm_int((uint32_t)((const int32_t*)k.data)[i], x); break;
+                case VR_HASH_LONG: x = mm_long((uint64_t)((const int64_t*)k.data)[i], x); break;
+                case VR_HASH_DOUBLE: {
+                    const double v = ((const double*)k.data)[i];
+                    uint64_t b;
+                    std::memcpy(&b, &v, 8);
+                    if (v == 0.0) b = 0;
+                    else if (std::isnan(v)) b = 0x7ff8000000000000ull;
+                    x = mm_long(b, x);
+                    break;
+                }
+                case VR_HASH_FLOAT: {
+                    const float f = (float)((const double*)k.data)[i];
+                    uint32_t b;
+                    std::memcpy(&b, &f, 4);
+                    if (f == 0.0f) b = 0;
+                    else if (std::isnan(f)) b = 0x7fc00000u;
+                    x = mm_int(b, x);
+                    break;
+                }
+                case VR_HASH_BOOL: x = mm_int(bit((const uint8_t*)k.data, i) ? 1 : 0, x); break;
+                case VR_HASH_DECIMAL128: {
+                    const uint64_t lo = ((const uint64_t*)k.data)[2 * i];
+                    const int64_t hi = ((const int64_t*)k.data)[2 * i + 1];
+                    const __int128 v = (__int128)(((unsigned __int128)(uint64_t)hi << 64) | lo);
+                    // toByteArray: the fewest big-endian bytes whose sign bit is the value's
+                    int nb = 1;
+                    while (nb < 16) {
+                        const __int128 lim = (__int128)1 << (8 * nb - 1);
+                        if (v >= -lim && v < lim) break;
+                        nb++;
+                    }
+                    uint8_t be[16];
+                    for (int j = 0; j < nb; j++) be[j] = (uint8_t)(v >> (8 * (nb - 1 - j)));
+                    x = mm_bytes(be, nb, x);
+                    break;
+                }
+                case VR_HASH_UTF8: {
+                    if (k.dictionary) {
+                        const int32_t code = ((const int32_t*)k.data)[i];
+                        const vr_column& dc = *k.dictionary;
+                        if (code < 0 || code >= dc.length) return fail(VR_ERR_INVALID_ARGUMENT, "a dictionary code outside its dictionary");
+                        x = mm_bytes((const uint8_t*)dc.data + dc.offsets[code], dc.offsets[code + 1] - dc.offsets[code], x);
+                    } else {
+                        x = mm_bytes((const uint8_t*)k.data + k.offsets[i], k.offsets[i + 1] - k.offsets[i], x);
+                    }
+                    break;
+                }
+                default: return fail(VR_ERR_INVALID_ARGUMENT, "unknown hash kind");
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; i++) {
+        if (hashes) hashes[i] = (int32_t)h[(size_t)i];
+        const int32_t r = (int32_t)h[(size_t)i] % parts;  // pmod (:365-368)
+        ids[i] = r < 0 ? r + parts : r;
+    }
+    return VR_OK;
+}
+
+// roundRobinIds (:411-422).
+int vro_round_robin_ids(int64_t n, int32_t parts, int32_t start, int32_t* ids, int32_t* next) {
+    int32_t p = start;
+    for (int64_t i = 0; i < n; i++) {
+        ids[i] = p;
+        if (++p == parts) p = 0;
+    }
+    if (next) *next = p;
+    return VR_OK;
+}
+
+// partitionOrder / partitionDestinations (:437-482): a counting sort, stable.
+int vro_partition_rows(const int32_t* ids, int64_t n, int32_t parts, int32_t* rows, int32_t* dest, int64_t* starts) {
+    std::vector<int64_t> cur((size_t)parts + 1, 0);
+    for (int64_t i = 0; i < n; i++) {
+        if (ids[i] < 0 || ids[i] >= parts) return fail(VR_ERR_INVALID_ARGUMENT, "a partition id outside [0, num_partitions)");
+        cur[(size_t)ids[i] + 1]++;
+    }
+    for (int32_t p = 0; p < parts; p++) cur[(size_t)p + 1] += cur[(size_t)p];
+    for (int32_t p = 0; p <= parts; p++) starts[p] = cur[(size_t)p];
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t slot = cur[(size_t)ids[i]]++;
+        if (rows) rows[slot] = (int32_t)i;
+        if (dest) dest[i] = (int32_t)slot;
+    }
+    return VR_OK;
 }
 
 }  // extern "C"

@@ -53,6 +53,9 @@ def lib():
         L.vro_compact.argtypes = [P, C.c_int32, C.c_int64, P, P, C.POINTER(C.c_int64)]
         L.vro_gather.argtypes = [C.POINTER(abi.VrColumn), P, C.c_int64, C.POINTER(abi.VrOutColumn)]
         L.vro_join_destroy.argtypes = [P]
+        L.vro_hash_partition_ids.argtypes = [P, P, C.c_int32, C.c_int64, C.c_int32, C.c_int32, P, P]
+        L.vro_round_robin_ids.argtypes = [C.c_int64, C.c_int32, C.c_int32, P, C.POINTER(C.c_int32)]
+        L.vro_partition_rows.argtypes = [P, C.c_int64, C.c_int32, P, P, P]
         L.vro_agg_partial_threads.argtypes = [C.POINTER(abi.VrAggSpec), P, P, P, C.c_int32, C.c_int32,
                                               C.POINTER(C.c_double)]
         L.vro_agg_partial_threads.restype = C.c_int64
@@ -234,3 +237,34 @@ def gather(col, idx, utf8_bytes=0):
         oc = OutColumn(col.type, max(1, m))
     _check(lib().vro_gather(C.byref(col.c()), idx.ctypes.data, m, C.byref(oc.c())))
     return oc
+
+
+def hash_partition_ids(keys, kinds, n, num_partitions, seed=42):
+    """Spark's Pmod(Murmur3Hash(keys, seed), num_partitions): (ids, hashes), int32 arrays of n."""
+    ids = np.zeros(max(1, n), np.int32)
+    hashes = np.zeros(max(1, n), np.int32)
+    ka = c_array(keys) if keys else None
+    kd = np.ascontiguousarray(kinds, np.int32)
+    _check(lib().vro_hash_partition_ids(C.cast(ka, C.c_void_p) if ka is not None else None, kd.ctypes.data, len(keys),
+                                        int(n), int(num_partitions), int(seed), ids.ctypes.data, hashes.ctypes.data))
+    return ids[:n], hashes[:n]
+
+
+def round_robin_ids(n, num_partitions, start):
+    """(ids, next start) of Spark's round-robin partitioning."""
+    ids = np.zeros(max(1, n), np.int32)
+    nxt = C.c_int32()
+    _check(lib().vro_round_robin_ids(int(n), int(num_partitions), int(start), ids.ctypes.data, C.byref(nxt)))
+    return ids[:n], nxt.value
+
+
+def partition_rows(ids, num_partitions):
+    """The stable grouping of rows by partition: (rows, dest, starts)."""
+    ids = np.ascontiguousarray(ids, np.int32)
+    n = len(ids)
+    rows = np.zeros(max(1, n), np.int32)
+    dest = np.zeros(max(1, n), np.int32)
+    starts = np.zeros(num_partitions + 1, np.int64)
+    _check(lib().vro_partition_rows(ids.ctypes.data, n, int(num_partitions), rows.ctypes.data, dest.ctypes.data,
+                                    starts.ctypes.data))
+    return rows[:n], dest[:n], starts

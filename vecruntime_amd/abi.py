@@ -47,6 +47,10 @@ RESULT = 2
 INNER, LEFT_SEMI, LEFT_ANTI, OUTER = 0, 1, 2, 3
 CMP_EQ, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE = 0, 1, 2, 3, 4, 5
 
+# vr_hash_kind: how a shuffle key is hashed (PartitionKernels.KeyKind)
+HASH_INT, HASH_LONG, HASH_DOUBLE, HASH_BOOL, HASH_UTF8, HASH_DECIMAL128, HASH_FLOAT = 0, 1, 2, 3, 4, 5, 6
+SPARK_SEED = 42
+
 
 class VrColumn(C.Structure):
     _fields_ = [

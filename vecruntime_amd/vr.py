@@ -70,6 +70,9 @@ _sig("vr_compact", [_P, _P, C.c_int32, C.c_int64, _P, _P, C.POINTER(C.c_int64), 
 _sig("vr_gather_utf8_bytes", [_P, C.POINTER(abi.VrColumn), _P, C.c_int64, C.POINTER(C.c_int64), _P])
 _sig("vr_gather", [_P, C.POINTER(abi.VrColumn), _P, C.c_int64, C.POINTER(abi.VrOutColumn), _P])
 _sig("vr_join_destroy", [_P])
+_sig("vr_hash_partition_ids", [_P, _P, _P, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _P, _P, _P])
+_sig("vr_round_robin_ids", [_P, C.c_int64, C.c_int32, C.c_int32, _P, C.POINTER(C.c_int32), _P])
+_sig("vr_partition_rows", [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P])
 
 
 def lib():
@@ -408,3 +411,38 @@ def gather_utf8_bytes(device, col, idx, m, stream=None):
 def gather(device, col, idx, m, out, stream=None):
     """vr_gather of a device column by a device int32 index tensor into a device output column."""
     check(_lib.vr_gather(device.h, C.byref(col.c()), _ptr(idx), int(m), C.byref(out.c()), _stream(stream)))
+
+
+def hash_partition_ids(device, keys, kinds, n, num_partitions, seed=abi.SPARK_SEED, want_hashes=False, stream=None):
+    """vr_hash_partition_ids over device key columns: the partition ids (int32 device tensor) and,
+    when asked, the hashes -- (ids, hashes or None)."""
+    import torch
+    ids = torch.empty(max(1, n), dtype=torch.int32, device="cuda")
+    hashes = torch.empty(max(1, n), dtype=torch.int32, device="cuda") if want_hashes else None
+    ka = c_array(keys) if keys else None
+    kd = (C.c_int32 * max(1, len(kinds)))(*kinds)
+    check(_lib.vr_hash_partition_ids(device.h, C.cast(ka, _P) if ka is not None else None, C.cast(kd, _P), len(keys),
+                                     int(n), int(num_partitions), int(seed), _ptr(ids), _ptr(hashes), _stream(stream)))
+    return ids[:n], (hashes[:n] if hashes is not None else None)
+
+
+def round_robin_ids(device, n, num_partitions, start, stream=None):
+    """vr_round_robin_ids: (ids as an int32 device tensor, the next batch's start)."""
+    import torch
+    ids = torch.empty(max(1, n), dtype=torch.int32, device="cuda")
+    nxt = C.c_int32()
+    check(_lib.vr_round_robin_ids(device.h, int(n), int(num_partitions), int(start), _ptr(ids), C.byref(nxt),
+                                  _stream(stream)))
+    return ids[:n], nxt.value
+
+
+def partition_rows(device, ids, n, num_partitions, want_rows=True, want_dest=True, stream=None):
+    """vr_partition_rows of a device int32 id tensor: (rows, dest, starts) device tensors, rows or
+    dest None when not asked for."""
+    import torch
+    rows = torch.empty(max(1, n), dtype=torch.int32, device="cuda") if want_rows else None
+    dest = torch.empty(max(1, n), dtype=torch.int32, device="cuda") if want_dest else None
+    starts = torch.empty(num_partitions + 1, dtype=torch.int64, device="cuda")
+    check(_lib.vr_partition_rows(device.h, _ptr(ids), int(n), int(num_partitions), _ptr(rows), _ptr(dest),
+                                 _ptr(starts), _stream(stream)))
+    return (rows[:n] if rows is not None else None), (dest[:n] if dest is not None else None), starts
