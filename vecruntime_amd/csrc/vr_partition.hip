@@ -221,7 +221,7 @@ __global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
 // et al., "GPU Multisplit", PPoPP 2016); the lowest of them adds the group's size to the wave's
 // LDS count for the id, and a lane's rank is the number of its peers below it.  Chunks in order,
 // waves in order, tiles in order: the grouping keeps row order within a partition.
-constexpr int MS_ROWS = 32;                   // rows per lane
+constexpr int MS_ROWS = 16;                   // rows per lane
 constexpr int MS_WAVE_ROWS = 64 * MS_ROWS;     // consecutive rows per wave
 constexpr int MS_TILE = 4 * MS_WAVE_ROWS;      // rows per block of 256 threads
 constexpr int MS_MAX_PARTS = 512;
@@ -274,35 +274,59 @@ __global__ void __launch_bounds__(256) ms_count_kernel(const int32_t* ids, int64
 }
 
 // offsets: the exclusive scan of counts (partition-major), the first slot of partition p's rows of
-// tile b.
+// tile b.  A tile's rows of one partition take consecutive slots, so each row is first placed in
+// LDS at its slot less the partition's delta (its first slot less the rows of lower partitions in
+// the tile) and the tile's `rows` then leave in runs of one partition each.
 __global__ void __launch_bounds__(256) ms_place_kernel(const int32_t* ids, int64_t n, int32_t parts, int bits,
                                                        int64_t tiles, const int32_t* offsets, int32_t* rows,
                                                        int32_t* dest) {
+    typedef hipcub::BlockScan<int32_t, 256> TileScan;
     __shared__ int32_t next[4 * MS_MAX_PARTS];  // per wave: the next free slot of each partition
+    __shared__ int32_t delta[MS_MAX_PARTS];
+    __shared__ int32_t srow[MS_TILE], sslot[MS_TILE];
+    __shared__ typename TileScan::TempStorage scan_tmp;
+    __shared__ int32_t s_live;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t base = (int64_t)blockIdx.x * MS_TILE + (int64_t)w * MS_WAVE_ROWS;
     uint32_t v[MS_ROWS], okmask;
     ms_load(ids, base, n, parts, v, okmask);
     for (int p = threadIdx.x; p < 4 * parts; p += 256) next[p] = 0;
     __syncthreads();
+    uint64_t peers[MS_ROWS];  // kept for the placing below
+#pragma unroll
     for (int c = 0; c < MS_ROWS; c++) {  // this wave's counts
         const bool ok = (okmask >> c) & 1u;
-        const uint64_t m = peer_lanes(v[c], bits, __ballot(ok));
-        if (ok && lane == __builtin_ctzll(m)) atomicAdd(&next[w * parts + v[c]], __popcll(m));
+        peers[c] = peer_lanes(v[c], bits, __ballot(ok));
+        if (ok && lane == __builtin_ctzll(peers[c])) atomicAdd(&next[w * parts + v[c]], __popcll(peers[c]));
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < parts; p += 256) {  // the tile's start, then the waves in order
-        int32_t run = offsets[(int64_t)p * tiles + blockIdx.x];
-        for (int q = 0; q < 4; q++) {
-            const int32_t t = next[q * parts + p];
-            next[q * parts + p] = run;
-            run += t;
+    // partitions 2t and 2t + 1: the tile's first slot, then the waves in order; the tile's total
+    int32_t first[2], total[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int p = 2 * threadIdx.x + k;
+        first[k] = total[k] = 0;
+        if (p < parts) {
+            int32_t run = first[k] = offsets[(int64_t)p * tiles + blockIdx.x];
+            for (int q = 0; q < 4; q++) {
+                const int32_t t = next[q * parts + p];
+                next[q * parts + p] = run;
+                run += t;
+            }
+            total[k] = run - first[k];
         }
     }
+    int32_t live;
+    TileScan(scan_tmp).ExclusiveSum(total, total, live);
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (2 * (int)threadIdx.x + k < parts) delta[2 * threadIdx.x + k] = first[k] - total[k];
+    if (threadIdx.x == 0) s_live = live;
     __syncthreads();
+#pragma unroll
     for (int c = 0; c < MS_ROWS; c++) {
         const bool ok = (okmask >> c) & 1u;
-        const uint64_t m = peer_lanes(v[c], bits, __ballot(ok));
+        const uint64_t m = peers[c];
         const int leader = ok ? __builtin_ctzll(m) : lane;
         int32_t at = 0;
         if (ok && lane == leader) at = atomicAdd(&next[w * parts + v[c]], __popcll(m));
@@ -311,9 +335,14 @@ __global__ void __launch_bounds__(256) ms_place_kernel(const int32_t* ids, int64
             const int64_t i = base + c * 64 + lane;
             const int32_t slot = at + __popcll(m & ((1ull << lane) - 1));
             if (dest) dest[i] = slot;
-            if (rows) rows[slot] = (int32_t)i;
+            const int32_t at_tile = slot - delta[v[c]];
+            srow[at_tile] = (int32_t)i;
+            sslot[at_tile] = slot;
         }
     }
+    __syncthreads();
+    if (rows)
+        for (int j = threadIdx.x; j < s_live; j += 256) rows[sslot[j]] = srow[j];
 }
 
 __global__ void __launch_bounds__(256) ms_starts_kernel(const int32_t* offsets, int32_t parts, int64_t tiles, int64_t n,

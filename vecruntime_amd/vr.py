@@ -11,6 +11,7 @@ been built raises, and there is no CPU fallback here.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -95,12 +96,18 @@ def check(st):
 
 
 def _stream(s):
-    """A HIP stream handle: None (the handle's own stream), an int, or a torch stream."""
-    if s is None:
+    """A HIP stream handle: None (the handle's own stream), an int, or a torch stream.
+
+    The handle's own stream is non-blocking, so nothing orders it after work torch has queued on
+    its current stream (a tensor still being filled or copied): a call that runs there first waits
+    for that work.  A call on a torch stream is ordered with it by the stream itself."""
+    h = s if s is None or isinstance(s, int) else s.cuda_stream
+    if not h:
+        t = sys.modules.get("torch")
+        if t is not None and t.cuda.is_initialized():
+            t.cuda.current_stream().synchronize()
         return None
-    if isinstance(s, int):
-        return s
-    return s.cuda_stream
+    return h
 
 
 def _ptr(a):
@@ -378,8 +385,8 @@ def filter_compare(device, col, op, value, n, selection=None, out=None, stream=N
     """vr_filter_compare over a device column: (selection bitmap as a uint8 device tensor padded to
     8 bytes, selected rows)."""
     import torch
-    if out is None:
-        out = torch.zeros(((n + 63) // 64) * 8, dtype=torch.uint8, device="cuda")
+    if out is None:  # every word is written
+        out = torch.empty(((n + 63) // 64) * 8, dtype=torch.uint8, device="cuda")
     sc = abi.scalar(value, col.type)
     cnt = C.c_int64()
     check(_lib.vr_filter_compare(device.h, C.byref(col.c()), int(op), C.byref(sc), _ptr(selection), int(n), _ptr(out),
