@@ -43,9 +43,9 @@ def test_each_kind_matches_oracle(dev, kind):
 
 
 def test_many_keys_dictionary_and_errors(dev):
-    """Seven keys of every kind, then the same seven twice over (fourteen) and three times (21 keys:
-    two launches over the running hash, with and without the hashes asked for); a dictionary key
-    with its dictionary on the host; no keys; a code outside its dictionary."""
+    """Eight keys (every kind and a dictionary string key with its dictionary on the host), then the
+    same eight twice and three times over (24 keys), with and without the hashes asked for; no
+    keys; a code outside its dictionary; a kind that does not fit its lane."""
     import torch
     from vecruntime_amd import vr
     rng = np.random.default_rng(77)
@@ -108,16 +108,20 @@ def test_partition_rows_matches_oracle(dev, parts):
 def test_partition_rows_rejects_an_id_outside(dev):
     import torch
     from vecruntime_amd import vr
-    for bad in (5, -1, 1 << 30):
-        ids = torch.tensor([0, 1, bad, 2], dtype=torch.int32, device="cuda")
-        with pytest.raises(vr.VrError) as e:
-            vr.partition_rows(dev, ids, 4, 5)
-        assert e.value.status == abi.VR_ERR_INVALID_ARGUMENT
+    for parts in (5, 1000):  # the counting sort, then the radix sort
+        for bad in (parts, -1, 1 << 30):
+            ids = torch.tensor([0, 1, bad, 2], dtype=torch.int32, device="cuda")
+            with pytest.raises(vr.VrError) as e:
+                vr.partition_rows(dev, ids, 4, parts)
+            assert e.value.status == abi.VR_ERR_INVALID_ARGUMENT
+    r, d, s = vr.partition_rows(dev, torch.tensor([2, 0, 1, 0, 2], dtype=torch.int32, device="cuda"), 5, 4)
+    assert r.cpu().tolist() == [1, 3, 2, 0, 4] and d.cpu().tolist() == [3, 0, 2, 1, 4]
+    assert s.cpu().tolist() == [0, 2, 3, 5, 5]
 
 
 def test_shuffle_map_side_at_size(dev):
-    """The map side of a 200-partition shuffle over 20M rows with a long and a string-free double
-    key: ids from the hash, then the grouping; properties checked at full size on the device (the
+    """The map side of a 200-partition shuffle over 20M rows with a long and a double key: ids
+    from the hash, then the grouping; properties checked at full size on the device (the
     ids of the grouped rows are sorted, rows keep their order within a partition, starts are the
     cumulative counts, dest inverts rows) and the first 200,000 rows against the oracle."""
     import torch
