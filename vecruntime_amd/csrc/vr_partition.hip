@@ -161,17 +161,65 @@ __device__ __forceinline__ uint32_t mix_key(const HashKey& k, int64_t i, uint32_
     }
 }
 
+constexpr int HASH_ROWS = 4;  // rows per thread per step, their loads issued before any is hashed
+
+// One key mixed into HASH_ROWS rows' running hashes: the kind is uniform, so the switch sits outside
+// the rows and every fixed-width value is loaded before the first is hashed.
+__device__ __forceinline__ void mix_rows(const HashKey& k, const int64_t (&i)[HASH_ROWS], const bool (&in)[HASH_ROWS],
+                                         uint32_t (&h)[HASH_ROWS], uint32_t* err) {
+    bool live[HASH_ROWS];
+#pragma unroll
+    for (int u = 0; u < HASH_ROWS; u++) live[u] = in[u] && valid_at(k.valid, i[u]);
+    switch (k.kind) {
+        case VR_HASH_INT: {
+            uint32_t x[HASH_ROWS];
+#pragma unroll
+            for (int u = 0; u < HASH_ROWS; u++) x[u] = in[u] ? (uint32_t)((const int32_t*)k.data)[i[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < HASH_ROWS; u++) if (live[u]) h[u] = hash_int(x[u], h[u]);
+            break;
+        }
+        case VR_HASH_LONG:
+        case VR_HASH_DOUBLE:
+        case VR_HASH_FLOAT: {
+            uint64_t x[HASH_ROWS];
+#pragma unroll
+            for (int u = 0; u < HASH_ROWS; u++) x[u] = in[u] ? ((const uint64_t*)k.data)[i[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < HASH_ROWS; u++) {
+                if (!live[u]) continue;
+                if (k.kind == VR_HASH_LONG) h[u] = hash_long(x[u], h[u]);
+                else if (k.kind == VR_HASH_DOUBLE) h[u] = hash_long(double_bits(__longlong_as_double((long long)x[u])), h[u]);
+                else h[u] = hash_int(float_bits((float)__longlong_as_double((long long)x[u])), h[u]);
+            }
+            break;
+        }
+        default:
+#pragma unroll
+            for (int u = 0; u < HASH_ROWS; u++) if (live[u]) h[u] = mix_key(k, i[u], h[u], err);
+    }
+}
+
 __global__ void __launch_bounds__(256) hash_ids_kernel(HashArgs a) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        uint32_t h = a.seed;
-        for (int c = 0; c < a.nkeys; c++) {  // a null key leaves the running hash as it was
-            const HashKey& k = a.keys[c];
-            if (valid_at(k.valid, i)) h = mix_key(k, i, h, a.err);
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < a.n; i0 += HASH_ROWS * stride) {
+        int64_t i[HASH_ROWS];
+        bool in[HASH_ROWS];
+        uint32_t h[HASH_ROWS];
+#pragma unroll
+        for (int u = 0; u < HASH_ROWS; u++) {
+            i[u] = i0 + u * stride;
+            in[u] = i[u] < a.n;
+            h[u] = a.seed;
         }
-        if (a.hashes) a.hashes[i] = h;
-        const int32_t r = (int32_t)h % a.parts;  // Pmod: the non-negative remainder
-        a.ids[i] = r < 0 ? r + a.parts : r;
+        for (int c = 0; c < a.nkeys; c++) mix_rows(a.keys[c], i, in, h, a.err);  // a null key leaves the hash
+#pragma unroll
+        for (int u = 0; u < HASH_ROWS; u++) {
+            if (!in[u]) continue;
+            if (a.hashes) a.hashes[i[u]] = h[u];
+            const int32_t r = (int32_t)h[u] % a.parts;  // Pmod: the non-negative remainder
+            a.ids[i[u]] = r < 0 ? r + a.parts : r;
+        }
     }
 }
 
@@ -429,7 +477,8 @@ vr_status vr_hash_partition_ids(vr_device* dev, const vr_column* keys, const int
     if (num_keys) VR_TRY_HIP(hipMemcpyAsync(kbuf.p, hk.data(), sizeof(HashKey) * (size_t)num_keys, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, err.as<uint32_t>());
     HashArgs a{kbuf.as<HashKey>(), num_keys, num_partitions, (uint32_t)seed, n, ids, (uint32_t*)hashes, err.as<uint32_t>()};
-    hipLaunchKernelGGL(hash_ids_kernel, dim3(grid_for((uint64_t)n, d.num_cus * 32)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(hash_ids_kernel, dim3(grid_for(((uint64_t)n + HASH_ROWS - 1) / HASH_ROWS, d.num_cus * 32)), dim3(256), 0,
+                       s, a);
     VR_TRY_HIP(hipGetLastError());
     uint32_t bad = 0;
     VR_TRY_HIP(hipMemcpyAsync(&bad, err.p, 4, hipMemcpyDeviceToHost, s));
