@@ -274,6 +274,15 @@ constexpr int MS_WAVE_ROWS = 64 * MS_ROWS;     // consecutive rows per wave
 constexpr int MS_TILE = 4 * MS_WAVE_ROWS;      // rows per block of 256 threads
 constexpr int MS_MAX_PARTS = 512;
 
+// The tile a block takes: blocks are dispatched to the 8 XCDs in turn, so XCD x gets a contiguous
+// run of tiles and its blocks' counts (adjacent words for one partition) meet in its own L2.  A
+// bijection of [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_tile() {
+    const uint32_t xcds = 8, g = gridDim.x, b = blockIdx.x;
+    const uint32_t q = g / xcds, r = g % xcds, x = b % xcds, k = b / xcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 __device__ __forceinline__ uint64_t peer_lanes(uint32_t id, int bits, uint64_t live) {
     uint64_t m = live;
     for (int b = 0; b < bits; b++) {
@@ -306,7 +315,8 @@ __global__ void __launch_bounds__(256) ms_count_kernel(const int32_t* ids, int64
     for (int p = threadIdx.x; p < parts; p += 256) hist[p] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int64_t base = (int64_t)blockIdx.x * MS_TILE + (int64_t)(threadIdx.x >> 6) * MS_WAVE_ROWS;
+    const uint32_t tile = xcd_tile();
+    const int64_t base = (int64_t)tile * MS_TILE + (int64_t)(threadIdx.x >> 6) * MS_WAVE_ROWS;
     uint32_t v[MS_ROWS], okmask;
     ms_load(ids, base, n, parts, v, okmask);
     bool bad = false;
@@ -318,7 +328,7 @@ __global__ void __launch_bounds__(256) ms_count_kernel(const int32_t* ids, int64
     }
     if (bad) *err = 1;
     __syncthreads();
-    for (int p = threadIdx.x; p < parts; p += 256) counts[(int64_t)p * tiles + blockIdx.x] = hist[p];
+    for (int p = threadIdx.x; p < parts; p += 256) counts[(int64_t)p * tiles + tile] = hist[p];
 }
 
 // offsets: the exclusive scan of counts (partition-major), the first slot of partition p's rows of
@@ -335,7 +345,8 @@ __global__ void __launch_bounds__(256) ms_place_kernel(const int32_t* ids, int64
     __shared__ typename TileScan::TempStorage scan_tmp;
     __shared__ int32_t s_live;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * MS_TILE + (int64_t)w * MS_WAVE_ROWS;
+    const uint32_t tile = xcd_tile();
+    const int64_t base = (int64_t)tile * MS_TILE + (int64_t)w * MS_WAVE_ROWS;
     uint32_t v[MS_ROWS], okmask;
     ms_load(ids, base, n, parts, v, okmask);
     for (int p = threadIdx.x; p < 4 * parts; p += 256) next[p] = 0;
@@ -355,7 +366,7 @@ __global__ void __launch_bounds__(256) ms_place_kernel(const int32_t* ids, int64
         const int p = 2 * threadIdx.x + k;
         first[k] = total[k] = 0;
         if (p < parts) {
-            int32_t run = first[k] = offsets[(int64_t)p * tiles + blockIdx.x];
+            int32_t run = first[k] = offsets[(int64_t)p * tiles + tile];
             for (int q = 0; q < 4; q++) {
                 const int32_t t = next[q * parts + p];
                 next[q * parts + p] = run;
