@@ -465,23 +465,28 @@ vr_status vr_hash_partition_ids(vr_device* dev, const vr_column* keys, const int
             if (dc.mem == VR_MEM_DEVICE) {
                 h.dict_off = dc.offsets;
                 h.dict_data = (const uint8_t*)dc.data;
-            } else {  // a host dictionary is copied over for this call
-                int32_t nbytes = dc.length ? dc.offsets[dc.length] : 0;
-                DevBuf off, bytes;
-                VR_TRY(off.ensure(&d, ((size_t)dc.length + 1) * 4));
-                VR_TRY(bytes.ensure(&d, (size_t)std::max(nbytes, 1)));
-                VR_TRY_HIP(hipMemcpyAsync(off.p, dc.offsets, ((size_t)dc.length + 1) * 4, hipMemcpyHostToDevice, s));
-                if (nbytes) VR_TRY_HIP(hipMemcpyAsync(bytes.p, dc.data, (size_t)nbytes, hipMemcpyHostToDevice, s));
-                h.dict_off = off.as<int32_t>();
-                h.dict_data = bytes.as<uint8_t>();
-                dicts.push_back(std::move(off));
-                dicts.push_back(std::move(bytes));
             }
         } else if (lane == VR_UTF8 && !k.offsets) {
             return fail(VR_ERR_INVALID_ARGUMENT, "a plain UTF8 key needs offsets");
         }
     }
     if (n == 0) return VR_OK;
+    // every key checked: host dictionaries are copied over for this call
+    for (int c = 0; c < num_keys; c++) {
+        const vr_column& k = keys[c];
+        if (k.type != VR_UTF8 || !k.dictionary || k.dictionary->mem == VR_MEM_DEVICE) continue;
+        const vr_column& dc = *k.dictionary;
+        const int32_t nbytes = dc.length ? dc.offsets[dc.length] : 0;
+        DevBuf off, bytes;
+        VR_TRY(off.ensure(&d, ((size_t)dc.length + 1) * 4));
+        VR_TRY(bytes.ensure(&d, (size_t)std::max(nbytes, 1)));
+        VR_TRY_HIP(hipMemcpyAsync(off.p, dc.offsets, ((size_t)dc.length + 1) * 4, hipMemcpyHostToDevice, s));
+        if (nbytes) VR_TRY_HIP(hipMemcpyAsync(bytes.p, dc.data, (size_t)nbytes, hipMemcpyHostToDevice, s));
+        hk[(size_t)c].dict_off = off.as<int32_t>();
+        hk[(size_t)c].dict_data = bytes.as<uint8_t>();
+        dicts.push_back(std::move(off));
+        dicts.push_back(std::move(bytes));
+    }
     DevBuf err, kbuf;
     VR_TRY(err.ensure(&d, 4));
     VR_TRY(kbuf.ensure(&d, sizeof(HashKey) * (size_t)std::max(1, num_keys)));
