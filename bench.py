@@ -383,6 +383,23 @@ def cpu_baseline(sample, threads):
     return nb * BATCH / secs, secs, groups
 
 
+def cpu_shuffle_baseline(keys, parts):
+    """The shuffle's map side on the host: the oracle's restatement of PartitionKernels (scalar C++,
+    one thread) over a sample of the same int64 keys -- hash partition ids, then the grouping."""
+    import time
+    from oracle import oracle
+    from vecruntime_amd import abi
+    from vecruntime_amd.columns import Column
+    col = [Column(abi.INT64, keys)]
+    t0 = time.perf_counter()
+    ids, _ = oracle.hash_partition_ids(col, [abi.HASH_LONG], len(keys), parts)
+    t1 = time.perf_counter()
+    oracle.partition_rows(ids, parts)
+    t2 = time.perf_counter()
+    return {"hash_ids_rows_per_s": len(keys) / (t1 - t0), "group_rows_rows_per_s": len(keys) / (t2 - t1), "cores": 1,
+            "kind": "port", "sample": "%d rows of the same keys, one thread, %.1f s" % (len(keys), t2 - t0)}
+
+
 def run_join(dev, args):
     """Join probe: 10M unique int64 build keys (int32 + decimal(7,2) payloads); probe batches of
     65,536 rows (90% hits), inner materialized / semi / anti, per-batch launches."""
@@ -518,6 +535,11 @@ def run_shuffle(dev, args):
         gbs = nbytes / (ms * 1e-3) / 1e9
         res[name] = {"ms": ms, "rows_per_s": n / (ms * 1e-3), "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
         log("shuffle %s: %.3f ms, %.0f GB/s (%.1f%% of %d)" % (name, ms, gbs, 100 * gbs / HBM_PEAK_GBS, HBM_PEAK_GBS))
+    if not args.skip_cpu:
+        res["cpu_baseline"] = cpu_shuffle_baseline(a[:20_000_000].cpu().numpy(), parts)
+        log("shuffle cpu (%s): hash_ids %.3g rows/s, group_rows %.3g rows/s"
+            % (res["cpu_baseline"]["sample"], res["cpu_baseline"]["hash_ids_rows_per_s"],
+               res["cpu_baseline"]["group_rows_rows_per_s"]))
     del a, ids, rows, dest, starts
     return res
 
