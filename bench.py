@@ -450,6 +450,36 @@ def run_join(dev, args):
         res[name] = dict(rows_per_s=npr / dt, ms=dt * 1e3, out_rows=total)
         log("join %s: %.1f ms, %.3e probe rows/s, %d out rows" % (name, dt * 1e3, npr / dt, total))
     bt.close()
+    if not args.skip_cpu:
+        sp = min(npr, 4_000_000)
+        res["cpu_baseline"] = cpu_join_baseline(spec, [x.cpu().numpy() for x in (bkeys, bpay, bdec)],
+                                                [x[:sp].cpu().numpy() for x in (pkeys, p1, p2)])
+        log("join cpu (%s): inner %.3g, semi %.3g, anti %.3g probe rows/s" % (
+            res["cpu_baseline"]["sample"], res["cpu_baseline"]["inner_rows_per_s"],
+            res["cpu_baseline"]["semi_rows_per_s"], res["cpu_baseline"]["anti_rows_per_s"]))
+    return res
+
+
+def cpu_join_baseline(spec, build, probe):
+    """The join leg on the host: the oracle's restatement of BuildTable.build and the probe (scalar
+    C++, one thread) over the same build side and a sample of the probe rows (one call per join type:
+    per-batch calls would time the Python wrapper's output allocations)."""
+    import time
+    from oracle.oracle import OracleJoin
+    from vecruntime_amd import abi
+    from vecruntime_amd.columns import Column
+    nb, npr = len(build[0]), len(probe[0])
+    t0 = time.perf_counter()
+    oj = OracleJoin(spec, [Column(abi.INT64, build[0]), Column(abi.INT32, build[1]), Column(abi.INT64, build[2])], nb)
+    res = {"build_s": time.perf_counter() - t0}
+    for name, jt in (("inner", abi.INNER), ("semi", abi.LEFT_SEMI), ("anti", abi.LEFT_ANTI)):
+        bo = [0, 1, 2] if jt == abi.INNER else []
+        cols = [Column(abi.INT64, probe[0]), Column(abi.INT32, probe[1]), Column(abi.INT32, probe[2])]
+        t1 = time.perf_counter()
+        oj.probe(cols, [0], npr, jt, probe_out=[0, 1, 2], build_out=bo, capacity=npr)
+        res[name + "_rows_per_s"] = npr / (time.perf_counter() - t1)
+    res.update({"cores": 1, "kind": "port", "sample": "build %d rows, %d probe rows in one call, one thread, %.1f s"
+                % (nb, npr, time.perf_counter() - t0)})
     return res
 
 
@@ -499,8 +529,37 @@ def run_filters(dev, args):
         res[name] = {"ms": ms, "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
         log("filters %s: %.3f ms, %.0f GB/s (%.1f%% of %d)" % (name, ms, gbs, 100 * gbs / HBM_PEAK_GBS, HBM_PEAK_GBS))
     res.update({"rows": n, "selected": m, "gather_rows": gm})
+    if not args.skip_cpu:
+        res["cpu_baseline"] = cpu_filter_baseline(f[:20_000_000].cpu().numpy(), a[:20_000_000].cpu().numpy(),
+                                                  b[:20_000_000].cpu().numpy(), a[:10_000_000].cpu().numpy(),
+                                                  idx[:4_000_000].cpu().numpy())
+        log("filters cpu (%s): compare %.3g, compact %.3g, gather %.3g rows/s" % (
+            res["cpu_baseline"]["sample"], res["cpu_baseline"]["compare_rows_per_s"],
+            res["cpu_baseline"]["compact_rows_per_s"], res["cpu_baseline"]["gather_rows_per_s"]))
     del f, a, b, sel, outs, idx, gout
     return res
+
+
+def cpu_filter_baseline(f, a, b, src, idx):
+    """The filter leg on the host: the oracle's restatement of CompareKernels / CompactKernels /
+    GatherKernels (scalar C++, one thread) over samples of the same columns and indices."""
+    import time
+    from oracle import oracle
+    from vecruntime_amd import abi
+    from vecruntime_amd.columns import Column
+    n = len(f)
+    t0 = time.perf_counter()
+    bits, m = oracle.filter_compare(Column(abi.INT32, f), abi.CMP_LT, 300, n)
+    t1 = time.perf_counter()
+    oracle.compact([Column(abi.INT64, a), Column(abi.FLOAT64, b), Column(abi.INT32, f)], n, bits,
+                   [abi.INT64, abi.FLOAT64, abi.INT32], m)
+    t2 = time.perf_counter()
+    oracle.gather(Column(abi.INT64, src), idx)
+    t3 = time.perf_counter()
+    return {"compare_rows_per_s": n / (t1 - t0), "compact_rows_per_s": n / (t2 - t1),
+            "gather_rows_per_s": len(idx) / (t3 - t2), "cores": 1, "kind": "port",
+            "sample": "%d rows compared and compacted (three columns), %d rows gathered from %d, one thread, %.1f s"
+                      % (n, len(idx), len(src), t3 - t0)}
 
 
 def run_shuffle(dev, args):
