@@ -401,20 +401,29 @@ def cpu_shuffle_baseline(keys, parts):
 
 
 def run_join(dev, args):
-    """Join probe: 10M unique int64 build keys (int32 + decimal(7,2) payloads); probe batches of
-    65,536 rows (90% hits), inner materialized / semi / anti, per-batch launches."""
+    """Join probe: 10M unique int64 build keys (int32 + decimal(7,2) payloads); 400M probe rows in
+    65,536-row batches (6,104 batches, 90% hits), two int32 payloads.  Inner materialized (the three
+    streamed and the three build columns), semi and anti (the three streamed columns), each as one
+    batched probe over every batch (vr_join_probe_batches) and as per-batch probes (vr_join_probe,
+    the drop-in), with the C calls' arguments marshalled before the timed region."""
+    import ctypes as C
     import torch
     from vecruntime_amd import abi, vr
+    from vecruntime_amd.columns import c_array
     nb, npr = args.join_build, args.join_probe
     g = torch.Generator(device="cuda")
     g.manual_seed(103)
     bkeys = scramble(torch.arange(nb, dtype=torch.int64, device="cuda"))
     bpay = torch.randint(-2 ** 31, 2 ** 31, (nb,), dtype=torch.int32, device="cuda", generator=g)
     bdec = torch.randint(0, 10 ** 7, (nb,), dtype=torch.int64, device="cuda", generator=g)
-    hit = torch.rand(npr, device="cuda", generator=g) < 0.9
-    pick = torch.randint(0, nb, (npr,), device="cuda", generator=g)
-    miss = scramble(torch.arange(nb, nb + npr, dtype=torch.int64, device="cuda"))
-    pkeys = torch.where(hit, bkeys[pick], miss)
+    pkeys = torch.empty(npr, dtype=torch.int64, device="cuda")
+    for s0 in range(0, npr, 1 << 26):  # in chunks: the temporaries of 400M rows at once are large
+        m = min(1 << 26, npr - s0)
+        hit = torch.rand(m, device="cuda", generator=g) < 0.9
+        pick = torch.randint(0, nb, (m,), device="cuda", generator=g)
+        miss = scramble(torch.arange(nb + s0, nb + s0 + m, dtype=torch.int64, device="cuda"))
+        pkeys[s0:s0 + m] = torch.where(hit, bkeys[pick], miss)
+        del hit, pick, miss
     p1 = torch.randint(-2 ** 31, 2 ** 31, (npr,), dtype=torch.int32, device="cuda", generator=g)
     p2 = torch.randint(-2 ** 31, 2 ** 31, (npr,), dtype=torch.int32, device="cuda", generator=g)
     spec = abi.JoinSpec([abi.INT64], [abi.INT64, abi.INT32, abi.INT64], [0], capacity_hint=nb)
@@ -425,46 +434,125 @@ def run_join(dev, args):
                vr.device_column(abi.INT64, bdec)], nb)
     bt.finish()
     build_s = time.perf_counter() - t0
-    stream = torch.cuda.current_stream()
-    cap = BATCH
+    expect_hits = int(torch.isin(pkeys[:4_000_000], bkeys).sum().item())  # a sample: the hit rate
+    batches = []
+    for s0 in range(0, npr, BATCH):
+        m = min(BATCH, npr - s0)
+        batches.append(([vr.device_column(abi.INT64, pkeys[s0:s0 + m]), vr.device_column(abi.INT32, p1[s0:s0 + m]),
+                         vr.device_column(abi.INT32, p2[s0:s0 + m])], m))
+    pb = vr.ProbeBatches(batches)
+    cap = npr
     outs_all = [torch.empty(cap, dtype=torch.int64, device="cuda"), torch.empty(cap, dtype=torch.int32, device="cuda"),
                 torch.empty(cap, dtype=torch.int32, device="cuda"), torch.empty(cap, dtype=torch.int64, device="cuda"),
                 torch.empty(cap, dtype=torch.int32, device="cuda"), torch.empty(cap, dtype=torch.int64, device="cuda")]
     types = [abi.INT64, abi.INT32, abi.INT32, abi.INT64, abi.INT32, abi.INT64]
-    res = {"build_rows": nb, "probe_rows": npr, "build_s": build_s}
+    stream = torch.cuda.current_stream()
+    res = {"build_rows": nb, "probe_rows": npr, "batches": len(batches), "build_s": build_s}
+    # bytes per probe row read (int64 key + two int32) and per output row written
+    out_bytes = {"inner": 16 + 20, "semi": 16, "anti": 16}
+    lib = vr.lib()
     for name, jt in (("inner", abi.INNER), ("semi", abi.LEFT_SEMI), ("anti", abi.LEFT_ANTI)):
         bo = [0, 1, 2] if jt == abi.INNER else []
         outs = [vr.device_out_column(t, cap, d) for t, d in zip(types[:3 + len(bo)], outs_all)]
-        total = 0
+        bt.probe_batches(pb, [0], jt, outputs=outs, probe_out=[0, 1, 2], build_out=bo, capacity=cap)  # warm
+        reps = 3
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for s0 in range(0, npr, BATCH):
-            m = min(BATCH, npr - s0)
-            cols = [vr.device_column(abi.INT64, pkeys[s0:s0 + m]), vr.device_column(abi.INT32, p1[s0:s0 + m]),
-                    vr.device_column(abi.INT32, p2[s0:s0 + m])]
-            _, _, _, r = bt.probe(cols, [0], m, jt, probe_out=[0, 1, 2], build_out=bo, outputs=outs, capacity=cap,
-                                  mem=abi.MEM_DEVICE, stream=stream, want_pairs=False)
-            total += r
+        e0.record(stream)
+        for _ in range(reps):
+            total, per = bt.probe_batches(pb, [0], jt, outputs=outs, probe_out=[0, 1, 2], build_out=bo, capacity=cap)
+        e1.record(stream)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        res[name] = dict(rows_per_s=npr / dt, ms=dt * 1e3, out_rows=total)
-        log("join %s: %.1f ms, %.3e probe rows/s, %d out rows" % (name, dt * 1e3, npr / dt, total))
+        wall = (time.perf_counter() - t0) / reps
+        dev_ms = e0.elapsed_time(e1) / reps
+        assert int(per.sum()) == total
+        alg = npr * 16 + total * out_bytes[name]
+        achieved = alg / (dev_ms * 1e-3) / 1e9
+        # the same rows through per-batch calls (the drop-in), arguments marshalled beforehand
+        ocs = c_array(outs, abi.VrOutColumn)
+        po, boa = abi.int_array([0, 1, 2]), abi.int_array(bo)
+        kc = abi.int_array([0])
+        calls = []
+        for cols, m in batches:
+            ca = c_array(cols)
+            pout = abi.VrProbeOutput(abi.MEM_DEVICE, cap, None, None, 3, C.cast(po, C.POINTER(C.c_int32)), len(bo),
+                                     C.cast(boa, C.POINTER(C.c_int32)), C.cast(ocs, C.POINTER(abi.VrOutColumn)), 0)
+            calls.append((ca, m, pout))
+        sh = stream.cuda_stream
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ptotal = 0
+        for ca, m, pout in calls:
+            st = lib.vr_join_probe(bt.h, C.cast(ca, C.c_void_p), 3, C.cast(kc, C.c_void_p), m, None, jt, 0,
+                                   C.byref(pout), sh)
+            vr.check(st)
+            ptotal += pout.rows
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t1
+        assert ptotal == total, (ptotal, total)
+        res[name] = {"out_rows": total, "batched": {"ms": wall * 1e3, "device_ms": dev_ms, "rows_per_s": npr / wall},
+                     "plain": {"ms": plain_s * 1e3, "rows_per_s": npr / plain_s, "calls": len(calls)},
+                     "batched_over_plain": plain_s / wall,
+                     "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                                  "kernel": "join_probe_kernel<true> (+ tile map, state clear)",
+                                  "algorithmic_bytes": alg,
+                                  "bytes_per_probe_row": 16, "bytes_per_out_row": out_bytes[name]}}
+        log("join %s: batched %.2f ms (device %.2f ms, %.3e probe rows/s, %.1f GB/s = %.1f%% of HBM), "
+            "plain %.1f ms over %d calls, %d out rows" % (name, wall * 1e3, dev_ms, npr / wall, achieved,
+                                                          100 * achieved / HBM_PEAK_GBS, plain_s * 1e3, len(calls),
+                                                          total))
+    res["hit_rate_sample"] = expect_hits / 4_000_000
+    res["checks"] = {"inner_equals_semi": res["inner"]["out_rows"] == res["semi"]["out_rows"],
+                     "semi_plus_anti": res["semi"]["out_rows"] + res["anti"]["out_rows"] == npr}
     bt.close()
+    del batches, pb, outs_all
     if not args.skip_cpu:
-        sp = min(npr, 4_000_000)
+        threads = cpu_threads()
+        sp = min(npr, threads * 1_000_000)
         res["cpu_baseline"] = cpu_join_baseline(spec, [x.cpu().numpy() for x in (bkeys, bpay, bdec)],
-                                                [x[:sp].cpu().numpy() for x in (pkeys, p1, p2)])
+                                                [x[:sp].cpu().numpy() for x in (pkeys, p1, p2)], threads)
         log("join cpu (%s): inner %.3g, semi %.3g, anti %.3g probe rows/s" % (
             res["cpu_baseline"]["sample"], res["cpu_baseline"]["inner_rows_per_s"],
             res["cpu_baseline"]["semi_rows_per_s"], res["cpu_baseline"]["anti_rows_per_s"]))
+    del pkeys, p1, p2, bkeys, bpay, bdec
+    torch.cuda.empty_cache()
     return res
 
 
-def cpu_join_baseline(spec, build, probe):
-    """The join leg on the host: the oracle's restatement of BuildTable.build and the probe (scalar
-    C++, one thread) over the same build side and a sample of the probe rows (one call per join type:
-    per-batch calls would time the Python wrapper's output allocations)."""
+def cpu_threads():
+    """Host threads for the CPU baselines: the cores this process may run on, at most 16 (the host
+    share of one GPU on the measurement box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cores": aff}
+
+
+def cpu_join_baseline(spec, build, probe, threads):
+    """The join leg on the host: the oracle's restatement of BuildTable.build (one thread) and the
+    probe (scalar C++) over the same build side and a sample of the probe rows, split over `threads`
+    threads that probe the one table at once (a broadcast build side shared by a task's threads)."""
     import time
+    from concurrent.futures import ThreadPoolExecutor
     from oracle.oracle import OracleJoin
     from vecruntime_amd import abi
     from vecruntime_amd.columns import Column
@@ -472,14 +560,26 @@ def cpu_join_baseline(spec, build, probe):
     t0 = time.perf_counter()
     oj = OracleJoin(spec, [Column(abi.INT64, build[0]), Column(abi.INT32, build[1]), Column(abi.INT64, build[2])], nb)
     res = {"build_s": time.perf_counter() - t0}
+    step = (npr + threads - 1) // threads
     for name, jt in (("inner", abi.INNER), ("semi", abi.LEFT_SEMI), ("anti", abi.LEFT_ANTI)):
         bo = [0, 1, 2] if jt == abi.INNER else []
-        cols = [Column(abi.INT64, probe[0]), Column(abi.INT32, probe[1]), Column(abi.INT32, probe[2])]
+        parts = []
+        for s0 in range(0, npr, step):
+            m = min(step, npr - s0)
+            parts.append(([Column(abi.INT64, probe[0][s0:s0 + m]), Column(abi.INT32, probe[1][s0:s0 + m]),
+                           Column(abi.INT32, probe[2][s0:s0 + m])], m))
+
+        def one(part):
+            cols, m = part
+            return oj.probe(cols, [0], m, jt, probe_out=[0, 1, 2], build_out=bo, capacity=m)[3]
+
         t1 = time.perf_counter()
-        oj.probe(cols, [0], npr, jt, probe_out=[0, 1, 2], build_out=bo, capacity=npr)
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, parts))
         res[name + "_rows_per_s"] = npr / (time.perf_counter() - t1)
-    res.update({"cores": 1, "kind": "port", "sample": "build %d rows, %d probe rows in one call, one thread, %.1f s"
-                % (nb, npr, time.perf_counter() - t0)})
+    res.update({"cores": threads, "kind": "port", **host_info(),
+                "sample": "build %d rows (one thread), %d probe rows in %d threads probing the one table, %.1f s"
+                % (nb, npr, threads, time.perf_counter() - t0)})
     return res
 
 
@@ -776,10 +876,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows", type=int, default=200_000_000)
     ap.add_argument("--skip-join", action="store_true")
+    ap.add_argument("--skip-groupby", action="store_true", help="run only the other legs (no headline line)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-batches", type=int, default=3052)
     ap.add_argument("--join-build", type=int, default=10_000_000)
-    ap.add_argument("--join-probe", type=int, default=40_000_000)
+    ap.add_argument("--join-probe", type=int, default=400_000_000)
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--uneven-steps", type=int, default=2)
@@ -813,6 +914,17 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     dev = vr.Device(0)
     n = args.rows
+    if args.skip_groupby:  # the other legs alone, for profiling them; no headline line
+        out = {}
+        if not args.skip_join:
+            out["join"] = run_join(dev, args)
+        if not args.skip_uneven:
+            out["batched_uneven"] = run_uneven(dev, args)
+        if not args.skip_serve:
+            out["serving"] = run_serve(dev, args)
+        dev.close()
+        print(json.dumps(out), flush=True)
+        return
     gb = run_groupby(dev, args, n)
     join = None if args.skip_join else run_join(dev, args)
     uneven = None if args.skip_uneven else run_uneven(dev, args)
@@ -826,9 +938,9 @@ def main():
     achieved = alg_bytes / (gb["update_kernel_ms"] * 1e-3) / 1e9
     cpu = None
     if gb["_sample"] is not None:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         rps, secs, _ = cpu_baseline(gb["_sample"], threads)
-        cpu = {"value": rps, "unit": "rows/s", "cores": threads, "kind": "port",
+        cpu = {"value": rps, "unit": "rows/s", "cores": threads, "kind": "port", **host_info(),
                "sample": "%d batches x 65,536 rows of the same input, %d threads, one partial table per thread, %.1f s"
                          % (gb["_sample"][1], threads, secs)}
     line = {

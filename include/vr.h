@@ -354,6 +354,20 @@ vr_status vr_join_probe(vr_join* join, const vr_column* columns, int32_t num_col
                         const int32_t* key_columns, int64_t n, const uint8_t* selection,
                         int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
                         void* stream);
+/* Probes many streamed batches with one launch: the probe iterator's per-batch loop
+ * (VectorHashJoinIterator.hasNext/probe/emitMatches/emitSemiAnti, VectorHashJoinExec.scala:1017-1182)
+ * over every batch a task holds, without a host round trip per batch.  Batch b's columns are
+ * columns[b * num_columns ...] (device-resident, the same lanes in every batch), its length
+ * lengths[b], its selection selections[b] (selections or an entry may be NULL).  The outputs of all
+ * batches follow each other in `out` (device memory), in batch order, each batch's in the order
+ * vr_join_probe gives it; probe_idx is the row within its batch.  batch_rows (NULL: not wanted)
+ * receives each batch's output rows; out->rows the total (with VR_ERR_CAPACITY when it passes
+ * out->capacity). */
+vr_status vr_join_probe_batches(vr_join* join, int32_t num_batches, const vr_column* columns,
+                                int32_t num_columns, const int64_t* lengths,
+                                const uint8_t* const* selections, const int32_t* key_columns,
+                                int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
+                                int64_t* batch_rows, void* stream);
 /* The build rows no probe has paired since the build, as (-1, row) pairs in build order:
  * probe_idx -1, build_idx the row, the streamed columns null (of the types `columns` gives) and the
  * build columns materialized (emitUnmatchedBuild, VectorHashJoinExec.scala:1457-1484).  Needs

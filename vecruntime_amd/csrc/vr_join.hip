@@ -27,16 +27,31 @@
 
 namespace vr {
 
-constexpr int BUCKET = 8;
+constexpr int BUCKET = 8;  // cells per bucket
 constexpr int PROBE_BLOCK = 256;
-constexpr int PROBE_ITEMS = 4;
+constexpr int PROBE_ITEMS = 8;  // rows per thread per tile
+constexpr int64_t PROBE_TILE = (int64_t)PROBE_BLOCK * PROBE_ITEMS;
+constexpr int MAX_PCOLS = 12;  // streamed column slots of a probe: 2 keys, 8 outputs, 2 residual operands
+// In the value word of a bucket's first cell: an insert walked on past this (full) bucket, so a
+// lookup that misses here must go on to the next bucket.  Row counts stay below 2^31, so the bit
+// is free in every value word.
+constexpr uint64_t PASSED = 1ull << 63;
 
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// A bucket is one 128-byte line of eight (key, value) cells, so a probe finds a key and its value
+// with one line read.  The bucket count is not a power of two: the table is sized to its load
+// limit (the home bucket is the hash's high half scaled to the bucket count), and probing goes on
+// bucket by bucket, wrapping at the end.
 struct JoinTable {
-    uint64_t* buckets;       // nbuckets * 16 words: keys[8], values[8]
-    uint64_t mask;           // nbuckets - 1
+    uint64_t* buckets;  // nb * 16 words: key, value per cell
+    uint64_t nb;        // buckets
     const uint32_t* range_rows;
-    uint64_t sentinel_value; // value of the key equal to EMPTY_KEY (0: absent)
+    uint64_t sentinel_value;  // value of the key equal to EMPTY_KEY (0: absent)
 };
+
+__host__ __device__ __forceinline__ uint64_t home_bucket(uint64_t key, uint64_t nb) { return ((fmix64(key) >> 32) * nb) >> 32; }
+__host__ __device__ __forceinline__ uint64_t next_bucket(uint64_t b, uint64_t nb) { return b + 1 == nb ? 0 : b + 1; }
 
 __device__ __forceinline__ bool key64_of(const DCol* k, int nk, int64_t i, uint64_t* key) {
     if (nk == 1) {
@@ -53,11 +68,10 @@ __device__ __forceinline__ bool key64_of(const DCol* k, int nk, int64_t i, uint6
     return true;
 }
 
-__device__ __forceinline__ uint64_t bucket_of(uint64_t key, uint64_t mask) { return fmix64(key) & mask; }
-
-// Build: claims (or finds) the key's cell; returns the global cell index.
+// Build: claims (or finds) the key's cell; returns the global cell index.  A key that goes on past
+// a full bucket marks that bucket passed.
 __global__ void join_insert_kernel(JoinTable t, DCol k0, DCol k1, int nk, int64_t n, uint32_t* row_cell,
-                                   uint32_t* cell_count, uint64_t* sentinel_rows) {
+                                   uint32_t* cell_count, uint64_t* sentinel_rows, uint32_t* passed) {
     DCol k[2] = {k0, k1};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t key;
@@ -70,14 +84,14 @@ __global__ void join_insert_kernel(JoinTable t, DCol k0, DCol k1, int nk, int64_
             atomic_add_u64(sentinel_rows, 1);
             continue;
         }
-        uint64_t b = bucket_of(key, t.mask);
+        uint64_t b = home_bucket(key, t.nb);
         uint32_t cell = 0xFFFFFFFFu;
-        while (cell == 0xFFFFFFFFu) {
-            uint64_t* keys = t.buckets + b * 16;
+        while (true) {
+            uint64_t* cells = t.buckets + b * 16;
             for (int j = 0; j < BUCKET; j++) {
-                uint64_t c = keys[j];
+                uint64_t c = cells[2 * j];
                 if (c == EMPTY_KEY) {
-                    c = atomic_cas_u64(&keys[j], EMPTY_KEY, key);
+                    c = atomic_cas_u64(&cells[2 * j], EMPTY_KEY, key);
                     if (c == EMPTY_KEY) c = key;
                 }
                 if (c == key) {
@@ -85,40 +99,82 @@ __global__ void join_insert_kernel(JoinTable t, DCol k0, DCol k1, int nk, int64_
                     break;
                 }
             }
-            b = (b + 1) & t.mask;
+            if (cell != 0xFFFFFFFFu) break;
+            atomicOr(&passed[b >> 5], 1u << (b & 31));  // every cell holds another key: go on
+            b = next_bucket(b, t.nb);
         }
         row_cell[i] = cell;
         atomicAdd(&cell_count[cell], 1u);
     }
 }
 
-// Writes each occupied cell's value from its row count and run start.
+// Writes each occupied cell's value from its row count and run start, and the passed flags.
 __global__ void join_values_kernel(JoinTable t, const uint32_t* cell_count, const uint32_t* cell_start, uint64_t ncells,
-                                   const uint32_t* range_rows) {
+                                   const uint32_t* range_rows, const uint32_t* passed) {
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t cnt = cell_count[c];
         if (!cnt) continue;
         uint32_t start = cell_start[c];
-        uint64_t payload = cnt == 1 ? range_rows[start] : start;
-        t.buckets[(c / BUCKET) * 16 + BUCKET + (c % BUCKET)] = ((uint64_t)cnt << 32) | payload;
+        uint64_t v = ((uint64_t)cnt << 32) | (cnt == 1 ? range_rows[start] : start);
+        const uint64_t b = c / BUCKET;
+        if (c % BUCKET == 0 && ((passed[b >> 5] >> (b & 31)) & 1u)) v |= PASSED;  // a passed bucket is full
+        t.buckets[b * 16 + 2 * (c % BUCKET) + 1] = v;
     }
 }
 
-// Probe lookup: the key's value word, 0 when absent.
+// The key's value word (0 when absent), walking from bucket b.
+__device__ __forceinline__ uint64_t lookup_from(const JoinTable& t, uint64_t key, uint64_t b) {
+    const u64x2* cells = (const u64x2*)t.buckets;
+    while (true) {
+        u64x2 x[BUCKET];
+#pragma unroll
+        for (int j = 0; j < BUCKET; j++) x[j] = cells[b * BUCKET + j];
+#pragma unroll
+        for (int j = 0; j < BUCKET; j++)
+            if (x[j].x == key) return x[j].y & ~PASSED;
+        if (!(x[0].y & PASSED)) return 0;  // no key of this home or before went on past here
+        b = next_bucket(b, t.nb);
+    }
+}
+
 __device__ __forceinline__ uint64_t lookup(const JoinTable& t, uint64_t key) {
     if (key == EMPTY_KEY) return t.sentinel_value;
-    uint64_t b = bucket_of(key, t.mask);
-    while (true) {
-        const uint64_t* keys = t.buckets + b * 16;
-        bool open = false;
-        for (int j = 0; j < BUCKET; j++) {
-            uint64_t c = keys[j];
-            if (c == key) return keys[BUCKET + j];
-            if (c == EMPTY_KEY) open = true;
-        }
-        if (open) return 0;  // buckets fill in order: an empty cell ends the chain
-        b = (b + 1) & t.mask;
+    return lookup_from(t, key, home_bucket(key, t.nb));
+}
+
+// Wave-cooperative lookup of the 64 rows the wave's lanes hold.  Eight lanes read one bucket, a
+// 16-byte cell each, so a load instruction covers eight rows (a line each) and eight of them cover
+// the wave; a ballot finds the matching cell and the bucket's passed flag, and a shuffle hands the
+// matching cell's value to the row's lane.  A row whose home bucket lacks the key but was passed
+// goes on bucket by bucket on its own lane (a few rows in a wave: looping the cooperative steps
+// for them instead holds every round of the tile back and measured slower).  Lanes with `active`
+// false read bucket 0 and get 0.  The whole wave must reach the call.
+__device__ __forceinline__ uint64_t coop_lookup(const JoinTable& t, bool active, uint64_t key) {
+    const int lane = threadIdx.x & 63, g = lane >> 3, j = lane & 7;
+    const uint32_t home = active ? (uint32_t)home_bucket(key, t.nb) : 0u;
+    const u64x2* cells = (const u64x2*)t.buckets;
+    u64x2 x[BUCKET];
+#pragma unroll
+    for (int s = 0; s < BUCKET; s++) {
+        const uint32_t bb = __shfl(home, g * 8 + s);
+        x[s] = cells[(uint64_t)bb * BUCKET + j];
     }
+    uint64_t res = 0;
+    bool cont = false;
+#pragma unroll
+    for (int s = 0; s < BUCKET; s++) {
+        const uint64_t kk = __shfl(key, g * 8 + s);
+        const uint64_t mb = __ballot(x[s].x == kk);
+        const uint64_t pb = __ballot(j == 0 && (x[s].y & PASSED));
+        const uint32_t gm = (uint32_t)(mb >> (g * 8)) & 0xFFu;
+        const uint64_t val = __shfl(x[s].y, g * 8 + (gm ? __ffs(gm) - 1 : 0));
+        if (j == s) {
+            res = gm ? (val & ~PASSED) : 0;
+            cont = !gm && ((pb >> (g * 8)) & 1ull);
+        }
+    }
+    if (active && cont) res = lookup_from(t, key, next_bucket(home, t.nb));
+    return active ? res : 0;
 }
 
 struct OutCol {
@@ -137,7 +193,7 @@ struct DenseIndex {
 };
 
 // The key's signed value (INT32 keys are read zero-extended into the 64-bit key).
-__device__ __forceinline__ int64_t key_value(int32_t type, uint64_t key) {
+__host__ __device__ __forceinline__ int64_t key_value(int32_t type, uint64_t key) {
     return type == VR_INT32 ? (int64_t)(int32_t)(uint32_t)key : (int64_t)key;
 }
 
@@ -147,34 +203,78 @@ __device__ __forceinline__ uint64_t dense_lookup(const DenseIndex& d, int32_t ty
     return off < d.range ? d.slots[off] : 0;
 }
 
+// A streamed column of one batch.
+struct PCol {
+    const uint8_t* valid;  // nullptr: no nulls
+    const void* data;
+};
+
+// One streamed batch of a probe launch.
+struct PBatch {
+    int64_t n;      // rows
+    int64_t tile0;  // the launch's first tile of this batch
+    const uint8_t* sel;
+    int64_t reserved;
+};
+
+// A probe launch over one batch (the batch in the arguments) or many (the batch table in device
+// memory).  A batch's streamed columns fill slots: the keys, then the columns to materialize, then
+// the streamed operands of a residual.
 struct ProbeArgs {
     JoinTable t;
     DenseIndex dense;
-    DCol keys[2];
     int32_t nk;
     int32_t join_type;
     int32_t drop_null_keys;  // null-aware anti: a null key counts as matched
     int32_t build_rows;      // 0: an empty build side
-    const uint8_t* sel;
-    int64_t n;
+    int32_t ncb;             // slots per batch
+    int32_t nprobe_out, nbuild_out;
+    int32_t ctype[MAX_PCOLS];      // lane type of each slot
+    int32_t build_from_key[8];     // build output c of a matched pair = key slot build_from_key[c] of the
+                                   // streamed row (equal keys are the same bits); -1: gathered by build row
+    DCol build_src[8];
+    OutCol outs[16];
     int64_t capacity;
     int32_t* probe_idx;
     int32_t* build_idx;
-    int32_t nprobe_out, nbuild_out;
-    DCol probe_src[8];
-    DCol build_src[8];
-    OutCol outs[16];
-    uint64_t* tile_state;   // per tile: (status << 62) | value, zeroed before the launch
-    uint64_t* tile_counter; // [0] next tile id, [1] total rows out
+    uint64_t* tile_state;    // per tile: (status << 62) | value, zeroed before the launch
+    uint64_t* tile_counter;  // [0] next tile id, [1] total rows out
     int64_t ntiles;
-    uint32_t* matched;      // build rows paired so far (one bit each), nullptr: not tracked
+    uint32_t* matched;       // build rows paired so far (one bit each), nullptr: not tracked
     // residual condition on the pairs (res_on 0: none): left + loff <op> right + roff, each operand
-    // a streamed column (indexed by the streamed row) or a build column (by the build row)
+    // a streamed column (a slot, indexed by the streamed row) or a build column (by the build row)
     int32_t res_on, res_op;
     int32_t res_lb, res_rb;
-    DCol res_l, res_r;
+    int32_t res_lslot, res_rslot;
+    DCol res_lbuild, res_rbuild;
     int64_t res_loff, res_roff;
+    const PBatch* batches;      // many batches: the batch table
+    const PCol* bcols;          // many batches: batch b's slots at [b * ncb, (b + 1) * ncb)
+    const int32_t* tile_batch;  // many batches: the batch of each tile
+    int64_t* batch_end;         // many batches: output rows through each batch's last tile
+    PBatch one;                 // one batch
+    PCol one_cols[MAX_PCOLS];
 };
+
+template <bool MULTI>
+__device__ __forceinline__ PCol slot_col(const ProbeArgs& a, const PCol* bc, int c) {
+    return MULTI ? bc[c] : a.one_cols[c];
+}
+
+// The 64-bit probe key of row i from the key slots; false when a key is null.
+__device__ __forceinline__ bool probe_key(const PCol& c0, const PCol& c1, int32_t t0, int nk, int64_t i, uint64_t* key) {
+    if (nk == 1) {
+        if (c0.valid && !bit_at(c0.valid, i)) return false;
+        if (t0 == VR_INT64) *key = (uint64_t)ld_stream((const int64_t*)c0.data + i);
+        else *key = (uint64_t)(uint32_t)ld_stream((const int32_t*)c0.data + i);
+        return true;
+    }
+    if ((c0.valid && !bit_at(c0.valid, i)) || (c1.valid && !bit_at(c1.valid, i))) return false;
+    const uint32_t a = (uint32_t)ld_stream((const int32_t*)c0.data + i);
+    const uint32_t b = (uint32_t)ld_stream((const int32_t*)c1.data + i);
+    *key = ((uint64_t)a << 32) | b;
+    return true;
+}
 
 // One operand of the residual plus its offset in the lane's width (wrapping); false when null.
 __device__ __forceinline__ bool res_operand(const DCol& c, int64_t row, int64_t off, int64_t* v) {
@@ -185,10 +285,9 @@ __device__ __forceinline__ bool res_operand(const DCol& c, int64_t row, int64_t 
 }
 
 // PairPredicate.test (VectorHashJoinExec.scala:1287-1303) for streamed row i and build row b.
-__device__ __forceinline__ bool res_passes(const ProbeArgs& a, int64_t i, int64_t b) {
+__device__ __forceinline__ bool res_passes(const ProbeArgs& a, const DCol& l, const DCol& r, int64_t i, int64_t b) {
     int64_t x, y;
-    if (!res_operand(a.res_l, a.res_lb ? b : i, a.res_loff, &x) || !res_operand(a.res_r, a.res_rb ? b : i, a.res_roff, &y))
-        return false;
+    if (!res_operand(l, a.res_lb ? b : i, a.res_loff, &x) || !res_operand(r, a.res_rb ? b : i, a.res_roff, &y)) return false;
     const int c = (x > y) - (x < y);
     switch (a.res_op) {
         case VR_CMP_EQ: return c == 0;
@@ -236,41 +335,129 @@ __device__ __forceinline__ void copy_value(const DCol& src, int64_t i, const Out
 }
 
 constexpr uint64_t ST_AGG = 1ull << 62, ST_PREFIX = 2ull << 62, ST_MASK = 3ull << 62;
+constexpr int PROBE_WAVES = PROBE_BLOCK / 64;
 
+// Exclusive prefix sum of v over the wave's lanes; *total = the wave's sum.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+// Decoupled look-back of tile `tile` (> 0) by one whole wave: each step reads the states of the 64
+// tiles before the window's end at once, waits until every one of them up to the nearest inclusive
+// prefix has published, and adds their values.  Returns the exclusive prefix of the tile.  Tile 0
+// publishes an inclusive prefix without waiting, and every tile has its id before it waits, so
+// every wait ends.
+__device__ __forceinline__ uint64_t wave_look_back(uint64_t* tile_state, int64_t tile) {
+    const int lane = threadIdx.x & 63;
+    uint64_t excl = 0;
+    int64_t end = tile - 1;  // the window is tiles end, end - 1, ..., end - 63
+    while (true) {
+        const int64_t idx = end - lane;
+        const uint64_t st = idx >= 0 ? ld_coherent(&tile_state[idx]) : ST_PREFIX;
+        const uint64_t status = st & ST_MASK;
+        const uint64_t pmask = __ballot(status == ST_PREFIX);
+        const uint64_t umask = __ballot(status == 0);
+        const int limit = pmask ? __ffsll((unsigned long long)pmask) - 1 : 63;  // lanes 0..limit count
+        const uint64_t need = limit == 63 ? ~0ull : ((2ull << limit) - 1);
+        if (umask & need) continue;  // a tile in the window has not published yet
+        uint64_t v = lane <= limit ? (st & ~ST_MASK) : 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+        excl += v;
+        if (pmask) return excl;
+        end -= 64;
+    }
+}
+
+// The probe: tiles of PROBE_TILE streamed rows, taken in order from an atomic counter across every
+// batch of the launch.  A tile's waves look their rows up (wave-cooperatively, the keys of all
+// their rows loaded first), count each row's output rows and scan the counts within the wave, with
+// no barrier between rounds; then the tile's offsets come from the waves' totals and its output
+// offset from a wave-wide decoupled look-back over the preceding tiles, and the rows write pairs
+// or materialized columns.  Outputs of all batches follow each other in batch order, each batch's
+// in the reference's order: streamed rows in order, each row's build rows in build order.
+template <bool MULTI>
 __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
-    typedef hipcub::BlockScan<uint32_t, PROBE_BLOCK> Scan;
-    __shared__ typename Scan::TempStorage scan_tmp;
     __shared__ int64_t tile_id_sh;
+    __shared__ int32_t batch_sh;
     __shared__ uint64_t base_sh;
-    __shared__ uint32_t sub_total[PROBE_ITEMS];
-    if (threadIdx.x == 0) tile_id_sh = (int64_t)atomic_add_u64(&a.tile_counter[0], 1);
+    __shared__ uint32_t wave_tot[PROBE_ITEMS][PROBE_WAVES];  // totals, then their exclusive prefix in the tile
+    __shared__ DCol psrc_sh[8], ksrc_sh[2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        const int64_t t = (int64_t)atomic_add_u64(&a.tile_counter[0], 1);
+        tile_id_sh = t;
+        batch_sh = MULTI ? a.tile_batch[t] : 0;
+    }
     __syncthreads();
     const int64_t tile = tile_id_sh;
-    const int64_t row0 = tile * PROBE_BLOCK * PROBE_ITEMS;
+    const int32_t bi = __builtin_amdgcn_readfirstlane(batch_sh);
+    const PBatch B = MULTI ? a.batches[bi] : a.one;
+    const PCol* bc = MULTI ? a.bcols + (int64_t)bi * a.ncb : nullptr;
+    const int64_t row0 = (tile - B.tile0) * PROBE_TILE;
+    const PCol k0 = slot_col<MULTI>(a, bc, 0);
+    const PCol k1 = a.nk > 1 ? slot_col<MULTI>(a, bc, 1) : PCol{nullptr, nullptr};
+    DCol resl = a.res_lbuild, resr = a.res_rbuild;
+    if (a.res_on) {
+        if (!a.res_lb) {
+            const PCol p = slot_col<MULTI>(a, bc, a.res_lslot);
+            resl = DCol{p.valid, p.data, nullptr, a.ctype[a.res_lslot], 0};
+        }
+        if (!a.res_rb) {
+            const PCol p = slot_col<MULTI>(a, bc, a.res_rslot);
+            resr = DCol{p.valid, p.data, nullptr, a.ctype[a.res_rslot], 0};
+        }
+    }
+    // the keys of all the thread's rows first: their loads overlap
+    uint64_t key[PROBE_ITEMS];
+    uint32_t flags = 0;  // bit r: row r is in (selected), bit 16 + r: its key is non-null
+#pragma unroll
+    for (int r = 0; r < PROBE_ITEMS; r++) {
+        const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+        const bool in = i < B.n && (!B.sel || bit_at(B.sel, i));
+        key[r] = 0;
+        const bool nonnull = in && probe_key(k0, k1, a.ctype[0], a.nk, i, &key[r]);
+        flags |= (in ? 1u : 0u) << r;
+        flags |= (nonnull ? 1u : 0u) << (16 + r);
+    }
     uint32_t cnt[PROBE_ITEMS], off[PROBE_ITEMS];
     uint32_t payload[PROBE_ITEMS], found[PROBE_ITEMS], npass[PROBE_ITEMS];
     const bool pairs = a.join_type == VR_JOIN_INNER || a.join_type == VR_JOIN_OUTER;
+#pragma unroll
     for (int r = 0; r < PROBE_ITEMS; r++) {
-        int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
-        uint32_t c = 0;
-        uint32_t pl = 0, fd = 0, np = 0;
-        if (i < a.n && (!a.sel || bit_at(a.sel, i))) {
-            uint64_t key;
-            bool nonnull = key64_of(a.keys, a.nk, i, &key);
-            uint64_t v = 0;
-            if (nonnull && a.build_rows) v = a.dense.slots ? dense_lookup(a.dense, a.keys[0].type, key) : lookup(a.t, key);
-            uint32_t matches = (uint32_t)(v >> 32);
+        const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+        const bool in = (flags >> r) & 1u, nonnull = (flags >> (16 + r)) & 1u;
+        uint64_t v = 0;
+        if (a.build_rows) {  // uniform: every lane of the wave takes the same branch
+            if (a.dense.slots) {
+                if (nonnull) v = dense_lookup(a.dense, a.ctype[0], key[r]);
+            } else {
+                v = coop_lookup(a.t, nonnull && key[r] != EMPTY_KEY, key[r]);
+                if (nonnull && key[r] == EMPTY_KEY) v = a.t.sentinel_value;
+            }
+        }
+        uint32_t c = 0, pl = 0, fd = 0, np = 0;
+        if (in) {
+            const uint32_t matches = (uint32_t)(v >> 32);
             pl = (uint32_t)v;
             fd = matches;
             np = matches;
             if (a.res_on && matches) {  // the candidates that pass the residual
                 np = 0;
-                for (uint32_t k = 0; k < matches; k++) np += res_passes(a, i, candidate(a, matches, pl, k)) ? 1u : 0u;
+                for (uint32_t k = 0; k < matches; k++) np += res_passes(a, resl, resr, i, candidate(a, matches, pl, k)) ? 1u : 0u;
             }
             if (pairs) {
                 c = np ? np : (a.join_type == VR_JOIN_OUTER ? 1u : 0u);  // (row, -1) when preserved
             } else {
-                bool matched = np > 0 || (a.drop_null_keys && a.build_rows && !nonnull);
+                const bool matched = np > 0 || (a.drop_null_keys && a.build_rows && !nonnull);
                 c = (matched == (a.join_type == VR_JOIN_LEFT_SEMI)) ? 1u : 0u;
             }
         }
@@ -279,36 +466,56 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
         found[r] = fd;
         npass[r] = np;
         uint32_t total;
-        Scan(scan_tmp).ExclusiveSum(c, off[r], total);
-        if (threadIdx.x == 0) sub_total[r] = total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        uint64_t local = 0;
-        for (int r = 0; r < PROBE_ITEMS; r++) local += sub_total[r];
-        uint64_t excl = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&a.tile_state[0], ST_PREFIX | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&a.tile_state[tile], ST_AGG | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = tile - 1;
-            while (true) {
-                uint64_t st = ld_coherent(&a.tile_state[j]);
-                if ((st & ST_MASK) == 0) continue;  // tile j has its id and publishes without waiting on us
-                excl += st & ~ST_MASK;
-                if ((st & ST_MASK) == ST_PREFIX) break;
-                j--;
-            }
-            __hip_atomic_store(&a.tile_state[tile], ST_PREFIX | (excl + local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (tile == a.ntiles - 1) __hip_atomic_store(&a.tile_counter[1], excl + local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        base_sh = excl;
+        off[r] = wave_excl_scan(c, &total);
+        if (lane == 0) wave_tot[r][wave] = total;
     }
     __syncthreads();
-    uint64_t base = base_sh;
+    if (wave == 0) {
+        // the waves' offsets within the tile, in row order: round-major, then wave
+        uint64_t local = 0;
+        if (lane == 0) {
+            uint32_t run = 0;
+            for (int r = 0; r < PROBE_ITEMS; r++)
+                for (int w = 0; w < PROBE_WAVES; w++) {
+                    const uint32_t t = wave_tot[r][w];
+                    wave_tot[r][w] = run;
+                    run += t;
+                }
+            local = run;
+        }
+        local = __shfl(local, 0);
+        uint64_t excl = 0;
+        if (tile == 0) {
+            if (lane == 0) __hip_atomic_store(&a.tile_state[0], ST_PREFIX | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(&a.tile_state[tile], ST_AGG | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            excl = wave_look_back(a.tile_state, tile);
+            if (lane == 0)
+                __hip_atomic_store(&a.tile_state[tile], ST_PREFIX | (excl + local), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            if (tile == a.ntiles - 1) __hip_atomic_store(&a.tile_counter[1], excl + local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (MULTI && a.batch_end && tile == B.tile0 + (B.n + PROBE_TILE - 1) / PROBE_TILE - 1)
+                a.batch_end[bi] = (int64_t)(excl + local);
+            base_sh = excl;
+        }
+    }
+    // the streamed columns the rows write from, staged in LDS: read from the batch table inside
+    // the loop, a descriptor would be read again after every store to an output (which could alias
+    // the table), a memory round trip per column and row
+    if (threadIdx.x < (unsigned)a.nprobe_out) {
+        const PCol p = slot_col<MULTI>(a, bc, a.nk + threadIdx.x);
+        psrc_sh[threadIdx.x] = DCol{p.valid, p.data, nullptr, a.ctype[a.nk + threadIdx.x], 0};
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + (unsigned)a.nk) {
+        const int q = threadIdx.x - 64;
+        const PCol p = slot_col<MULTI>(a, bc, q);
+        ksrc_sh[q] = DCol{p.valid, p.data, nullptr, a.ctype[q], 0};
+    }
+    __syncthreads();
+    const uint64_t base = base_sh;
     for (int r = 0; r < PROBE_ITEMS; r++) {
-        int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
-        uint64_t o = base + off[r];
+        const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+        uint64_t o = base + wave_tot[r][wave] + off[r];
         uint32_t k = 0;  // the next candidate of the row's key
         for (uint32_t m = 0; m < cnt[r]; m++, o++) {
             if ((int64_t)o >= a.capacity) break;
@@ -316,20 +523,31 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
             if (pairs && npass[r]) {  // the next passing candidate; a row with none is padded
                 while (k < found[r]) {
                     brow = candidate(a, found[r], payload[r], k++);
-                    if (!a.res_on || res_passes(a, i, brow)) break;
+                    if (!a.res_on || res_passes(a, resl, resr, i, brow)) break;
                 }
             }
             if (a.matched && brow >= 0) {  // buildMatched(r) = true; bits only ever get set
                 uint32_t* w = a.matched + (brow >> 5);
-                const uint32_t b = 1u << (brow & 31);
-                if (!(*w & b)) atomicOr(w, b);
+                const uint32_t bit = 1u << (brow & 31);
+                if (!(*w & bit)) atomicOr(w, bit);
             }
             if (a.probe_idx) a.probe_idx[o] = (int32_t)i;
             if (a.build_idx) a.build_idx[o] = (int32_t)brow;
-            for (int c = 0; c < a.nprobe_out; c++) copy_value(a.probe_src[c], i, a.outs[c], (int64_t)o);
-            for (int c = 0; c < a.nbuild_out; c++) copy_value(a.build_src[c], brow, a.outs[a.nprobe_out + c], (int64_t)o);
+            for (int c = 0; c < a.nprobe_out; c++) copy_value(psrc_sh[c], i, a.outs[c], (int64_t)o);
+            for (int c = 0; c < a.nbuild_out; c++) {
+                const int ks = a.build_from_key[c];
+                if (ks >= 0 && brow >= 0) copy_value(ksrc_sh[ks], i, a.outs[a.nprobe_out + c], (int64_t)o);
+                else copy_value(a.build_src[c], brow, a.outs[a.nprobe_out + c], (int64_t)o);
+            }
         }
-        base += sub_total[r];
+    }
+}
+
+// The batch of every tile of a many-batch launch.
+__global__ void tile_batch_kernel(const PBatch* batches, int32_t nbatch, int32_t* tile_batch) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nbatch; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t0 = batches[b].tile0, nt = (batches[b].n + PROBE_TILE - 1) / PROBE_TILE;
+        for (int64_t k = 0; k < nt; k++) tile_batch[t0 + k] = (int32_t)b;
     }
 }
 
@@ -338,7 +556,7 @@ __global__ void zero_u64_kernel(uint64_t* p, uint64_t n) {
 }
 __global__ void fill_keys_kernel(uint64_t* buckets, uint64_t nbuckets) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbuckets * 16; i += (uint64_t)gridDim.x * blockDim.x)
-        buckets[i] = (i % 16) < BUCKET ? EMPTY_KEY : 0ull;
+        buckets[i] = (i % 2) == 0 ? EMPTY_KEY : 0ull;
 }
 // Build statistics: stats[1] += cells holding rows (distinct keys), stats[2] = max rows of a cell.
 __global__ __launch_bounds__(256) void cell_stats_kernel(const uint32_t* cell_count, uint64_t ncells, uint64_t* stats) {
@@ -363,7 +581,7 @@ __global__ __launch_bounds__(256) void cell_stats_kernel(const uint32_t* cell_co
 __global__ void key_range_kernel(const uint64_t* buckets, uint64_t nbuckets, int32_t type, uint64_t* stats) {
     int64_t lo = INT64_MAX, hi = INT64_MIN;
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nbuckets * BUCKET; c += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t key = buckets[(c / BUCKET) * 16 + (c % BUCKET)];
+        const uint64_t key = buckets[2 * c];
         if (key == EMPTY_KEY) continue;
         const int64_t v = key_value(type, key);
         lo = v < lo ? v : lo;
@@ -383,10 +601,9 @@ __global__ void range_init_kernel(uint64_t* stats) {
 // slots[key - min] = the cell's value word, for every occupied cell.
 __global__ void dense_fill_kernel(const uint64_t* buckets, uint64_t nbuckets, int32_t type, int64_t min, uint64_t* slots) {
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nbuckets * BUCKET; c += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t* b = buckets + (c / BUCKET) * 16;
-        const uint64_t key = b[c % BUCKET];
+        const uint64_t key = buckets[2 * c];
         if (key == EMPTY_KEY) continue;
-        slots[(uint64_t)key_value(type, key) - (uint64_t)min] = b[BUCKET + (c % BUCKET)];
+        slots[(uint64_t)key_value(type, key) - (uint64_t)min] = buckets[2 * c + 1] & ~PASSED;
     }
 }
 __global__ void store_u64_kernel(uint64_t* p, uint64_t v) {
@@ -461,8 +678,11 @@ struct JoinImpl {
     DevBuf dense;
     int64_t dense_min = 0;
     uint64_t dense_range = 0;
-    // probe scratch
-    DevBuf tile_state, probe_stage, out_stage;
+    // Pinned staging of host-resident streamed columns, shared by the probes of the handle: held
+    // under `stage_mu` from the copy until the kernel reading it has finished.  Every other piece of
+    // a probe's scratch (tile states, output staging, batch tables) belongs to the call, so probes
+    // of a finished table may run from several threads at once.
+    std::mutex stage_mu;
     Staging staging;
 };
 
@@ -626,9 +846,12 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     if (j.finished) return fail(VR_ERR_STATE, "the build side is already finished");
     hipStream_t s = j.dev->pick(stream);
     const int64_t n = j.rows;
-    uint64_t cells_wanted = (uint64_t)((double)std::max<int64_t>(n, 1) / 0.75) + 1;  // at most 3/4 of the cells
-    uint64_t nb = 1;
-    while (nb * BUCKET < cells_wanted) nb <<= 1;
+    // at most half of the cells hold keys.  A probe reads one bucket line per row unless its home
+    // bucket was passed; a few such rows hold their whole wave back, so fewer of them pays more than
+    // a table that stays in the 256 MB MALL: 10M build keys at 70% load (218 MB) probed 400M rows
+    // in 15.4 ms (semi join), at 50% (305 MB) in 13.5 ms, at 40% in 13.3 ms
+    const uint64_t cells_wanted = (uint64_t)((double)std::max<int64_t>(n, 1) / 0.50) + 1;
+    const uint64_t nb = (cells_wanted + BUCKET - 1) / BUCKET;
     j.nbuckets = nb;
     const uint64_t ncells = nb * BUCKET;
     int cap = j.dev->num_cus * 16;
@@ -638,10 +861,14 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     VR_TRY(j.cell_count.ensure(j.dev, ncells * 4 + 8));
     VR_TRY(j.cell_start.ensure(j.dev, ncells * 4 + 8));
     VR_TRY(j.small.ensure(j.dev, 64));
+    const uint64_t passed_words = (nb + 63) / 64;  // one bit per bucket, in 64-bit words
+    DevBuf passed;
+    VR_TRY(passed.ensure(j.dev, passed_words * 8));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(ncells / 2 + 1, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint64_t>(),
                        ncells / 2 + 1);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(passed_words, 256, cap)), dim3(256), 0, s, passed.as<uint64_t>(), passed_words);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>(), 8ull);
-    JoinTable t{j.buckets.as<uint64_t>(), nb - 1, nullptr, 0};
+    JoinTable t{j.buckets.as<uint64_t>(), nb, nullptr, 0};
     DCol k[2]{};
     for (size_t q = 0; q < j.key_types.size(); q++) {
         auto& bc = j.cols[j.key_cols[q]];
@@ -652,7 +879,7 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     if (n > 0)
         hipLaunchKernelGGL(join_insert_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, t, k[0], k[1],
                            (int)j.key_types.size(), n, j.row_cell.as<uint32_t>(), j.cell_count.as<uint32_t>(),
-                           j.small.as<uint64_t>());
+                           j.small.as<uint64_t>(), passed.as<uint32_t>());
     VR_TRY_HIP(hipGetLastError());
     // run starts, then the build rows ordered by cell (stable: build order inside a key)
     size_t tmp = 0;
@@ -674,7 +901,7 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
                                                       j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
     }
     hipLaunchKernelGGL(join_values_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, t, j.cell_count.as<uint32_t>(),
-                       j.cell_start.as<uint32_t>(), ncells, j.range_rows.as<uint32_t>());
+                       j.cell_start.as<uint32_t>(), ncells, j.range_rows.as<uint32_t>(), passed.as<uint32_t>());
     VR_TRY_HIP(hipGetLastError());
     // statistics (distinct keys, longest chain) reduced on the device into small[1], small[2]
     hipLaunchKernelGGL(cell_stats_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint32_t>(),
@@ -735,10 +962,11 @@ vr_status vr_join_info(vr_join* h, int64_t* build_rows, int64_t* distinct_keys, 
     return VR_OK;
 }
 
-// Where a call's outputs go: the caller's device buffers, or a device staging area copied back to
-// the caller's host buffers after the kernel.
+// Where a call's outputs go: the caller's device buffers, or a device staging area of the call's
+// own, copied back to the caller's host buffers after the kernel.
 struct OutPlan {
     bool host = false;
+    DevBuf stage;
     uint8_t* ob = nullptr;
     size_t pidx_off = 0, bidx_off = 0;
     std::vector<size_t> voff, doff;
@@ -773,8 +1001,8 @@ static vr_status plan_outputs(JoinImpl& j, const vr_probe_output* out, const std
         }
     }
     if (need) {
-        VR_TRY(j.out_stage.ensure(j.dev, need));
-        P.ob = j.out_stage.as<uint8_t>();
+        VR_TRY(P.stage.ensure(j.dev, need));
+        P.ob = P.stage.as<uint8_t>();
     }
     *pidx = P.host ? (out->probe_idx ? (int32_t*)(P.ob + P.pidx_off) : nullptr) : out->probe_idx;
     *bidx = want_build_idx ? (P.host ? (out->build_idx ? (int32_t*)(P.ob + P.bidx_off) : nullptr) : out->build_idx) : nullptr;
@@ -811,92 +1039,88 @@ static vr_status copy_back(const vr_probe_output* out, const OutPlan& P, int64_t
     return VR_OK;
 }
 
-vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_columns, const int32_t* key_columns, int64_t n,
-                        const uint8_t* selection, int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
-                        void* stream) {
-    if (!h || !out || (!columns && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
-    JoinImpl& j = h->j;
+// What a probe call reads and writes, checked against the table and the output description: the
+// streamed column (index into a batch's `columns`) of each slot, and the output lanes.
+struct ProbeShape {
+    std::vector<int32_t> slot_col;
+    std::vector<int32_t> out_types;
+    bool pairs = false;
+};
+
+// The checks of a probe call that need no streamed columns.
+static vr_status check_probe_call(const JoinImpl& j, int32_t join_type, const vr_probe_output* out) {
     if (!j.finished) return fail(VR_ERR_STATE, "probe before vr_join_build_finish");
     if (join_type < VR_JOIN_INNER || join_type > VR_JOIN_OUTER) return fail(VR_ERR_INVALID_ARGUMENT, "join type");
     const bool pairs = join_type == VR_JOIN_INNER || join_type == VR_JOIN_OUTER;
     if (!pairs && out->num_build_out) return fail(VR_ERR_INVALID_ARGUMENT, "semi/anti joins output streamed columns only");
-    if (out->num_probe_out + out->num_build_out > 16 || out->num_probe_out > 8 || out->num_build_out > 8)
+    if (out->num_probe_out < 0 || out->num_build_out < 0 || out->num_probe_out + out->num_build_out > 16 ||
+        out->num_probe_out > 8 || out->num_build_out > 8)
         return fail(VR_ERR_UNSUPPORTED, "at most 8 + 8 output columns");
     if (join_type == VR_JOIN_OUTER)
         for (int q = 0; q < out->num_build_out; q++)
             if (!out->columns[out->num_probe_out + q].validity)
                 return fail(VR_ERR_INVALID_ARGUMENT, "an outer join's build columns need validity for the padded rows");
-    out->rows = 0;
-    if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
-    if (n == 0) return VR_OK;
-    if (n >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "probe batches past 2^31 rows");
-    hipStream_t s = j.dev->pick(stream);
-    ProbeArgs a{};
-    a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets - 1, j.range_rows.as<uint32_t>(), j.sentinel_value};
+    return VR_OK;
+}
+
+// Fills everything of the launch arguments but the batches (after check_probe_call).
+static vr_status prepare_probe(JoinImpl& j, const vr_column* columns, int32_t num_columns, const int32_t* key_columns,
+                               int32_t join_type, int32_t drop_null_keys, const vr_probe_output* out, ProbeArgs& a,
+                               ProbeShape& sh) {
+    const bool pairs = join_type == VR_JOIN_INNER || join_type == VR_JOIN_OUTER;
+    sh.pairs = pairs;
+    a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets, j.range_rows.as<uint32_t>(), j.sentinel_value};
     a.dense = DenseIndex{j.dense_range ? j.dense.as<uint64_t>() : nullptr, j.dense_min, j.dense_range};
     a.matched = (pairs && j.track) ? j.matched.as<uint32_t>() : nullptr;
     a.nk = (int)j.key_types.size();
     a.join_type = join_type;
     a.drop_null_keys = drop_null_keys;
     a.build_rows = j.rows > 0 ? 1 : 0;
-    a.n = n;
     a.capacity = out->capacity;
-    // streamed columns: device pointers, or staged from host memory
-    std::vector<std::pair<const void*, size_t>> pieces;
-    std::vector<const void**> patch;
-    auto want = [&](const void* p, size_t bytes, const void** dst, int32_t mem) {
-        *dst = p;
-        if (p && bytes && mem == VR_MEM_HOST) {
-            pieces.push_back({p, bytes});
-            patch.push_back(dst);
-        }
+    auto add_slot = [&](int32_t col, int32_t type) {
+        a.ctype[sh.slot_col.size()] = type;
+        sh.slot_col.push_back(col);
+        return (int32_t)sh.slot_col.size() - 1;
     };
-    int32_t mem = VR_MEM_DEVICE;
     for (int q = 0; q < a.nk; q++) {
-        int kc = key_columns[q];
+        const int kc = key_columns[q];
         if (kc < 0 || kc >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "probe key column out of range");
-        const vr_column& c = columns[kc];
-        if (c.type != j.key_types[q]) return fail(VR_ERR_INVALID_ARGUMENT, "probe key lane differs from the build key");
-        a.keys[q].type = c.type;
-        want(c.validity, bitmap_bytes(n), (const void**)&a.keys[q].valid, c.mem);
-        want(c.data, lane_bytes(c.type, n), &a.keys[q].data, c.mem);
-        mem = c.mem;
+        if (columns[kc].type != j.key_types[q]) return fail(VR_ERR_INVALID_ARGUMENT, "probe key lane differs from the build key");
+        add_slot(kc, columns[kc].type);
     }
     a.nprobe_out = out->num_probe_out;
     a.nbuild_out = out->num_build_out;
-    std::vector<int32_t> types;
     for (int q = 0; q < a.nprobe_out; q++) {
-        int pc = out->probe_out_columns[q];
+        const int pc = out->probe_out_columns[q];
         if (pc < 0 || pc >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "probe output column out of range");
-        const vr_column& c = columns[pc];
-        if (c.type == VR_UTF8 || c.type == VR_BOOL) return fail(VR_ERR_UNSUPPORTED, "UTF8/BOOL outputs run on the CPU kernels");
-        a.probe_src[q].type = c.type;
-        want(c.validity, bitmap_bytes(n), (const void**)&a.probe_src[q].valid, c.mem);
-        want(c.data, lane_bytes(c.type, n), &a.probe_src[q].data, c.mem);
-        types.push_back(c.type);
+        const int32_t t = columns[pc].type;
+        if (t == VR_UTF8 || t == VR_BOOL) return fail(VR_ERR_UNSUPPORTED, "UTF8/BOOL outputs run on the CPU kernels");
+        add_slot(pc, t);
+        sh.out_types.push_back(t);
     }
     for (int q = 0; q < a.nbuild_out; q++) {
-        int bc = out->build_out_columns[q];
+        const int bc = out->build_out_columns[q];
         if (bc < 0 || bc >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "build output column out of range");
         a.build_src[q].type = j.col_types[bc];
         a.build_src[q].data = j.cols[bc].data.p;
         a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
-        types.push_back(j.col_types[bc]);
+        a.build_from_key[q] = -1;
+        for (int k = 0; k < a.nk; k++)
+            if (j.key_cols[k] == bc) a.build_from_key[q] = k;  // a matched pair's build key = its streamed key
+        sh.out_types.push_back(j.col_types[bc]);
     }
     if (const vr_join_residual* res = out->residual) {
         if (res->op < VR_CMP_EQ || res->op > VR_CMP_GE) return fail(VR_ERR_INVALID_ARGUMENT, "residual operator");
         if (drop_null_keys) return fail(VR_ERR_UNSUPPORTED, "a null-aware anti join with a residual runs on the CPU kernels");
         int32_t types[2];
-        DCol* dst[2] = {&a.res_l, &a.res_r};
+        DCol* dst[2] = {&a.res_lbuild, &a.res_rbuild};
+        int32_t* slot[2] = {&a.res_lslot, &a.res_rslot};
         const int32_t side[2] = {res->left_side, res->right_side}, col[2] = {res->left_column, res->right_column};
         for (int q = 0; q < 2; q++) {
             if (side[q] == 0) {
                 if (col[q] < 0 || col[q] >= num_columns) return fail(VR_ERR_INVALID_ARGUMENT, "residual column out of range");
-                const vr_column& c = columns[col[q]];
-                types[q] = c.type;
-                dst[q]->type = c.type;
-                want(c.validity, bitmap_bytes(n), (const void**)&dst[q]->valid, c.mem);
-                want(c.data, lane_bytes(c.type, n), &dst[q]->data, c.mem);
+                types[q] = columns[col[q]].type;
+                *slot[q] = add_slot(col[q], types[q]);
             } else if (side[q] == 1) {
                 if (col[q] < 0 || col[q] >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "residual column out of range");
                 types[q] = j.col_types[col[q]];
@@ -916,10 +1140,46 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         a.res_loff = res->left_offset;
         a.res_roff = res->right_offset;
     }
-    a.sel = selection;
-    if (selection && mem == VR_MEM_HOST) want(selection, bitmap_bytes(n), (const void**)&a.sel, VR_MEM_HOST);
+    a.ncb = (int32_t)sh.slot_col.size();
+    return VR_OK;
+}
+
+vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_columns, const int32_t* key_columns, int64_t n,
+                        const uint8_t* selection, int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
+                        void* stream) {
+    if (!h || !out || (!columns && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    VR_TRY(check_probe_call(j, join_type, out));
+    out->rows = 0;
+    if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
+    if (n == 0) return VR_OK;
+    if (n >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "probe batches past 2^31 rows");
+    ProbeArgs a{};
+    ProbeShape sh;
+    VR_TRY(prepare_probe(j, columns, num_columns, key_columns, join_type, drop_null_keys, out, a, sh));
+    hipStream_t s = j.dev->pick(stream);
+    a.one = PBatch{n, 0, selection, 0};
+    // streamed columns: device pointers, or staged from host memory
+    std::vector<std::pair<const void*, size_t>> pieces;
+    std::vector<const void**> patch;
+    auto want = [&](const void* p, size_t bytes, const void** dst, int32_t mem) {
+        *dst = p;
+        if (p && bytes && mem == VR_MEM_HOST) {
+            pieces.push_back({p, bytes});
+            patch.push_back(dst);
+        }
+    };
+    for (size_t q = 0; q < sh.slot_col.size(); q++) {
+        const vr_column& c = columns[sh.slot_col[q]];
+        want(c.validity, bitmap_bytes(n), (const void**)&a.one_cols[q].valid, c.mem);
+        want(c.data, lane_bytes(c.type, n), &a.one_cols[q].data, c.mem);
+    }
+    if (selection && columns[key_columns[0]].mem == VR_MEM_HOST)
+        want(selection, bitmap_bytes(n), (const void**)&a.one.sel, VR_MEM_HOST);
+    std::unique_lock<std::mutex> stage_lock(j.stage_mu, std::defer_lock);
     int slot = -1;
     if (!pieces.empty()) {
+        stage_lock.lock();  // the shared pinned staging, until the kernel reading it has finished
         size_t total = 0;
         for (auto& p : pieces) total = align_up(total, 256) + p.second;
         uint8_t *host, *dev;
@@ -934,30 +1194,119 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         VR_TRY(j.staging.upload(slot, total, s));
     }
     OutPlan P;
-    VR_TRY(plan_outputs(j, out, types, pairs, a.outs, &a.probe_idx, &a.build_idx, s, P));
+    VR_TRY(plan_outputs(j, out, sh.out_types, sh.pairs, a.outs, &a.probe_idx, &a.build_idx, s, P));
     const int64_t cap = out->capacity;
-    // tiles and their look-back state
-    const int64_t T = (int64_t)PROBE_BLOCK * PROBE_ITEMS;
-    a.ntiles = (n + T - 1) / T;
-    VR_TRY(j.tile_state.ensure(j.dev, (size_t)(a.ntiles + 2) * 8));
-    a.tile_counter = j.tile_state.as<uint64_t>();
-    a.tile_state = j.tile_state.as<uint64_t>() + 2;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(a.ntiles + 2, 256, 1024)), dim3(256), 0, s, j.tile_state.as<uint64_t>(),
+    // tiles and their look-back state, the call's own
+    a.ntiles = (n + PROBE_TILE - 1) / PROBE_TILE;
+    DevBuf tiles;
+    VR_TRY(tiles.ensure(j.dev, (size_t)(a.ntiles + 2) * 8));
+    a.tile_counter = tiles.as<uint64_t>();
+    a.tile_state = tiles.as<uint64_t>() + 2;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(a.ntiles + 2, 256, 1024)), dim3(256), 0, s, tiles.as<uint64_t>(),
                        (uint64_t)(a.ntiles + 2));
-    hipLaunchKernelGGL(join_probe_kernel, dim3((unsigned)a.ntiles), dim3(PROBE_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(join_probe_kernel<false>, dim3((unsigned)a.ntiles), dim3(PROBE_BLOCK), 0, s, a);
     VR_TRY_HIP(hipGetLastError());
     uint64_t rows = 0;
-    VR_TRY_HIP(hipMemcpyAsync(&rows, j.tile_state.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipMemcpyAsync(&rows, tiles.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
     VR_TRY_HIP(hipStreamSynchronize(s));
     if (slot >= 0) VR_TRY(j.staging.done(slot, s));
+    if (stage_lock.owns_lock()) stage_lock.unlock();
     int64_t written = std::min<int64_t>((int64_t)rows, cap);
-    VR_TRY(copy_back(out, P, written, pairs));
+    VR_TRY(copy_back(out, P, written, sh.pairs));
     out->rows = (int64_t)rows;
     j.dev->count(&vr_stats::calls, 1);
     j.dev->count(&vr_stats::input_rows, n);
     j.dev->count(&vr_stats::output_rows, written);
     j.dev->count(&vr_stats::kernel_launches, 2);
     if ((int64_t)rows > cap) return fail(VR_ERR_CAPACITY, "probe output needs " + std::to_string(rows) + " rows");
+    return VR_OK;
+}
+
+vr_status vr_join_probe_batches(vr_join* h, int32_t num_batches, const vr_column* columns, int32_t num_columns,
+                                const int64_t* lengths, const uint8_t* const* selections, const int32_t* key_columns,
+                                int32_t join_type, int32_t drop_null_keys, vr_probe_output* out, int64_t* batch_rows,
+                                void* stream) {
+    if (!h || !out || num_batches < 0 || (num_batches && (!columns || !lengths))) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    VR_TRY(check_probe_call(j, join_type, out));
+    out->rows = 0;
+    if (num_batches == 0) return VR_OK;
+    if (out->mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "batched probes write device outputs (vr_join_probe stages host ones)");
+    ProbeArgs a{};
+    ProbeShape sh;
+    VR_TRY(prepare_probe(j, columns, num_columns, key_columns, join_type, drop_null_keys, out, a, sh));
+    const int ncb = a.ncb;
+    std::vector<PBatch> batches((size_t)num_batches);
+    std::vector<PCol> bcols((size_t)num_batches * ncb);
+    int64_t tiles = 0, rows_in = 0;
+    for (int32_t b = 0; b < num_batches; b++) {
+        const int64_t n = lengths[b];
+        if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
+        if (n >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "probe batches past 2^31 rows");
+        const vr_column* cb = columns + (size_t)b * num_columns;
+        for (int q = 0; q < ncb; q++) {
+            const vr_column& c = cb[sh.slot_col[q]];
+            if (c.type != a.ctype[q]) return fail(VR_ERR_INVALID_ARGUMENT, "a batch's column lane differs from the first batch's");
+            if (n && c.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "batched probes read device-resident batches");
+            if (n && c.length < n) return fail(VR_ERR_INVALID_ARGUMENT, "a batch's column is shorter than the batch");
+            bcols[(size_t)b * ncb + q] = PCol{c.validity, c.data};
+        }
+        batches[b] = PBatch{n, tiles, selections ? selections[b] : nullptr, 0};
+        tiles += (n + PROBE_TILE - 1) / PROBE_TILE;
+        rows_in += n;
+    }
+    if (tiles == 0) {
+        if (batch_rows) std::fill(batch_rows, batch_rows + num_batches, 0);
+        return VR_OK;
+    }
+    if (tiles >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "more than 2^31 probe tiles in one call");
+    hipStream_t s = j.dev->pick(stream);
+    // the call's device tables: batches, their columns, tile -> batch, tile states, batch ends
+    const size_t off_cols = align_up(sizeof(PBatch) * (size_t)num_batches, 256);
+    const size_t off_map = off_cols + align_up(sizeof(PCol) * bcols.size(), 256);
+    const size_t off_state = off_map + align_up(4 * (size_t)tiles, 256);
+    const size_t off_end = off_state + align_up(8 * (size_t)(tiles + 2), 256);
+    const size_t bytes = off_end + 8 * (size_t)num_batches;
+    DevBuf desc;
+    VR_TRY(desc.ensure(j.dev, bytes));
+    uint8_t* d = desc.as<uint8_t>();
+    VR_TRY_HIP(hipMemcpyAsync(d, batches.data(), sizeof(PBatch) * batches.size(), hipMemcpyHostToDevice, s));
+    VR_TRY_HIP(hipMemcpyAsync(d + off_cols, bcols.data(), sizeof(PCol) * bcols.size(), hipMemcpyHostToDevice, s));
+    a.batches = (const PBatch*)d;
+    a.bcols = (const PCol*)(d + off_cols);
+    a.tile_batch = (const int32_t*)(d + off_map);
+    a.tile_counter = (uint64_t*)(d + off_state);
+    a.tile_state = a.tile_counter + 2;
+    a.batch_end = (int64_t*)(d + off_end);
+    a.ntiles = tiles;
+    OutPlan P;
+    VR_TRY(plan_outputs(j, out, sh.out_types, sh.pairs, a.outs, &a.probe_idx, &a.build_idx, s, P));
+    const int cap = j.dev->num_cus * 16;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(tiles + 2, 256, cap)), dim3(256), 0, s, a.tile_counter, (uint64_t)(tiles + 2));
+    hipLaunchKernelGGL(tile_batch_kernel, dim3(grid_of(num_batches, 256, cap)), dim3(256), 0, s, a.batches, num_batches,
+                       (int32_t*)a.tile_batch);
+    hipLaunchKernelGGL(join_probe_kernel<true>, dim3((unsigned)tiles), dim3(PROBE_BLOCK), 0, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t rows = 0;
+    std::vector<int64_t> ends(batch_rows ? (size_t)num_batches : 0);
+    VR_TRY_HIP(hipMemcpyAsync(&rows, a.tile_counter + 1, 8, hipMemcpyDeviceToHost, s));
+    if (batch_rows) VR_TRY_HIP(hipMemcpyAsync(ends.data(), a.batch_end, 8 * (size_t)num_batches, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    if (batch_rows) {  // an empty batch wrote no end: it ends where the batch before it ends
+        int64_t prev = 0;
+        for (int32_t b = 0; b < num_batches; b++) {
+            const int64_t e = batches[b].n ? ends[b] : prev;
+            batch_rows[b] = e - prev;
+            prev = e;
+        }
+    }
+    out->rows = (int64_t)rows;
+    const int64_t written = std::min<int64_t>((int64_t)rows, out->capacity);
+    j.dev->count(&vr_stats::calls, 1);
+    j.dev->count(&vr_stats::input_rows, rows_in);
+    j.dev->count(&vr_stats::output_rows, written);
+    j.dev->count(&vr_stats::kernel_launches, 3);
+    if ((int64_t)rows > out->capacity) return fail(VR_ERR_CAPACITY, "probe output needs " + std::to_string(rows) + " rows");
     return VR_OK;
 }
 
@@ -1021,8 +1370,8 @@ vr_status vr_join_unmatched_build(vr_join* h, vr_probe_output* out, void* stream
 vr_status vr_join_memory(vr_join* h, int64_t* bytes) {
     if (!h || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     JoinImpl& j = h->j;
-    int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.tile_state.bytes +
-                          j.out_stage.bytes + j.dense.bytes + j.matched.bytes + j.unmatched.bytes);
+    int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.dense.bytes +
+                          j.matched.bytes + j.unmatched.bytes);
     for (auto& c : j.cols) b += (int64_t)(c.data.bytes + c.valid.bytes);
     *bytes = b;
     return VR_OK;

@@ -62,6 +62,8 @@ _sig("vr_join_build_finish", [_P, _P])
 _sig("vr_join_info", [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
 _sig("vr_join_probe", [_P, _P, C.c_int32, _P, C.c_int64, _P, C.c_int32, C.c_int32,
                        C.POINTER(abi.VrProbeOutput), _P])
+_sig("vr_join_probe_batches", [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32,
+                               C.POINTER(abi.VrProbeOutput), _P, _P])
 _sig("vr_join_unmatched_build", [_P, C.POINTER(abi.VrProbeOutput), _P])
 _sig("vr_join_memory", [_P, C.POINTER(C.c_int64)])
 _sig("vr_filter_compare", [_P, C.POINTER(abi.VrColumn), C.c_int32, C.POINTER(abi.VrScalar), _P, C.c_int64, _P,
@@ -96,16 +98,17 @@ def check(st):
 
 
 def _stream(s):
-    """A HIP stream handle: None (the handle's own stream), an int, or a torch stream.
+    """A HIP stream handle: an int, a torch stream, or None.
 
-    The handle's own stream is non-blocking, so nothing orders it after work torch has queued on
-    its current stream (a tensor still being filled or copied): a call that runs there first waits
-    for that work.  A call on a torch stream is ordered with it by the stream itself."""
+    None is torch's current stream once torch has initialized the device (else the handle's own
+    stream).  Work on torch's stream is ordered with the work that filled the input tensors and with
+    whatever torch's caching allocator later gives their memory to, so a call may return while its
+    kernels still read the tensors and the caller may drop them at once."""
     h = s if s is None or isinstance(s, int) else s.cuda_stream
     if not h:
         t = sys.modules.get("torch")
         if t is not None and t.cuda.is_initialized():
-            t.cuda.current_stream().synchronize()
+            return t.cuda.current_stream().cuda_stream or None
         return None
     return h
 
@@ -334,6 +337,26 @@ class BuildTable:
                     build_idx[:r] if build_idx is not None else None, outputs, r)
         return probe_idx, build_idx, outputs, r
 
+    def probe_batches(self, batches, key_columns, join_type, outputs=(), probe_out=(), build_out=(), capacity=None,
+                      probe_idx=None, build_idx=None, drop_null_keys=False, residual=None, stream=None):
+        """vr_join_probe_batches over a ProbeBatches set into device outputs (torch tensors):
+        returns (total rows, rows of each batch)."""
+        oa = c_array(list(outputs), abi.VrOutColumn)
+        po, bo = abi.int_array(list(probe_out)), abi.int_array(list(build_out))
+        kc = abi.int_array(list(key_columns))
+        pairs = join_type in (abi.INNER, abi.OUTER)
+        out = abi.VrProbeOutput(abi.MEM_DEVICE, int(capacity), _ptr(probe_idx), _ptr(build_idx) if pairs else None,
+                                len(probe_out), C.cast(po, C.POINTER(C.c_int32)), len(build_out),
+                                C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        if residual is not None:
+            out.residual = C.addressof(residual)
+        rows = np.zeros(max(1, batches.count), np.int64)
+        check(_lib.vr_join_probe_batches(self.h, batches.count, C.cast(batches.cols, _P), batches.ncols,
+                                         batches.lengths.ctypes.data, C.cast(batches.sels, _P) if batches.has_sel else None,
+                                         C.cast(kc, _P), join_type, 1 if drop_null_keys else 0, C.byref(out),
+                                         rows.ctypes.data, _stream(stream)))
+        return out.rows, rows[:batches.count]
+
     def unmatched_build(self, probe_types=(), build_out=(), capacity=None, stream=None):
         """vr_join_unmatched_build into host buffers: (probe_idx (-1), build_idx, output columns, rows)."""
         if capacity is None:
@@ -349,6 +372,24 @@ class BuildTable:
         check(_lib.vr_join_unmatched_build(self.h, C.byref(out), _stream(stream)))
         r = out.rows
         return pidx[:r], bidx[:r], outputs, r
+
+
+class ProbeBatches:
+    """The streamed batches of one vr_join_probe_batches call, marshalled once: `batches` is a list
+    of (columns, n) or (columns, n, selection) with device-resident columns, every batch with the
+    same column lanes."""
+
+    def __init__(self, batches):
+        self.count = len(batches)
+        self.ncols = len(batches[0][0]) if batches else 0
+        self._keep = [b[0] for b in batches]
+        flat = [c for b in batches for c in b[0]]
+        self.cols = c_array(flat)
+        self.lengths = np.array([int(b[1]) for b in batches] or [0], np.int64)
+        sels = [b[2] if len(b) > 2 else None for b in batches]
+        self.has_sel = any(x is not None for x in sels)
+        self._sel_keep = sels
+        self.sels = (C.c_void_p * max(1, self.count))(*[_ptr(x) for x in sels])
 
 
 def agg_key_dictionary(handle, c):
