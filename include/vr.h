@@ -131,6 +131,7 @@ typedef struct vr_stats {
     int64_t partitioned_updates;  /* aggregate calls that took the partitioned update */
     int64_t specialized_updates;  /* of those, calls that ran kernels compiled for their spec */
     int64_t dense_join_builds;    /* join builds that took the direct key index */
+    int64_t device_allocations;   /* device blocks the library handed out (new or from its cache) */
 } vr_stats;
 vr_status vr_stats_get(vr_device* dev, vr_stats* out);
 
@@ -354,6 +355,18 @@ vr_status vr_join_probe(vr_join* join, const vr_column* columns, int32_t num_col
                         const int32_t* key_columns, int64_t n, const uint8_t* selection,
                         int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
                         void* stream);
+/* vr_join_probe without any device allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  It has no
+ * counterpart in the reference.  Columns, selection and outputs must be device-resident; a call
+ * takes at most (131,070 * 2,048) = 268,431,360 rows (VR_ERR_UNSUPPORTED past that).  Its kernels
+ * keep their scratch (tile states, the row count) in a 1 MiB array of the library on each device,
+ * held until the process ends and shared by every call on that device: calls that overlap on one
+ * device race, so a caller issues them one at a time per device.  Results are those of
+ * vr_join_probe on the same inputs. */
+vr_status vr_join_probe_noalloc(vr_join* join, const vr_column* columns, int32_t num_columns,
+                                const int32_t* key_columns, int64_t n, const uint8_t* selection,
+                                int32_t join_type, int32_t drop_null_keys, vr_probe_output* out,
+                                void* stream);
 /* Probes many streamed batches with one launch: the probe iterator's per-batch loop
  * (VectorHashJoinIterator.hasNext/probe/emitMatches/emitSemiAnti, VectorHashJoinExec.scala:1017-1182)
  * over every batch a task holds, without a host round trip per batch.  Batch b's columns are

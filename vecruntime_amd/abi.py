@@ -116,6 +116,7 @@ class VrStats(C.Structure):
         ("partitioned_updates", C.c_int64),
         ("specialized_updates", C.c_int64),
         ("dense_join_builds", C.c_int64),
+        ("device_allocations", C.c_int64),
     ]
 
 

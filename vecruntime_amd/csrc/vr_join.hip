@@ -762,6 +762,31 @@ static vr_status build_dense(JoinImpl& j, bool candidate, int64_t lo, int64_t hi
     return VR_OK;
 }
 
+// The allocation-free probe's scratch: the tiles' look-back state and the output row count,
+// in a device array of the library (each device holds its own copy from when the library's code
+// is loaded on it until the process ends).  Host code never passes the array's name to a kernel
+// or a copy: the name is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress, once per device.
+constexpr uint64_t NOALLOC_SCRATCH_WORDS = 1ull << 17;  // 1 MiB: two counters + 131,070 tiles
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_WORDS];
+
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, uint64_t*> g_scratch_ptr;
+
+// The address of `scratch` on device `ordinal`, which must be the current device.
+static vr_status scratch_on(int ordinal, uint64_t** out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(ordinal);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        VR_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(VR_ERR_DEVICE, "no device address for the probe scratch");
+        it = g_scratch_ptr.emplace(ordinal, (uint64_t*)p).first;
+    }
+    *out = it->second;
+    return VR_OK;
+}
+
 extern "C" {
 
 vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out) {
@@ -1219,6 +1244,54 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     j.dev->count(&vr_stats::output_rows, written);
     j.dev->count(&vr_stats::kernel_launches, 2);
     if ((int64_t)rows > cap) return fail(VR_ERR_CAPACITY, "probe output needs " + std::to_string(rows) + " rows");
+    return VR_OK;
+}
+
+vr_status vr_join_probe_noalloc(vr_join* h, const vr_column* columns, int32_t num_columns, const int32_t* key_columns,
+                                int64_t n, const uint8_t* selection, int32_t join_type, int32_t drop_null_keys,
+                                vr_probe_output* out, void* stream) {
+    if (!h || !out || (!columns && n)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    JoinImpl& j = h->j;
+    VR_TRY(check_probe_call(j, join_type, out));
+    out->rows = 0;
+    if (out->mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "the allocation-free probe writes device outputs only");
+    if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
+    if (n == 0) return VR_OK;
+    const int64_t ntiles = (n + PROBE_TILE - 1) / PROBE_TILE;
+    if ((uint64_t)ntiles + 2 > NOALLOC_SCRATCH_WORDS)
+        return fail(VR_ERR_UNSUPPORTED, "the allocation-free probe takes at most " +
+                                            std::to_string((NOALLOC_SCRATCH_WORDS - 2) * PROBE_TILE) + " rows per call");
+    ProbeArgs a{};
+    ProbeShape sh;
+    VR_TRY(prepare_probe(j, columns, num_columns, key_columns, join_type, drop_null_keys, out, a, sh));
+    for (size_t q = 0; q < sh.slot_col.size(); q++) {
+        const vr_column& c = columns[sh.slot_col[q]];
+        if (c.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "the allocation-free probe reads device-resident columns only");
+        a.one_cols[q] = PCol{c.validity, c.data};
+    }
+    a.one = PBatch{n, 0, selection, 0};
+    VR_TRY_HIP(hipSetDevice(j.dev->ordinal));
+    uint64_t* sp = nullptr;
+    VR_TRY(scratch_on(j.dev->ordinal, &sp));
+    hipStream_t s = j.dev->pick(stream);
+    OutPlan P;  // device outputs: no staging
+    VR_TRY(plan_outputs(j, out, sh.out_types, sh.pairs, a.outs, &a.probe_idx, &a.build_idx, s, P));
+    a.ntiles = ntiles;
+    a.tile_counter = sp;
+    a.tile_state = sp + 2;
+    // cleared by a kernel, not a memset: the call may be captured into a graph
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(ntiles + 2, 256, 1024)), dim3(256), 0, s, sp, (uint64_t)(ntiles + 2));
+    hipLaunchKernelGGL(join_probe_kernel<false>, dim3((unsigned)ntiles), dim3(PROBE_BLOCK), 0, s, a);
+    VR_TRY_HIP(hipGetLastError());
+    uint64_t rows = 0;
+    VR_TRY_HIP(hipMemcpyAsync(&rows, sp + 1, 8, hipMemcpyDeviceToHost, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    out->rows = (int64_t)rows;
+    j.dev->count(&vr_stats::calls, 1);
+    j.dev->count(&vr_stats::input_rows, n);
+    j.dev->count(&vr_stats::output_rows, std::min<int64_t>((int64_t)rows, out->capacity));
+    j.dev->count(&vr_stats::kernel_launches, 2);
+    if ((int64_t)rows > out->capacity) return fail(VR_ERR_CAPACITY, "probe output needs " + std::to_string(rows) + " rows");
     return VR_OK;
 }
 

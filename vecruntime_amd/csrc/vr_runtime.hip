@@ -33,6 +33,7 @@ vr_status Device::alloc(void** p, size_t bytes) {
     const size_t want = size_class(bytes);
     {
         std::lock_guard<std::mutex> g(mu);
+        stats.device_allocations++;
         if (budget > 0 && allocated + (int64_t)want > budget)
             return fail(VR_ERR_OOM, "HBM budget of " + std::to_string(budget) + " bytes exceeded");
         auto it = cache.lower_bound(want);

@@ -62,6 +62,8 @@ _sig("vr_join_build_finish", [_P, _P])
 _sig("vr_join_info", [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
 _sig("vr_join_probe", [_P, _P, C.c_int32, _P, C.c_int64, _P, C.c_int32, C.c_int32,
                        C.POINTER(abi.VrProbeOutput), _P])
+_sig("vr_join_probe_noalloc", [_P, _P, C.c_int32, _P, C.c_int64, _P, C.c_int32, C.c_int32,
+                               C.POINTER(abi.VrProbeOutput), _P])
 _sig("vr_join_probe_batches", [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32,
                                C.POINTER(abi.VrProbeOutput), _P, _P])
 _sig("vr_join_unmatched_build", [_P, C.POINTER(abi.VrProbeOutput), _P])
@@ -336,6 +338,24 @@ class BuildTable:
             return (probe_idx[:r] if probe_idx is not None else None,
                     build_idx[:r] if build_idx is not None else None, outputs, r)
         return probe_idx, build_idx, outputs, r
+
+    def probe_noalloc(self, columns, key_columns, n, join_type, outputs=(), probe_out=(), build_out=(), capacity=None,
+                      probe_idx=None, build_idx=None, selection=None, drop_null_keys=False, residual=None, stream=None):
+        """vr_join_probe_noalloc over device columns into device outputs (torch tensors): the rows
+        produced."""
+        ca = c_array(columns)
+        kc = abi.int_array(list(key_columns))
+        oa = c_array(list(outputs), abi.VrOutColumn)
+        po, bo = abi.int_array(list(probe_out)), abi.int_array(list(build_out))
+        pairs = join_type in (abi.INNER, abi.OUTER)
+        out = abi.VrProbeOutput(abi.MEM_DEVICE, int(capacity), _ptr(probe_idx), _ptr(build_idx) if pairs else None,
+                                len(probe_out), C.cast(po, C.POINTER(C.c_int32)), len(build_out),
+                                C.cast(bo, C.POINTER(C.c_int32)), C.cast(oa, C.POINTER(abi.VrOutColumn)), 0)
+        if residual is not None:
+            out.residual = C.addressof(residual)
+        check(_lib.vr_join_probe_noalloc(self.h, C.cast(ca, _P), len(columns), C.cast(kc, _P), int(n), _ptr(selection),
+                                         join_type, 1 if drop_null_keys else 0, C.byref(out), _stream(stream)))
+        return out.rows
 
     def probe_batches(self, batches, key_columns, join_type, outputs=(), probe_out=(), build_out=(), capacity=None,
                       probe_idx=None, build_idx=None, drop_null_keys=False, residual=None, stream=None):
