@@ -17,7 +17,8 @@
 //     Math.addExact when checked (:363-601), strict DoubleSum -- one running sum per group, rows
 //     added in row order across batches (:202-206,234-236,276-280), WideLongSum's 128-bit add
 //     with carry and sticky overflow (:749-781,817-824), LongMinMax / DoubleMinMax offer with
-//     CompareOp.nanSafeCompare (:1248-1258,1327-1334; CompareOp.java:68-82);
+//     CompareOp.nanSafeCompare (:1248-1258,1327-1334; CompareOp.java:68-82), Decimal128MinMax
+//     offer by Decimal128.compare (:1357-1462; Decimal128.java:145);
 //   * the partial buffers and results the aggregate functions emit
 //     (spark/src/main/scala/io/vecruntime/spark/agg/VectorAggregates.scala:118-518,905-1040);
 //   * BuildTable.build's head/next chains in build order and the probe's pair order, semi/anti
@@ -386,7 +387,8 @@ struct Acc {
     std::vector<double> dsum;    // DoubleSum
     std::vector<int64_t> lo, hi; // WideLongSum
     std::vector<uint8_t> overflowed, any, sumNull, merged;
-    std::vector<int64_t> lbest;
+    std::vector<int64_t> lbest;  // LongMinMax; the low limb for Decimal128MinMax
+    std::vector<int64_t> hbest;  // Decimal128MinMax: the high limb
     std::vector<double> dbest;
     std::vector<int64_t> wraps;  // exact ungrouped final merge: multiples of 2^128 beyond lo/hi
     bool checkedOverflow = false;
@@ -404,6 +406,7 @@ struct Acc {
         sumNull.resize(g, 0);
         merged.resize(g, 0);
         lbest.resize(g, 0);
+        hbest.resize(g, 0);
         dbest.resize(g, 0.0);
         wraps.resize(g, 0);
     }
@@ -532,7 +535,24 @@ struct Acc {
                 case VR_AGG_MERGE_MAX: {
                     if (!valid(*v, i)) break;
                     bool isMin = d.kind == VR_AGG_MIN || d.kind == VR_AGG_MERGE_MIN;
-                    if (v->type == VR_FLOAT64) {
+                    if (v->type == VR_DECIMAL128) {
+                        // Decimal128MinMax.offer (GroupedAccumulators.java:1410-1422): the first
+                        // value, then one strictly better by Decimal128.compare (signed high limb,
+                        // then unsigned low limb, Decimal128.java:145)
+                        const int64_t xl = ((const int64_t*)v->data)[2 * i], xh = ((const int64_t*)v->data)[2 * i + 1];
+                        if (!any[g]) {
+                            lbest[g] = xl;
+                            hbest[g] = xh;
+                            any[g] = 1;
+                        } else {
+                            const int c = xh != hbest[g] ? (xh < hbest[g] ? -1 : 1)
+                                                         : ((uint64_t)xl == (uint64_t)lbest[g] ? 0 : ((uint64_t)xl < (uint64_t)lbest[g] ? -1 : 1));
+                            if (isMin ? c < 0 : c > 0) {
+                                lbest[g] = xl;
+                                hbest[g] = xh;
+                            }
+                        }
+                    } else if (v->type == VR_FLOAT64) {
                         double x = ((const double*)v->data)[i];
                         if (!any[g]) {
                             dbest[g] = x;
@@ -744,7 +764,11 @@ int emit_one(const Acc& a, int64_t g, vr_out_column* out, int64_t r) {
         case VR_AGG_MERGE_MIN:
         case VR_AGG_MERGE_MAX: {
             bool has = a.any[g];
-            if (a.inType == VR_FLOAT64) put_f64(out[0], r, has, a.dbest[g]);
+            if (a.inType == VR_DECIMAL128) {
+                set_bit(out[0].validity, r, has);
+                ((int64_t*)out[0].data)[2 * r] = has ? a.lbest[g] : 0;
+                ((int64_t*)out[0].data)[2 * r + 1] = has ? a.hbest[g] : 0;
+            } else if (a.inType == VR_FLOAT64) put_f64(out[0], r, has, a.dbest[g]);
             else if (a.inType == VR_INT32) {
                 set_bit(out[0].validity, r, has);
                 ((int32_t*)out[0].data)[r] = has ? (int32_t)a.lbest[g] : 0;

@@ -226,6 +226,37 @@ def test_checked_long_sum_overflow(dev):
     assert g.emit()[1][0].values(1) == [-(1 << 63)]
 
 
+def test_checked_long_sum_first_prefix_in_deterministic_mode(dev):
+    """The reference's checked LongSum adds with Math.addExact row by row (GroupedAccumulators.java:
+    563-601), so [Long.MAX, 1, -1] throws although its total fits.  The deterministic mode folds
+    each group's rows in row order and raises the same, also when the overflowing prefix spans two
+    batches; the oracle agrees.  Prefixes that all fit give the sum."""
+    from oracle.oracle import OracleError
+    from vecruntime_amd import vr
+    big = (1 << 63) - 1
+    spec = abi.AggSpec([abi.INT32], [abi.INT64], [abi.agg_desc(abi.SUM_LONG, 0, flags=abi.CHECKED)],
+                       deterministic=True)
+    for parts in ([([1, 1, 1, 2], [big, 1, -1, 5])], [([1, 2], [big, 5]), ([3, 1, 1], [0, 1, -1])]):
+        g, o = vr.GroupedAggregate(dev, spec), OracleAgg(spec)
+        raised = []
+        for k, v in parts:
+            g.update([Column.ints(k)], [Column.longs(v)], len(k))
+            try:
+                o.update([Column.ints(k)], [Column.longs(v)], len(k))
+            except OracleError as oe:  # the oracle throws where the reference does: in the update
+                raised.append(oe.status)
+        assert raised == [abi.VR_ERR_ARITHMETIC_OVERFLOW]
+        with pytest.raises(vr.VrError) as e:  # the device reports it when the group is emitted
+            g.emit()
+        assert e.value.status == abi.VR_ERR_ARITHMETIC_OVERFLOW
+        g.close()
+    g = vr.GroupedAggregate(dev, spec)
+    g.update([Column.ints([1, 1, 1, 1])], [Column.longs([big, -1, 1, -(1 << 63) + 1])], 4)
+    g.update([Column.ints([1])], [Column.longs([-(1 << 63)])], 1)
+    assert g.emit()[1][0].values(1) == [-(1 << 63)]  # every prefix fits; the total is Long.MIN_VALUE
+    g.close()
+
+
 def test_null_and_sentinel_keys(dev):
     """Null keys group together apart from 0; a key equal to the table's empty marker groups too."""
     from vecruntime_amd import vr
@@ -407,7 +438,12 @@ def test_long_batch_over_few_groups_is_split(dev, batched):
         g.update(dk, dv, n)
     o = OracleAgg(spec)
     o.update([Column.longs(keys)], [Column.doubles(dbl), Column.longs(i64)], n)
-    compare(g, o, spec, rel=1e-9)
+    # double sums within 1e-12 of the group's sum of magnitudes (the non-finite values are compared
+    # as such before any tolerance)
+    uk, inv = np.unique(keys, return_inverse=True)
+    mag = np.bincount(inv, np.where(np.isfinite(dbl), np.abs(dbl), 0.0))
+    scale = {(int(k),): [0.0, 0.0, 0.0, 0.0, float(m)] for k, m in zip(uk, mag)}
+    assert compare(g, o, spec, rel=1e-12, scale=scale) < 1e-12
     assert g.memory() < 2 ** 30  # sized by its groups plus one launch's room, not by 9M rows
 
 

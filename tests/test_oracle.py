@@ -436,3 +436,67 @@ def test_ungrouped_closed_forms_and_empty_input():
     assert eo[0].values(1) == [0] and eo[1].values(1) == [0]
     assert eo[2].values(1) == [None] and eo[3].values(1) == [None]
     assert eo[6].values(1) == [None] and eo[7].values(1) == [None]
+
+
+def _decimal_pool(rng, n):
+    """Unscaled 128-bit values: the extremes, values that differ only in a low limb whose top bit
+    is set or clear, negative and positive values sharing a high limb, and random ones."""
+    lo_big = (1 << 63) + 5
+    special = [pyref.INT128_MIN, pyref.INT128_MAX, 0, -1, 1, (7 << 64) + lo_big, (7 << 64) + 3,
+               -(7 << 64) + lo_big, -(7 << 64) + 3, (1 << 64) - 1, -(1 << 64), lo_big, 3]
+    out = []
+    for _ in range(n):
+        r = rng.random()
+        if r < 0.4:
+            out.append(special[int(rng.integers(0, len(special)))])
+        elif r < 0.7:
+            out.append(int(rng.integers(-10 ** 18, 10 ** 18)) * int(rng.integers(1, 10 ** 18)))
+        else:
+            out.append(int(rng.integers(-2 ** 62, 2 ** 62)) << int(rng.integers(0, 64)))
+    return out
+
+
+def test_decimal128_min_max_match_pyref():
+    """Decimal128MinMax (GroupedAccumulators.java:1357-1462): the first value, then one strictly
+    better by signed 128-bit order (Decimal128.compare, Decimal128.java:145); a group with no
+    value emits null.  Against the Python restatement's exact integers over three batches with nulls
+    and a selection."""
+    rng = np.random.default_rng(128)
+    batches, pb = [], []
+    for n in (700, 1, 1300):
+        keys = rng.integers(0, 40, n)
+        vals = _decimal_pool(rng, n)
+        for i in np.flatnonzero(keys == 38):  # one high limb: the low limb's order decides
+            vals[i] = (7 << 64) + ((1 << 63) + 5 if rng.random() < 0.5 else 3)
+        nul = (rng.random(n) < 0.15) | (keys == 39)
+        sel = rng.random(n) < 0.8
+        batches.append((keys, vals, nul, sel, n))
+        pb.append(dict(keys=[[int(k) for k in keys]], values=[[None if nul[i] else vals[i] for i in range(n)]],
+                       selection=[bool(x) for x in sel]))
+    aggs = [abi.agg_desc(abi.MIN, 0), abi.agg_desc(abi.MAX, 0), abi.agg_desc(abi.COUNT, 0)]
+    spec = abi.AggSpec([abi.INT32], [abi.DECIMAL128], aggs)
+    agg = OracleAgg(spec)
+    for keys, vals, nul, sel, n in batches:
+        agg.update([Column.ints(keys)], [Column.decimals(vals, nulls=nul)], n, selection=pack_bits(sel))
+    paggs = [dict(kind=a.kind, input=a.input, flags=a.flags, in_type="decimal") for a in aggs]
+    order, outs = pyref.aggregate(pb, [False], paggs)
+    G = agg.num_groups()
+    assert G == len(order)
+    kout, (mn, mx, cnt) = agg.emit()
+    assert kout[0].values(G) == [o[0] for o in order]
+    assert mn.values(G) == [o[0][0] for o in outs]
+    assert mx.values(G) == [o[1][0] for o in outs]
+    assert cnt.values(G) == [o[2][0] for o in outs]
+    assert any(v is None for v in mn.values(G))  # key 39 holds only nulls
+    # a compare that takes the low limb as signed (a plausible slip) picks differently on this data
+    def signed_lo_key(v):
+        lo = v & ((1 << 64) - 1)
+        return (v >> 64, lo - (1 << 64) if lo >= (1 << 63) else lo)
+    wrong = {}
+    for b in pb:
+        for i, (k, v) in enumerate(zip(b["keys"][0], b["values"][0])):
+            if v is None or not b["selection"][i]:
+                continue
+            if k not in wrong or signed_lo_key(v) < signed_lo_key(wrong[k]):
+                wrong[k] = v
+    assert any(wrong.get(o[0]) != e[0][0] for o, e in zip(order, outs))

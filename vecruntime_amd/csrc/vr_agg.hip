@@ -93,8 +93,8 @@ __global__ __launch_bounds__(AGG_BLOCK) void agg_update_kernel(UpdateArgs a) {
     uint64_t* lkeys = lds;
     uint64_t* lstate = lds + E;
     // 64-bit keys are cached by key (an LDS hit touches no global memory); wide keys, and every
-    // key in the deterministic mode (which records each row's slot), by slot.
-    const bool by_key = a.ks.mode != KM_WIDE && !a.det;
+    // key of an update that records each row's slot (for an ordered fold), by slot.
+    const bool by_key = a.ks.mode != KM_WIDE && !a.det && !a.row_slot;
     if (threadIdx.x == 0) inserted = 0;
     if (E) {
         for (int e = threadIdx.x; e < E; e += blockDim.x) lkeys[e] = EMPTY_KEY;
@@ -254,10 +254,12 @@ __global__ void migrate_kernel(TableView from, UpdateArgs a) {
     }
 }
 
-// Deterministic mode: after a radix sort of (slot, row) pairs by slot, the first position of each
-// slot's run folds that run's double values in row order onto the group's running sum -- Spark's
-// `sum += x` chain across batches (GroupedAccumulators.DoubleSum(strict), GroupedAccumulators.java:
-// 202-206,276-280).
+// After a radix sort of (slot, row) pairs by slot, the first position of each slot's run folds that
+// run's values in row order onto the group's state:
+//  * deterministic mode, double sums: Spark's `sum += x` chain across batches
+//    (GroupedAccumulators.DoubleSum(strict), GroupedAccumulators.java:202-206,276-280);
+//  * 128-bit MIN/MAX (state lo, hi, any): Decimal128MinMax.offer -- the first value, then one
+//    strictly better by signed 128-bit order (GroupedAccumulators.java:1410-1422, Decimal128.java:145).
 __global__ void ordered_fold_kernel(const uint32_t* slots, const uint32_t* rows, int64_t n, UpdateArgs a) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         uint32_t s = slots[p];
@@ -265,14 +267,53 @@ __global__ void ordered_fold_kernel(const uint32_t* slots, const uint32_t* rows,
         uint64_t* g = a.t.state + (uint64_t)s * a.t.stride;
         for (int k = 0; k < a.nops; k++) {
             const Op& op = a.ops[k];
-            if (op.code != OP_SUM_F64) continue;
             const DCol& c = a.one.vals[op.col];
-            double acc = __longlong_as_double((long long)g[op.off]);
-            for (int64_t q = p; q < n && slots[q] == s; q++) {
-                uint32_t i = rows[q];
-                if (is_valid(c, i)) acc += ((const double*)c.data)[i];
+            if (op.code == OP_SUM_F64 && a.det) {
+                double acc = __longlong_as_double((long long)g[op.off]);
+                for (int64_t q = p; q < n && slots[q] == s; q++) {
+                    uint32_t i = rows[q];
+                    if (is_valid(c, i)) acc += ((const double*)c.data)[i];
+                }
+                g[op.off] = (uint64_t)__double_as_longlong(acc);
+            } else if (op.code == OP_SUM_I64 && ((a.checked_sums >> k) & 1u)) {
+                // LongSum(checked) (GroupedAccumulators.java:563-601): Math.addExact per row, so any
+                // prefix of the group's rows in row order outside the long range is an error.  The
+                // state already holds this batch's rows: the total before them is the state's exact
+                // value less their sum.
+                const bool i32 = c.type == VR_INT32;
+                __int128 run = 0;
+                for (int64_t q = p; q < n && slots[q] == s; q++) {
+                    const uint32_t i = rows[q];
+                    if (is_valid(c, i)) run += i32 ? (int64_t)((const int32_t*)c.data)[i] : ((const int64_t*)c.data)[i];
+                }
+                __int128 acc = (__int128)g[op.off] + ((__int128)(int64_t)g[op.off + 1] << 32) - run;
+                bool over = acc < (__int128)INT64_MIN || acc > (__int128)INT64_MAX;
+                for (int64_t q = p; q < n && slots[q] == s && !over; q++) {
+                    const uint32_t i = rows[q];
+                    if (!is_valid(c, i)) continue;
+                    acc += i32 ? (int64_t)((const int32_t*)c.data)[i] : ((const int64_t*)c.data)[i];
+                    over = acc < (__int128)INT64_MIN || acc > (__int128)INT64_MAX;
+                }
+                if (over) atomicOr((unsigned long long*)&a.t.counters[C_ERROR], (unsigned long long)ERR_LONG_OVERFLOW);
+            } else if (op.code == OP_MIN_D128 || op.code == OP_MAX_D128) {
+                const bool mn = op.code == OP_MIN_D128;
+                uint64_t lo = g[op.off], hi = g[op.off + 1], any = g[op.off + 2];
+                for (int64_t q = p; q < n && slots[q] == s; q++) {
+                    const uint32_t i = rows[q];
+                    if (!is_valid(c, i)) continue;
+                    const uint64_t xl = ((const uint64_t*)c.data)[2 * (uint64_t)i];
+                    const uint64_t xh = ((const uint64_t*)c.data)[2 * (uint64_t)i + 1];
+                    const int cmp = xh != hi ? ((int64_t)xh < (int64_t)hi ? -1 : 1) : (xl == lo ? 0 : (xl < lo ? -1 : 1));
+                    if (!any || (mn ? cmp < 0 : cmp > 0)) {
+                        lo = xl;
+                        hi = xh;
+                        any = 1;
+                    }
+                }
+                g[op.off] = lo;
+                g[op.off + 1] = hi;
+                g[op.off + 2] = any;
             }
-            g[op.off] = (uint64_t)__double_as_longlong(acc);
         }
     }
 }
@@ -702,6 +743,9 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
                 } else if (e.in_type == VR_INT64) {
                     bool has = in && ((g[a.flags_off] >> e.bit) & 1ull);
                     put_i64(o0, o, has, has ? (int64_t)(s[0] ^ 0x8000000000000000ull) : 0, in);
+                } else if (e.in_type == VR_DECIMAL128) {
+                    const bool has = in && s[2] != 0;  // Decimal128MinMax.writeTo: null without a value
+                    put_d128(o0, o, has, has ? s[0] : 0, has ? s[1] : 0, in);
                 } else {
                     uint64_t code = in ? s[0] : 0;
                     bool has = in && code != (isMin ? ~0ull : 0ull);
@@ -755,6 +799,14 @@ struct AggImpl {
     int64_t rows_seen = 0;
     int32_t epoch = 0;
     bool has_f64_minmax = false;
+    // Every update records its rows' slots and folds some states in row order after the kernel
+    // (ordered_fold_kernel): the deterministic mode, or a 128-bit MIN/MAX.
+    bool ordered = false;
+    // Integer and decimal sums are kept as sums of 32-bit pieces in 64-bit words: exact until one
+    // group has folded 2^32 rows.  The handle refuses an update past that many rows since its
+    // creation or last reset (the caller emits and resets, or runs the CPU kernels).
+    bool half_sums = false;
+    uint64_t rows_since_reset = 0;
 
     std::vector<HostDictionary> dicts;
     Staging staging;
@@ -907,6 +959,7 @@ static vr_status compile_spec(AggImpl& h) {
     };
     add_op(OP_ROWS, -1, 1);
     int min_max_i64 = 0;
+    uint32_t checked_sums = 0;
     h.emits.clear();
     h.out_types.clear();
     for (auto& d : h.aggs) {
@@ -965,6 +1018,7 @@ static vr_status compile_spec(AggImpl& h) {
             case VR_AGG_SUM_LONG:
                 if (in_type != VR_INT32 && in_type != VR_INT64) return fail(VR_ERR_UNSUPPORTED, "sum over a non-integral lane");
                 k = add_op(OP_SUM_I64, d.input, 2);
+                if (h.det && (d.flags & VR_AGG_CHECKED) && k < 32) checked_sums |= 1u << k;
                 nulls();
                 h.out_types.push_back(VR_INT64);
                 break;
@@ -1007,6 +1061,9 @@ static vr_status compile_spec(AggImpl& h) {
                 } else if (in_type == VR_FLOAT64) {
                     k = add_op(mn ? OP_MIN_F64 : OP_MAX_F64, d.input, 4);
                     h.has_f64_minmax = true;
+                } else if (in_type == VR_DECIMAL128) {
+                    k = add_op(mn ? OP_MIN_D128 : OP_MAX_D128, d.input, 3);
+                    h.ordered = true;
                 } else {
                     return fail(VR_ERR_UNSUPPORTED, "min/max over this lane runs on the CPU kernels");
                 }
@@ -1079,6 +1136,7 @@ static vr_status compile_spec(AggImpl& h) {
     }
     h.stride = off;
     p.det = h.det ? 1 : 0;
+    p.checked_sums = checked_sums;
     p.filter_col = -1;
     return VR_OK;
 }
@@ -1559,7 +1617,7 @@ static_assert(PA_CHUNK > 2 * PA_GAP, "a share outlasts a tile's straddling gap")
 static bool partitioned_applies(const AggImpl& h, int64_t rows, RecLayout* L, const vr_launch_config* cfg) {
     const int32_t mode = (cfg && cfg->partitioned) ? cfg->partitioned : h.cfg.partitioned;
     if (mode < 0 || (mode == 0 && rows < h.part_min_rows)) return false;
-    if (h.det || h.proto.ks.mode == KM_WIDE || rows <= 0) return false;
+    if (h.ordered || h.proto.ks.mode == KM_WIDE || rows <= 0) return false;
     // block numbers of the record pool fit a merge list entry; one chain per lane of a merge wave
     const uint64_t cus = (uint64_t)h.dev->num_cus;
     if (pool_blocks(rows, cus, 1ull << PART_MAX_PBITS) >= (1ull << PB_ID_BITS)) return false;
@@ -1798,6 +1856,7 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     h.dev->count(&vr_stats::partitioned_updates, 1);
     if (specialized) h.dev->count(&vr_stats::specialized_updates, 1);
     h.rows_seen += rows;
+    h.rows_since_reset += (uint64_t)rows;
     h.order_count = -1;
     h.dev->count(&vr_stats::kernel_launches, launches);
     return VR_OK;
@@ -1820,10 +1879,10 @@ static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, 
     a.total_tiles = (n + T - 1) / T;
     a.epoch = ++h.epoch;
     int launches = 0;
-    if (h.det || n <= GUARD_ROWS) {
-        // the deterministic mode records each row's slot: one launch, no growth inside it
+    if (h.ordered || n <= GUARD_ROWS) {
+        // an ordered fold needs each row's slot: one launch, no growth inside it
         VR_TRY(ensure_capacity(h, n, s));
-        if (h.det) {
+        if (h.ordered) {
             VR_TRY(h.row_slot.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
             a.row_slot = h.row_slot.as<uint32_t>();
         }
@@ -1837,7 +1896,7 @@ static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, 
     } else {
         VR_TRY(launch_guarded<false>(h, a, grid, block, lds_bytes, s, &launches));
     }
-    if (h.det && n > 0) {
+    if (h.ordered && n > 0) {
         // order the batch's rows by slot (stable: row order kept inside a slot), then fold
         VR_TRY(h.sort_keys.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
         VR_TRY(h.sort_vals.ensure(h.dev, (size_t)n * sizeof(uint32_t)));
@@ -1864,6 +1923,7 @@ static vr_status run_update(AggImpl& h, BatchDesc& b, int64_t n, hipStream_t s, 
         launches++;
     }
     h.rows_seen += n;
+    h.rows_since_reset += (uint64_t)n;
     h.order_count = -1;
     h.dev->count(&vr_stats::kernel_launches, launches);
     return VR_OK;
@@ -1971,10 +2031,13 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     if (spec->num_values > 0) h.value_types.assign(spec->value_types, spec->value_types + spec->num_values);
     if (spec->num_aggs > 0) h.aggs.assign(spec->aggs, spec->aggs + spec->num_aggs);
     h.det = spec->deterministic != 0;
+    h.ordered = h.det;
     if (spec->max_load_percent < 0 || spec->max_load_percent > 90)
         return fail(VR_ERR_INVALID_ARGUMENT, "max_load_percent outside [0, 90]");
     if (spec->max_load_percent) h.max_load = spec->max_load_percent / 100.0;
     VR_TRY(compile_spec(h));
+    for (int k = 0; k < h.proto.nops; k++)
+        if (h.proto.ops[k].code == OP_SUM_I64 || h.proto.ops[k].code == OP_SUM_D128) h.half_sums = true;
     h.dicts.resize(h.key_types.size());
     VR_TRY(init_table(h, h.ungrouped ? 1 : spec->capacity_hint));
     if (h.ungrouped) {
@@ -1985,13 +2048,23 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     return VR_OK;
 }
 
+// The bound of the 32-bit-piece sums: `rows` more rows must keep every group below 2^32 rows.
+static vr_status check_row_bound(const AggImpl& h, uint64_t rows) {
+    if (h.half_sums && h.rows_since_reset + rows >= (1ull << 32))
+        return fail(VR_ERR_UNSUPPORTED, "an aggregate with integer or decimal sums takes fewer than 2^32 rows between "
+                                        "resets (emit and reset the handle, or run the CPU kernels)");
+    return VR_OK;
+}
+
 vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* values, int64_t n,
                         const uint8_t* selection, void* stream) {
     if (!agg || (!keys && !agg->h.ungrouped)) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
     AggImpl& h = agg->h;
     if (n == 0) return VR_OK;
-    if ((uint64_t)n >= (1ull << 32) && h.det) return fail(VR_ERR_UNSUPPORTED, "deterministic batches are below 2^32 rows");
+    if ((uint64_t)n >= (1ull << 32) && h.ordered)
+        return fail(VR_ERR_UNSUPPORTED, "deterministic batches and batches of a 128-bit MIN/MAX are below 2^32 rows");
+    VR_TRY(check_row_bound(h, (uint64_t)n));
     hipStream_t s = h.dev->pick(stream);
     VR_TRY(enter(h, s));
     vr_column zk;
@@ -2035,6 +2108,12 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     const int nk = (int)h.key_types.size(), nv = (int)h.value_types.size();
     if (bs->filter_col >= nv || (bs->filter_col >= 0 && h.value_types[bs->filter_col] != VR_INT32))
         return fail(VR_ERR_INVALID_ARGUMENT, "the filter column must be an INT32 value column");
+    uint64_t all_rows = 0;
+    for (int b = 0; b < nb; b++) {
+        if (bs->lengths[b] < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative batch length");
+        all_rows += (uint64_t)bs->lengths[b];
+    }
+    VR_TRY(check_row_bound(h, all_rows));
     hipStream_t s = h.dev->pick(stream);
     vr_batch_set with_keys;
     std::vector<vr_column> zkeys;
@@ -2050,8 +2129,8 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
         bs = &with_keys;
     }
     // One persistent launch needs device-resident columns and one dictionary per UTF8 key; anything
-    // else (and the deterministic mode, which orders each batch) takes the per-batch path.
-    bool fused = !h.det;
+    // else (and an ordered fold, which orders each batch) takes the per-batch path.
+    bool fused = !h.ordered;
     for (int b = 0; b < nb && fused; b++) {
         for (int c = 0; c < nk; c++) {
             const vr_column& k = bs->keys[(size_t)b * nk + c];
@@ -2148,6 +2227,7 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
         launches++;
     }
     h.rows_seen += rows;
+    h.rows_since_reset += (uint64_t)rows;
     h.order_count = -1;
     h.dev->count(&vr_stats::kernel_launches, launches);
     h.dev->count(&vr_stats::calls, 1);
@@ -2331,6 +2411,7 @@ vr_status vr_agg_reset(vr_agg* agg) {
     }
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
     h.groups_upper = 0;
+    h.rows_since_reset = 0;
     if (h.ungrouped) VR_TRY(insert_single_group(h, s));
     VR_TRY_HIP(hipStreamSynchronize(s));
     h.order_count = -1;

@@ -56,7 +56,11 @@ enum OpCode : int32_t {
     OP_MAX_I64 = 8,
     OP_MIN_F64 = 9,
     OP_MAX_F64 = 10,
-    OP_FIRST_ROW = 11
+    OP_FIRST_ROW = 11,
+    // 128-bit MIN/MAX (lo, hi, any): no 128-bit atomics on this part, so they are folded after the
+    // update in row order over the rows sorted by slot (ordered_fold_kernel), never per row
+    OP_MIN_D128 = 12,
+    OP_MAX_D128 = 13
 };
 
 // counters[] words
@@ -139,6 +143,8 @@ struct UpdateArgs {
     int32_t filter_col;    // INT32 value column filtered by value < filter_threshold, -1: none
     int32_t filter_threshold;
     int32_t det;           // deterministic mode: double sums folded by the ordered pass
+    uint32_t checked_sums; // deterministic mode: bit k = op k is a checked bigint sum, whose row-order
+                           // prefixes the ordered pass checks (Math.addExact per row)
     int32_t num_batches;   // 1: `one` below; >1: `batches` (device array)
     int32_t epoch;         // update sequence number: the NaN flag holds the last epoch that saw one
     int32_t pad;
@@ -542,6 +548,8 @@ __device__ __forceinline__ void apply_one(const UpdateArgs& a, const Op& op, con
                 if (*w > gi) atomic_min_u64(w, gi);
                 break;
             }
+            case OP_MIN_D128:
+            case OP_MAX_D128: break;  // folded in row order after the update
         }
     }
 }
@@ -634,6 +642,8 @@ __device__ __forceinline__ void flush_entry(const UpdateArgs& a, const uint64_t*
             case OP_FIRST_ROW:
                 if (l[0] < *w) atomic_min_u64(w, l[0]);
                 break;
+            case OP_MIN_D128:
+            case OP_MAX_D128: break;  // folded in row order after the update
         }
     }
 }
