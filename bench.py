@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BATCH = 65536
+# the uneven batched workload's columns: filter int32, keys int32 + int64 + int32 dictionary code,
+# decimal(15,2) as int64, double (every row's line is read at 30% selectivity)
+UNEVEN_BYTES_PER_ROW = 4 + 4 + 8 + 4 + 8 + 8
 # HBM bytes per launch of the batched update kernel on the 200M-row group-by, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a 1 GiB device copy
 # (profiles/agg_update_groupby200M_pmc_traffic.md).  Measured for the kernel as committed with it.
@@ -343,9 +346,18 @@ def run_uneven(dev, args):
             agg.update_prepared(prep, n, 0, thr, stream=stream)
         torch.cuda.synchronize()
 
+    graph = [None]
+
+    def graphed():  # the captured step: one HIP graph of the table's reset and the update
+        if graph[0] is None:
+            graph[0] = agg.capture(batches, max_groups=int(passing * 0.8), filter_col=0, threshold=thr,
+                                   config=tuned_cfg)
+        graph[0].launch(stream)
+        torch.cuda.synchronize()
+
     groups = {}
     for name, fn in (("batched_tuned", lambda: batched(tuned_cfg)), ("batched_default", lambda: batched(default_cfg)),
-                     ("plain", plain)):
+                     ("plain", plain), ("graph", graphed)):
         fn()  # warm (and sizes the table)
         ts = []
         for _ in range(max(1, args.uneven_steps)):
@@ -356,9 +368,28 @@ def run_uneven(dev, args):
         groups[name] = agg.num_groups()
         res[name] = dict(ms=dt * 1e3, rows_per_s=total / dt)
         log("uneven %s: %.1f ms, %.3e rows/s, %d groups" % (name, dt * 1e3, total / dt, groups[name]))
+    # the captured step's device time, with HIP events on its stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(1, args.uneven_steps)
+    e0.record(stream)
+    for _ in range(reps):
+        graph[0].launch(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dev_ms = e0.elapsed_time(e1) / reps
+    alg = total * UNEVEN_BYTES_PER_ROW
+    achieved = alg / (dev_ms * 1e-3) / 1e9
+    res["graph"].update({"device_ms": dev_ms, "captured": True})
+    res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "agg_update_kernel<true> in the captured step (table reset + update)",
+                       "algorithmic_bytes": alg, "bytes_per_row": UNEVEN_BYTES_PER_ROW}
+    log("uneven graph step: %.1f ms device, %.1f GB/s = %.1f%% of HBM" % (dev_ms, achieved, 100 * achieved / HBM_PEAK_GBS))
+    res["graph_over_plain"] = res["graph"]["rows_per_s"] / res["plain"]["rows_per_s"]
     res["batched_over_plain"] = res["batched_tuned"]["rows_per_s"] / res["plain"]["rows_per_s"]
     res["groups"] = groups["plain"]
     res["paths_agree"] = len(set(groups.values())) == 1
+    graph[0].close()
     agg.close()
     del batches, prepared, cols
     torch.cuda.empty_cache()
@@ -912,6 +943,9 @@ def main():
     import torch
     from vecruntime_amd import vr
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    # every leg on one side stream: the library's calls and the HIP events that time them share it
+    # (torch's legacy default stream is not ordered with the library's non-blocking streams)
+    torch.cuda.set_stream(torch.cuda.Stream())
     dev = vr.Device(0)
     n = args.rows
     if args.skip_groupby:  # the other legs alone, for profiling them; no headline line

@@ -261,6 +261,24 @@ vr_status vr_agg_set_launch_config(vr_agg* agg, const vr_launch_config* cfg);
 
 vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* batches,
                                 const vr_launch_config* cfg, void* stream);
+/* A captured batched update (no counterpart in the reference, whose per-batch loop is
+ * VectorGroupedAggregateIterator.consume, VectorHashAggregateExec.scala:584-602): one HIP graph
+ * of a whole step over a fixed set of device-resident batches -- the table cleared, every batch
+ * folded by one persistent launch (the set's filter fused), the NaN-payload pass -- replayed by
+ * vr_agg_graph_launch with no host round trip and no allocation.  The caller refills the batches'
+ * buffers between replays as it likes; after a replay, vr_agg_num_groups / vr_agg_emit read the
+ * step's groups as after any update.  The table is sized for `max_groups` groups when the graph
+ * is captured and nothing grows inside it: a step with more groups than the table holds sets the
+ * table-full error, which the next read reports as VR_ERR_OOM.  A replay after the handle's table
+ * moved (an update that grew it) returns VR_ERR_STATE: capture again.  A graph may outlive its
+ * handle: its launches then return VR_ERR_STATE and it is still destroyed with
+ * vr_agg_graph_destroy.  Not for deterministic specs, 128-bit MIN/MAX or ungrouped aggregates
+ * (VR_ERR_UNSUPPORTED); UTF8 keys need one dictionary per key over the set. */
+typedef struct vr_agg_graph vr_agg_graph;
+vr_status vr_agg_graph_create(vr_agg* agg, const vr_batch_set* batches, const vr_launch_config* cfg,
+                              int64_t max_groups, vr_agg_graph** out);
+vr_status vr_agg_graph_launch(vr_agg_graph* graph, void* stream);
+vr_status vr_agg_graph_destroy(vr_agg_graph* graph);
 /* Measurement aid (no reference counterpart): while timing is on, every update-kernel launch of the
  * handle is bracketed by HIP events on its stream; vr_agg_kernel_time waits for them and returns
  * the summed device time and the launch count since timing was switched on or last read. */

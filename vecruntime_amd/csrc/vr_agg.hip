@@ -343,6 +343,9 @@ __device__ __forceinline__ void nan_payload_rows(const UpdateArgs& a, const Batc
             if (slot == ~0ull) {
                 bool ins;
                 slot = row_slot_of(a, b, i, &ins, false);
+                // a key the table does not hold went to the spare row when its region was full
+                // (the update then fails with the table-full error)
+                if (slot == ~0ull) slot = a.t.slots + 2;
             }
             uint64_t* w = a.t.state + slot * (uint64_t)a.t.stride + op.off;
             if (w[2] == (uint64_t)(b.row_base + i)) w[3] = bits;
@@ -2005,6 +2008,31 @@ using namespace vr;
 struct vr_agg {
     DeviceRef owner;  // destroyed after the state below
     AggImpl h;
+    std::vector<vr_agg_graph*> graphs;  // live graphs captured from this handle (detached at destroy)
+};
+
+// A captured batched update: a HIP graph of one whole step over a fixed batch set (the table
+// cleared, every batch folded with one unguarded persistent launch, the NaN-payload pass), with the
+// device copies of the batch descriptors and key dictionary maps it reads.
+struct vr_agg_graph {
+    DeviceRef owner;     // destroyed after the buffers below
+    vr_agg* agg = nullptr;  // null once the aggregate is destroyed: launches are then refused
+    hipEvent_t done = nullptr;  // recorded after each launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf descs, remaps;
+    int64_t rows = 0;
+    // the table the graph was captured against: a replay after the table moved is refused
+    void *cells = nullptr, *state = nullptr, *counters = nullptr;
+    uint64_t slots = 0;
+    ~vr_agg_graph() {
+        if (done) {
+            (void)hipEventSynchronize(done);  // a replay may still read the graph's buffers
+            (void)hipEventDestroy(done);
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 extern "C" {
@@ -2418,6 +2446,166 @@ vr_status vr_agg_reset(vr_agg* agg) {
     return leave(h, s);
 }
 
+vr_status vr_agg_graph_create(vr_agg* agg, const vr_batch_set* bs, const vr_launch_config* cfg, int64_t max_groups,
+                              vr_agg_graph** out) {
+    if (!agg || !bs || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    AggImpl& h = agg->h;
+    const int nb = bs->num_batches;
+    if (nb <= 0) return fail(VR_ERR_INVALID_ARGUMENT, "a graph needs at least one batch");
+    if (max_groups <= 0) return fail(VR_ERR_INVALID_ARGUMENT, "max_groups must be positive");
+    if (h.ordered) return fail(VR_ERR_UNSUPPORTED, "updates that fold in row order after a sort are not captured");
+    if (h.ungrouped) return fail(VR_ERR_UNSUPPORTED, "ungrouped aggregates are not captured");
+    const int nk = (int)h.key_types.size(), nv = (int)h.value_types.size();
+    if (bs->filter_col >= nv || (bs->filter_col >= 0 && h.value_types[bs->filter_col] != VR_INT32))
+        return fail(VR_ERR_INVALID_ARGUMENT, "the filter column must be an INT32 value column");
+    uint64_t all_rows = 0;
+    for (int b = 0; b < nb; b++) {
+        if (bs->lengths[b] < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative batch length");
+        all_rows += (uint64_t)bs->lengths[b];
+    }
+    if (h.half_sums && all_rows >= (1ull << 32))
+        return fail(VR_ERR_UNSUPPORTED, "an aggregate with integer or decimal sums takes fewer than 2^32 rows per step");
+    for (int b = 0; b < nb; b++) {
+        for (int c = 0; c < nk; c++) {
+            const vr_column& k = bs->keys[(size_t)b * nk + c];
+            if (k.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "a captured update reads device-resident batches");
+            if (k.type == VR_UTF8 && (!k.dictionary || !bs->keys[c].dictionary || k.dictionary->data != bs->keys[c].dictionary->data))
+                return fail(VR_ERR_UNSUPPORTED, "a captured update takes one dictionary per UTF8 key");
+        }
+        for (int c = 0; c < nv; c++)
+            if (bs->values[(size_t)b * nv + c].mem != VR_MEM_DEVICE)
+                return fail(VR_ERR_UNSUPPORTED, "a captured update reads device-resident batches");
+    }
+    std::unique_ptr<vr_agg_graph> g(new vr_agg_graph());
+    g->owner.hold(agg->owner.dev);
+    g->agg = agg;
+    hipStream_t s = h.dev->stream;
+    VR_TRY(enter(h, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    // the table sized for max_groups up front: nothing grows inside the graph (a step with more
+    // groups than the table holds sets the table-full error, reported by the next read as OOM)
+    const uint64_t want = next_pow2((uint64_t)((double)max_groups / h.max_load) + 1);
+    if (want > h.slots) VR_TRY(grow_table(h, want, s));
+    UpdateArgs a = h.proto;
+    a.filter_col = bs->filter_col;
+    a.filter_threshold = bs->filter_threshold;
+    int grid, block;
+    size_t lds_bytes;
+    a.total_tiles = 1;
+    launch_shape(h, cfg, 0, a, &grid, &block, &lds_bytes, true);
+    const int64_t T = (int64_t)block * a.rows_per_thread;
+    std::vector<BatchDesc> descs((size_t)nb);
+    std::vector<std::vector<int32_t>> ids;
+    int64_t tiles = 0, rows = 0;
+    for (int b = 0; b < nb; b++) {
+        StagePlan plan;
+        int32_t mem;
+        const int64_t n = bs->lengths[b];
+        VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, n, nullptr, descs[b], plan, ids, &mem));
+        descs[b].row_base = rows;  // every replay starts from a cleared table
+        descs[b].tile_begin = tiles;
+        tiles += (n + T - 1) / T;
+        rows += n;
+    }
+    // the dictionary maps the descriptors point at, copied to the graph (the handle's own map of a
+    // key changes when another dictionary is mapped)
+    size_t remap_bytes = 0;
+    for (int c = 0; c < nk; c++)
+        if (h.key_types[c] == VR_UTF8) remap_bytes += align_up((size_t)descs[0].keys[c].remap_len * 4, 256);
+    if (remap_bytes) {
+        VR_TRY(g->remaps.ensure(h.dev, remap_bytes));
+        size_t off = 0;
+        for (int c = 0; c < nk; c++) {
+            if (h.key_types[c] != VR_UTF8) continue;
+            int32_t* dst = (int32_t*)(g->remaps.as<uint8_t>() + off);
+            VR_TRY_HIP(hipMemcpyAsync(dst, descs[0].keys[c].remap, (size_t)descs[0].keys[c].remap_len * 4,
+                                      hipMemcpyDeviceToDevice, s));
+            for (int b = 0; b < nb; b++) descs[b].keys[c].remap = dst;
+            off += align_up((size_t)descs[0].keys[c].remap_len * 4, 256);
+        }
+    }
+    VR_TRY(g->descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
+    VR_TRY_HIP(hipMemcpyAsync(g->descs.p, descs.data(), sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
+    VR_TRY_HIP(hipStreamSynchronize(s));
+    const int cap = h.dev->num_cus * std::max(1, (cfg && cfg->blocks_per_cu) ? cfg->blocks_per_cu : h.cfg.blocks_per_cu);
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+    a.batches = g->descs.as<BatchDesc>();
+    a.num_batches = nb;
+    a.tile_lo = 0;
+    a.total_tiles = tiles;
+    a.epoch = ++h.epoch;
+    a.t = view_of(h);
+    // capture: every node a kernel (no memset nodes), no read-back, no allocation
+    hipStream_t cs = nullptr;
+    VR_TRY_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const int W = h.proto.ks.width;
+        const int fcap = h.dev->num_cus * 16;
+        const uint64_t empty = h.proto.ks.mode == KM_WIDE ? 0ull : EMPTY_KEY;
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(h.slots * W, 256, fcap)), dim3(256), 0, cs, h.cells.as<uint64_t>(),
+                           (uint64_t)(h.slots * W), empty);
+        UpdateArgs ia = h.proto;
+        ia.t.stride = h.stride;
+        hipLaunchKernelGGL(init_state_kernel, dim3(grid_for((h.slots + 3 + 63) / 64 * 64, 256, fcap)), dim3(256), 0, cs,
+                           h.state.as<uint64_t>(), (uint64_t)(h.slots + 3), ia);
+        const uint64_t rg_words = ((1ull << h.pbits) + 1) / 2;
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(rg_words, 256, 64)), dim3(256), 0, cs, h.rgroups.as<uint64_t>(),
+                           rg_words, 0ull);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, cs, h.counters.as<uint64_t>(), (uint64_t)C_WORDS, 0ull);
+        if (rows > 0) hipLaunchKernelGGL(agg_update_kernel<true>, dim3(grid), dim3(block), lds_bytes, cs, a);
+        if (rows > 0 && h.has_f64_minmax)
+            hipLaunchKernelGGL(nan_payload_kernel<true>, dim3(grid), dim3(block), 0, cs, a);
+        hipError_t le = hipGetLastError();
+        e = hipStreamEndCapture(cs, &g->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return hip_fail(e, "capturing the batched update");
+    VR_TRY_HIP(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+    g->rows = rows;
+    g->cells = h.cells.p;
+    g->state = h.state.p;
+    g->counters = h.counters.p;
+    g->slots = h.slots;
+    VR_TRY(leave(h, s));
+    agg->graphs.push_back(g.get());
+    *out = g.release();
+    return VR_OK;
+}
+
+vr_status vr_agg_graph_launch(vr_agg_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(VR_ERR_INVALID_ARGUMENT, "null graph");
+    if (!g->agg) return fail(VR_ERR_STATE, "the aggregate the graph was captured from was destroyed");
+    AggImpl& h = g->agg->h;
+    if (h.cells.p != g->cells || h.state.p != g->state || h.counters.p != g->counters || h.slots != g->slots)
+        return fail(VR_ERR_STATE, "the aggregate's table moved since the graph was captured (capture it again)");
+    hipStream_t s = h.dev->pick(stream);
+    VR_TRY(enter(h, s));
+    VR_TRY_HIP(hipGraphLaunch(g->exec, s));
+    VR_TRY_HIP(hipEventRecord(g->done, s));
+    // the host's view of the table after the step: cleared, then folded over the graph's rows
+    h.groups_upper = g->rows;
+    h.rows_seen = g->rows;
+    h.rows_since_reset = (uint64_t)g->rows;
+    h.order_count = -1;
+    h.dev->count(&vr_stats::calls, 1);
+    h.dev->count(&vr_stats::input_rows, g->rows);
+    h.dev->count(&vr_stats::kernel_launches, 1);  // one graph launch
+    return leave(h, s);
+}
+
+vr_status vr_agg_graph_destroy(vr_agg_graph* g) {
+    if (!g) return VR_OK;
+    if (g->agg) {
+        std::vector<vr_agg_graph*>& live = g->agg->graphs;
+        live.erase(std::remove(live.begin(), live.end(), g), live.end());
+    }
+    delete g;
+    return VR_OK;
+}
+
 vr_status vr_agg_set_timing(vr_agg* agg, int32_t enable) {
     if (!agg) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     agg->h.timing = enable != 0;
@@ -2445,6 +2633,7 @@ vr_status vr_agg_destroy(vr_agg* agg) {
     if (!agg) return VR_OK;
     (void)hipStreamSynchronize(agg->h.dev->stream);
     if (agg->h.last_ev) (void)hipEventSynchronize(agg->h.last_ev);  // work queued on a caller's stream
+    for (vr_agg_graph* g : agg->graphs) g->agg = nullptr;  // a graph outliving its aggregate stays destroyable
     delete agg;
     return VR_OK;
 }

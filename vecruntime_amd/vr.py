@@ -48,6 +48,9 @@ _sig("vr_agg_update_batches", [_P, C.POINTER(abi.VrBatchSet), C.POINTER(abi.VrLa
 _sig("vr_agg_num_groups", [_P, C.POINTER(C.c_int64)])
 _sig("vr_agg_set_launch_config", [_P, C.POINTER(abi.VrLaunchConfig)])
 _sig("vr_agg_set_timing", [_P, C.c_int32])
+_sig("vr_agg_graph_create", [_P, C.POINTER(abi.VrBatchSet), C.POINTER(abi.VrLaunchConfig), C.c_int64, C.POINTER(_P)])
+_sig("vr_agg_graph_launch", [_P, _P])
+_sig("vr_agg_graph_destroy", [_P])
 _sig("vr_agg_kernel_time", [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)])
 _sig("vr_agg_output_layout", [_P, C.POINTER(C.c_int32), _P, C.c_int32])
 _sig("vr_agg_emit", [_P, C.c_int64, C.c_int64, _P, _P, _P])
@@ -102,17 +105,33 @@ def check(st):
 def _stream(s):
     """A HIP stream handle: an int, a torch stream, or None.
 
-    None is torch's current stream once torch has initialized the device (else the handle's own
-    stream).  Work on torch's stream is ordered with the work that filled the input tensors and with
-    whatever torch's caching allocator later gives their memory to, so a call may return while its
-    kernels still read the tensors and the caller may drop them at once."""
+    None is torch's current stream once torch has initialized the device, so that the call is
+    ordered with the work that filled its input tensors and with whatever torch's caching allocator
+    later gives their memory to.  torch's legacy default stream has handle 0, which the library
+    takes for the handle's own (non-blocking) stream: that stream is not ordered with the legacy
+    one, so the call first waits for torch's work and the wrapper synchronizes after a call that
+    returns with work queued (_settle).  Without torch, None is the handle's own stream."""
     h = s if s is None or isinstance(s, int) else s.cuda_stream
     if not h:
         t = sys.modules.get("torch")
         if t is not None and t.cuda.is_initialized():
-            return t.cuda.current_stream().cuda_stream or None
+            cur = t.cuda.current_stream()
+            if cur.cuda_stream:
+                return cur.cuda_stream
+            cur.synchronize()
         return None
     return h
+
+
+def _settle(device_h, s):
+    """After a call that may return with kernels queued: when it ran on the handle's own stream
+    while torch uses its legacy default stream, wait for it (the caller may drop the inputs)."""
+    h = s if s is None or isinstance(s, int) else s.cuda_stream
+    if h:
+        return
+    t = sys.modules.get("torch")
+    if t is not None and t.cuda.is_initialized() and not t.cuda.current_stream().cuda_stream:
+        check(_lib.vr_device_sync(device_h, None))
 
 
 def _ptr(a):
@@ -180,6 +199,7 @@ class GroupedAggregate:
     def update(self, keys, values, n, selection=None, stream=None):
         ka, va = c_array(keys), c_array(values)
         check(_lib.vr_agg_update(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), _ptr(selection), _stream(stream)))
+        _settle(self.device.h, stream)
 
     @staticmethod
     def prepare(keys, values):
@@ -190,11 +210,13 @@ class GroupedAggregate:
     def update_prepared(self, prep, n, filter_col=-1, threshold=0, stream=None):
         """vr_agg_update_filtered over prepare()d columns (filter_col -1: no filter)."""
         check(_lib.vr_agg_update_filtered(self.h, prep[2], prep[3], int(n), filter_col, threshold, _stream(stream)))
+        _settle(self.device.h, stream)
 
     def update_filtered(self, keys, values, n, filter_col, threshold, stream=None):
         ka, va = c_array(keys), c_array(values)
         check(_lib.vr_agg_update_filtered(self.h, C.cast(ka, _P), C.cast(va, _P), int(n), filter_col, threshold,
                                           _stream(stream)))
+        _settle(self.device.h, stream)
 
     @staticmethod
     def prepare_batches(batches, filter_col=-1, threshold=0):
@@ -207,6 +229,17 @@ class GroupedAggregate:
                             ln.ctypes.data_as(C.POINTER(C.c_int64)), filter_col, threshold)
         return PreparedBatches(bs, (ka, va, ln, batches))
 
+    def capture(self, batches, max_groups, filter_col=-1, threshold=0, config=None):
+        """vr_agg_graph_create: an AggGraph replaying one whole step (table cleared, every batch
+        folded) over `batches` (list of (keys, values, n) with device columns, or a prepare_batches()
+        result)."""
+        if not isinstance(batches, PreparedBatches):
+            batches = self.prepare_batches(batches, filter_col, threshold)
+        cfg = config if config is not None else abi.launch_config()
+        h = _P()
+        check(_lib.vr_agg_graph_create(self.h, C.byref(batches.bs), C.byref(cfg), int(max_groups), C.byref(h)))
+        return AggGraph(self, h, batches)
+
     def update_batches(self, batches, filter_col=-1, threshold=0, config=None, stream=None):
         """batches: list of (keys, values, n), or a prepare_batches() result (which carries its own
         filter).  One persistent launch when every column is device resident."""
@@ -214,6 +247,7 @@ class GroupedAggregate:
             batches = self.prepare_batches(batches, filter_col, threshold)
         cfg = config if config is not None else abi.launch_config()
         check(_lib.vr_agg_update_batches(self.h, C.byref(batches.bs), C.byref(cfg), _stream(stream)))
+        _settle(self.device.h, stream)
         self._keep = [batches]
 
     def set_timing(self, on):
@@ -251,6 +285,7 @@ class GroupedAggregate:
             outs = [OutColumn(t, m) for t in self.output_types()]
         ka, oa = c_array(keys, abi.VrOutColumn), c_array(outs, abi.VrOutColumn)
         check(_lib.vr_agg_emit(self.h, start, stop, C.cast(ka, _P), C.cast(oa, _P), _stream(stream)))
+        _settle(self.device.h, stream)
         return keys, outs
 
     def key_dictionary(self, c):
@@ -394,6 +429,28 @@ class BuildTable:
         return pidx[:r], bidx[:r], outputs, r
 
 
+class AggGraph:
+    """A captured batched update of a GroupedAggregate (vr_agg_graph_*); keeps its batches alive."""
+
+    def __init__(self, agg, h, batches):
+        self.agg, self.h, self._batches = agg, h, batches
+
+    def launch(self, stream=None):
+        check(_lib.vr_agg_graph_launch(self.h, _stream(stream)))
+        _settle(self.agg.device.h, stream)
+
+    def close(self):
+        if self.h:
+            _lib.vr_agg_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class ProbeBatches:
     """The streamed batches of one vr_join_probe_batches call, marshalled once: `batches` is a list
     of (columns, n) or (columns, n, selection) with device-resident columns, every batch with the
@@ -452,12 +509,14 @@ def filter_compare(device, col, op, value, n, selection=None, out=None, stream=N
     cnt = C.c_int64()
     check(_lib.vr_filter_compare(device.h, C.byref(col.c()), int(op), C.byref(sc), _ptr(selection), int(n), _ptr(out),
                                  C.byref(cnt), _stream(stream)))
+    _settle(device.h, stream)
     return out, cnt.value
 
 
 def compact_utf8_bytes(device, col, n, selection, stream=None):
     b = C.c_int64()
     check(_lib.vr_compact_utf8_bytes(device.h, C.byref(col.c()), int(n), _ptr(selection), C.byref(b), _stream(stream)))
+    _settle(device.h, stream)
     return b.value
 
 
@@ -467,18 +526,21 @@ def compact(device, columns, n, selection, outputs, stream=None):
     rows = C.c_int64()
     check(_lib.vr_compact(device.h, C.cast(ca, _P), len(columns), int(n), _ptr(selection), C.cast(oa, _P), C.byref(rows),
                           _stream(stream)))
+    _settle(device.h, stream)
     return rows.value
 
 
 def gather_utf8_bytes(device, col, idx, m, stream=None):
     b = C.c_int64()
     check(_lib.vr_gather_utf8_bytes(device.h, C.byref(col.c()), _ptr(idx), int(m), C.byref(b), _stream(stream)))
+    _settle(device.h, stream)
     return b.value
 
 
 def gather(device, col, idx, m, out, stream=None):
     """vr_gather of a device column by a device int32 index tensor into a device output column."""
     check(_lib.vr_gather(device.h, C.byref(col.c()), _ptr(idx), int(m), C.byref(out.c()), _stream(stream)))
+    _settle(device.h, stream)
 
 
 def hash_partition_ids(device, keys, kinds, n, num_partitions, seed=abi.SPARK_SEED, want_hashes=False, stream=None):
@@ -491,6 +553,7 @@ def hash_partition_ids(device, keys, kinds, n, num_partitions, seed=abi.SPARK_SE
     kd = (C.c_int32 * max(1, len(kinds)))(*kinds)
     check(_lib.vr_hash_partition_ids(device.h, C.cast(ka, _P) if ka is not None else None, C.cast(kd, _P), len(keys),
                                      int(n), int(num_partitions), int(seed), _ptr(ids), _ptr(hashes), _stream(stream)))
+    _settle(device.h, stream)
     return ids[:n], (hashes[:n] if hashes is not None else None)
 
 
@@ -501,6 +564,7 @@ def round_robin_ids(device, n, num_partitions, start, stream=None):
     nxt = C.c_int32()
     check(_lib.vr_round_robin_ids(device.h, int(n), int(num_partitions), int(start), _ptr(ids), C.byref(nxt),
                                   _stream(stream)))
+    _settle(device.h, stream)
     return ids[:n], nxt.value
 
 
@@ -513,4 +577,5 @@ def partition_rows(device, ids, n, num_partitions, want_rows=True, want_dest=Tru
     starts = torch.empty(num_partitions + 1, dtype=torch.int64, device="cuda")
     check(_lib.vr_partition_rows(device.h, _ptr(ids), int(n), int(num_partitions), _ptr(rows), _ptr(dest),
                                  _ptr(starts), _stream(stream)))
+    _settle(device.h, stream)
     return (rows[:n] if rows is not None else None), (dest[:n] if dest is not None else None), starts
