@@ -220,6 +220,11 @@ vr_status vr_agg_key_dictionary(vr_agg* agg, int32_t c, int32_t* offsets, int64_
                                 int64_t* num_bytes);
 /* Device bytes held by the handle (GroupKeyTable.memoryBytes, GroupKeyTable.java:2596). */
 vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes);
+/* Compiles ahead of time the kernels the partitioned update runs for `spec` (hipRTC, seconds per
+ * spec, otherwise paid by the spec's first large update).  No counterpart in the reference (its
+ * kernels are compiled with the JVM): a start-up aid for a server that knows the specs it will
+ * see.  Safe from several threads at once; VR_OK also when the spec takes no compiled kernels. */
+vr_status vr_agg_prewarm(vr_device* dev, const vr_agg_spec* spec);
 /* Drops every group, keeping the allocation (emit-and-reset, VectorHashAggregateExec.scala:193-246). */
 vr_status vr_agg_reset(vr_agg* agg);
 vr_status vr_agg_destroy(vr_agg* agg);

@@ -2,8 +2,9 @@
  * vr_serve.h -- continuous-batching query server of libvrgpu.so.
  *
  * Queries (a group-by over device-resident batches, or a join build + probe) are submitted from
- * any thread; the server admits them in arrival order as soon as a worker is free and the query's
- * estimated device memory fits the HBM budget, runs each admitted query on its worker's own HIP
+ * any thread; the server admits a queued query as soon as a worker is free and the query's
+ * estimated device memory fits the HBM budget (smallest input first, with aging, or in arrival
+ * order: vr_server_config.admission), runs each admitted query on its worker's own HIP
  * stream (so admitted queries overlap on the device), and keeps the query's hash table resident
  * across its batches.  A finished query's results stay in server-owned device memory, counted
  * against the budget, until vr_server_release.
@@ -25,10 +26,19 @@ extern "C" {
 
 typedef struct vr_server vr_server;
 
+enum vr_admission {
+    /* A free worker takes the queued query with the least input rows divided by
+     * (1 + its wait / aging_ms), among those whose memory fits the budget: small queries pass
+     * large ones, and a large query's claim grows with its wait, so none waits for ever. */
+    VR_ADMIT_SIZE_AWARE = 0,
+    VR_ADMIT_FIFO = 1, /* arrival order; the head query waits until its memory fits */
+};
+
 typedef struct vr_server_config {
     int32_t workers;           /* concurrent queries (each on its own stream); 0: 4 */
-    int32_t reserved;
+    int32_t admission;         /* vr_admission */
     int64_t hbm_budget_bytes;  /* device bytes admitted queries may hold at once; 0: 200 GB */
+    int64_t aging_ms;          /* VR_ADMIT_SIZE_AWARE: wait that halves a query's size; 0: 100 */
 } vr_server_config;
 
 enum vr_query_kind { VR_QUERY_AGG = 0, VR_QUERY_JOIN = 1 };

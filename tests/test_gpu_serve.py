@@ -236,3 +236,37 @@ def test_failed_query_reports_its_status(dev):
         assert st["failed"] == 1 and st["reserved_bytes"] == 0
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("admission", ["size-aware", "fifo"])
+def test_admission_order(dev, admission):
+    """One worker busy with a large group-by; a second large one is queued, then three small ones.
+    Size-aware admission (aging long enough not to matter here) starts the small ones first;
+    arrival-order admission starts the second large one first.  Every result matches the oracle."""
+    import time
+    from vecruntime_amd.serve import Server, prewarm
+    big = AggCase(11, 100_000, [2_000_000] * 3, "long")
+    smalls = [AggCase(12 + i, 500, [20_000], "long") for i in range(3)]
+    prewarm(dev, [big.spec])
+    mode = abi.ADMIT_SIZE_AWARE if admission == "size-aware" else abi.ADMIT_FIFO
+    srv = Server(dev, workers=1, admission=mode, aging_ms=10 ** 9)
+    try:
+        order = [big, big] + smalls
+        sub, tickets = [], []
+        for c in order:
+            sub.append(time.perf_counter())
+            tickets.append(c.submit(srv))
+        admitted = []
+        for c, t, at in zip(order, tickets, sub):
+            res = srv.wait(t)
+            assert res.status == abi.VR_OK
+            admitted.append(at * 1e3 + res.queue_ms)
+            compare_case(c, c.rows_of(srv, t, res), c.oracle_rows())
+            srv.release(t)
+        if admission == "size-aware":
+            assert max(admitted[2:]) < admitted[1]
+        else:
+            assert admitted[1] < min(admitted[2:])
+        assert srv.stats()["peak_inflight"] == 1
+    finally:
+        srv.close()

@@ -216,11 +216,15 @@ QUERY_AGG, QUERY_JOIN = 0, 1
 MAX_RESULT_COLUMNS = 40
 
 
+ADMIT_SIZE_AWARE, ADMIT_FIFO = 0, 1
+
+
 class VrServerConfig(C.Structure):
     _fields_ = [
         ("workers", C.c_int32),
-        ("reserved", C.c_int32),
+        ("admission", C.c_int32),
         ("hbm_budget_bytes", C.c_int64),
+        ("aging_ms", C.c_int64),
     ]
 
 

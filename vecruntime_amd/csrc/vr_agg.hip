@@ -2405,6 +2405,21 @@ vr_status vr_agg_key_dictionary(vr_agg* agg, int32_t c, int32_t* offsets, int64_
     return VR_OK;
 }
 
+vr_status vr_agg_prewarm(vr_device* dev, const vr_agg_spec* spec) {
+    if (!dev || !spec) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
+    vr_agg* agg = nullptr;
+    VR_TRY(vr_agg_create(dev, spec, &agg));
+    std::unique_ptr<vr_agg, vr_status (*)(vr_agg*)> guard(agg, vr_agg_destroy);
+    AggImpl& h = agg->h;
+    RecLayout L;
+    // an input large enough for the partitioned update; the spec decides whether it applies
+    if (h.cfg.specialize < 0 || !partitioned_applies(h, h.part_min_rows, &L, nullptr)) return VR_OK;
+    ColProg prog[VR_MAX_VALUES];
+    const int np = column_programs(h, L, prog);
+    (void)part_jit(h.dev->ordinal, prog, np, L);  // a failed compile leaves the generic kernels in use
+    return VR_OK;
+}
+
 vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
     if (!agg || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     AggImpl& h = agg->h;

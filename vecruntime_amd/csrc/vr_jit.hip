@@ -11,6 +11,7 @@
 
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -23,9 +24,17 @@ namespace vr {
 
 namespace {
 
+// One entry per device ordinal + source.  The map's lock is held only to find or add an entry;
+// the compile runs under the entry's own lock, so a compile (seconds) holds up only the callers
+// that want the same kernels, and a cached entry is returned at once.
+struct JitEntry {
+    std::mutex mu;
+    bool done = false;
+    PartJit* jit = nullptr;  // nullptr once done: the compile failed
+};
 std::mutex g_mu;
 bool g_disabled = false;
-std::map<std::string, PartJit*> g_cache;  // key: device ordinal + source; nullptr: failed
+std::map<std::string, std::shared_ptr<JitEntry>> g_cache;
 
 std::string prog_literal(const ColProg& c) {
     char b[256];
@@ -119,13 +128,20 @@ PartJit* compile(int ordinal, const std::string& src, const RecLayout& L) {
 const PartJit* part_jit(int ordinal, const ColProg* prog, int nprog, const RecLayout& L) {
     const std::string src = jit_source(prog, nprog, L);
     const std::string key = std::to_string(ordinal) + "\n" + src;
-    std::lock_guard<std::mutex> g(g_mu);
-    if (g_disabled) return nullptr;
-    auto it = g_cache.find(key);
-    if (it != g_cache.end()) return it->second;
-    PartJit* j = compile(ordinal, src, L);
-    g_cache[key] = j;
-    return j;
+    std::shared_ptr<JitEntry> e;
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+        if (g_disabled) return nullptr;
+        std::shared_ptr<JitEntry>& slot = g_cache[key];
+        if (!slot) slot = std::make_shared<JitEntry>();
+        e = slot;
+    }
+    std::lock_guard<std::mutex> g(e->mu);
+    if (!e->done) {
+        e->jit = compile(ordinal, src, L);
+        e->done = true;
+    }
+    return e->jit;
 }
 
 void part_jit_disable() {

@@ -1,7 +1,8 @@
 // Continuous-batching query server: the vr_server_* entry points of include/vr_serve.h.
 //
-// One queue, first-come first-admitted: the head query is admitted when a worker is free and its
-// estimated device bytes fit the budget (a query larger than the whole budget is admitted alone).
+// One queue.  A free worker admits the queued query whose estimated device bytes fit the budget
+// and whose input rows, divided by (1 + its wait / aging), are least (or, in FIFO mode, the head
+// query once it fits); a query larger than the whole budget is admitted alone.
 // Each worker owns a HIP stream; an admitted query runs every launch on it, so several admitted
 // queries overlap on the device.  A group-by keeps its table resident across its batches and
 // emits every group into server-owned device columns; a join builds, probes once and keeps its
@@ -45,6 +46,7 @@ struct Ticket {
     int64_t id = 0;
     std::unique_ptr<QueryCopy> qc;
     int64_t est = 0;       // bytes reserved while it runs
+    double rows = 0;       // input rows: the size admission orders by
     int64_t kept = 0;      // bytes its results hold until released
     Clock::time_point submit, admit, done;
     bool finished = false;
@@ -93,6 +95,8 @@ struct vr_server {
     vr_device* dev = nullptr;
     DeviceRef ref;
     int64_t budget = 0;
+    int32_t admission = VR_ADMIT_SIZE_AWARE;
+    double aging_ms = 100;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Ticket*> queue;
@@ -140,7 +144,13 @@ static vr_status run_agg(vr_server* srv, Ticket* t, hipStream_t s) {
     vr_agg_spec spec = c.spec;
     if (spec.capacity_hint <= 0) spec.capacity_hint = c.q.expected_groups;
     VR_TRY(vr_agg_create(srv->dev, &spec, &agg));
-    std::unique_ptr<vr_agg, vr_status (*)(vr_agg*)> guard(agg, vr_agg_destroy);
+    // on an early return the worker's stream may still run this query's kernels: wait for them
+    // before the handle's blocks go back to the shared cache
+    auto destroy = [s](vr_agg* a) {
+        (void)hipStreamSynchronize(s);
+        vr_agg_destroy(a);
+    };
+    std::unique_ptr<vr_agg, decltype(destroy)> guard(agg, destroy);
     VR_TRY(vr_agg_update_batches(agg, &c.bs, nullptr, s));
     int64_t g = 0;
     VR_TRY(vr_agg_num_groups(agg, &g));
@@ -173,7 +183,11 @@ static vr_status run_join(vr_server* srv, Ticket* t, hipStream_t s) {
     QueryCopy& c = *t->qc;
     vr_join* j = nullptr;
     VR_TRY(vr_join_create(srv->dev, &c.jspec, &j));
-    std::unique_ptr<vr_join, vr_status (*)(vr_join*)> guard(j, vr_join_destroy);
+    auto destroy = [s](vr_join* h) {  // as in run_agg: the stream's work ends before the blocks return
+        (void)hipStreamSynchronize(s);
+        vr_join_destroy(h);
+    };
+    std::unique_ptr<vr_join, decltype(destroy)> guard(j, destroy);
     VR_TRY(vr_join_build_append(j, c.build_cols.data(), c.q.build_rows, s));
     VR_TRY(vr_join_build_finish(j, s));
     int64_t cap = c.q.expected_rows > 0 ? c.q.expected_rows : c.q.probe_rows;
@@ -197,6 +211,7 @@ static vr_status run_join(vr_server* srv, Ticket* t, hipStream_t s) {
             break;
         }
         // the output outgrew the hint: size it to the rows the probe reported and probe again
+        (void)hipStreamSynchronize(s);
         free_results(srv, t);
         t->kept = 0;
         cap = out.rows;
@@ -207,20 +222,42 @@ static vr_status run_join(vr_server* srv, Ticket* t, hipStream_t s) {
     return VR_OK;
 }
 
+// The queued query a free worker admits next (srv->mu held), or queue.end() when none may start:
+// the budget must hold its bytes unless nothing runs.
+static std::deque<Ticket*>::iterator pick(vr_server* srv) {
+    auto fits = [&](const Ticket* t) { return srv->reserved + t->est <= srv->budget || srv->running == 0; };
+    if (srv->queue.empty()) return srv->queue.end();
+    if (srv->admission == VR_ADMIT_FIFO) return fits(srv->queue.front()) ? srv->queue.begin() : srv->queue.end();
+    const Clock::time_point now = Clock::now();
+    auto best = srv->queue.end();
+    double best_score = 0;
+    for (auto it = srv->queue.begin(); it != srv->queue.end(); ++it) {
+        if (!fits(*it)) continue;
+        const double score = (*it)->rows / (1.0 + ms_between((*it)->submit, now) / srv->aging_ms);
+        if (best == srv->queue.end() || score < best_score) {
+            best = it;
+            best_score = score;
+        }
+    }
+    if (best == srv->queue.end() && srv->running == 0) best = srv->queue.begin();
+    return best;
+}
+
 static void worker(vr_server* srv) {
     (void)hipSetDevice(srv->dev->d.ordinal);
     hipStream_t s = nullptr;
     (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     while (true) {
         std::unique_lock<std::mutex> lk(srv->mu);
+        std::deque<Ticket*>::iterator next;
         srv->cv_work.wait(lk, [&] {
             if (srv->queue.empty()) return srv->stopping;
-            Ticket* h = srv->queue.front();
-            return srv->reserved + h->est <= srv->budget || srv->running == 0;
+            next = pick(srv);
+            return next != srv->queue.end();
         });
         if (srv->queue.empty()) break;
-        Ticket* t = srv->queue.front();
-        srv->queue.pop_front();
+        Ticket* t = *next;
+        srv->queue.erase(next);
         srv->reserved += t->est;
         srv->running++;
         srv->stats.peak_inflight = std::max(srv->stats.peak_inflight, srv->running);
@@ -232,7 +269,11 @@ static void worker(vr_server* srv) {
             hipError_t e = hipStreamSynchronize(s);
             if (e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
         }
-        if (st != VR_OK) free_results(srv, t), t->kept = 0;
+        if (st != VR_OK) {  // kernels queued before the failure may still write the result blocks
+            (void)hipStreamSynchronize(s);
+            free_results(srv, t);
+            t->kept = 0;
+        }
         lk.lock();
         t->done = Clock::now();
         t->res.status = st;
@@ -262,6 +303,12 @@ vr_status vr_server_create(vr_device* dev, const vr_server_config* cfg, vr_serve
     int workers = cfg && cfg->workers > 0 ? cfg->workers : 4;
     if (workers > 16) workers = 16;
     srv->budget = cfg && cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (200ll << 30);
+    if (cfg && (cfg->admission < VR_ADMIT_SIZE_AWARE || cfg->admission > VR_ADMIT_FIFO || cfg->aging_ms < 0)) {
+        delete srv;
+        return fail(VR_ERR_INVALID_ARGUMENT, "admission mode or aging");
+    }
+    if (cfg) srv->admission = cfg->admission;
+    if (cfg && cfg->aging_ms > 0) srv->aging_ms = (double)cfg->aging_ms;
     for (int w = 0; w < workers; w++) srv->threads.emplace_back(worker, srv);
     *out = srv;
     return VR_OK;
@@ -317,6 +364,10 @@ vr_status vr_server_submit(vr_server* srv, const vr_query* q, int64_t* ticket) {
     }
     auto t = std::make_unique<Ticket>();
     t->est = q->kind == VR_QUERY_AGG ? estimate_agg(*qc) : estimate_join(*qc);
+    if (q->kind == VR_QUERY_AGG)
+        for (int64_t n : qc->lengths) t->rows += (double)n;
+    else
+        t->rows = (double)q->build_rows + (double)q->probe_rows;
     t->qc = std::move(qc);
     t->submit = Clock::now();
     std::lock_guard<std::mutex> lk(srv->mu);

@@ -39,11 +39,20 @@ class QueryResult:
         self.agg = r.agg
 
 
+def prewarm(device, specs, threads=4):
+    """vr_agg_prewarm for each abi.AggSpec, `threads` compiles at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+    specs = list(specs)
+    with ThreadPoolExecutor(max(1, min(threads, len(specs) or 1))) as ex:
+        for st in ex.map(lambda sp: _lib.vr_agg_prewarm(device.h, C.byref(sp.c)), specs):
+            check(st)
+
+
 class Server:
-    def __init__(self, device, workers=4, hbm_budget=0):
+    def __init__(self, device, workers=4, hbm_budget=0, admission=abi.ADMIT_SIZE_AWARE, aging_ms=0):
         self.dev = device
         self.h = _P()
-        cfg = abi.VrServerConfig(int(workers), 0, int(hbm_budget))
+        cfg = abi.VrServerConfig(int(workers), int(admission), int(hbm_budget), int(aging_ms))
         check(_lib.vr_server_create(device.h, C.byref(cfg), C.byref(self.h)))
         self._keep = {}
 

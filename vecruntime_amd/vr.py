@@ -57,6 +57,7 @@ _sig("vr_agg_emit", [_P, C.c_int64, C.c_int64, _P, _P, _P])
 _sig("vr_agg_key_dictionary", [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64),
                                C.POINTER(C.c_int64)])
 _sig("vr_agg_memory", [_P, C.POINTER(C.c_int64)])
+_sig("vr_agg_prewarm", [_P, C.POINTER(abi.VrAggSpec)])
 _sig("vr_agg_reset", [_P])
 _sig("vr_agg_destroy", [_P])
 _sig("vr_join_create", [_P, C.POINTER(abi.VrJoinSpec), C.POINTER(_P)])
