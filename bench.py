@@ -204,6 +204,20 @@ def run_groupby(dev, args, n):
         dt = (time.perf_counter() - t0) / steps
         res[name] = dict(ms_per_step=dt * 1e3, rows_per_s=n / dt, steps=steps)
         log("%s: %.2f ms/step, %.3e rows/s" % (name, dt * 1e3, n / dt))
+    # a cold table: a fresh handle with no capacity hint (2,048 slots at first, grown as groups
+    # arrive), created, updated and emitted inside the timed region
+    cold = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fresh = vr.GroupedAggregate(dev, groupby_spec(0))
+        fresh.update_batches(batch_set, stream=stream)
+        fresh.emit(0, groups, ek, eo, stream=stream)
+        torch.cuda.synchronize()
+        cold.append(time.perf_counter() - t0)
+        fresh.close()
+    res["cold"] = {"ms": min(cold) * 1e3, "rows_per_s": n / min(cold)}
+    log("cold table: %.2f ms/step" % (min(cold) * 1e3))
     # the dominant kernel, timed with HIP events the library records around each of its launches
     reps = max(1, args.steps)
     agg.set_timing(True)
@@ -220,15 +234,135 @@ def run_groupby(dev, args, n):
     res["checks"] = checks
     # a bounded sample copied to the host for the CPU baseline
     res["_sample"] = None
-    if not args.skip_cpu:
-        s_batches = min(len(batches), args.cpu_batches)
-        m = s_batches * BATCH
-        host = dict(keys=keys[:m].cpu().numpy(), dec=dec[:m].cpu().numpy(), dbl=dbl[:m].cpu().numpy(),
-                    i32=i32[:m].cpu().numpy(), valid=[v[:m // 8].cpu().numpy() for v in valid])
-        res["_sample"] = (host, s_batches)
+    res["_host"] = None
+    if not args.skip_cpu or not args.skip_pcie:
+        host = dict(keys=keys.cpu().numpy(), dec=dec.cpu().numpy(), dbl=dbl.cpu().numpy(),
+                    i32=i32.cpu().numpy(), valid=[v.cpu().numpy() for v in valid])
+        res["_host"] = host
+        if not args.skip_cpu:  # whole batches
+            res["_sample"] = (host, min(n // BATCH, args.cpu_batches))
     agg.close()
     del batches, keys, dec, dbl, i32, valid
     torch.cuda.empty_cache()
+    return res
+
+
+def host_groupby_batches(host, pinned=False):
+    """The group-by input as host-resident Arrow batches (the reference's off-heap buffers):
+    pageable numpy arrays, or page-locked copies when `pinned`."""
+    import torch
+    from vecruntime_amd import abi
+    from vecruntime_amd.columns import Column
+    src = host
+    if pinned:
+        def pin(a):
+            t = torch.empty(a.shape, dtype=getattr(torch, a.dtype.name), pin_memory=True)
+            t.numpy()[...] = a
+            return t.numpy()
+        src = dict(keys=pin(host["keys"]), dec=pin(host["dec"]), dbl=pin(host["dbl"]), i32=pin(host["i32"]),
+                   valid=[pin(v) for v in host["valid"]])
+    out = []
+    n = len(src["keys"])
+    for s0 in range(0, n, BATCH):
+        m = min(BATCH, n - s0)
+        vb = [v[s0 // 8:] for v in src["valid"]]
+        out.append(([Column(abi.INT64, src["keys"][s0:s0 + m])],
+                    [Column(abi.INT64, src["dec"][s0:s0 + m], vb[0], length=m),
+                     Column(abi.FLOAT64, src["dbl"][s0:s0 + m], vb[1], length=m),
+                     Column(abi.INT32, src["i32"][s0:s0 + m], vb[2], length=m)], m))
+    return out, src
+
+
+def run_pcie(dev, args, host, groups):
+    """End to end from host memory: the same batches fed from host-resident buffers, each staged
+    to the device by the library (pinned staging buffers, copies on its side stream) and folded by
+    the per-batch update.  Reports rows/s and the PCIe bytes/s this sustains."""
+    import torch
+    from vecruntime_amd import vr
+    n = len(host["keys"])
+    bytes_in = n * (8 + 8 + 8 + 4 + 3 / 8.0)
+    res = {"rows": n, "host_bytes": bytes_in}
+    agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000))
+    for kind in ("pageable", "pinned"):
+        batches, keep = host_groupby_batches(host, pinned=kind == "pinned")
+        agg.reset()
+        agg.update_batches(batches[:64])  # warm-up: staging buffers sized, table touched
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(2):
+            agg.reset()
+            t0 = time.perf_counter()
+            agg.update_batches(batches)
+            g = agg.num_groups()
+            times.append(time.perf_counter() - t0)
+        dt = min(times)
+        res[kind] = {"ms": dt * 1e3, "rows_per_s": n / dt, "host_GB_per_s": bytes_in / dt / 1e9,
+                     "groups_match_device_resident": g == groups}
+        log("pcie %s: %.1f ms, %.3e rows/s, %.1f GB/s from host" % (kind, dt * 1e3, n / dt, bytes_in / dt / 1e9))
+        del batches, keep
+    agg.close()
+    return res
+
+
+def run_cfg1(dev, args):
+    """The reference's CPU-path case: one Arrow batch of 1,000,000 rows, int64 key uniform over
+    1,000 values, COUNT(*), SUM(int64 uniform [-1e6, 1e6]), SUM(float64 uniform [0, 1)), no nulls.
+    Timed device-resident and from host memory; checked against per-key sums computed by torch;
+    the CPU port of the reference kernels timed on the same batch."""
+    import torch
+    from vecruntime_amd import abi, vr
+    from vecruntime_amd.columns import Column
+    n = 1_000_000
+    rng = np.random.default_rng(101)
+    keys = rng.integers(0, 1000, n).astype(np.int64)
+    lv = rng.integers(-10 ** 6, 10 ** 6 + 1, n).astype(np.int64)
+    dv = rng.random(n)
+    spec = abi.AggSpec([abi.INT64], [abi.INT64, abi.FLOAT64],
+                       [abi.agg_desc(abi.COUNT_STAR), abi.agg_desc(abi.SUM_LONG, 0), abi.agg_desc(abi.SUM_DOUBLE, 1)],
+                       key_nullable=[False])
+    tk, tl, td = (torch.from_numpy(a).cuda() for a in (keys, lv, dv))
+    dcols = ([vr.device_column(abi.INT64, tk)], [vr.device_column(abi.INT64, tl), vr.device_column(abi.FLOAT64, td)])
+    hcols = ([Column.longs(keys)], [Column.longs(lv), Column.doubles(dv)])
+    agg = vr.GroupedAggregate(dev, spec)
+    res = {"rows": n, "groups": 1000}
+    for kind, (kc, vc) in (("device_resident", dcols), ("from_host", hcols)):
+        for _ in range(3):
+            agg.reset()
+            agg.update(kc, vc, n)
+            agg.num_groups()
+        reps = 20
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            agg.reset()
+            agg.update(kc, vc, n)
+            g = agg.num_groups()
+        dt = (time.perf_counter() - t0) / reps
+        res[kind] = {"ms": dt * 1e3, "rows_per_s": n / dt}
+    # exact check of every group against torch
+    ek, eo = agg.emit()
+    G = agg.num_groups()
+    k = torch.from_numpy(ek[0].data[:G].copy())
+    cnt = torch.bincount(torch.from_numpy(keys), minlength=1000)
+    ls = torch.zeros(1000, dtype=torch.int64).index_add_(0, torch.from_numpy(keys), torch.from_numpy(lv))
+    ds = torch.zeros(1000, dtype=torch.float64).index_add_(0, torch.from_numpy(keys), torch.from_numpy(dv))
+    ok = G == 1000 and bool((torch.from_numpy(eo[0].data[:G].copy()) == cnt[k]).all())
+    ok = ok and bool((torch.from_numpy(eo[1].data[:G].copy()) == ls[k]).all())
+    rel = float(((torch.from_numpy(eo[2].data[:G].copy()) - ds[k]).abs() / ds[k].abs()).max())
+    res["checks"] = {"count_and_long_sum_exact": ok, "double_sum_max_rel_err": rel, "ok": ok and rel < 1e-12}
+    agg.close()
+    if not args.skip_cpu:
+        from oracle.oracle import OracleAgg  # the CPU port: the baseline, timed on the same batch
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 2.0:
+            o = OracleAgg(spec)
+            o.update(hcols[0], hcols[1], n)
+            del o
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        res["cpu_baseline"] = {"value": n / dt, "unit": "rows/s", "cores": 1, "kind": "port", **host_info(),
+                               "sample": "the whole 1,000,000-row batch, one thread, %d runs" % reps}
+    log("cfg1: %s" % json.dumps(res))
     return res
 
 
@@ -939,6 +1073,7 @@ def main():
     ap.add_argument("--uneven-steps", type=int, default=2)
     ap.add_argument("--tune-budget", type=float, default=60.0)
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-pcie", action="store_true")
     ap.add_argument("--skip-filters", action="store_true")
     ap.add_argument("--filter-rows", type=int, default=200_000_000)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -982,7 +1117,11 @@ def main():
         dev.close()
         print(json.dumps(out), flush=True)
         return
+    cfg1 = run_cfg1(dev, args)
     gb = run_groupby(dev, args, n)
+    pcie = None
+    if gb["_host"] is not None and not args.skip_pcie:
+        pcie = run_pcie(dev, args, gb["_host"], gb["groups"])
     join = None if args.skip_join else run_join(dev, args)
     uneven = None if args.skip_uneven else run_uneven(dev, args)
     serve = None if args.skip_serve else run_serve(dev, args)
@@ -1026,6 +1165,9 @@ def main():
         "cpu_baseline": cpu,
         "plain": {"rows_per_s": gb["plain"]["rows_per_s"], "ms_per_step": gb["plain"]["ms_per_step"],
                   "batched_over_plain": gb["batched"]["rows_per_s"] / gb["plain"]["rows_per_s"]},
+        "cold_table": gb["cold"],
+        "from_host": pcie,
+        "cfg1_1m_rows_1k_groups": cfg1,
         "join": join,
         "batched_uneven": uneven,
         "serving": serve,

@@ -1298,12 +1298,7 @@ static vr_status zero_key_column(AggImpl& h, int64_t n, hipStream_t s, vr_column
 // ---- columns -> device descriptors (staging host buffers)
 
 struct StagePlan {
-    struct Piece {
-        const void* src;
-        size_t bytes;
-        const void** dst;  // device pointer to patch
-    };
-    std::vector<Piece> pieces;
+    std::vector<StagePiece> pieces;
     size_t total = 0;
     void add(const void* src, size_t bytes, const void** dst) {
         if (!src || !bytes) return;
@@ -1408,19 +1403,7 @@ static vr_status describe(AggImpl& h, const vr_column* keys, const vr_column* va
 }
 
 static vr_status stage(AggImpl& h, StagePlan& plan, hipStream_t s, int* slot) {
-    *slot = -1;
-    if (plan.pieces.empty()) return VR_OK;
-    uint8_t *host, *dev;
-    VR_TRY(h.staging.begin(h.dev, plan.total, &host, &dev, slot));
-    size_t off = 0;
-    for (auto& p : plan.pieces) {
-        off = align_up(off, 256);
-        std::memcpy(host + off, p.src, p.bytes);
-        *p.dst = dev + off;
-        off += p.bytes;
-    }
-    VR_TRY(h.staging.upload(*slot, plan.total, s));
-    return VR_OK;
+    return h.staging.stage(h.dev, plan.pieces, s, slot);
 }
 
 static void launch_shape(AggImpl& h, const vr_launch_config* cfg, int64_t n, UpdateArgs& a, int* grid,

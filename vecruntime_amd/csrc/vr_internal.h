@@ -112,9 +112,17 @@ struct DevBuf {
     T* as() const { return (T*)p; }
 };
 
+// A host buffer to stage: its bytes and the device pointer to patch with its staged copy.
+struct StagePiece {
+    const void* src;
+    size_t bytes;
+    const void** dst;
+};
+
 // Pinned host staging: host columns are copied into a pinned buffer, then to the device on the
 // side stream; the compute stream waits on the copy's event.  Two slots alternate so the copy of
-// batch k+1 overlaps the kernels of batch k.
+// batch k+1 overlaps the kernels of batch k.  A column already in page-locked host memory is
+// copied by DMA straight from the caller's buffer (no host copy).
 struct Staging {
     Device* dev = nullptr;
     struct Slot {
@@ -129,11 +137,15 @@ struct Staging {
     ~Staging();
     // Reserves `bytes` in the next slot: returns host (pinned) and device pointers.
     vr_status begin(Device* d, size_t bytes, uint8_t** host, uint8_t** device, int* which);
-    // Uploads [0, bytes) of the slot on the side stream and makes `compute` wait for it.
-    vr_status upload(int which, size_t bytes, hipStream_t compute);
     // Marks the slot as read by the work queued on `compute` so far.
     vr_status done(int which, hipStream_t compute);
+    // Stages the pieces through the next slot (256-byte aligned, in order), patches their device
+    // pointers and makes `compute` wait for the copies; *which = -1 when there is nothing to stage.
+    vr_status stage(Device* d, const std::vector<StagePiece>& pieces, hipStream_t compute, int* which);
 };
+
+// True when `p` is page-locked host memory the DMA engine can read.
+bool host_pinned(const void* p);
 
 // Bytes of an Arrow buffer for n rows.
 inline size_t bitmap_bytes(int64_t n) { return (size_t)((n + 7) / 8); }

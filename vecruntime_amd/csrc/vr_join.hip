@@ -1185,14 +1185,10 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     hipStream_t s = j.dev->pick(stream);
     a.one = PBatch{n, 0, selection, 0};
     // streamed columns: device pointers, or staged from host memory
-    std::vector<std::pair<const void*, size_t>> pieces;
-    std::vector<const void**> patch;
+    std::vector<StagePiece> pieces;
     auto want = [&](const void* p, size_t bytes, const void** dst, int32_t mem) {
         *dst = p;
-        if (p && bytes && mem == VR_MEM_HOST) {
-            pieces.push_back({p, bytes});
-            patch.push_back(dst);
-        }
+        if (p && bytes && mem == VR_MEM_HOST) pieces.push_back({p, bytes, dst});
     };
     for (size_t q = 0; q < sh.slot_col.size(); q++) {
         const vr_column& c = columns[sh.slot_col[q]];
@@ -1205,18 +1201,7 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     int slot = -1;
     if (!pieces.empty()) {
         stage_lock.lock();  // the shared pinned staging, until the kernel reading it has finished
-        size_t total = 0;
-        for (auto& p : pieces) total = align_up(total, 256) + p.second;
-        uint8_t *host, *dev;
-        VR_TRY(j.staging.begin(j.dev, total, &host, &dev, &slot));
-        size_t off = 0;
-        for (size_t q = 0; q < pieces.size(); q++) {
-            off = align_up(off, 256);
-            std::memcpy(host + off, pieces[q].first, pieces[q].second);
-            *patch[q] = dev + off;
-            off += pieces[q].second;
-        }
-        VR_TRY(j.staging.upload(slot, total, s));
+        VR_TRY(j.staging.stage(j.dev, pieces, s, &slot));
     }
     OutPlan P;
     VR_TRY(plan_outputs(j, out, sh.out_types, sh.pairs, a.outs, &a.probe_idx, &a.build_idx, s, P));

@@ -137,14 +137,43 @@ vr_status Staging::begin(Device* d, size_t bytes, uint8_t** host, uint8_t** devi
     return VR_OK;
 }
 
-vr_status Staging::upload(int which, size_t bytes, hipStream_t compute) {
-    Slot& s = slot[which];
-    if (bytes) {
-        VR_TRY_HIP(hipMemcpyAsync(s.device.p, s.pinned, bytes, hipMemcpyHostToDevice, dev->copy_stream));
-        dev->count(&vr_stats::h2d_bytes, (int64_t)bytes);
+bool host_pinned(const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is not known to the runtime
+        return false;
     }
+    return at.type == hipMemoryTypeHost;
+}
+
+vr_status Staging::stage(Device* d, const std::vector<StagePiece>& pieces, hipStream_t compute, int* which) {
+    *which = -1;
+    if (pieces.empty()) return VR_OK;
+    size_t total = 0;
+    for (auto& p : pieces) total = align_up(total, 256) + p.bytes;
+    uint8_t *host, *device;
+    VR_TRY(begin(d, total, &host, &device, which));
+    Slot& s = slot[*which];
+    size_t off = 0;
+    bool direct = false;
+    for (auto& p : pieces) {
+        off = align_up(off, 256);
+        const void* from = p.src;
+        if (host_pinned(p.src)) {
+            direct = true;
+        } else {
+            std::memcpy(host + off, p.src, p.bytes);
+            from = host + off;
+        }
+        VR_TRY_HIP(hipMemcpyAsync(device + off, from, p.bytes, hipMemcpyHostToDevice, dev->copy_stream));
+        *p.dst = device + off;
+        off += p.bytes;
+    }
+    dev->count(&vr_stats::h2d_bytes, (int64_t)total);
     VR_TRY_HIP(hipEventRecord(s.copied, dev->copy_stream));
     VR_TRY_HIP(hipStreamWaitEvent(compute, s.copied, 0));
+    // the caller may reuse its buffers once the call returns: the copies out of them end first
+    if (direct) VR_TRY_HIP(hipEventSynchronize(s.copied));
     return VR_OK;
 }
 
