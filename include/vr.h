@@ -79,7 +79,8 @@ enum vr_type {
 enum vr_mem { VR_MEM_HOST = 0, VR_MEM_DEVICE = 1 };
 
 /* An Arrow-layout input column (the reference's VectorBuffers, VectorBuffers.java:48-215).
- * `validity` is NULL when no row is null.  For a dictionary-encoded UTF8 column `data` holds
+ * `validity` is NULL when no row is null.  A plain UTF8 column has `offsets` (n + 1 int32) and its
+ * bytes in `data`, both in the column's memory; for a dictionary-encoded UTF8 column `data` holds
  * int32 codes and `dictionary` points at a plain UTF8 column in host memory. */
 typedef struct vr_column {
     int32_t type;
@@ -147,8 +148,8 @@ enum vr_agg_kind {
     VR_AGG_SUM_WIDE = 4,     /* SUM over a decimal lane into Decimal(p+10,s) (WideDecimalSumAgg) */
     VR_AGG_AVG_DOUBLE = 5,   /* AVG over FLOAT64, buffer (sum, count) (AverageAgg) */
     VR_AGG_AVG_WIDE = 6,     /* AVG over a decimal lane, buffer (sum, count) (WideDecimalAvgAgg) */
-    VR_AGG_MIN = 7,          /* MIN over INT32/INT64/FLOAT64 (MinMaxAgg) */
-    VR_AGG_MAX = 8,          /* MAX over INT32/INT64/FLOAT64 (MinMaxAgg) */
+    VR_AGG_MIN = 7,          /* MIN over INT32/INT64/FLOAT64/DECIMAL128 (MinMaxAgg) */
+    VR_AGG_MAX = 8,          /* MAX over INT32/INT64/FLOAT64/DECIMAL128 (MinMaxAgg) */
     /* Merge modes (PartialMerge / Final): inputs are partial buffer columns
      * (VectorAggregates.compileMerge, VectorAggregates.scala:1141). */
     VR_AGG_MERGE_COUNT = 16,      /* input: count bigint */
@@ -182,7 +183,9 @@ typedef struct vr_agg_desc {
 typedef struct vr_agg_spec {
     int32_t num_keys;             /* 0: ungrouped aggregation, one group (one output row even for no
                                      input: AggKernels' reductions); keys are then passed as NULL */
-    const int32_t* key_types;     /* vr_type per key; UTF8 keys must be dictionary encoded */
+    const int32_t* key_types;     /* vr_type per key; a UTF8 key, plain or dictionary encoded, in host
+                                     or device memory, is mapped to the handle's dictionary ids on the
+                                     device (ids in the dictionary's own order, not first-seen order) */
     const uint8_t* key_nullable;  /* may be NULL: every key nullable */
     int32_t num_values;
     const int32_t* value_types;   /* vr_type per value column */
@@ -309,7 +312,11 @@ enum vr_join_type {
 
 typedef struct vr_join_spec {
     int32_t num_keys;
-    const int32_t* key_types;      /* INT32 / INT64 / dictionary ids */
+    const int32_t* key_types;      /* 1-4 keys of INT32 / INT64 / FLOAT64 (bit for bit) / DECIMAL128 /
+                                      UTF8 (plain or dictionary encoded: ids of the build side's device
+                                      dictionary; a dictionary with null entries is refused); more than
+                                      64 bits of key lanes are looked up by a 64-bit hash of the lanes
+                                      and compared lane by lane.  A UTF8 column can only be a key. */
     int32_t num_columns;           /* build-side columns kept for the output (keys included) */
     const int32_t* column_types;
     const int32_t* key_columns;    /* index into the build columns of each key */

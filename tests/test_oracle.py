@@ -500,3 +500,23 @@ def test_decimal128_min_max_match_pyref():
             if k not in wrong or signed_lo_key(v) < signed_lo_key(wrong[k]):
                 wrong[k] = v
     assert any(wrong.get(o[0]) != e[0][0] for o, e in zip(order, outs))
+
+
+@pytest.mark.parametrize("kinds", [k for k in __import__("join_key_cases").KIND_SETS])
+def test_join_wide_and_string_keys_match_nested_loop(kinds):
+    """Key tuples over more than 64 bits (INT64 pairs, DECIMAL128, four mixed lanes), FLOAT64 keys
+    compared bit for bit (-0.0 and 0.0 differ, NaN equals its own bits) and UTF8 keys (plain build,
+    dictionary-encoded probe): the oracle's pairs, semi and anti rows equal the nested loop's."""
+    from join_key_cases import make_case
+    case = make_case(sum(map(ord, "".join(kinds))), kinds, 300, 500, 60, dict_probe=("utf8" in kinds))
+    j = OracleJoin(case["spec"], case["build"], case["nb"])
+    keys = list(range(case["nk"]))
+    p, b, _, _ = j.probe(case["probe"], keys, case["npr"], abi.INNER, capacity=case["npr"] * case["nb"])
+    want = pyref.join(case["pybuild"], case["pyprobe"], 0)
+    assert len(want) > 100
+    assert list(zip(p.tolist(), b.tolist())) == want
+    for jt in (abi.LEFT_SEMI, abi.LEFT_ANTI):
+        p, _, _, _ = j.probe(case["probe"], keys, case["npr"], jt)
+        assert p.tolist() == pyref.join(case["pybuild"], case["pyprobe"], jt)
+    p, _, _, _ = j.probe(case["probe"], keys, case["npr"], abi.LEFT_ANTI, drop_null_keys=True)
+    assert p.tolist() == pyref.join(case["pybuild"], case["pyprobe"], 2, drop_null_keys=True)

@@ -19,7 +19,8 @@ ARCH = os.environ.get("VR_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["vr_runtime.hip", "vr_agg.hip", "vr_join.hip", "vr_serve.hip", "vr_jit.hip", "vr_filter.hip", "vr_partition.hip"]
+SOURCES = ["vr_runtime.hip", "vr_strings.hip", "vr_agg.hip", "vr_join.hip", "vr_serve.hip", "vr_jit.hip", "vr_filter.hip",
+           "vr_partition.hip"]
 
 # Device headers compiled into the library as text for the run-time specialization (vr_jit.hip):
 # (name the kernel source includes them by, path)

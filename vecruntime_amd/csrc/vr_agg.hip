@@ -13,6 +13,7 @@
 #include "vr_agg_part.h"
 #include "vr_internal.h"
 #include "vr_jit.h"
+#include "vr_strings.h"
 
 namespace vr {
 
@@ -781,6 +782,17 @@ static int key_words(int32_t type) {
     }
 }
 
+// A UTF8 key column's dictionary on the device, and the code -> id map of the last input
+// dictionary mapped through it (kept while the same dictionary keeps arriving).
+struct StrKey {
+    StrTable table;
+    DevBuf remap;
+    int32_t remap_len = 0;
+    const void* last_dict = nullptr;
+    int64_t last_len = -1;
+    const int32_t* last_offsets = nullptr;
+};
+
 struct AggImpl {
     Device* dev = nullptr;
     std::vector<int32_t> key_types, value_types;
@@ -811,7 +823,7 @@ struct AggImpl {
     bool half_sums = false;
     uint64_t rows_since_reset = 0;
 
-    std::vector<HostDictionary> dicts;
+    std::vector<StrKey> strs;  // UTF8 keys: each one's device dictionary
     Staging staging;
     DevBuf ids_stage;  // host-mapped UTF8 ids (pinned copies go through `staging`)
 
@@ -1307,53 +1319,64 @@ struct StagePlan {
     }
 };
 
+// The device id arrays of a call's plain UTF8 key columns: freed only once the stream's queued
+// work, which reads them, has finished.
+struct KeyIds {
+    std::vector<DevBuf> bufs;
+    hipStream_t s = nullptr;
+    explicit KeyIds(hipStream_t stream) : s(stream) {}
+    ~KeyIds() {
+        if (!bufs.empty()) (void)hipStreamSynchronize(s);
+    }
+};
+
+// Key c's ids for a UTF8 column, from the handle's device dictionary of that key
+// (GroupKeyTable.toIds, GroupKeyTable.java:1308-1430): a dictionary-encoded column maps its
+// dictionary's entries once per dictionary (the code -> id map stays on the device while the same
+// dictionary keeps arriving); a plain column, in host or device memory, maps its rows into a
+// device array of ids kept in `ids_store` until the update's kernels have run.
 static vr_status map_utf8_key(AggImpl& h, int c, const vr_column& col, int64_t n, DCol& d, StagePlan& plan,
-                              std::vector<int32_t>& ids_store) {
-    HostDictionary& dict = h.dicts[c];
+                              std::vector<DevBuf>& ids_store, hipStream_t s) {
+    StrKey& sk = h.strs[c];
     if (col.dictionary) {
         const vr_column& dc = *col.dictionary;
         if (dc.mem != VR_MEM_HOST || dc.type != VR_UTF8 || !dc.offsets)
             return fail(VR_ERR_UNSUPPORTED, "a key dictionary must be a plain UTF8 column in host memory");
-        if (dict.last_dict != dc.data || dict.last_len != dc.length || dict.last_hash != (uint64_t)(uintptr_t)dc.offsets) {
-            dict.host_remap.assign((size_t)std::max<int64_t>(dc.length, 1), -1);
-            for (int64_t e = 0; e < dc.length; e++) {
-                if (dc.validity && !((dc.validity[e >> 3] >> (e & 7)) & 1)) continue;
-                int32_t s0 = dc.offsets[e], s1 = dc.offsets[e + 1];
-                dict.host_remap[e] = dict.index_of((const char*)dc.data + s0, s1 - s0, true);
-            }
-            VR_TRY_HIP(hipStreamSynchronize(h.dev->stream));  // a kernel may still read the old map
-            VR_TRY(dict.remap.ensure(h.dev, dict.host_remap.size() * sizeof(int32_t)));
-            VR_TRY_HIP(hipMemcpy(dict.remap.p, dict.host_remap.data(), dict.host_remap.size() * sizeof(int32_t),
-                                 hipMemcpyHostToDevice));
-            dict.last_dict = dc.data;
-            dict.last_len = dc.length;
-            dict.last_hash = (uint64_t)(uintptr_t)dc.offsets;
+        if (sk.last_dict != dc.data || sk.last_len != dc.length || sk.last_offsets != dc.offsets) {
+            VR_TRY_HIP(hipStreamSynchronize(s));  // a kernel may still read the old map
+            VR_TRY(sk.remap.ensure(h.dev, (size_t)std::max<int64_t>(dc.length, 1) * sizeof(int32_t)));
+            // a null entry maps to -1: the row is null through its dictionary
+            VR_TRY(sk.table.map_host(h.dev, dc.offsets, (const uint8_t*)dc.data, dc.validity, dc.length, true, -1,
+                                     sk.remap.as<int32_t>(), s));
+            sk.remap_len = (int32_t)std::max<int64_t>(dc.length, 1);
+            sk.last_dict = dc.data;
+            sk.last_len = dc.length;
+            sk.last_offsets = dc.offsets;
         }
-        d.remap = dict.remap.as<int32_t>();
-        d.remap_len = (int32_t)dict.host_remap.size();
+        d.remap = sk.remap.as<int32_t>();
+        d.remap_len = sk.remap_len;
         d.data = col.data;
         if (col.mem == VR_MEM_HOST) plan.add(col.data, (size_t)n * 4, &d.data);
         return VR_OK;
     }
-    if (col.mem != VR_MEM_HOST) return fail(VR_ERR_UNSUPPORTED, "plain UTF8 keys in device memory run on the CPU kernels");
-    ids_store.resize((size_t)n);
-    for (int64_t i = 0; i < n; i++) {
-        if (col.validity && !((col.validity[i >> 3] >> (i & 7)) & 1)) {
-            ids_store[i] = 0;
-            continue;
-        }
-        int32_t s0 = col.offsets[i], s1 = col.offsets[i + 1];
-        ids_store[i] = dict.index_of((const char*)col.data + s0, s1 - s0, true);
+    if (!col.offsets) return fail(VR_ERR_INVALID_ARGUMENT, "a plain UTF8 key column needs its offsets");
+    ids_store.emplace_back();
+    DevBuf& ids = ids_store.back();
+    VR_TRY(ids.ensure(h.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+    if (col.mem == VR_MEM_HOST) {
+        VR_TRY(sk.table.map_host(h.dev, col.offsets, (const uint8_t*)col.data, col.validity, n, true, 0, ids.as<int32_t>(), s));
+    } else {
+        const StrRows rows{col.offsets, (const uint8_t*)col.data, col.validity, n};
+        VR_TRY(sk.table.map(h.dev, rows, true, 0, ids.as<int32_t>(), s));
     }
     d.remap = nullptr;
-    d.data = ids_store.data();
-    plan.add(ids_store.data(), (size_t)n * 4, &d.data);
+    d.data = ids.p;
     return VR_OK;
 }
 
 static vr_status describe(AggImpl& h, const vr_column* keys, const vr_column* values, int64_t n,
-                          const uint8_t* selection, BatchDesc& b, StagePlan& plan,
-                          std::vector<std::vector<int32_t>>& id_store, int32_t* sel_mem) {
+                          const uint8_t* selection, BatchDesc& b, StagePlan& plan, std::vector<DevBuf>& id_store,
+                          int32_t* sel_mem, hipStream_t s) {
     std::memset(&b, 0, sizeof(b));
     b.n = n;
     int32_t mem = -1;
@@ -1362,7 +1385,6 @@ static vr_status describe(AggImpl& h, const vr_column* keys, const vr_column* va
         return VR_OK;
     };
     const int nk = (int)h.key_types.size(), nv = (int)h.value_types.size();
-    id_store.resize(nk);
     for (int c = 0; c < nk; c++) {
         const vr_column& col = keys[c];
         VR_TRY(note_mem(col.mem));
@@ -1374,7 +1396,7 @@ static vr_status describe(AggImpl& h, const vr_column* keys, const vr_column* va
         d.valid = col.validity;
         if (col.mem == VR_MEM_HOST) plan.add(col.validity, bitmap_bytes(n), (const void**)&d.valid);
         if (col.type == VR_UTF8) {
-            VR_TRY(map_utf8_key(h, c, col, n, d, plan, id_store[c]));
+            VR_TRY(map_utf8_key(h, c, col, n, d, plan, id_store, s));
         } else {
             d.data = col.data;
             if (col.mem == VR_MEM_HOST) plan.add(col.data, lane_bytes(col.type, n), &d.data);
@@ -2049,7 +2071,7 @@ vr_status vr_agg_create(vr_device* dev, const vr_agg_spec* spec, vr_agg** out) {
     VR_TRY(compile_spec(h));
     for (int k = 0; k < h.proto.nops; k++)
         if (h.proto.ops[k].code == OP_SUM_I64 || h.proto.ops[k].code == OP_SUM_D128) h.half_sums = true;
-    h.dicts.resize(h.key_types.size());
+    h.strs.resize(h.key_types.size());
     VR_TRY(init_table(h, h.ungrouped ? 1 : spec->capacity_hint));
     if (h.ungrouped) {
         VR_TRY(insert_single_group(h, h.dev->stream));
@@ -2085,9 +2107,9 @@ vr_status vr_agg_update(vr_agg* agg, const vr_column* keys, const vr_column* val
     }
     BatchDesc b;
     StagePlan plan;
-    std::vector<std::vector<int32_t>> ids;
+    KeyIds ids(s);
     int32_t mem;
-    VR_TRY(describe(h, keys, values, n, selection, b, plan, ids, &mem));
+    VR_TRY(describe(h, keys, values, n, selection, b, plan, ids.bufs, &mem, s));
     int slot;
     VR_TRY(stage(h, plan, s, &slot));
     VR_TRY(run_update(h, b, n, s, nullptr));
@@ -2169,12 +2191,12 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
         VR_TRY(enter(h, s));
         static thread_local std::vector<BatchDesc> pdescs;
         pdescs.resize(nb);
-        std::vector<std::vector<int32_t>> ids;
+        KeyIds ids(s);
         for (int b = 0; b < nb; b++) {
             StagePlan plan;
             int32_t mem;
             VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, bs->lengths[b], nullptr, pdescs[b],
-                            plan, ids, &mem));
+                            plan, ids.bufs, &mem, s));
         }
         VR_TRY(update_partitioned(h, pdescs.data(), nb, total, bs->filter_col, bs->filter_threshold, layout, cfg, s));
         h.dev->count(&vr_stats::calls, 1);
@@ -2195,13 +2217,14 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     static thread_local std::vector<BatchDesc> descs;
     descs.resize(nb);
     int64_t tiles = 0, rows = 0;
-    std::vector<std::vector<int32_t>> ids;
+    KeyIds ids(s);
     for (int b = 0; b < nb; b++) {
         StagePlan plan;
         int32_t mem;
         int64_t n = bs->lengths[b];
         if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative batch length");
-        VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, n, nullptr, descs[b], plan, ids, &mem));
+        VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, n, nullptr, descs[b], plan, ids.bufs,
+                        &mem, s));
         descs[b].row_base = h.rows_seen + rows;
         descs[b].tile_begin = tiles;
         tiles += (n + T - 1) / T;
@@ -2371,20 +2394,15 @@ vr_status vr_agg_key_dictionary(vr_agg* agg, int32_t c, int32_t* offsets, int64_
     AggImpl& h = agg->h;
     if (c < 0 || c >= (int)h.key_types.size() || h.key_types[c] != VR_UTF8)
         return fail(VR_ERR_INVALID_ARGUMENT, "not a UTF8 key column");
-    const HostDictionary& d = h.dicts[c];
-    int64_t total = 0;
-    for (auto& s : d.values) total += (int64_t)s.size();
-    *count = (int64_t)d.values.size();
-    *num_bytes = total;
+    std::vector<int32_t> offs;
+    std::vector<uint8_t> data;
+    VR_TRY(h.strs[c].table.read(&offs, &data, h.dev->stream));
+    *count = (int64_t)offs.size() - 1;
+    *num_bytes = (int64_t)data.size();
     if (!offsets || !bytes) return VR_OK;
-    if (offsets_capacity < *count + 1 || bytes_capacity < total) return fail(VR_ERR_CAPACITY, "dictionary buffers too small");
-    int64_t o = 0;
-    offsets[0] = 0;
-    for (size_t i = 0; i < d.values.size(); i++) {
-        std::memcpy(bytes + o, d.values[i].data(), d.values[i].size());
-        o += (int64_t)d.values[i].size();
-        offsets[i + 1] = (int32_t)o;
-    }
+    if (offsets_capacity < *count + 1 || bytes_capacity < *num_bytes) return fail(VR_ERR_CAPACITY, "dictionary buffers too small");
+    std::memcpy(offsets, offs.data(), offs.size() * sizeof(int32_t));
+    if (!data.empty()) std::memcpy(bytes, data.data(), data.size());
     return VR_OK;
 }
 
@@ -2413,6 +2431,7 @@ vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
                        h.p_cand_count.bytes + h.p_hist.bytes + h.p_hot_keys.bytes + h.p_hot_part.bytes +
                        h.p_hot_total.bytes + h.p_pool.bytes + h.p_next.bytes + h.p_cnt.bytes + h.p_heads.bytes +
                        h.p_resume.bytes + h.p_tile_batch.bytes + h.p_pending.bytes);
+    for (auto& sk : h.strs) *bytes += sk.table.memory() + (int64_t)sk.remap.bytes;
     return VR_OK;
 }
 
@@ -2493,13 +2512,14 @@ vr_status vr_agg_graph_create(vr_agg* agg, const vr_batch_set* bs, const vr_laun
     launch_shape(h, cfg, 0, a, &grid, &block, &lds_bytes, true);
     const int64_t T = (int64_t)block * a.rows_per_thread;
     std::vector<BatchDesc> descs((size_t)nb);
-    std::vector<std::vector<int32_t>> ids;
+    KeyIds ids(s);
     int64_t tiles = 0, rows = 0;
     for (int b = 0; b < nb; b++) {
         StagePlan plan;
         int32_t mem;
         const int64_t n = bs->lengths[b];
-        VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, n, nullptr, descs[b], plan, ids, &mem));
+        VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, n, nullptr, descs[b], plan, ids.bufs,
+                        &mem, s));
         descs[b].row_base = rows;  // every replay starts from a cleared table
         descs[b].tile_begin = tiles;
         tiles += (n + T - 1) / T;

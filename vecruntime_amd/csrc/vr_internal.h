@@ -162,24 +162,6 @@ inline size_t lane_bytes(int32_t type, int64_t n) {
 }
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// Host-side string dictionary of one UTF8 key column (the table's own ids, first-use order), with
-// the device copy of the last input dictionary's code -> id map (GroupKeyTable.toIds,
-// GroupKeyTable.java:1308-1430).
-struct HostDictionary {
-    std::vector<std::string> values;
-    std::vector<std::pair<uint64_t, int32_t>> index;  // open addressing: (fingerprint, id)
-    size_t mask = 0;
-    // Cache of the last input dictionary mapped: its identity and the device remap array.
-    const void* last_dict = nullptr;
-    int64_t last_len = -1;
-    uint64_t last_hash = 0;
-    DevBuf remap;
-    std::vector<int32_t> host_remap;
-    int32_t index_of(const char* s, int32_t len, bool insert);
-};
-
-uint64_t hash_bytes64(const void* p, size_t n);
-
 }  // namespace vr
 
 struct vr_device {

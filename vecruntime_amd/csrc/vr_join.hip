@@ -24,6 +24,7 @@
 
 #include "vr_device.h"
 #include "vr_internal.h"
+#include "vr_strings.h"
 
 namespace vr {
 
@@ -31,7 +32,7 @@ constexpr int BUCKET = 8;  // cells per bucket
 constexpr int PROBE_BLOCK = 256;
 constexpr int PROBE_ITEMS = 8;  // rows per thread per tile
 constexpr int64_t PROBE_TILE = (int64_t)PROBE_BLOCK * PROBE_ITEMS;
-constexpr int MAX_PCOLS = 12;  // streamed column slots of a probe: 2 keys, 8 outputs, 2 residual operands
+constexpr int MAX_PCOLS = 14;  // streamed column slots of a probe: 4 keys, 8 outputs, 2 residual operands
 // In the value word of a bucket's first cell: an insert walked on past this (full) bucket, so a
 // lookup that misses here must go on to the next bucket.  Row counts stay below 2^31, so the bit
 // is free in every value word.
@@ -48,16 +49,105 @@ struct JoinTable {
     uint64_t nb;        // buckets
     const uint32_t* range_rows;
     uint64_t sentinel_value;  // value of the key equal to EMPTY_KEY (0: absent)
+    int32_t wide;             // cells hold (fingerprint << 32) | build row: keys compared in the build columns
+    int32_t pad;
 };
 
 __host__ __device__ __forceinline__ uint64_t home_bucket(uint64_t key, uint64_t nb) { return ((fmix64(key) >> 32) * nb) >> 32; }
 __host__ __device__ __forceinline__ uint64_t next_bucket(uint64_t b, uint64_t nb) { return b + 1 == nb ? 0 : b + 1; }
 
+// ---- wide keys: more than 64 bits of key lanes (up to four columns, DECIMAL128 included)
+//
+// The table is keyed on a 64-bit hash of the key's lanes: a kernel hashes every build row into a
+// 64-bit column, which the 64-bit build inserts as it would a key; a second kernel then checks
+// that the rows of every cell hold one key (on a collision the build hashes again with another
+// seed).  A probe looks its row's hash up like a 64-bit key and compares its lanes with the first
+// build row of the run it found.  UTF8 key lanes hold the handle's dictionary ids, so equal
+// strings are equal lanes.
+constexpr int MAX_JOIN_KEYS = 4;
+
+struct WideKeys {
+    const uint8_t* valid[MAX_JOIN_KEYS];
+    const void* data[MAX_JOIN_KEYS];
+    int32_t width[MAX_JOIN_KEYS];  // bytes per row: 4, 8 or 16
+    int32_t nk;
+    int32_t pad;
+    uint64_t seed;
+};
+
+__host__ __device__ __forceinline__ int key_lane_width(int32_t type) {
+    return type == VR_INT64 || type == VR_FLOAT64 ? 8 : type == VR_DECIMAL128 ? 16 : 4;
+}
+
+__device__ __forceinline__ uint64_t wide_mix(uint64_t h, uint64_t w) { return fmix64(h ^ (w * 0xC2B2AE3D27D4EB4Full)); }
+
+// The key's hash from its lanes at row i (data[c] + i * width[c]).
+__device__ __forceinline__ uint64_t wide_hash(const void* const* data, const int32_t* width, int nk, uint64_t seed, int64_t i) {
+    uint64_t h = seed + (uint64_t)nk;
+    for (int c = 0; c < nk; c++) {
+        if (width[c] == 4) {
+            h = wide_mix(h, (uint64_t)ld_stream((const uint32_t*)data[c] + i));
+        } else {
+            const uint64_t* p = (const uint64_t*)data[c] + (width[c] == 16 ? 2 * i : i);
+            h = wide_mix(h, ld_stream(p));
+            if (width[c] == 16) h = wide_mix(h, ld_stream(p + 1));
+        }
+    }
+    return h;
+}
+
+// Row i of lanes `a` equals row j of lanes `b` (same widths).
+__device__ __forceinline__ bool wide_equal(const void* const* a, int64_t i, const void* const* b, int64_t j, const int32_t* width,
+                                           int nk) {
+    for (int c = 0; c < nk; c++) {
+        if (width[c] == 4) {
+            if (((const uint32_t*)a[c])[i] != ((const uint32_t*)b[c])[j]) return false;
+        } else if (width[c] == 8) {
+            if (((const uint64_t*)a[c])[i] != ((const uint64_t*)b[c])[j]) return false;
+        } else {
+            const uint64_t* x = (const uint64_t*)a[c] + 2 * i;
+            const uint64_t* y = (const uint64_t*)b[c] + 2 * j;
+            if (x[0] != y[0] || x[1] != y[1]) return false;
+        }
+    }
+    return true;
+}
+
+// hashes[i] = the hash of build row i; valid: bit i set when no key lane of row i is null (each
+// thread writes one 32-row word).
+__global__ void wide_hash_kernel(WideKeys k, int64_t n, uint64_t* hashes, uint32_t* valid) {
+    const int64_t words = (n + 31) / 32;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t bits = 0;
+        for (int64_t i = w * 32; i < n && i < w * 32 + 32; i++) {
+            bool ok = true;
+            for (int c = 0; c < k.nk; c++)
+                if (k.valid[c] && !bit_at(k.valid[c], i)) ok = false;
+            hashes[i] = ok ? wide_hash(k.data, k.width, k.nk, k.seed, i) : 0;
+            if (ok) bits |= 1u << (i & 31);
+        }
+        valid[w] = bits;
+    }
+}
+
+// Counts the build rows whose key differs from the first row of their cell's run (a hash
+// collision): row_cell from the insert, runs in range_rows from cell_start; the rows of the key
+// equal to EMPTY_KEY (cell 0xFFFFFFFE) run from `sentinel_start`.
+__global__ void wide_verify_kernel(WideKeys k, int64_t n, const uint32_t* row_cell, const uint32_t* cell_start,
+                                   const uint32_t* range_rows, uint32_t sentinel_start, uint64_t* collisions) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t cell = row_cell[i];
+        if (cell == 0xFFFFFFFFu) continue;  // a null key
+        const uint32_t first = range_rows[cell == 0xFFFFFFFEu ? sentinel_start : cell_start[cell]];
+        if (!wide_equal(k.data, i, k.data, (int64_t)first, k.width, k.nk)) atomic_add_u64(collisions, 1);
+    }
+}
+
 __device__ __forceinline__ bool key64_of(const DCol* k, int nk, int64_t i, uint64_t* key) {
     if (nk == 1) {
         const DCol& c = k[0];
         if (c.valid && !bit_at(c.valid, i)) return false;
-        if (c.type == VR_INT64) *key = (uint64_t)ld_stream((const int64_t*)c.data + i);
+        if (key_lane_width(c.type) == 8) *key = (uint64_t)ld_stream((const int64_t*)c.data + i);
         else *key = (uint64_t)(uint32_t)ld_stream((const int32_t*)c.data + i);
         return true;
     }
@@ -222,6 +312,7 @@ struct PBatch {
 // the streamed operands of a residual.
 struct ProbeArgs {
     JoinTable t;
+    WideKeys bk;  // wide keys: the build side's key lanes
     DenseIndex dense;
     int32_t nk;
     int32_t join_type;
@@ -265,7 +356,7 @@ __device__ __forceinline__ PCol slot_col(const ProbeArgs& a, const PCol* bc, int
 __device__ __forceinline__ bool probe_key(const PCol& c0, const PCol& c1, int32_t t0, int nk, int64_t i, uint64_t* key) {
     if (nk == 1) {
         if (c0.valid && !bit_at(c0.valid, i)) return false;
-        if (t0 == VR_INT64) *key = (uint64_t)ld_stream((const int64_t*)c0.data + i);
+        if (key_lane_width(t0) == 8) *key = (uint64_t)ld_stream((const int64_t*)c0.data + i);
         else *key = (uint64_t)(uint32_t)ld_stream((const int32_t*)c0.data + i);
         return true;
     }
@@ -390,7 +481,7 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     __shared__ int32_t batch_sh;
     __shared__ uint64_t base_sh;
     __shared__ uint32_t wave_tot[PROBE_ITEMS][PROBE_WAVES];  // totals, then their exclusive prefix in the tile
-    __shared__ DCol psrc_sh[8], ksrc_sh[2];
+    __shared__ DCol psrc_sh[8], ksrc_sh[MAX_JOIN_KEYS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x == 0) {
         const int64_t t = (int64_t)atomic_add_u64(&a.tile_counter[0], 1);
@@ -405,6 +496,14 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     const int64_t row0 = (tile - B.tile0) * PROBE_TILE;
     const PCol k0 = slot_col<MULTI>(a, bc, 0);
     const PCol k1 = a.nk > 1 ? slot_col<MULTI>(a, bc, 1) : PCol{nullptr, nullptr};
+    const void* pd[MAX_JOIN_KEYS] = {nullptr, nullptr, nullptr, nullptr};  // wide keys: the streamed key lanes
+    const uint8_t* pv[MAX_JOIN_KEYS] = {nullptr, nullptr, nullptr, nullptr};
+    if (a.t.wide)
+        for (int c = 0; c < a.nk; c++) {
+            const PCol p = slot_col<MULTI>(a, bc, c);
+            pd[c] = p.data;
+            pv[c] = p.valid;
+        }
     DCol resl = a.res_lbuild, resr = a.res_rbuild;
     if (a.res_on) {
         if (!a.res_lb) {
@@ -424,7 +523,15 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
         const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
         const bool in = i < B.n && (!B.sel || bit_at(B.sel, i));
         key[r] = 0;
-        const bool nonnull = in && probe_key(k0, k1, a.ctype[0], a.nk, i, &key[r]);
+        bool nonnull;
+        if (a.t.wide) {  // the key's hash; a null lane makes the key null
+            nonnull = in;
+            for (int c = 0; c < a.nk; c++)
+                if (pv[c] && !bit_at(pv[c], i)) nonnull = false;
+            if (nonnull) key[r] = wide_hash(pd, a.bk.width, a.nk, a.bk.seed, i);
+        } else {
+            nonnull = in && probe_key(k0, k1, a.ctype[0], a.nk, i, &key[r]);
+        }
         flags |= (in ? 1u : 0u) << r;
         flags |= (nonnull ? 1u : 0u) << (16 + r);
     }
@@ -442,6 +549,11 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
             } else {
                 v = coop_lookup(a.t, nonnull && key[r] != EMPTY_KEY, key[r]);
                 if (nonnull && key[r] == EMPTY_KEY) v = a.t.sentinel_value;
+                if (a.t.wide && v) {  // the run's key must be the row's: its first build row decides
+                    const uint32_t m = (uint32_t)(v >> 32), pl = (uint32_t)v;
+                    const int64_t first = m == 1 ? (int64_t)pl : (int64_t)a.t.range_rows[pl];
+                    if (!wide_equal(pd, i, a.bk.data, first, a.bk.width, a.nk)) v = 0;
+                }
             }
         }
         uint32_t c = 0, pl = 0, fd = 0, np = 0;
@@ -684,7 +796,54 @@ struct JoinImpl {
     // of a finished table may run from several threads at once.
     std::mutex stage_mu;
     Staging staging;
+    // wide keys (more than 64 bits of key lanes): cells hold fingerprints and build rows
+    bool wide = false;
+    uint64_t seed = 0x9E3779B97F4A7C15ull;  // wide keys: the hash's seed (another on a collision)
+    // UTF8 keys: each one's dictionary of the build side's strings (a build column of a UTF8 key
+    // holds the rows' ids; a probe maps its strings to those ids, unknown ones to -1)
+    std::vector<StrTable> strs;
 };
+
+__global__ void remap_codes_kernel(const int32_t* codes, int64_t n, const int32_t* remap, int32_t m, int32_t* ids) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = codes[i];
+        ids[i] = c >= 0 && c < m ? remap[c] : -1;
+    }
+}
+
+// ids[0, n) (device) of UTF8 key k's column `col`: inserting the build side's strings, or looking
+// a probe's up (unknown: -1).  A dictionary-encoded column maps its dictionary's entries, then its
+// codes through them.
+static vr_status map_key_strings(JoinImpl& j, int k, const vr_column& col, int64_t n, bool insert, int32_t* ids,
+                                 hipStream_t s) {
+    StrTable& t = j.strs[(size_t)k];
+    if (col.dictionary) {
+        const vr_column& dc = *col.dictionary;
+        if (dc.mem != VR_MEM_HOST || dc.type != VR_UTF8 || !dc.offsets)
+            return fail(VR_ERR_UNSUPPORTED, "a key dictionary must be a plain UTF8 column in host memory");
+        if (dc.validity)
+            for (int64_t e = 0; e < dc.length; e++)
+                if (!((dc.validity[e >> 3] >> (e & 7)) & 1))
+                    return fail(VR_ERR_UNSUPPORTED, "join keys over a dictionary with null entries run on the CPU kernels");
+        const int32_t m = (int32_t)dc.length;
+        DevBuf remap, codes;
+        VR_TRY(remap.ensure(j.dev, (size_t)std::max(m, 1) * 4));
+        VR_TRY(t.map_host(j.dev, dc.offsets, (const uint8_t*)dc.data, nullptr, m, insert, -1, remap.as<int32_t>(), s));
+        const int32_t* cp = (const int32_t*)col.data;
+        if (col.mem == VR_MEM_HOST) {
+            VR_TRY(codes.ensure(j.dev, (size_t)n * 4));
+            VR_TRY_HIP(hipMemcpyAsync(codes.p, col.data, (size_t)n * 4, hipMemcpyHostToDevice, s));
+            cp = codes.as<int32_t>();
+        }
+        hipLaunchKernelGGL(remap_codes_kernel, dim3(grid_of(n, 256, 4096)), dim3(256), 0, s, cp, n, remap.as<int32_t>(), m, ids);
+        VR_TRY_HIP(hipGetLastError());
+        VR_TRY_HIP(hipStreamSynchronize(s));  // the map and codes are freed on return
+        return VR_OK;
+    }
+    if (!col.offsets) return fail(VR_ERR_INVALID_ARGUMENT, "a plain UTF8 key column needs its offsets");
+    if (col.mem == VR_MEM_HOST) return t.map_host(j.dev, col.offsets, (const uint8_t*)col.data, col.validity, n, insert, 0, ids, s);
+    return t.map(j.dev, StrRows{col.offsets, (const uint8_t*)col.data, col.validity, n}, insert, 0, ids, s);
+}
 
 static vr_status grow_col(JoinImpl& j, JoinImpl::BuildCol& c, int32_t type, size_t want_rows, hipStream_t s) {
     if (want_rows <= c.cap_rows) return VR_OK;
@@ -714,6 +873,20 @@ __global__ void append_bits_kernel(uint8_t* dst, int64_t at, const uint8_t* src,
         if (v) atomicOr(w, 1u << (o & 31));
         else atomicAnd(w, ~(1u << (o & 31)));
     }
+}
+
+// The build side's key lanes, for wide-key inserts and comparisons.
+static WideKeys wide_keys(const JoinImpl& j) {
+    WideKeys wk{};
+    wk.nk = (int32_t)j.key_types.size();
+    for (int q = 0; q < wk.nk; q++) {
+        const auto& bc = j.cols[j.key_cols[q]];
+        wk.data[q] = bc.data.p;
+        wk.valid[q] = bc.has_valid ? bc.valid.as<uint8_t>() : nullptr;
+        wk.width[q] = key_lane_width(j.key_types[q]);
+    }
+    wk.seed = j.seed;
+    return wk;
 }
 
 }  // namespace vr
@@ -791,7 +964,7 @@ extern "C" {
 
 vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out) {
     if (!dev || !spec || !out) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
-    if (spec->num_keys < 1 || spec->num_keys > 2) return fail(VR_ERR_UNSUPPORTED, "one or two join keys run on the device");
+    if (spec->num_keys < 1 || spec->num_keys > MAX_JOIN_KEYS) return fail(VR_ERR_UNSUPPORTED, "one to four join keys run on the device");
     if (spec->num_columns < 1 || spec->num_columns > 8) return fail(VR_ERR_UNSUPPORTED, "at most 8 build columns");
     std::unique_ptr<vr_join> h(new vr_join());
     h->owner.hold(dev);
@@ -800,15 +973,23 @@ vr_status vr_join_create(vr_device* dev, const vr_join_spec* spec, vr_join** out
     j.key_types.assign(spec->key_types, spec->key_types + spec->num_keys);
     j.col_types.assign(spec->column_types, spec->column_types + spec->num_columns);
     j.key_cols.assign(spec->key_columns, spec->key_columns + spec->num_keys);
+    int key_bytes = 0;
     for (int k = 0; k < spec->num_keys; k++) {
         int32_t t = j.key_types[k];
         if (j.key_cols[k] < 0 || j.key_cols[k] >= spec->num_columns || j.col_types[j.key_cols[k]] != t)
             return fail(VR_ERR_INVALID_ARGUMENT, "join key column does not match the build columns");
-        if (t != VR_INT32 && t != VR_INT64) return fail(VR_ERR_UNSUPPORTED, "join keys other than INT32/INT64 run on the CPU kernels");
-        if (spec->num_keys == 2 && t != VR_INT32) return fail(VR_ERR_UNSUPPORTED, "two-key joins take two INT32 keys");
+        if (t == VR_BOOL) return fail(VR_ERR_UNSUPPORTED, "BOOL join keys run on the CPU kernels");
+        key_bytes += key_lane_width(t);
     }
-    for (int32_t t : j.col_types)
-        if (t == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "UTF8 payloads run on the CPU kernels");
+    // one lane of at most 8 bytes, or two of 4: the key is its 64 bits; else cells hold fingerprints
+    j.wide = !(spec->num_keys == 1 && key_bytes <= 8) && !(spec->num_keys == 2 && key_bytes == 8);
+    for (int c = 0; c < spec->num_columns; c++) {
+        if (j.col_types[c] != VR_UTF8) continue;
+        bool key = false;
+        for (int k = 0; k < spec->num_keys; k++) key |= j.key_cols[k] == c;
+        if (!key) return fail(VR_ERR_UNSUPPORTED, "UTF8 payloads run on the CPU kernels");
+    }
+    j.strs.resize((size_t)spec->num_keys);
     if (spec->dense_index != 0 && spec->dense_index != -1) return fail(VR_ERR_INVALID_ARGUMENT, "dense_index is 0 or -1");
     j.dense_mode = spec->dense_index;
     j.track = spec->track_matched != 0;
@@ -839,7 +1020,13 @@ vr_status vr_join_build_append(vr_join* h, const vr_column* columns, int64_t n, 
             return fail(VR_ERR_UNSUPPORTED, "BOOL build payloads run on the CPU kernels");
         }
         size_t w = lane_bytes(col.type, 1);
-        VR_TRY_HIP(hipMemcpyAsync(bc.data.as<uint8_t>() + (size_t)j.rows * w, col.data, (size_t)n * w, kind, s));
+        if (col.type == VR_UTF8) {  // a UTF8 key: the rows' ids in the key's dictionary
+            int k = 0;
+            while (j.key_cols[k] != (int32_t)c) k++;
+            VR_TRY(map_key_strings(j, k, col, n, true, bc.data.as<int32_t>() + j.rows, s));
+        } else {
+            VR_TRY_HIP(hipMemcpyAsync(bc.data.as<uint8_t>() + (size_t)j.rows * w, col.data, (size_t)n * w, kind, s));
+        }
         if (col.validity) {
             const uint8_t* src = col.validity;
             DevBuf tmp;
@@ -880,88 +1067,120 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
     j.nbuckets = nb;
     const uint64_t ncells = nb * BUCKET;
     int cap = j.dev->num_cus * 16;
-    VR_TRY(j.buckets.ensure(j.dev, nb * 16 * sizeof(uint64_t)));
-    hipLaunchKernelGGL(fill_keys_kernel, dim3(grid_of(nb * 16, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(), nb);
-    VR_TRY(j.row_cell.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
-    VR_TRY(j.cell_count.ensure(j.dev, ncells * 4 + 8));
-    VR_TRY(j.cell_start.ensure(j.dev, ncells * 4 + 8));
-    VR_TRY(j.small.ensure(j.dev, 64));
-    const uint64_t passed_words = (nb + 63) / 64;  // one bit per bucket, in 64-bit words
-    DevBuf passed;
-    VR_TRY(passed.ensure(j.dev, passed_words * 8));
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(ncells / 2 + 1, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint64_t>(),
-                       ncells / 2 + 1);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(passed_words, 256, cap)), dim3(256), 0, s, passed.as<uint64_t>(), passed_words);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>(), 8ull);
-    JoinTable t{j.buckets.as<uint64_t>(), nb, nullptr, 0};
-    DCol k[2]{};
-    for (size_t q = 0; q < j.key_types.size(); q++) {
-        auto& bc = j.cols[j.key_cols[q]];
-        k[q].type = j.key_types[q];
-        k[q].data = bc.data.p;
-        k[q].valid = bc.has_valid ? bc.valid.as<uint8_t>() : nullptr;
-    }
-    if (n > 0)
-        hipLaunchKernelGGL(join_insert_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, t, k[0], k[1],
-                           (int)j.key_types.size(), n, j.row_cell.as<uint32_t>(), j.cell_count.as<uint32_t>(),
-                           j.small.as<uint64_t>(), passed.as<uint32_t>());
-    VR_TRY_HIP(hipGetLastError());
-    // run starts, then the build rows ordered by cell (stable: build order inside a key)
-    size_t tmp = 0;
-    VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, j.cell_count.as<uint32_t>(), j.cell_start.as<uint32_t>(),
-                                                (int64_t)ncells, s));
-    VR_TRY(j.cub_tmp.ensure(j.dev, tmp));
-    VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(j.cub_tmp.p, tmp, j.cell_count.as<uint32_t>(), j.cell_start.as<uint32_t>(),
-                                                (int64_t)ncells, s));
-    VR_TRY(j.range_rows.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
-    VR_TRY(j.sorted_cells.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
-    VR_TRY(j.rows_in.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
-    if (n > 0) {
-        hipLaunchKernelGGL(iota_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, j.rows_in.as<uint32_t>(), n);
-        size_t t2 = 0;
-        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, j.row_cell.as<uint32_t>(), j.sorted_cells.as<uint32_t>(),
-                                                      j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
-        VR_TRY(j.cub_tmp.ensure(j.dev, std::max(tmp, t2)));
-        VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(j.cub_tmp.p, t2, j.row_cell.as<uint32_t>(), j.sorted_cells.as<uint32_t>(),
-                                                      j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
-    }
-    hipLaunchKernelGGL(join_values_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, t, j.cell_count.as<uint32_t>(),
-                       j.cell_start.as<uint32_t>(), ncells, j.range_rows.as<uint32_t>(), passed.as<uint32_t>());
-    VR_TRY_HIP(hipGetLastError());
-    // statistics (distinct keys, longest chain) reduced on the device into small[1], small[2]
-    hipLaunchKernelGGL(cell_stats_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint32_t>(),
-                       ncells, j.small.as<uint64_t>());
-    VR_TRY_HIP(hipGetLastError());
-    const bool dense_candidate = j.dense_mode == 0 && j.key_types.size() == 1 && n > 0;
-    if (dense_candidate) {
-        hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>());
-        hipLaunchKernelGGL(key_range_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(), nb,
-                           j.key_types[0], j.small.as<uint64_t>());
-        VR_TRY_HIP(hipGetLastError());
-    }
-    // the key equal to the empty marker: its rows follow every cell's rows in range_rows
+    const bool dense_candidate = j.dense_mode == 0 && j.key_types.size() == 1 && n > 0 &&
+                                 (j.key_types[0] == VR_INT32 || j.key_types[0] == VR_INT64);
+    DevBuf hashes, hvalid;  // wide keys: the build rows' hashes and which rows have no null key lane
     uint64_t small[5] = {0, 0, 0, 0, 0};
-    uint32_t last[2] = {0, 0};
-    VR_TRY_HIP(hipMemcpyAsync(small, j.small.p, sizeof(small), hipMemcpyDeviceToHost, s));
-    VR_TRY_HIP(hipMemcpyAsync(&last[0], j.cell_start.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
-    VR_TRY_HIP(hipMemcpyAsync(&last[1], j.cell_count.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
-    VR_TRY_HIP(hipStreamSynchronize(s));
-    const uint64_t sentinel_rows = small[0];
-    j.sentinel_value = 0;
-    if (sentinel_rows) {
-        // sorted by cell, the 0xFFFFFFFE rows sit just before the 0xFFFFFFFF (null) rows, in build order
-        uint32_t total_cells_rows = last[0] + last[1];
-        uint64_t payload = total_cells_rows;
-        if (sentinel_rows == 1) {
-            uint32_t row = 0;
-            VR_TRY_HIP(hipMemcpyAsync(&row, j.range_rows.as<uint32_t>() + total_cells_rows, 4, hipMemcpyDeviceToHost, s));
-            VR_TRY_HIP(hipStreamSynchronize(s));
-            payload = row;
+    uint64_t sentinel_rows = 0;
+    for (int attempt = 0;; attempt++) {
+        VR_TRY(j.buckets.ensure(j.dev, nb * 16 * sizeof(uint64_t)));
+        hipLaunchKernelGGL(fill_keys_kernel, dim3(grid_of(nb * 16, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(), nb);
+        VR_TRY(j.row_cell.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+        VR_TRY(j.cell_count.ensure(j.dev, ncells * 4 + 8));
+        VR_TRY(j.cell_start.ensure(j.dev, ncells * 4 + 8));
+        VR_TRY(j.small.ensure(j.dev, 64));
+        const uint64_t passed_words = (nb + 63) / 64;  // one bit per bucket, in 64-bit words
+        DevBuf passed;
+        VR_TRY(passed.ensure(j.dev, passed_words * 8));
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(ncells / 2 + 1, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint64_t>(),
+                           ncells / 2 + 1);
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(passed_words, 256, cap)), dim3(256), 0, s, passed.as<uint64_t>(), passed_words);
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>(), 8ull);
+        JoinTable t{j.buckets.as<uint64_t>(), nb, nullptr, 0, j.wide ? 1 : 0, 0};
+        if (j.wide) {  // the rows' hashes, inserted as 64-bit keys
+            VR_TRY(hashes.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 8));
+            VR_TRY(hvalid.ensure(j.dev, ((size_t)(n + 31) / 32 + 2) * 4));
+            const WideKeys wk = wide_keys(j);
+            if (n > 0) {
+                hipLaunchKernelGGL(wide_hash_kernel, dim3(grid_of((n + 31) / 32, 256, cap)), dim3(256), 0, s, wk, n,
+                                   hashes.as<uint64_t>(), hvalid.as<uint32_t>());
+                const DCol hk{hvalid.as<uint8_t>(), hashes.p, nullptr, VR_INT64, 0};
+                hipLaunchKernelGGL(join_insert_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, t, hk, DCol{}, 1, n,
+                                   j.row_cell.as<uint32_t>(), j.cell_count.as<uint32_t>(), j.small.as<uint64_t>(),
+                                   passed.as<uint32_t>());
+            }
+        } else {
+            DCol k[2]{};
+            for (size_t q = 0; q < j.key_types.size(); q++) {
+                auto& bc = j.cols[j.key_cols[q]];
+                k[q].type = j.key_types[q];
+                k[q].data = bc.data.p;
+                k[q].valid = bc.has_valid ? bc.valid.as<uint8_t>() : nullptr;
+            }
+            if (n > 0)
+                hipLaunchKernelGGL(join_insert_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, t, k[0], k[1],
+                                   (int)j.key_types.size(), n, j.row_cell.as<uint32_t>(), j.cell_count.as<uint32_t>(),
+                                   j.small.as<uint64_t>(), passed.as<uint32_t>());
         }
-        j.sentinel_value = (sentinel_rows << 32) | payload;
+        VR_TRY_HIP(hipGetLastError());
+        // run starts, then the build rows ordered by cell (stable: build order inside a key)
+        size_t tmp = 0;
+        VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, j.cell_count.as<uint32_t>(), j.cell_start.as<uint32_t>(),
+                                                    (int64_t)ncells, s));
+        VR_TRY(j.cub_tmp.ensure(j.dev, tmp));
+        VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(j.cub_tmp.p, tmp, j.cell_count.as<uint32_t>(), j.cell_start.as<uint32_t>(),
+                                                    (int64_t)ncells, s));
+        VR_TRY(j.range_rows.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+        VR_TRY(j.sorted_cells.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+        VR_TRY(j.rows_in.ensure(j.dev, (size_t)std::max<int64_t>(n, 1) * 4));
+        if (n > 0) {
+            hipLaunchKernelGGL(iota_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, j.rows_in.as<uint32_t>(), n);
+            size_t t2 = 0;
+            VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, j.row_cell.as<uint32_t>(), j.sorted_cells.as<uint32_t>(),
+                                                          j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
+            VR_TRY(j.cub_tmp.ensure(j.dev, std::max(tmp, t2)));
+            VR_TRY_HIP(hipcub::DeviceRadixSort::SortPairs(j.cub_tmp.p, t2, j.row_cell.as<uint32_t>(), j.sorted_cells.as<uint32_t>(),
+                                                          j.rows_in.as<uint32_t>(), j.range_rows.as<uint32_t>(), (int)n, 0, 32, s));
+        }
+        hipLaunchKernelGGL(join_values_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, t, j.cell_count.as<uint32_t>(),
+                           j.cell_start.as<uint32_t>(), ncells, j.range_rows.as<uint32_t>(), passed.as<uint32_t>());
+        VR_TRY_HIP(hipGetLastError());
+        // statistics (distinct keys, longest chain) reduced on the device into small[1], small[2]
+        hipLaunchKernelGGL(cell_stats_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.cell_count.as<uint32_t>(),
+                           ncells, j.small.as<uint64_t>());
+        VR_TRY_HIP(hipGetLastError());
+        if (dense_candidate) {
+            hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>());
+            hipLaunchKernelGGL(key_range_kernel, dim3(grid_of(ncells, 256, cap)), dim3(256), 0, s, j.buckets.as<uint64_t>(), nb,
+                               j.key_types[0], j.small.as<uint64_t>());
+            VR_TRY_HIP(hipGetLastError());
+        }
+        // the key equal to the empty marker: its rows follow every cell's rows in range_rows
+        uint32_t last[2] = {0, 0};
+        VR_TRY_HIP(hipMemcpyAsync(small, j.small.p, sizeof(small), hipMemcpyDeviceToHost, s));
+        VR_TRY_HIP(hipMemcpyAsync(&last[0], j.cell_start.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
+        VR_TRY_HIP(hipMemcpyAsync(&last[1], j.cell_count.as<uint32_t>() + ncells - 1, 4, hipMemcpyDeviceToHost, s));
+        VR_TRY_HIP(hipStreamSynchronize(s));
+        sentinel_rows = small[0];
+        j.sentinel_value = 0;
+        if (sentinel_rows) {
+            // sorted by cell, the 0xFFFFFFFE rows sit just before the 0xFFFFFFFF (null) rows, in build order
+            uint32_t total_cells_rows = last[0] + last[1];
+            uint64_t payload = total_cells_rows;
+            if (sentinel_rows == 1) {
+                uint32_t row = 0;
+                VR_TRY_HIP(hipMemcpyAsync(&row, j.range_rows.as<uint32_t>() + total_cells_rows, 4, hipMemcpyDeviceToHost, s));
+                VR_TRY_HIP(hipStreamSynchronize(s));
+                payload = row;
+            }
+            j.sentinel_value = (sentinel_rows << 32) | payload;
+        }
+        j.distinct = (int64_t)small[1] + (sentinel_rows ? 1 : 0);
+        j.max_chain = std::max<int64_t>((int64_t)small[2], (int64_t)sentinel_rows);
+        if (!j.wide || n == 0) break;
+        // wide keys: every run must hold one key (two keys with one 64-bit hash share a cell)
+        uint64_t collisions = 0;
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, j.small.as<uint64_t>() + 6, 1ull);
+        hipLaunchKernelGGL(wide_verify_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, wide_keys(j), n,
+                           j.row_cell.as<uint32_t>(), j.cell_start.as<uint32_t>(), j.range_rows.as<uint32_t>(),
+                           last[0] + last[1], j.small.as<uint64_t>() + 6);
+        VR_TRY_HIP(hipGetLastError());
+        VR_TRY_HIP(hipMemcpyAsync(&collisions, j.small.as<uint64_t>() + 6, 8, hipMemcpyDeviceToHost, s));
+        VR_TRY_HIP(hipStreamSynchronize(s));
+        if (!collisions) break;
+        if (attempt == 3) return fail(VR_ERR_UNSUPPORTED, "wide join keys whose hashes keep colliding run on the CPU kernels");
+        j.seed = fmix64(j.seed + (uint64_t)attempt + 1);  // build again with another hash
     }
-    j.distinct = (int64_t)small[1] + (sentinel_rows ? 1 : 0);
-    j.max_chain = std::max<int64_t>((int64_t)small[2], (int64_t)sentinel_rows);
     VR_TRY(build_dense(j, dense_candidate, (int64_t)small[3], (int64_t)small[4], sentinel_rows != 0, s));
     if (j.track) {
         const uint64_t words = ((uint64_t)std::max<int64_t>(n, 1) + 63) / 64;
@@ -1094,7 +1313,8 @@ static vr_status prepare_probe(JoinImpl& j, const vr_column* columns, int32_t nu
                                ProbeShape& sh) {
     const bool pairs = join_type == VR_JOIN_INNER || join_type == VR_JOIN_OUTER;
     sh.pairs = pairs;
-    a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets, j.range_rows.as<uint32_t>(), j.sentinel_value};
+    a.t = JoinTable{j.buckets.as<uint64_t>(), j.nbuckets, j.range_rows.as<uint32_t>(), j.sentinel_value, j.wide ? 1 : 0, 0};
+    if (j.wide) a.bk = wide_keys(j);
     a.dense = DenseIndex{j.dense_range ? j.dense.as<uint64_t>() : nullptr, j.dense_min, j.dense_range};
     a.matched = (pairs && j.track) ? j.matched.as<uint32_t>() : nullptr;
     a.nk = (int)j.key_types.size();
@@ -1126,6 +1346,7 @@ static vr_status prepare_probe(JoinImpl& j, const vr_column* columns, int32_t nu
     for (int q = 0; q < a.nbuild_out; q++) {
         const int bc = out->build_out_columns[q];
         if (bc < 0 || bc >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "build output column out of range");
+        if (j.col_types[bc] == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "UTF8/BOOL outputs run on the CPU kernels");
         a.build_src[q].type = j.col_types[bc];
         a.build_src[q].data = j.cols[bc].data.p;
         a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
@@ -1190,10 +1411,18 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
         *dst = p;
         if (p && bytes && mem == VR_MEM_HOST) pieces.push_back({p, bytes, dst});
     };
+    std::vector<DevBuf> key_ids;  // UTF8 keys: the rows' ids, read by the probe kernel
     for (size_t q = 0; q < sh.slot_col.size(); q++) {
         const vr_column& c = columns[sh.slot_col[q]];
         want(c.validity, bitmap_bytes(n), (const void**)&a.one_cols[q].valid, c.mem);
-        want(c.data, lane_bytes(c.type, n), &a.one_cols[q].data, c.mem);
+        if (q < (size_t)a.nk && c.type == VR_UTF8) {
+            key_ids.emplace_back();
+            VR_TRY(key_ids.back().ensure(j.dev, (size_t)n * 4));
+            VR_TRY(map_key_strings(j, (int)q, c, n, false, key_ids.back().as<int32_t>(), s));
+            a.one_cols[q].data = key_ids.back().p;
+        } else {
+            want(c.data, lane_bytes(c.type, n), &a.one_cols[q].data, c.mem);
+        }
     }
     if (selection && columns[key_columns[0]].mem == VR_MEM_HOST)
         want(selection, bitmap_bytes(n), (const void**)&a.one.sel, VR_MEM_HOST);
@@ -1252,6 +1481,7 @@ vr_status vr_join_probe_noalloc(vr_join* h, const vr_column* columns, int32_t nu
     for (size_t q = 0; q < sh.slot_col.size(); q++) {
         const vr_column& c = columns[sh.slot_col[q]];
         if (c.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "the allocation-free probe reads device-resident columns only");
+        if (c.type == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "the allocation-free probe takes no UTF8 keys (their ids need memory)");
         a.one_cols[q] = PCol{c.validity, c.data};
     }
     a.one = PBatch{n, 0, selection, 0};
@@ -1297,6 +1527,8 @@ vr_status vr_join_probe_batches(vr_join* h, int32_t num_batches, const vr_column
     std::vector<PBatch> batches((size_t)num_batches);
     std::vector<PCol> bcols((size_t)num_batches * ncb);
     int64_t tiles = 0, rows_in = 0;
+    hipStream_t s = j.dev->pick(stream);
+    std::vector<DevBuf> key_ids;  // UTF8 keys: each batch's ids, read by the probe kernel
     for (int32_t b = 0; b < num_batches; b++) {
         const int64_t n = lengths[b];
         if (n < 0) return fail(VR_ERR_INVALID_ARGUMENT, "negative row count");
@@ -1308,6 +1540,12 @@ vr_status vr_join_probe_batches(vr_join* h, int32_t num_batches, const vr_column
             if (n && c.mem != VR_MEM_DEVICE) return fail(VR_ERR_UNSUPPORTED, "batched probes read device-resident batches");
             if (n && c.length < n) return fail(VR_ERR_INVALID_ARGUMENT, "a batch's column is shorter than the batch");
             bcols[(size_t)b * ncb + q] = PCol{c.validity, c.data};
+            if (n && q < a.nk && c.type == VR_UTF8) {
+                key_ids.emplace_back();
+                VR_TRY(key_ids.back().ensure(j.dev, (size_t)n * 4));
+                VR_TRY(map_key_strings(j, q, c, n, false, key_ids.back().as<int32_t>(), s));
+                bcols[(size_t)b * ncb + q].data = key_ids.back().p;
+            }
         }
         batches[b] = PBatch{n, tiles, selections ? selections[b] : nullptr, 0};
         tiles += (n + PROBE_TILE - 1) / PROBE_TILE;
@@ -1318,7 +1556,6 @@ vr_status vr_join_probe_batches(vr_join* h, int32_t num_batches, const vr_column
         return VR_OK;
     }
     if (tiles >= (1ll << 31)) return fail(VR_ERR_UNSUPPORTED, "more than 2^31 probe tiles in one call");
-    hipStream_t s = j.dev->pick(stream);
     // the call's device tables: batches, their columns, tile -> batch, tile states, batch ends
     const size_t off_cols = align_up(sizeof(PBatch) * (size_t)num_batches, 256);
     const size_t off_map = off_cols + align_up(sizeof(PCol) * bcols.size(), 256);
@@ -1408,6 +1645,7 @@ vr_status vr_join_unmatched_build(vr_join* h, vr_probe_output* out, void* stream
     for (int q = 0; q < a.nbuild_out; q++) {
         int bc = out->build_out_columns[q];
         if (bc < 0 || bc >= (int)j.cols.size()) return fail(VR_ERR_INVALID_ARGUMENT, "build output column out of range");
+        if (j.col_types[bc] == VR_UTF8) return fail(VR_ERR_UNSUPPORTED, "UTF8/BOOL outputs run on the CPU kernels");
         a.build_src[q].type = j.col_types[bc];
         a.build_src[q].data = j.cols[bc].data.p;
         a.build_src[q].valid = j.cols[bc].has_valid ? j.cols[bc].valid.as<uint8_t>() : nullptr;
@@ -1431,6 +1669,7 @@ vr_status vr_join_memory(vr_join* h, int64_t* bytes) {
     int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.dense.bytes +
                           j.matched.bytes + j.unmatched.bytes);
     for (auto& c : j.cols) b += (int64_t)(c.data.bytes + c.valid.bytes);
+    for (auto& t : j.strs) b += t.memory();
     *bytes = b;
     return VR_OK;
 }

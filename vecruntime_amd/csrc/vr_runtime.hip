@@ -184,53 +184,6 @@ vr_status Staging::done(int which, hipStream_t compute) {
     return VR_OK;
 }
 
-uint64_t hash_bytes64(const void* p, size_t n) {
-    // FNV-1a over the bytes, then a 64-bit finalizer (murmur3 fmix64).
-    const uint8_t* b = (const uint8_t*)p;
-    uint64_t h = 0xcbf29ce484222325ull ^ n;
-    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
-    h ^= h >> 33;
-    h *= 0xff51afd7ed558ccdull;
-    h ^= h >> 33;
-    h *= 0xc4ceb9fe1a85ec53ull;
-    h ^= h >> 33;
-    return h;
-}
-
-int32_t HostDictionary::index_of(const char* s, int32_t len, bool insert) {
-    uint64_t fp = hash_bytes64(s, (size_t)len);
-    if (index.empty()) {
-        index.assign(1024, {0, -1});
-        mask = 1023;
-    }
-    size_t pos = fp & mask;
-    while (true) {
-        auto& e = index[pos];
-        if (e.second < 0) break;
-        if (e.first == fp && values[e.second].size() == (size_t)len &&
-            std::memcmp(values[e.second].data(), s, (size_t)len) == 0)
-            return e.second;
-        pos = (pos + 1) & mask;
-    }
-    if (!insert) return -1;
-    int32_t id = (int32_t)values.size();
-    values.emplace_back(s, (size_t)len);
-    index[pos] = {fp, id};
-    if (values.size() * 2 > index.size()) {  // grow at half load
-        std::vector<std::pair<uint64_t, int32_t>> old;
-        old.swap(index);
-        index.assign(old.size() * 2, {0, -1});
-        mask = index.size() - 1;
-        for (auto& e : old) {
-            if (e.second < 0) continue;
-            size_t p = e.first & mask;
-            while (index[p].second >= 0) p = (p + 1) & mask;
-            index[p] = e;
-        }
-    }
-    return id;
-}
-
 static void device_free(vr_device* dev) {
     (void)hipStreamSynchronize(dev->d.stream);
     (void)hipStreamSynchronize(dev->d.copy_stream);

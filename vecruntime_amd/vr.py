@@ -482,10 +482,11 @@ def agg_key_dictionary(handle, c):
     return [raw[offs[i]:offs[i + 1]].decode("utf-8") for i in range(count.value)]
 
 
-def device_column(type, data, validity=None, length=None, dictionary=None):
-    """An input column over device tensors (torch); `dictionary` is a host Column."""
+def device_column(type, data, validity=None, length=None, dictionary=None, offsets=None):
+    """An input column over device tensors (torch); `dictionary` is a host Column; a plain UTF8
+    column passes its bytes as `data` and its int32 `offsets`."""
     from .columns import Column
-    return Column(type, data, validity, dictionary=dictionary, length=length, mem=abi.MEM_DEVICE)
+    return Column(type, data, validity, offsets=offsets, dictionary=dictionary, length=length, mem=abi.MEM_DEVICE)
 
 
 def device_out_column(type, capacity, data, validity=None):
