@@ -78,7 +78,7 @@ def gen_groupby(n, seed=102):
     return keys, dec, dbl, i32, valid
 
 
-def groupby_spec(capacity_hint):
+def groupby_spec(capacity_hint, max_load_percent=0):
     from vecruntime_amd import abi
     return abi.AggSpec(
         [abi.INT64], [abi.INT64, abi.FLOAT64, abi.INT32],
@@ -86,7 +86,7 @@ def groupby_spec(capacity_hint):
          abi.agg_desc(abi.AVG_WIDE, 0, buffer_precision=22, buffer_scale=2),
          abi.agg_desc(abi.AVG_DOUBLE, 1), abi.agg_desc(abi.MIN, 2), abi.agg_desc(abi.MAX, 2),
          abi.agg_desc(abi.COUNT_STAR)],
-        key_nullable=[False], capacity_hint=capacity_hint)
+        key_nullable=[False], capacity_hint=capacity_hint, max_load_percent=max_load_percent)
 
 
 def groupby_batches(keys, dec, dbl, i32, valid, n):
@@ -157,9 +157,44 @@ def verify_groupby(keys, dec, dbl, i32, valid, n, groups, ek, eo):
     return out
 
 
+def tune_groupby(dev, args, batches, stream):
+    """The autotuner's descent over the partitioned path's knobs and the load factor, each
+    configuration timed (median of 3) on the first quarter of the batches -- outside the timed
+    steps; cached per device and workload."""
+    import torch
+    from vecruntime_amd import autotune, vr
+    sample = vr.GroupedAggregate.prepare_batches(batches[: max(1, len(batches) // 4)])
+    srows = sum(b[2] for b in batches[: max(1, len(batches) // 4)])
+    handles = {}
+
+    def run(c):
+        m = c["max_load_percent"]
+        if m not in handles:
+            handles[m] = vr.GroupedAggregate(dev, groupby_spec(12_000_000, m))
+        h = handles[m]
+        h.reset()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        h.update_batches(sample, config=autotune.to_launch_config(c), stream=stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    key = autotune.signature(torch.cuda.get_device_properties(0).name, groupby_spec(0), "groupby-zipf1.1-10M")
+    t0 = time.time()
+    best, trials = autotune.tuned(key, run, autotune.SPACES["partitioned"], reps=3, budget_s=args.tune_budget)
+    for h in handles.values():
+        h.close()
+    info = {"sample_rows": srows, "seconds": time.time() - t0, "configurations": len(trials) if trials else None}
+    if trials:
+        info["sample_ms_best"] = trials[0][0] * 1e3
+        info["sample_ms_default"] = next((t * 1e3 for t, c in trials if c == autotune.DEFAULT), None)
+    log("autotune (group-by): %s, %s" % (best, json.dumps(info)))
+    return best, info
+
+
 def run_groupby(dev, args, n):
     import torch
-    from vecruntime_amd import abi, vr
+    from vecruntime_amd import abi, autotune, vr
     log("generating %d rows of the group-by input on the device" % n)
     t0 = time.time()
     keys, dec, dbl, i32, valid = gen_groupby(n)
@@ -167,14 +202,16 @@ def run_groupby(dev, args, n):
     log("generated in %.1f s: %d batches" % (time.time() - t0, len(batches)))
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
-    agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000))
+    best, tuning = tune_groupby(dev, args, batches, stream)
+    tuned_cfg = autotune.to_launch_config(best)
+    agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000, best["max_load_percent"]))
 
     batch_set = vr.GroupedAggregate.prepare_batches(batches)  # as the plain path's prepare(), once
 
-    def step_batched(emit_cols):
-        agg.reset()
-        agg.update_batches(batch_set, stream=stream)
-        agg.emit(0, emit_cols[2], emit_cols[0], emit_cols[1], stream=stream)
+    def step_batched(emit_cols, h=agg, cfg=tuned_cfg):
+        h.reset()
+        h.update_batches(batch_set, config=cfg, stream=stream)
+        h.emit(0, emit_cols[2], emit_cols[0], emit_cols[1], stream=stream)
 
     prepared = [(vr.GroupedAggregate.prepare(k, v), m) for k, v, m in batches]
 
@@ -184,7 +221,7 @@ def run_groupby(dev, args, n):
             agg.update_prepared(prep, m, stream=stream)
         agg.emit(0, emit_cols[2], emit_cols[0], emit_cols[1], stream=stream)
 
-    agg.update_batches(batches, stream=stream)
+    agg.update_batches(batch_set, config=tuned_cfg, stream=stream)
     groups = agg.num_groups()
     ek, eo = emit_buffers(agg, groups)
     cols = (ek, eo, groups)
@@ -204,6 +241,23 @@ def run_groupby(dev, args, n):
         dt = (time.perf_counter() - t0) / steps
         res[name] = dict(ms_per_step=dt * 1e3, rows_per_s=n / dt, steps=steps)
         log("%s: %.2f ms/step, %.3e rows/s" % (name, dt * 1e3, n / dt))
+    # the library's defaults beside the tuned configuration (a handle of the default load factor)
+    res["tuning"] = dict(tuning, tuned=best)
+    if best != autotune.DEFAULT:
+        dh = agg if best["max_load_percent"] == 50 else vr.GroupedAggregate(dev, groupby_spec(12_000_000))
+        dcfg = abi.launch_config()
+        step_batched(cols, dh, dcfg)  # warm (sizes the table)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step_batched(cols, dh, dcfg)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        res["tuning"]["default_ms_per_step"] = dt * 1e3
+        if dh is not agg:
+            dh.close()
+        log("default configuration: %.2f ms/step" % (dt * 1e3))
+    res["tuning"]["tuned_ms_per_step"] = res["batched"]["ms_per_step"]
     # a cold table: a fresh handle with no capacity hint (2,048 slots at first, grown as groups
     # arrive), created, updated and emitted inside the timed region
     cold = []
@@ -458,20 +512,28 @@ def run_uneven(dev, args):
     props = torch.cuda.get_device_properties(0)
     key = autotune.signature(props.name, uneven_spec(0), "uneven-filter30")
     t0 = time.time()
-    best, results = autotune.tuned(key, run, autotune.candidates(), reps=2, budget_s=args.tune_budget)
+    best, results = autotune.tuned(key, run, autotune.SPACES["persistent"], reps=3, budget_s=args.tune_budget)
     for h in handles.values():
         h.close()
-    log("autotune: %s (%.1f s%s)" % (best, time.time() - t0, "" if results else ", cached"))
+    log("autotune: %s (%.1f s, %s configurations%s)" % (best, time.time() - t0, len(results) if results else "-",
+                                                         "" if results else ", cached"))
     default_cfg = abi.launch_config()
     tuned_cfg = autotune.to_launch_config(best)
     agg = vr.GroupedAggregate(dev, uneven_spec(int(passing * 0.8), best["max_load_percent"]))
     agg.set_launch_config(tuned_cfg)  # the per-batch launches use the tuned shape too
     prepared = [(vr.GroupedAggregate.prepare(k, v), n) for k, v, n in batches]
-    res = {"rows": total, "batches": len(batches), "passing_rows": passing, "tuned": best}
+    res = {"rows": total, "batches": len(batches), "passing_rows": passing, "tuned": best,
+           "tuning": {"sample_rows": srows, "configurations": len(results) if results else None,
+                      "sample_ms_best": results[0][0] * 1e3 if results else None,
+                      "sample_ms_default": next((t * 1e3 for t, c in results if c == autotune.DEFAULT), None)
+                      if results else None}}
 
-    def batched(cfg):
-        agg.reset()
-        agg.update_batches(batches, filter_col=0, threshold=thr, config=cfg, stream=stream)
+    # the default configuration on a handle of the default load factor
+    agg_def = agg if best["max_load_percent"] == 50 else vr.GroupedAggregate(dev, uneven_spec(int(passing * 0.8), 50))
+
+    def batched(cfg, h=agg):
+        h.reset()
+        h.update_batches(batches, filter_col=0, threshold=thr, config=cfg, stream=stream)
         torch.cuda.synchronize()
 
     def plain():
@@ -490,8 +552,9 @@ def run_uneven(dev, args):
         torch.cuda.synchronize()
 
     groups = {}
-    for name, fn in (("batched_tuned", lambda: batched(tuned_cfg)), ("batched_default", lambda: batched(default_cfg)),
-                     ("plain", plain), ("graph", graphed)):
+    for name, fn, h in (("batched_tuned", lambda: batched(tuned_cfg), agg),
+                        ("batched_default", lambda: batched(default_cfg, agg_def), agg_def),
+                        ("plain", plain, agg), ("graph", graphed, agg)):
         fn()  # warm (and sizes the table)
         ts = []
         for _ in range(max(1, args.uneven_steps)):
@@ -499,7 +562,7 @@ def run_uneven(dev, args):
             fn()
             ts.append(time.perf_counter() - t)
         dt = min(ts)
-        groups[name] = agg.num_groups()
+        groups[name] = h.num_groups()
         res[name] = dict(ms=dt * 1e3, rows_per_s=total / dt)
         log("uneven %s: %.1f ms, %.3e rows/s, %d groups" % (name, dt * 1e3, total / dt, groups[name]))
     # the captured step's device time, with HIP events on its stream
@@ -524,6 +587,8 @@ def run_uneven(dev, args):
     res["groups"] = groups["plain"]
     res["paths_agree"] = len(set(groups.values())) == 1
     graph[0].close()
+    if agg_def is not agg:
+        agg_def.close()
     agg.close()
     del batches, prepared, cols
     torch.cuda.empty_cache()
@@ -1071,7 +1136,7 @@ def main():
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--uneven-steps", type=int, default=2)
-    ap.add_argument("--tune-budget", type=float, default=60.0)
+    ap.add_argument("--tune-budget", type=float, default=45.0)
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-pcie", action="store_true")
     ap.add_argument("--skip-filters", action="store_true")
