@@ -1,7 +1,7 @@
 """Host-resident input through the library's staging (MI355X): batches in pageable host memory
-(copied into the library's pinned buffers) and in page-locked host memory (copied by DMA straight
-from the caller's buffers) give the oracle's groups and join rows; a caller that overwrites its
-page-locked buffer right after a call does not change that call's result."""
+(copied into the library's pinned buffers) and in page-locked host memory (large buffers copied by
+DMA straight from the caller's) give the oracle's groups and join rows; a caller that overwrites
+its page-locked buffer right after a call does not change that call's result."""
 import numpy as np
 import pytest
 
@@ -50,7 +50,7 @@ def test_host_batches_match_oracle(dev, memory):
     rng = np.random.default_rng(77)
     g, o = vr.GroupedAggregate(dev, SPEC), OracleAgg(SPEC)
     keep = []
-    for n in (65_536, 1, 40_000, 65_536, 777):
+    for n in (65_536, 1, 40_000, 600_000, 65_536, 777):  # 600,000 rows: 4.8 MB lanes, copied by DMA as they are
         k, v0, v1, v2, vb = batch(rng, n)
         o.update([Column.longs(k)], [Column.longs(v0), Column.doubles(v1), Column(abi.INT32, v2, vb.copy(), length=n)], n)
         if memory == "pinned":
@@ -96,3 +96,44 @@ def test_host_probe_matches_oracle(dev, memory):
         if jt == abi.INNER:
             assert list(gb[:r]) == list(want_b)
     bt.close()
+
+
+@pytest.mark.parametrize("memory", ["pageable", "pinned"])
+def test_host_batch_set_in_chunks_matches_oracle(dev, memory):
+    """vr_agg_update_batches over host-resident batches (about 400 MB, so the library stages them in
+    more than one chunk) with the fused filter: the groups equal the oracle's."""
+    from vecruntime_amd import vr
+    rng = np.random.default_rng(79)
+    n_batches, bn = 220, 65_536
+    N = n_batches * bn + 777
+    k = (rng.integers(0, 20_000, N) * 1_000_003).astype(np.int64)
+    v0 = rng.integers(-10 ** 9, 10 ** 9, N)
+    v1 = rng.lognormal(0, 1, N)
+    v2 = rng.integers(-2 ** 31, 2 ** 31, N).astype(np.int32)
+    nulls = rng.random(N) < 0.05
+    vb = pack_bits(~nulls)
+    keep = []
+    if memory == "pinned":
+        (k, tk), (v0, t0), (v1, t1), (v2, t2), (vb, tb) = (pinned(a) for a in (k, v0, v1, v2, vb))
+        keep += [tk, t0, t1, t2, tb]
+    batches = []
+    for s0 in range(0, N, bn):
+        m = min(bn, N - s0)
+        batches.append(([Column.longs(k[s0:s0 + m])],
+                        [Column.longs(v0[s0:s0 + m]), Column.doubles(v1[s0:s0 + m]),
+                         Column(abi.INT32, v2[s0:s0 + m], vb[s0 // 8:], length=m)], m))
+    thr = 0
+    g, o = vr.GroupedAggregate(dev, SPEC), OracleAgg(SPEC)
+    g.update_batches(batches, filter_col=2, threshold=thr)
+    for kc, vc, m in batches:
+        f = np.asarray(vc[2].data[:m])
+        ok = f < thr
+        valid = np.unpackbits(np.asarray(vc[2].validity[: (m + 7) // 8]), bitorder="little", count=m).astype(bool)
+        o.update(kc, vc, m, pack_bits(ok & valid))
+    G = o.num_groups()
+    assert g.num_groups() == G
+    gk, go = g.emit()
+    ok_, oo = o.emit()
+    worst = compare_groups(group_rows(gk, go, G, SPEC, {}), group_rows(ok_, oo, G, SPEC, {}), out_roles(SPEC))
+    assert worst < 1e-12
+    g.close()

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <future>
 #include <memory>
 
 #include "vr_agg_part.h"
@@ -2133,6 +2134,98 @@ vr_status vr_agg_update_filtered(vr_agg* agg, const vr_column* keys, const vr_co
     return st;
 }
 
+// Host-resident batch sets: consecutive batches are staged in chunks of up to 256 MiB (page-locked
+// buffers copied by DMA as they are, pageable ones through the pinned buffer by several host
+// threads, consecutive pieces in one copy) on the copy stream, and each chunk is folded by the
+// device-resident batched update while the copies of the next chunk run.
+static vr_status update_batches_from_host(vr_agg* agg, const vr_batch_set* bs, const vr_launch_config* cfg, hipStream_t s) {
+    AggImpl& h = agg->h;
+    const int nb = bs->num_batches;
+    const int nk = (int)h.key_types.size(), nv = (int)h.value_types.size();
+    constexpr size_t CHUNK = (size_t)256 << 20;
+    // the bytes of a host column of n rows to stage, and its pieces (pointers patched in `col`)
+    auto col_bytes = [](const vr_column& col, int64_t n) -> size_t {
+        if (col.mem != VR_MEM_HOST || n <= 0) return 0;
+        size_t b = col.validity ? bitmap_bytes(n) + 256 : 0;
+        if (col.type == VR_UTF8 && !col.dictionary) return b + (size_t)(n + 1) * 4 + (col.offsets ? (size_t)std::max(col.offsets[n], 0) : 0) + 512;
+        return b + lane_bytes(col.type, n) + 256;
+    };
+    auto col_pieces = [](vr_column& col, int64_t n, std::vector<StagePiece>& pieces) {
+        if (col.mem != VR_MEM_HOST || n <= 0) return;
+        col.mem = VR_MEM_DEVICE;
+        if (col.validity) pieces.push_back({col.validity, bitmap_bytes(n), (const void**)&col.validity});
+        if (col.type == VR_UTF8 && !col.dictionary) {
+            if (!col.offsets) return;  // refused by the update
+            pieces.push_back({col.offsets, (size_t)(n + 1) * 4, (const void**)&col.offsets});
+            const size_t nbytes = (size_t)std::max(col.offsets[n], 0);
+            if (nbytes) pieces.push_back({col.data, nbytes, &col.data});
+        } else if (lane_bytes(col.type, n)) {
+            pieces.push_back({col.data, lane_bytes(col.type, n), &col.data});
+        }
+    };
+    struct Chunk {
+        int b0 = 0, b1 = 0;
+        std::vector<vr_column> keys, vals;
+        std::vector<StagePiece> pieces;
+        int slot = -1;
+    };
+    std::vector<std::pair<int, int>> bounds;
+    for (int b0 = 0; b0 < nb;) {
+        size_t bytes = 0;
+        int b1 = b0;
+        while (b1 < nb) {
+            size_t bb = 0;
+            for (int c = 0; c < nk; c++) bb += col_bytes(bs->keys[(size_t)b1 * nk + c], bs->lengths[b1]);
+            for (int c = 0; c < nv; c++) bb += col_bytes(bs->values[(size_t)b1 * nv + c], bs->lengths[b1]);
+            if (b1 > b0 && bytes + bb > CHUNK) break;
+            bytes += bb;
+            b1++;
+        }
+        bounds.push_back({b0, b1});
+        b0 = b1;
+    }
+    auto prepare = [&](Chunk& ch, int k) -> vr_status {
+        ch.b0 = bounds[(size_t)k].first;
+        ch.b1 = bounds[(size_t)k].second;
+        const int m = ch.b1 - ch.b0;
+        ch.keys.assign(bs->keys + (size_t)ch.b0 * nk, bs->keys + (size_t)ch.b1 * nk);
+        ch.vals.assign(bs->values + (size_t)ch.b0 * nv, bs->values + (size_t)ch.b1 * nv);
+        ch.pieces.clear();
+        for (int b = 0; b < m; b++) {
+            const int64_t n = bs->lengths[ch.b0 + b];
+            for (int c = 0; c < nk; c++) col_pieces(ch.keys[(size_t)b * nk + c], n, ch.pieces);
+            for (int c = 0; c < nv; c++) col_pieces(ch.vals[(size_t)b * nv + c], n, ch.pieces);
+        }
+        return h.staging.stage(h.dev, ch.pieces, s, &ch.slot, true);
+    };
+    Chunk cur, next;
+    VR_TRY(prepare(cur, 0));
+    for (size_t k = 0; k < bounds.size(); k++) {
+        if (cur.slot >= 0) VR_TRY(h.staging.ready(cur.slot, s));
+        // the next chunk's host copies and DMA on another thread, beside this chunk's fold
+        std::future<vr_status> staged;
+        if (k + 1 < bounds.size())
+            staged = std::async(std::launch::async, [&, k]() -> vr_status {
+                if (hipSetDevice(h.dev->ordinal) != hipSuccess) return VR_ERR_DEVICE;
+                return prepare(next, (int)k + 1);
+            });
+        vr_batch_set cb = *bs;
+        cb.num_batches = cur.b1 - cur.b0;
+        cb.keys = cur.keys.data();
+        cb.values = cur.vals.data();
+        cb.lengths = bs->lengths + cur.b0;
+        const vr_status st = vr_agg_update_batches(agg, &cb, cfg, s);
+        const vr_status st_next = staged.valid() ? staged.get() : VR_OK;
+        VR_TRY(st);
+        if (st_next != VR_OK) return fail(st_next, "staging a chunk of host-resident batches failed");
+        if (cur.slot >= 0) VR_TRY(h.staging.done(cur.slot, s));
+        std::swap(cur, next);
+    }
+    // every copy out of the caller's page-locked buffers has ended before the call returns
+    VR_TRY_HIP(hipStreamSynchronize(h.dev->copy_stream));
+    return VR_OK;
+}
+
 vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_launch_config* cfg, void* stream) {
     if (!agg || !bs) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     AggImpl& h = agg->h;
@@ -2161,6 +2254,13 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
         with_keys.keys = zkeys.data();
         bs = &with_keys;
     }
+    // host-resident columns: staged in chunks, each folded as a device-resident set
+    bool host = false;
+    for (int b = 0; b < nb && !host; b++) {
+        for (int c = 0; c < nk; c++) host |= bs->keys[(size_t)b * nk + c].mem == VR_MEM_HOST;
+        for (int c = 0; c < nv; c++) host |= bs->values[(size_t)b * nv + c].mem == VR_MEM_HOST;
+    }
+    if (host && !h.ordered) return update_batches_from_host(agg, bs, cfg, s);
     // One persistent launch needs device-resident columns and one dictionary per UTF8 key; anything
     // else (and an ordered fold, which orders each batch) takes the per-batch path.
     bool fused = !h.ordered;

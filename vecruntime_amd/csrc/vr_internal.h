@@ -121,8 +121,8 @@ struct StagePiece {
 
 // Pinned host staging: host columns are copied into a pinned buffer, then to the device on the
 // side stream; the compute stream waits on the copy's event.  Two slots alternate so the copy of
-// batch k+1 overlaps the kernels of batch k.  A column already in page-locked host memory is
-// copied by DMA straight from the caller's buffer (no host copy).
+// batch k+1 overlaps the kernels of batch k.  A large buffer (4 MiB or more) already in
+// page-locked host memory is copied by DMA straight from the caller's buffer (no host copy).
 struct Staging {
     Device* dev = nullptr;
     struct Slot {
@@ -141,7 +141,11 @@ struct Staging {
     vr_status done(int which, hipStream_t compute);
     // Stages the pieces through the next slot (256-byte aligned, in order), patches their device
     // pointers and makes `compute` wait for the copies; *which = -1 when there is nothing to stage.
-    vr_status stage(Device* d, const std::vector<StagePiece>& pieces, hipStream_t compute, int* which);
+    // With `defer` the copies are only queued: the caller calls ready() before the kernels that
+    // read the slot, and keeps page-locked sources unchanged until the copy stream has drained.
+    vr_status stage(Device* d, const std::vector<StagePiece>& pieces, hipStream_t compute, int* which,
+                    bool defer = false);
+    vr_status ready(int which, hipStream_t compute);
 };
 
 // True when `p` is page-locked host memory the DMA engine can read.

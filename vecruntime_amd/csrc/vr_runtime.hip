@@ -1,5 +1,7 @@
 // Device handle, errors, allocation accounting and pinned staging for libvrgpu.so.
+#include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "vr_internal.h"
 
@@ -146,7 +148,30 @@ bool host_pinned(const void* p) {
     return at.type == hipMemoryTypeHost;
 }
 
-vr_status Staging::stage(Device* d, const std::vector<StagePiece>& pieces, hipStream_t compute, int* which) {
+// Copies `n` bytes in up to `threads` host threads (pageable sources into the pinned buffer).
+static void parallel_copies(const std::vector<std::pair<uint8_t*, const uint8_t*>>& dst_src,
+                            const std::vector<size_t>& bytes, size_t total, int threads) {
+    if (threads <= 1 || total < ((size_t)16 << 20)) {
+        for (size_t q = 0; q < bytes.size(); q++) std::memcpy(dst_src[q].first, dst_src[q].second, bytes[q]);
+        return;
+    }
+    // thread t copies bytes [t * total / threads, (t + 1) * total / threads) of the concatenation
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t] {
+            const size_t lo = total * (size_t)t / (size_t)threads, hi = total * (size_t)(t + 1) / (size_t)threads;
+            size_t at = 0;
+            for (size_t q = 0; q < bytes.size() && at < hi; q++) {
+                const size_t a = std::max(lo, at), b = std::min(hi, at + bytes[q]);
+                if (a < b) std::memcpy(dst_src[q].first + (a - at), dst_src[q].second + (a - at), b - a);
+                at += bytes[q];
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+vr_status Staging::stage(Device* d, const std::vector<StagePiece>& pieces, hipStream_t compute, int* which, bool defer) {
     *which = -1;
     if (pieces.empty()) return VR_OK;
     size_t total = 0;
@@ -154,26 +179,57 @@ vr_status Staging::stage(Device* d, const std::vector<StagePiece>& pieces, hipSt
     uint8_t *host, *device;
     VR_TRY(begin(d, total, &host, &device, which));
     Slot& s = slot[*which];
-    size_t off = 0;
-    bool direct = false;
-    for (auto& p : pieces) {
+    // where each piece goes; a large page-locked source is copied by DMA as it is, every other
+    // piece first into the pinned buffer (at its own offset, so consecutive pieces go in one copy:
+    // a DMA per small piece costs more than the host copy)
+    std::vector<size_t> offs(pieces.size());
+    std::vector<char> pinned(pieces.size());
+    std::vector<std::pair<uint8_t*, const uint8_t*>> dst_src;
+    std::vector<size_t> sizes;
+    size_t off = 0, pageable = 0;
+    for (size_t q = 0; q < pieces.size(); q++) {
         off = align_up(off, 256);
-        const void* from = p.src;
-        if (host_pinned(p.src)) {
+        offs[q] = off;
+        pinned[q] = pieces[q].bytes >= ((size_t)4 << 20) && host_pinned(pieces[q].src) ? 1 : 0;
+        if (!pinned[q]) {
+            dst_src.push_back({host + off, (const uint8_t*)pieces[q].src});
+            sizes.push_back(pieces[q].bytes);
+            pageable += pieces[q].bytes;
+        }
+        *pieces[q].dst = device + off;
+        off += pieces[q].bytes;
+    }
+    parallel_copies(dst_src, sizes, pageable, (int)std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency())));
+    bool direct = false;
+    size_t run_lo = 0, run_hi = 0;  // a run of pinned-buffer pieces not copied yet
+    auto flush = [&]() -> vr_status {
+        if (run_hi > run_lo)
+            VR_TRY_HIP(hipMemcpyAsync(device + run_lo, host + run_lo, run_hi - run_lo, hipMemcpyHostToDevice, dev->copy_stream));
+        run_lo = run_hi = 0;
+        return VR_OK;
+    };
+    for (size_t q = 0; q < pieces.size(); q++) {
+        if (pinned[q]) {
+            VR_TRY(flush());
+            VR_TRY_HIP(hipMemcpyAsync(device + offs[q], pieces[q].src, pieces[q].bytes, hipMemcpyHostToDevice, dev->copy_stream));
             direct = true;
         } else {
-            std::memcpy(host + off, p.src, p.bytes);
-            from = host + off;
+            if (run_hi == run_lo) run_lo = offs[q];
+            run_hi = offs[q] + pieces[q].bytes;
         }
-        VR_TRY_HIP(hipMemcpyAsync(device + off, from, p.bytes, hipMemcpyHostToDevice, dev->copy_stream));
-        *p.dst = device + off;
-        off += p.bytes;
     }
+    VR_TRY(flush());
     dev->count(&vr_stats::h2d_bytes, (int64_t)total);
     VR_TRY_HIP(hipEventRecord(s.copied, dev->copy_stream));
+    if (defer) return VR_OK;  // the caller waits (ready) and, for page-locked sources, synchronizes
     VR_TRY_HIP(hipStreamWaitEvent(compute, s.copied, 0));
     // the caller may reuse its buffers once the call returns: the copies out of them end first
     if (direct) VR_TRY_HIP(hipEventSynchronize(s.copied));
+    return VR_OK;
+}
+
+vr_status Staging::ready(int which, hipStream_t compute) {
+    VR_TRY_HIP(hipStreamWaitEvent(compute, slot[which].copied, 0));
     return VR_OK;
 }
 
