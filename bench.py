@@ -230,7 +230,10 @@ def run_groupby(dev, args, n):
     checks = verify_groupby(keys, dec, dbl, i32, valid, n, groups, ek, eo)
     log("checks: %s" % json.dumps(checks))
     res = {}
-    for name, fn, steps in (("batched", step_batched, args.steps), ("plain", step_plain, max(1, args.steps // 2))):
+    legs = [("batched", step_batched, args.steps)]
+    if not args.skip_plain:
+        legs.append(("plain", step_plain, max(1, args.steps // 2)))
+    for name, fn, steps in legs:
         for _ in range(args.warmup):
             fn(cols)
         torch.cuda.synchronize()
@@ -1139,6 +1142,7 @@ def main():
     ap.add_argument("--tune-budget", type=float, default=45.0)
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-pcie", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true", help="no per-batch leg (for profiling the batched update)")
     ap.add_argument("--skip-filters", action="store_true")
     ap.add_argument("--filter-rows", type=int, default=200_000_000)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -1229,7 +1233,7 @@ def main():
                      "avg_launch_ms": gb["update_kernel_avg_launch_ms"], "algorithmic_bytes_per_step": alg_bytes},
         "cpu_baseline": cpu,
         "plain": {"rows_per_s": gb["plain"]["rows_per_s"], "ms_per_step": gb["plain"]["ms_per_step"],
-                  "batched_over_plain": gb["batched"]["rows_per_s"] / gb["plain"]["rows_per_s"]},
+                  "batched_over_plain": gb["batched"]["rows_per_s"] / gb["plain"]["rows_per_s"]} if "plain" in gb else None,
         "cold_table": gb["cold"],
         "from_host": pcie,
         "cfg1_1m_rows_1k_groups": cfg1,
