@@ -839,9 +839,9 @@ struct AggImpl {
     DevBuf batch_descs;
     DevBuf resume;  // guarded launches: the tile each workgroup resumes at
 
-    // partitioned update: hot-set sampling, record pool and chains, merge resume points
+    // partitioned update: hot-set sampling, record counts and ranges, record pool, merge resume points
     DevBuf p_cand, p_cand_count, p_hist, p_hot_keys, p_hot_part, p_hot_total;
-    DevBuf p_pool, p_next, p_cnt, p_heads, p_resume, p_tile_batch, p_pending;
+    DevBuf p_pool, p_count, p_base, p_rtot, p_rstart, p_resume, p_tile_batch, p_pending;
     // Ungrouped aggregation (no grouping key: AggKernels' reductions): one group, keyed by a
     // constant INT64 column of zeros the library supplies, present from creation on so that an
     // empty input still emits its one row.
@@ -1224,6 +1224,7 @@ static vr_status read_counters(AggImpl& h, uint64_t* out, hipStream_t s) {
     VR_TRY_HIP(hipStreamSynchronize(s));
     if (out[C_ERROR] & ERR_TABLE_FULL)
         return fail(VR_ERR_OOM, "a hash table region is full (keys that do not spread over the table)");
+    if (out[C_ERROR] & ERR_INTERNAL) return fail(VR_ERR_DEVICE, "the partitioned update's record counts disagree");
     return VR_OK;
 }
 
@@ -1611,26 +1612,16 @@ static int merge_cache_entries(const AggImpl& h) {
     return 0;
 }
 
-// Record blocks one partitioned update over `rows` rows may take from the pool: a block per
-// PART_BLK records (every active row a record) plus the open last block of each of the grid * P
-// chains; each refill of a workgroup's share leaves under PA_GAP blocks unused (the region whose
-// blocks straddle the end of the old share), and a share is refilled only after at least
-// PA_CHUNK - PA_GAP of its blocks went to records; the last share of each workgroup may go unused.
-static uint64_t pool_blocks(int64_t rows, uint64_t grid, uint64_t P) {
-    const uint64_t used = (uint64_t)(rows + PART_BLK - 1) / PART_BLK + grid * P;
-    const uint64_t refills = used / (uint64_t)(PA_CHUNK - PA_GAP) + grid;
-    return used + refills * (uint64_t)PA_GAP + grid * (uint64_t)PA_CHUNK + 1;
-}
-static_assert(PA_CHUNK > 2 * PA_GAP, "a share outlasts a tile's straddling gap");
+// Record blocks of the pool for one partitioned update over `rows` rows: every active row may be
+// a record, and each of the P regions' ranges ends in a partly filled block.
+static uint64_t pool_blocks(int64_t rows, uint64_t P) { return (uint64_t)(rows + PART_BLK - 1) / PART_BLK + P; }
 
 static bool partitioned_applies(const AggImpl& h, int64_t rows, RecLayout* L, const vr_launch_config* cfg) {
     const int32_t mode = (cfg && cfg->partitioned) ? cfg->partitioned : h.cfg.partitioned;
     if (mode < 0 || (mode == 0 && rows < h.part_min_rows)) return false;
     if (h.ordered || h.proto.ks.mode == KM_WIDE || rows <= 0) return false;
-    // block numbers of the record pool fit a merge list entry; one chain per lane of a merge wave
-    const uint64_t cus = (uint64_t)h.dev->num_cus;
-    if (pool_blocks(rows, cus, 1ull << PART_MAX_PBITS) >= (1ull << PB_ID_BITS)) return false;
-    if (cus > (uint64_t)PB_BLOCK) return false;
+    // record positions are 32-bit
+    if (pool_blocks(rows, 1ull << PART_MAX_PBITS) * PART_BLK >= (1ull << 32)) return false;
     if (!record_layout(h, *L)) return false;
     ColProg prog[VR_MAX_VALUES];
     if (column_programs(h, *L, prog) > PART_MAX_PROG) return false;
@@ -1716,27 +1707,46 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
         pa.hot_part = h.p_hot_part.as<uint64_t>();
         pa.hot_total = h.p_hot_total.as<uint64_t>();
     }
-    // the record pool
-    const uint64_t max_blocks = pool_blocks(rows, (uint64_t)grid_a, (uint64_t)P);
-    if (max_blocks >= (1ull << PB_ID_BITS)) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
-    VR_TRY(h.p_pool.ensure(h.dev, max_blocks * PART_BLK * L.words * sizeof(uint32_t)));
-    VR_TRY(h.p_next.ensure(h.dev, max_blocks * sizeof(uint32_t)));
-    VR_TRY(h.p_cnt.ensure(h.dev, max_blocks * sizeof(uint32_t)));
-    VR_TRY(h.p_heads.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
+    // the record pool and its ranges: the count pass, a scan down each region's column of counts,
+    // a scan over the regions
+    const uint64_t nblocks = pool_blocks(rows, (uint64_t)P);
+    if (nblocks * PART_BLK >= (1ull << 32)) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
+    VR_TRY(h.p_pool.ensure(h.dev, nblocks * PART_BLK * L.words * sizeof(uint32_t)));
+    VR_TRY(h.p_count.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
+    VR_TRY(h.p_base.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
+    VR_TRY(h.p_rtot.ensure(h.dev, (size_t)P * sizeof(uint32_t)));
+    VR_TRY(h.p_rstart.ensure(h.dev, (size_t)(P + 1) * sizeof(uint32_t)));
     VR_TRY(h.p_resume.ensure(h.dev, (size_t)P * sizeof(int32_t)));
-    VR_TRY(h.p_pending.ensure(h.dev, 2 * max_blocks * sizeof(uint64_t)));
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, h.counters.as<uint64_t>() + C_POOL, 2ull, 0ull);
+    VR_TRY(h.p_pending.ensure(h.dev, 2 * nblocks * sizeof(uint64_t)));
     pa.pool = h.p_pool.as<uint32_t>();
-    pa.block_next = h.p_next.as<uint32_t>();
-    pa.block_cnt = h.p_cnt.as<uint32_t>();
-    pa.heads = h.p_heads.as<uint32_t>();
+    pa.count = h.p_count.as<uint32_t>();
+    pa.base = h.p_base.as<uint32_t>();
+    pa.rtot = h.p_rtot.as<uint32_t>();
+    pa.rstart = h.p_rstart.as<uint32_t>();
     pa.resume_step = h.p_resume.as<int32_t>();
     pa.tile_batch = h.p_tile_batch.as<uint32_t>();
     pa.pending = h.p_pending.as<uint64_t>();
-    pa.max_blocks = max_blocks;
+    pa.pool_blocks = nblocks;
+    pa.pool_records = (uint32_t)(nblocks * PART_BLK);
     pa.u = a;
+    const size_t lds_c = count_lds_bytes(K, H, P);
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
     const size_t lds_b = merge_lds_bytes(E, S);
+    if (jit) {
+        if (launch_jit(jit->count, (unsigned)grid_a, PA_BLOCK, lds_c, s, pa) != VR_OK) {
+            // the module's kernels do not launch here: the generic kernels from the next call on
+            (void)hipGetLastError();
+            part_jit_disable();
+            return fail(VR_ERR_DEVICE, "the partitioned update's compiled kernels failed to launch");
+        }
+    } else {
+        VR_TRY(allow_lds(part_count_kernel<RuntimeProgs>, lds_c));
+        hipLaunchKernelGGL(part_count_kernel<RuntimeProgs>, dim3(grid_a), dim3(PA_BLOCK), lds_c, s, pa);
+    }
+    hipLaunchKernelGGL(part_col_scan_kernel, dim3((unsigned)((P + 63) / 64)), dim3(1024), 0, s, pa);
+    hipLaunchKernelGGL(part_region_scan_kernel, dim3(1), dim3(1024), 0, s, pa);
+    VR_TRY_HIP(hipGetLastError());
+    *launches += 3;
     if (jit && launch_jit(jit->scatter, (unsigned)grid_a, PA_BLOCK, lds_a, s, pa) != VR_OK) {
         // the module's kernels do not launch here: the generic kernels from the next call on (this
         // call's tiles are cut for the compiled kernel's rows per thread)
@@ -1824,10 +1834,12 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
     if (spec_mode >= 0) {
         ColProg prog[VR_MAX_VALUES];
         const int np = column_programs(h, L, prog);
-        jit = part_jit(h.dev->ordinal, prog, np, L);
+        jit = part_jit(h.dev->ordinal, prog, np, L, h.proto.ks.mode == KM_I64);
     }
     int rpt = (cfg && cfg->rows_per_thread) ? cfg->rows_per_thread : h.cfg.rows_per_thread;
-    a.rows_per_thread = std::max(1, std::min(jit ? PA_JIT_RPT : PA_MAX_RPT, rpt));
+    // the compiled kernels take PA_JIT_RPT rows per thread (a constant of theirs); the generic ones
+    // the configured number, up to PA_MAX_RPT
+    a.rows_per_thread = jit ? PA_JIT_RPT : std::max(1, std::min(PA_MAX_RPT, rpt));
     const int64_t T = (int64_t)PA_BLOCK * a.rows_per_thread;
     int64_t tiles = 0, seen = 0;
     for (int b = 0; b < nb; b++) {
@@ -2517,7 +2529,7 @@ vr_status vr_agg_prewarm(vr_device* dev, const vr_agg_spec* spec) {
     if (h.cfg.specialize < 0 || !partitioned_applies(h, h.part_min_rows, &L, nullptr)) return VR_OK;
     ColProg prog[VR_MAX_VALUES];
     const int np = column_programs(h, L, prog);
-    (void)part_jit(h.dev->ordinal, prog, np, L);  // a failed compile leaves the generic kernels in use
+    (void)part_jit(h.dev->ordinal, prog, np, L, h.proto.ks.mode == KM_I64);  // a failed compile leaves the generic kernels in use
     return VR_OK;
 }
 
@@ -2529,7 +2541,7 @@ vr_status vr_agg_memory(vr_agg* agg, int64_t* bytes) {
                        h.sort_vals.bytes + h.sort_vals_in.bytes + h.firsts.bytes + h.firsts_sorted.bytes +
                        h.batch_descs.bytes + h.rgroups.bytes + h.resume.bytes + h.p_cand.bytes +
                        h.p_cand_count.bytes + h.p_hist.bytes + h.p_hot_keys.bytes + h.p_hot_part.bytes +
-                       h.p_hot_total.bytes + h.p_pool.bytes + h.p_next.bytes + h.p_cnt.bytes + h.p_heads.bytes +
+                       h.p_hot_total.bytes + h.p_pool.bytes + h.p_count.bytes + h.p_base.bytes + h.p_rtot.bytes + h.p_rstart.bytes +
                        h.p_resume.bytes + h.p_tile_batch.bytes + h.p_pending.bytes);
     for (auto& sk : h.strs) *bytes += sk.table.memory() + (int64_t)sk.remap.bytes;
     return VR_OK;

@@ -3,24 +3,27 @@
 //
 // Device-scope atomics on this part run at ~24 G/s whatever the working set, while LDS atomics run
 // at ~2 T/s and a wave streams a block of 64 records at HBM speed.  The update is therefore split
-// into two launches in which no global state row is ever updated by two workgroups:
+// into launches in which no global state row is ever updated by two workgroups:
 //
-//   A (scatter)  Each workgroup streams tiles of rows once.  Rows whose key is in the hot set (the
-//                most frequent keys of a sample of the input) fold into a per-workgroup LDS copy
-//                of those groups.  Every other row becomes a record -- the key, the value lanes the
-//                aggregates read, the row index when a double MIN/MAX needs it, one validity word
-//                -- appended to its region's chain of 64-record blocks: a workgroup keeps one open
-//                block per region, takes new blocks from a pool with one atomic per block and
-//                links each to the previous one (block_next), so the records of region r are the
-//                chains heads[w][r] of all scatter workgroups w.
+//   A1 (count)   Each workgroup walks the tiles its scatter workgroup will walk, reading only the
+//                keys (and the selection and filter), and counts per region the rows that will
+//                become records: rows whose key is not in the hot set (the most frequent keys of a
+//                sample of the input).  Two small scans turn the counts into a range of the
+//                record pool per (workgroup, region), the ranges of a region contiguous and each
+//                region starting on a block of PART_BLK records.
+//   A2 (scatter) Each workgroup streams its tiles once.  Hot rows fold into a per-workgroup LDS
+//                copy of the hot groups; every other row becomes a record -- the key, the value
+//                lanes the aggregates read, the row index when a double MIN/MAX needs it, one
+//                validity word -- written at its region's LDS cursor inside the workgroup's
+//                range.  The waves never wait for each other inside the tile loop.
 //   B (merge)    One workgroup per region owns that region's slots and state rows.  It first
 //                merges the hot set's partials (reduced over the scatter workgroups) of its region,
-//                then each of its waves walks its share of the chains on its own (no barrier),
+//                then its waves take the region's record blocks from an LDS counter (no barrier),
 //                folding records into an LDS cache of groups with LDS atomics, and at the end the
-//                cache is merged into the region -- new slots claimed by CAS, state rows by plain
-//                read-modify-write, since no other workgroup touches the region.  A record whose
-//                key finds the cache full around it is marked in a pending map and folded by a
-//                further pass after that merge.
+//                cache is merged into the region -- new slots claimed without device atomics, state
+//                rows by plain read-modify-write, since no other workgroup touches the region.  A
+//                record whose key finds the cache full around it is marked in a pending map and
+//                folded by a further pass after that merge.
 //
 // Growth.  Before a merge adds groups to its region the workgroup checks that the region can
 // take every cached group as a new one; when it cannot, the workgroup stops and leaves in
@@ -57,13 +60,16 @@ struct PartArgs {
     uint64_t* hot_part;        // [grid_a][nhot][stride]
     uint64_t* hot_total;       // [nhot][stride]
     uint32_t* pool;            // block b holds records [b * 64, b * 64 + 64) of rl.words words
-    uint32_t* block_next;      // the chain's previous block
-    uint32_t* block_cnt;       // records in the block
-    uint32_t* heads;           // [grid_a][nparts] newest block of each chain
+    uint32_t* count;           // [grid_a][nparts] records of each scatter workgroup per region
+    uint32_t* base;            // [grid_a][nparts] where they start inside the region's records
+    uint32_t* rtot;            // [nparts] records per region
+    uint32_t* rstart;          // [nparts + 1] first record of each region (a multiple of PART_BLK)
     int32_t* resume_step;      // [nparts] merge: see above; -1 when merged
     const uint32_t* tile_batch;  // scatter: the batch of every tile
-    uint64_t* pending;         // merge: 2 x max_blocks words, the records of a block pending in a pass
-    uint64_t max_blocks;
+    uint64_t* pending;         // merge: 2 x pool_blocks words, the records of a block pending in a pass
+    uint64_t pool_blocks;      // blocks of the pool
+    uint32_t pool_records;     // records the pool holds (pool_blocks * PART_BLK)
+    uint32_t pad3;
 };
 
 // Where the kernels read the column programs and the record layout.  RuntimeProgs reads them from
@@ -76,6 +82,10 @@ struct RuntimeProgs {
     static __device__ __forceinline__ const ColProg& prog(const PartArgs& a, int q) { return a.prog[q]; }
     static __device__ __forceinline__ int valid_off(const PartArgs& a) { return a.rl.valid_off; }
     static __device__ __forceinline__ int row_off(const PartArgs& a) { return a.rl.row_off; }
+    // the key is one 64-bit lane (loaded with the tile; other key modes load theirs per row)
+    static __device__ __forceinline__ bool fast_key(const PartArgs& a) { return a.u.ks.mode == KM_I64; }
+    // rows per thread per tile (at most max_rpt)
+    static __device__ __forceinline__ int rpt(const PartArgs& a) { return a.u.rows_per_thread; }
 };
 
 // Folds one value of a column into the state words at `e` (an LDS entry: LDS atomics), as its
@@ -239,7 +249,7 @@ __global__ __launch_bounds__(256) void hot_pick_kernel(const uint64_t* cand, con
     }
 }
 
-// ------------------------------------------------------------------ phase A: scatter
+// ------------------------------------------------------------------ phase A: count and scatter
 
 // LDS rows of the hot set and of the merge cache: the key, then the state words, padded to an odd
 // number of 8-byte words so that the lanes of a wave updating one word of different rows spread
@@ -253,82 +263,42 @@ __host__ __device__ inline int lds_row_words(int stride) { return (stride + 1) |
 constexpr int HB = 8;
 __device__ __forceinline__ uint32_t hot_tag(uint64_t h) { return (uint32_t)(h >> 16) & 0xFFFF0000u; }
 
-// Number of `key` in the hot set, or HOT_EMPTY.
+// Number of `key` in the hot set (keys hrow[k * HW]), or HOT_EMPTY.  A bucket's entries are tested
+// without branches (a mask of tag matches, a mask of empty entries); only a tag match reads a key,
+// and a second match in one bucket (a 16-bit tag collision) takes the slow loop.
 __device__ __forceinline__ uint32_t hot_find(const uint32_t* hidx, const uint64_t* hrow, int HW, uint32_t bmask,
                                              uint64_t h, uint64_t key) {
     const uint32_t tag = hot_tag(h);
     uint32_t b = (uint32_t)h & bmask;
-    while (true) {
+    for (uint32_t probes = 0; probes <= bmask; probes++) {
         const uint4* q = (const uint4*)(hidx + b * HB);
         const uint4 x = q[0], y = q[1];
         const uint32_t e[HB] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
-        bool full = true;
+        uint32_t match = 0, empty = 0, cand = HOT_EMPTY;
 #pragma unroll
-        for (int j = 0; j < HB; j++) {
-            if (e[j] == HOT_EMPTY) full = false;
-            else if ((e[j] & 0xFFFF0000u) == tag && hrow[(e[j] & 0xFFFFu) * HW] == key) return e[j] & 0xFFFFu;
+        for (int j = HB - 1; j >= 0; j--) {
+            const bool m = e[j] != HOT_EMPTY && (e[j] & 0xFFFF0000u) == tag;
+            match |= (uint32_t)m << j;
+            empty |= (uint32_t)(e[j] == HOT_EMPTY) << j;
+            cand = m ? (e[j] & 0xFFFFu) : cand;  // the first match
         }
-        if (!full) return HOT_EMPTY;
+        if (match) {
+            if (hrow[(size_t)cand * HW] == key) return cand;
+            if (match & (match - 1)) {
+#pragma unroll
+                for (int j = 0; j < HB; j++)
+                    if (((match >> j) & 1u) && hrow[(size_t)(e[j] & 0xFFFFu) * HW] == key) return e[j] & 0xFFFFu;
+            }
+        }
+        if (empty) return HOT_EMPTY;
         b = (b + 1) & bmask;
     }
+    return HOT_EMPTY;
 }
 
-// LDS bytes of the scatter kernel.
-__host__ __device__ inline size_t scatter_lds_bytes(int nhot, int hot_slots, int stride, int nparts) {
-    return (size_t)nhot * lds_row_words(stride) * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 16;
-}
-
-// Word o of a record held in registers (o uniform): a select per word, no indexed register access.
-template <int RW>
-__device__ __forceinline__ void put_word(uint32_t (&w)[RW], int o, uint32_t x) {
-#pragma unroll
-    for (int j = 0; j < RW; j++)
-        if (j == o) w[j] = x;
-}
-
-// RW: 32-bit words per record (the layout's `words`).  PS: where the column programs come from.
-template <int RW, class PS>
-__global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
-    constexpr int MR = PS::max_rpt;  // rows per thread per tile at most (registers per row)
-    extern __shared__ uint64_t lds[];
-    const UpdateArgs& u = a.u;
-    const int S = u.t.stride;
-    const int P = a.nparts;
-    const int H = a.hot_slots;
-    const int K = a.nhot;
-    // LDS: hot keys [K], hot states [K][S], hot index [H] (u32), per region: fill, start, cur, first
-    const int HW = lds_row_words(S);
-    uint64_t* hrow = lds;  // row k: key k of the hot set, its state words
-    uint32_t* hidx = (uint32_t*)(hrow + (size_t)K * HW);
-    uint32_t* fill = hidx + H;   // positions taken in the region's chain of this workgroup
-    uint32_t* start = fill + P;  // `fill` at the start of the tile
-    uint32_t* cur = start + P;   // the chain's newest block
-    uint32_t* first = cur + P;   // first block taken for the tile's records
-    // blocks taken from the pool and not handed out yet: [s_next, s_end); a tile needing more takes
-    // a new range [s_alt, s_alt_end) for the regions the rest cannot hold, which then replaces it
-    __shared__ uint32_t s_next, s_end, s_need, s_alt, s_alt_end, s_fail;
-
-    for (int w = threadIdx.x; w < K * HW; w += blockDim.x) {
-        const int j = w % HW;
-        hrow[w] = j == 0 ? a.hot_keys[w / HW] : (j <= S ? u.ident[j - 1] : 0);
-    }
-    for (int q = threadIdx.x; q < H; q += blockDim.x) hidx[q] = HOT_EMPTY;
-    for (int p = threadIdx.x; p < P; p += blockDim.x) {
-        fill[p] = 0;
-        start[p] = 0;
-        cur[p] = NO_BLOCK;
-        first[p] = NO_BLOCK;
-    }
-    if (threadIdx.x == 0) {
-        s_next = 0;
-        s_end = 0;
-        s_need = 0;
-        s_alt = 0;
-        s_alt_end = 0;
-        s_fail = 0;
-    }
-    __syncthreads();
-    const uint32_t hbmask = (uint32_t)(H / HB - 1);
+// Enters the keys hrow[k * HW], k < K, into the hot set's LDS index (EMPTY_KEY entries are not in
+// the set).  Called by every thread of the workgroup, between two barriers.
+__device__ __forceinline__ void build_hot_index(uint32_t* hidx, const uint64_t* hrow, int HW, int K, uint32_t hbmask) {
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
         const uint64_t key = hrow[(size_t)k * HW];
         if (key == EMPTY_KEY) continue;
@@ -340,51 +310,265 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             if (j < HB) break;
         }
     }
-    __syncthreads();
+}
 
-    const int RPT = u.rows_per_thread;
+// The descriptor of a tile's batch, through the constant address space: the tile number is the
+// same in every lane, so the batch number and the descriptor's fields come in scalar loads.
+__device__ __forceinline__ const BatchDesc& tile_desc(const PartArgs& a, int64_t tile) {
+    const __attribute__((address_space(4))) uint32_t* tb = (const __attribute__((address_space(4))) uint32_t*)a.tile_batch;
+    return batch_at(a.u.batches, tb[tile]);
+}
+
+// Whether row i of a tile takes part: inside its batch, selected, and passing the fused filter.
+// sbyte / fbyte: the selection / filter-validity bitmap bytes holding the row's bit; fval: the
+// filter lane.
+__device__ __forceinline__ bool row_active(const UpdateArgs& u, int64_t i, int64_t n, uint32_t sbyte, uint32_t fbyte,
+                                           int32_t fval) {
+    const int bit = (int)(i & 7);
+    if (i >= n || !((sbyte >> bit) & 1u)) return false;
+    return u.filter_col < 0 || (((fbyte >> bit) & 1u) && fval < u.filter_threshold);
+}
+
+// What becomes of an active row's key: false for a reserved row (the null key, a key equal to the
+// empty marker); else *k = its entry in the hot set (HOT_EMPTY: none, the row becomes a record) and
+// *p = its region.  The count pass and the scatter both classify through this function, so their
+// record counts agree.  Nothing here branches on the row: the callers predicate what follows.
+__device__ __forceinline__ bool key_class(const uint32_t* hidx, const uint64_t* hrow, int HW, uint32_t hbmask, int K,
+                                          int pbits, uint64_t key, bool nul, uint32_t* k, uint32_t* p) {
+    const uint64_t h = fmix64(key);
+    *k = K ? hot_find(hidx, hrow, HW, hbmask, h, key) : HOT_EMPTY;
+    *p = (uint32_t)region_of_hash(pbits, h);
+    return !nul && key != EMPTY_KEY;
+}
+
+// LDS bytes of the count pass: the hot keys, their index, a counter per region.
+__host__ __device__ inline size_t count_lds_bytes(int nhot, int hot_slots, int nparts) {
+    return (size_t)nhot * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 4;
+}
+
+// Phase A, first pass: the records each scatter workgroup will write to each region.  Workgroup w
+// walks the tiles scatter workgroup w walks and classifies their rows as the scatter does, reading
+// only the keys, the selection and the filter lane; count[w * nparts + p] = its rows that become
+// records of region p.  Rows of the reserved rows (null keys, the empty marker) are folded here.
+template <class PS>
+__global__ __launch_bounds__(PA_BLOCK) void part_count_kernel(PartArgs a) {
+    constexpr int MR = PA_JIT_RPT;
+    extern __shared__ uint64_t lds[];
+    const UpdateArgs& u = a.u;
+    const int P = a.nparts;
+    const int H = a.hot_slots;
+    const int K = a.nhot;
+    uint64_t* hk = lds;
+    uint32_t* hidx = (uint32_t*)(hk + K);
+    uint32_t* cnt = hidx + H;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) hk[k] = a.hot_keys[k];
+    for (int q = threadIdx.x; q < H; q += blockDim.x) hidx[q] = HOT_EMPTY;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = 0;
+    __syncthreads();
+    const uint32_t hbmask = (uint32_t)(H / HB - 1);
+    build_hot_index(hidx, hk, 1, K, hbmask);
+    __syncthreads();
+    const int RPT = PS::rpt(a);
     const int64_t T = (int64_t)PA_BLOCK * RPT;
     const int pbits = u.t.pbits;
-    // Tiles are grid-strided, so that the CUs read neighbouring addresses at any time.  The batch
-    // descriptor of a tile is in LDS while the tile runs (its column pointers are read there once
-    // per column and tile); the next tile's is fetched meanwhile into the other buffer.
-    constexpr int DESC_WORDS = (int)(sizeof(BatchDesc) / 8);
-    __shared__ BatchDesc lb[2];
-    int64_t tile = u.tile_lo + blockIdx.x;
-    if (tile < u.total_tiles) {
-        const uint64_t* src = (const uint64_t*)&batch_at(u.batches, a.tile_batch[tile]);
-        for (int w = threadIdx.x; w < DESC_WORDS; w += blockDim.x) ((uint64_t*)&lb[0])[w] = src[w];
+    const bool fast_key = PS::fast_key(a);
+    const bool has_filter = u.filter_col >= 0;
+    for (int64_t tile = u.tile_lo + blockIdx.x; tile < u.total_tiles; tile += gridDim.x) {
+        const BatchDesc& b = tile_desc(a, tile);
+        const int64_t row0 = (tile - b.tile_begin) * T;
+        const DCol& fcol = b.vals[has_filter ? u.filter_col : 0];
+        const DCol& kcol = b.keys[0];
+        uint32_t sbyte[MR], fbyte[MR], kbyte[MR], klo[MR], khi[MR];
+        int32_t fval[MR];
+#pragma unroll
+        for (int r = 0; r < MR; r++) {
+            sbyte[r] = fbyte[r] = kbyte[r] = klo[r] = khi[r] = 0;
+            fval[r] = 0;
+            if (r >= RPT) continue;
+            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+            const int64_t ci = i < b.n ? i : b.n - 1;
+            sbyte[r] = *gptr(b.sel ? b.sel + (ci >> 3) : k_all_ones);
+            fbyte[r] = *gptr(has_filter && fcol.valid ? fcol.valid + (ci >> 3) : k_all_ones);
+            fval[r] = (int32_t)*gptr(has_filter ? (const uint32_t*)fcol.data + ci : k_zero_words);
+            const uint32_t* kp = fast_key ? (const uint32_t*)kcol.data + 2 * ci : k_zero_words;
+            kbyte[r] = *gptr(fast_key && kcol.valid ? kcol.valid + (ci >> 3) : k_all_ones);
+            klo[r] = *gptr(kp);
+            khi[r] = *gptr(kp + 1);
+        }
+#pragma unroll
+        for (int r = 0; r < MR; r++) {
+            if (r >= RPT) continue;
+            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+            if (!row_active(u, i, b.n, sbyte[r], fbyte[r], fval[r])) continue;
+            uint64_t key = 0;
+            bool nul;
+            if (fast_key) {
+                nul = !((kbyte[r] >> (int)(i & 7)) & 1u);
+                key = (uint64_t)klo[r] | ((uint64_t)khi[r] << 32);
+            } else {
+                uint64_t slot;
+                nul = !key64_of(u, b, i, &key, &slot);
+            }
+            uint32_t k, p;
+            const bool keyed = key_class(hidx, hk, 1, hbmask, K, pbits, key, nul, &k, &p);
+            if (keyed && k == HOT_EMPTY) {
+                atomicAdd(&cnt[p], 1u);
+            } else if (!keyed) {
+                // a reserved row (the null of a 64-bit lane, or a key equal to the empty marker):
+                // folded here, with device atomics, so that the scatter's row loop never holds
+                // the generic per-row operations
+                const uint64_t slot = nul ? u.t.slots + 1 : u.t.slots;
+                mark_reserved(u.t.counters, nul ? RES_NULL : RES_SENTINEL);
+                apply_ops(u, b, i, u.t.state + slot * (uint64_t)u.t.stride);
+            }
+        }
+        vmem_wait_all();  // the tile's loads are in (see the scatter): the next tile's issue at once
     }
     __syncthreads();
-    int cur_buf = 0;
-    for (; tile < u.total_tiles; tile += gridDim.x) {
-        const int64_t ntile = tile + gridDim.x;
-        const bool more = ntile < u.total_tiles;
-        uint64_t next_word = 0;
-        if (more && threadIdx.x < DESC_WORDS)
-            next_word = ((const uint64_t*)&batch_at(u.batches, a.tile_batch[ntile]))[threadIdx.x];
-        const BatchDesc& b = lb[cur_buf];
+    uint32_t* out = a.count + (size_t)blockIdx.x * P;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) out[p] = cnt[p];
+}
+
+// base[w * nparts + p] = records of region p that the scatter workgroups before w write (a scan
+// down each column of count); rtot[p] = the column's total.  A workgroup takes 64 regions, each of
+// its 16 waves a sixteenth of the workgroups.
+__global__ __launch_bounds__(1024) void part_col_scan_kernel(PartArgs a) {
+    __shared__ uint32_t part[16][64];
+    const int G = a.grid_a, P = a.nparts;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    const int per = (G + 15) / 16;
+    const int w0 = wv * per < G ? wv * per : G, w1 = w0 + per < G ? w0 + per : G;
+    uint32_t s = 0;
+    if (p < P)
+        for (int w = w0; w < w1; w++) s += a.count[(size_t)w * P + p];
+    part[wv][lane] = s;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int j = 0; j < wv; j++) off += part[j][lane];
+    if (p >= P) return;
+    for (int w = w0; w < w1; w++) {
+        const uint32_t c = a.count[(size_t)w * P + p];
+        a.base[(size_t)w * P + p] = off;
+        off += c;
+    }
+    if (wv == 15) a.rtot[p] = off;
+}
+
+// rstart[p] = where region p's records start: its range is padded to whole blocks of PART_BLK
+// records, so that no block of the pool holds two regions' records (rstart[nparts]: the end).
+__global__ __launch_bounds__(1024) void part_region_scan_kernel(PartArgs a) {
+    __shared__ uint32_t ws[1024];
+    const int P = a.nparts;
+    const int per = (P + 1023) / 1024;
+    const int p0 = threadIdx.x * per;
+    auto padded = [&](int q) -> uint32_t { return (a.rtot[q] + PART_BLK - 1) / PART_BLK * PART_BLK; };
+    uint32_t s = 0;
+    for (int j = 0; j < per; j++)
+        if (p0 + j < P) s += padded(p0 + j);
+    ws[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint32_t x = threadIdx.x >= (unsigned)d ? ws[threadIdx.x - d] : 0;
+        __syncthreads();
+        ws[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t off = ws[threadIdx.x] - s;
+    for (int j = 0; j < per; j++) {
+        if (p0 + j >= P) break;
+        a.rstart[p0 + j] = off;
+        off += padded(p0 + j);
+    }
+    if (threadIdx.x == 1023) a.rstart[P] = ws[1023];
+}
+
+// LDS bytes of the scatter: the hot rows, their index, a record cursor per region.
+__host__ __device__ inline size_t scatter_lds_bytes(int nhot, int hot_slots, int stride, int nparts) {
+    return (size_t)nhot * lds_row_words(stride) * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 4;
+}
+
+// Word o of a record held in registers (o uniform): a select per word, no indexed register access.
+template <int RW>
+__device__ __forceinline__ void put_word(uint32_t (&w)[RW], int o, uint32_t x) {
+#pragma unroll
+    for (int j = 0; j < RW; j++)
+        if (j == o) w[j] = x;
+}
+
+// Phase A, second pass.  RW: 32-bit words per record (the layout's `words`); PS: where the column
+// programs come from.  The workgroup's record cursor of region p starts at rstart[p] + base[w][p],
+// so the records of a region are contiguous in the pool (workgroup by workgroup) and a row's
+// record position is one LDS atomic away: no wave ever waits for another inside the tile loop.
+template <int RW, class PS>
+__global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
+    constexpr int MR = PS::max_rpt;  // rows per thread per tile at most (registers per row)
+    extern __shared__ uint64_t lds[];
+    const UpdateArgs& u = a.u;
+    const int S = u.t.stride;
+    const int P = a.nparts;
+    const int H = a.hot_slots;
+    const int K = a.nhot;
+    // LDS: hot keys and states [K][HW], hot index [H] (u32), record cursor per region [P] (u32)
+    const int HW = lds_row_words(S);
+    uint64_t* hrow = lds;  // row k: key k of the hot set, its state words
+    uint32_t* hidx = (uint32_t*)(hrow + (size_t)K * HW);
+    uint32_t* cursor = hidx + H;
+    const uint32_t* wbase = a.base + (size_t)blockIdx.x * P;
+    for (int w = threadIdx.x; w < K * HW; w += blockDim.x) {
+        const int j = w % HW;
+        hrow[w] = j == 0 ? a.hot_keys[w / HW] : (j <= S ? u.ident[j - 1] : 0);
+    }
+    for (int q = threadIdx.x; q < H; q += blockDim.x) hidx[q] = HOT_EMPTY;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) cursor[p] = a.rstart[p] + wbase[p];
+    __syncthreads();
+    const uint32_t hbmask = (uint32_t)(H / HB - 1);
+    build_hot_index(hidx, hrow, HW, K, hbmask);
+    __syncthreads();
+
+    const int RPT = PS::rpt(a);
+    const int64_t T = (int64_t)PA_BLOCK * RPT;
+    const int pbits = u.t.pbits;
+    const bool fast_key = PS::fast_key(a);
+    const bool has_filter = u.filter_col >= 0;
+    // A tile's records are built in registers and stored at the start of the next tile, before
+    // its loads: the loads then never wait behind the stores (memory operations complete in
+    // order), and a store's registers are written again only after the next tile's loads arrived,
+    // by when the store has read them.  Records wider than 32 bytes are stored at once (their
+    // registers would not fit).
+    constexpr bool kDelay = RW <= 8;
+    uint32_t rec[MR][RW];
+    uint32_t rec_at[MR];  // the pool position of rec[r]
+    uint32_t rec_on = 0;  // bit r: rec[r] holds a record to store
+#pragma unroll
+    for (int r = 0; r < MR; r++) rec_at[r] = 0;
+    auto store_record = [&](int r) __attribute__((always_inline)) {
+        if (!((rec_on >> r) & 1u)) return;
+        rec_on &= ~(1u << r);
+        uint4* dst = (uint4*)(a.pool + (uint64_t)rec_at[r] * RW);
+#pragma unroll
+        for (int q = 0; q < RW / 4; q++)
+            dst[q] = make_uint4(rec[r][4 * q], rec[r][4 * q + 1], rec[r][4 * q + 2], rec[r][4 * q + 3]);
+    };
+    // Tiles are grid-strided (the CUs read neighbouring addresses at any time), the same tiles per
+    // workgroup as the count pass.
+    for (int64_t tile = u.tile_lo + blockIdx.x; tile < u.total_tiles; tile += gridDim.x) {
+        const BatchDesc& b = tile_desc(a, tile);
         const int64_t row0 = (tile - b.tile_begin) * T;
-        // per row: key, and its fate -- ent >= 0: hot entry; -1: nothing (inactive or folded);
-        // -2: a record of region pv at chain position vv
-        uint64_t kk[MR];
-        int32_t ent[MR];
-        uint32_t pv[MR], vv[MR];
-        bool nul[MR];
+        if (kDelay) {
+#pragma unroll
+            for (int r = 0; r < MR; r++) store_record(r);
+        }
         // Every load of the tile first, with no branch between them (one round trip per tile): the
         // row index is clamped into the batch, a lane is read as 32-bit words at i * width (the
         // second word re-reads the first for a 4-byte lane), bitmaps the batch lacks are read from
         // k_all_ones.  Validity, selection and the filter are applied afterwards.
-        uint32_t vm[MR];
-        uint64_t xa[MR][PART_MAX_PROG], xb[MR][PART_MAX_PROG];
         uint32_t vbyte[MR][PART_MAX_PROG];
         uint32_t lo[MR][PART_MAX_PROG], hi[MR][PART_MAX_PROG];
         uint32_t w2[MR][PART_MAX_PROG], w3[MR][PART_MAX_PROG];
         uint32_t sbyte[MR], fbyte[MR], kbyte[MR];
         int32_t fval[MR];
         uint32_t klo[MR], khi[MR];
-        const bool fast_key = u.ks.mode == KM_I64;
-        const bool has_filter = u.filter_col >= 0;
         const DCol& fcol = b.vals[has_filter ? u.filter_col : 0];
         const DCol& kcol = b.keys[0];
 #pragma unroll
@@ -394,12 +578,11 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
             sbyte[r] = *gptr(b.sel ? b.sel + (ci >> 3) : k_all_ones);
             fbyte[r] = *gptr(has_filter && fcol.valid ? fcol.valid + (ci >> 3) : k_all_ones);
             fval[r] = (int32_t)*gptr(has_filter ? (const uint32_t*)fcol.data + ci : k_zero_words);
-            if (fast_key) {
-                kbyte[r] = *gptr(kcol.valid ? kcol.valid + (ci >> 3) : k_all_ones);
-                const uint32_t* kp = (const uint32_t*)kcol.data + 2 * ci;
-                klo[r] = *gptr(kp);
-                khi[r] = *gptr(kp + 1);
-            }
+            // the 64-bit key lane (other key modes read their lanes while classifying)
+            const uint32_t* kp = fast_key ? (const uint32_t*)kcol.data + 2 * ci : k_zero_words;
+            kbyte[r] = *gptr(fast_key && kcol.valid ? kcol.valid + (ci >> 3) : k_all_ones);
+            klo[r] = *gptr(kp);
+            khi[r] = *gptr(kp + 1);
 #pragma unroll
             for (int q = 0; q < PART_MAX_PROG; q++) {
                 const bool on = q < PS::nprog(a);
@@ -414,192 +597,123 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
                 w3[r][q] = *gptr(vp + (wd >= 16 ? 3 : 0));
             }
         }
+        // The rows' fates, with no branch on the row: hot entry (hot_m), record (rec_m), neither.
+        // fate[r]: the hot entry << 16 | the region (both below 2^16); vm: 4 validity bits per row.
+        uint32_t fate[MR];
+        uint32_t hot_m = 0, rec_m = 0, vm = 0;
 #pragma unroll
         for (int r = 0; r < MR; r++) {
-            ent[r] = -1;
-            kk[r] = 0;
-            nul[r] = false;
-            pv[r] = 0;
-            vv[r] = 0;
-            vm[r] = 0;
+            fate[r] = 0;
+            if (r >= RPT) break;
             const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+            const bool act = row_active(u, i, b.n, sbyte[r], fbyte[r], fval[r]);
             const int bit = (int)(i & 7);
 #pragma unroll
-            for (int q = 0; q < PART_MAX_PROG; q++) {
-                const int wd = q < PS::nprog(a) ? PS::prog(a, q).width : 0;
-                xa[r][q] = (uint64_t)lo[r][q] | (wd >= 8 ? (uint64_t)hi[r][q] << 32 : 0ull);
-                xb[r][q] = wd >= 16 ? ((uint64_t)w2[r][q] | ((uint64_t)w3[r][q] << 32)) : 0ull;
-                if (q < PS::nprog(a) && ((vbyte[r][q] >> bit) & 1u)) vm[r] |= 1u << q;
-            }
-            if (r >= RPT || i >= b.n || !((sbyte[r] >> bit) & 1u)) continue;
-            if (has_filter && !(((fbyte[r] >> bit) & 1u) && fval[r] < u.filter_threshold)) continue;
+            for (int q = 0; q < PART_MAX_PROG; q++)
+                if (q < PS::nprog(a) && ((vbyte[r][q] >> bit) & 1u)) vm |= 1u << (4 * r + q);
+            uint64_t key = 0;
+            bool nul;
             if (fast_key) {
-                nul[r] = !((kbyte[r] >> bit) & 1u);
-                kk[r] = (uint64_t)klo[r] | ((uint64_t)khi[r] << 32);
+                nul = !((kbyte[r] >> bit) & 1u);
+                key = (uint64_t)klo[r] | ((uint64_t)khi[r] << 32);
             } else {
                 uint64_t slot;
-                nul[r] = !key64_of(u, b, i, &kk[r], &slot);
+                nul = act ? !key64_of(u, b, i, &key, &slot) : true;
             }
-            ent[r] = -3;  // active, not classified yet
+            uint32_t k, p;
+            const bool keyed = key_class(hidx, hrow, HW, hbmask, K, pbits, key, nul, &k, &p) && act;
+            // rows of the reserved rows were folded by the count pass
+            hot_m |= (uint32_t)(keyed && k != HOT_EMPTY) << r;
+            rec_m |= (uint32_t)(keyed && k == HOT_EMPTY) << r;
+            fate[r] = (k << 16) | p;
         }
+        // record positions: the LDS atomics of every record row issued together
+        uint32_t pos[MR];
 #pragma unroll
         for (int r = 0; r < MR; r++) {
-            if (ent[r] != -3) continue;
+            pos[r] = 0;
+            if ((rec_m >> r) & 1u) pos[r] = atomicAdd(&cursor[fate[r] & 0xFFFFu], 1u);
+        }
+        // hot rows: COUNT(*), then each column's program, into the workgroup's LDS copy
+#pragma unroll
+        for (int r = 0; r < MR; r++) {
+            if (!((hot_m >> r) & 1u)) continue;
             const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-            const uint64_t key = kk[r];
-            if (nul[r] || key == EMPTY_KEY) {
-                // a reserved row (the null of a 64-bit lane, or a key equal to the empty marker)
-                const uint64_t slot = nul[r] ? u.t.slots + 1 : u.t.slots;
-                mark_reserved(u.t.counters, nul[r] ? RES_NULL : RES_SENTINEL);
-                apply_ops(u, b, i, u.t.state + slot * (uint64_t)S);
-                ent[r] = -1;
-                continue;
-            }
-            const uint64_t h = fmix64(key);
-            if (K) {
-                const uint32_t k = hot_find(hidx, hrow, HW, hbmask, h, key);
-                if (k != HOT_EMPTY) {
-                    ent[r] = (int32_t)k;
-                    continue;
-                }
-            }
-            const uint32_t p = (uint32_t)region_of_hash(pbits, h);
-            pv[r] = p;
-            vv[r] = atomicAdd(&fill[p], 1u);
-            ent[r] = -2;
-        }
-        __syncthreads();
-        // blocks for the positions [start, fill) the tile took in each chain, from the
-        // workgroup's share of the pool (refilled with one device atomic per PA_CHUNK blocks; the
-        // one region whose blocks straddle the end of the old share leaves under PA_GAP unused)
-        for (int p = threadIdx.x; p < P; p += blockDim.x) {
-            const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - (start[p] + PART_BLK - 1) / PART_BLK;
-            if (need) atomicAdd(&s_need, need);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t need = s_need;
-            if (!s_fail && s_end - s_next < need) {
-                const uint32_t take = need > (uint32_t)PA_CHUNK ? need : (uint32_t)PA_CHUNK;
-                const uint64_t b0 = atomic_add_u64(&u.t.counters[C_POOL], take);
-                if (b0 + take > a.max_blocks) {
-                    atomicOr((unsigned long long*)&u.t.counters[C_ERROR], (unsigned long long)ERR_TABLE_FULL);
-                    s_fail = 1;  // nothing handed out from here on: the launch fails
+            uint64_t* e = hrow + (size_t)(fate[r] >> 16) * HW + 1;
+            atomic_add_u64(e, 1);
+#pragma unroll
+            for (int q = 0; q < PART_MAX_PROG; q++) {
+                if (q >= PS::nprog(a)) break;
+                const ColProg& cp = PS::prog(a, q);
+                const int wd = cp.width;
+                const uint64_t xa = (uint64_t)lo[r][q] | (wd >= 8 ? (uint64_t)hi[r][q] << 32 : 0ull);
+                const uint64_t xb = wd >= 16 ? ((uint64_t)w2[r][q] | ((uint64_t)w3[r][q] << 32)) : 0ull;
+                const bool v = (vm >> (4 * r + q)) & 1u;
+                if (cp.flags & CP_GENERIC) {
+                    const RegSrc src{v, xa, xb, (uint64_t)(b.row_base + i), cp.type};
+                    for (int k = 0; k < u.nops; k++)
+                        if (u.ops[k].col == cp.col) apply_one(u, u.ops[k], src, e);
                 } else {
-                    s_alt = (uint32_t)b0;
-                    s_alt_end = (uint32_t)(b0 + take);
-                }
-            }
-            s_need = 0;
-        }
-        __syncthreads();
-        const bool refilled = s_alt_end != 0;  // else the share holds the whole tile
-        for (int p = threadIdx.x; p < P; p += blockDim.x) {
-            const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
-            const uint32_t need = (fill[p] + PART_BLK - 1) / PART_BLK - have;
-            if (need == 0) continue;
-            if (s_fail) {
-                first[p] = NO_BLOCK;
-                continue;
-            }
-            uint32_t b0 = atomicAdd(&s_next, need);
-            // the rest of the old share cannot hold it: the new range (which holds the whole tile)
-            if (refilled && b0 + need > s_end) b0 = atomicAdd(&s_alt, need);
-            first[p] = b0;
-            for (uint32_t j = 0; j < need; j++) {
-                a.block_next[b0 + j] = j ? b0 + j - 1 : cur[p];
-                a.block_cnt[b0 + j] = PART_BLK;
-            }
-        }
-        __syncthreads();
-        // record addresses of the cold rows
-        uint32_t* rec[MR];
-        uint32_t vbits[MR];
-#pragma unroll
-        for (int r = 0; r < MR; r++) {
-            rec[r] = nullptr;
-            vbits[r] = 0;
-            if (ent[r] != -2) continue;
-            const uint32_t p = pv[r], v = vv[r];
-            const uint32_t have = (start[p] + PART_BLK - 1) / PART_BLK;
-            const uint32_t j = v / PART_BLK;
-            uint32_t blk = j < have ? cur[p] : first[p];
-            if (blk == NO_BLOCK) {  // the pool ran out: the launch fails
-                ent[r] = -1;
-                continue;
-            }
-            if (j >= have) blk += j - have;
-            rec[r] = a.pool + ((uint64_t)blk * PART_BLK + (v % PART_BLK)) * (uint64_t)RW;
-        }
-        // COUNT(*): word 0 of every state row
-#pragma unroll
-        for (int r = 0; r < MR; r++)
-            if (ent[r] >= 0) atomic_add_u64(hrow + (size_t)ent[r] * HW + 1, 1);
-        // each column's program on the hot rows, the column's field of the records
-#pragma unroll
-        for (int q = 0; q < PART_MAX_PROG; q++) {
-            if (q >= PS::nprog(a)) break;
-            const ColProg cp = PS::prog(a, q);
-            const int o = cp.rec_off;
-#pragma unroll
-            for (int r = 0; r < MR; r++) {
-                const bool v = (vm[r] >> q) & 1u;
-                if (ent[r] >= 0) {
-                    uint64_t* e = hrow + (size_t)ent[r] * HW + 1;
-                    if (cp.flags & CP_GENERIC) {
-                        const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-                        const RegSrc src{v, xa[r][q], xb[r][q], (uint64_t)(b.row_base + i), cp.type};
-                        for (int k = 0; k < u.nops; k++)
-                            if (u.ops[k].col == cp.col) apply_one(u, u.ops[k], src, e);
-                    } else {
-                        apply_prog(u, cp, v, xa[r][q], xb[r][q], e);
-                    }
-                } else if (ent[r] == -2) {
-                    if (v) vbits[r] |= 1u << cp.col;
-                    if (o < 0 || !v) continue;
-                    uint32_t* dst = rec[r] + o;
-                    if (cp.type == VR_INT32) {
-                        *dst = (uint32_t)xa[r][q];
-                    } else {
-                        *(uint64_t*)dst = xa[r][q];
-                        if (cp.type == VR_DECIMAL128) *(uint64_t*)(dst + 2) = xb[r][q];
-                    }
+                    apply_prog(u, cp, v, xa, xb, e);
                 }
             }
         }
+        // records: key, the carried lanes, validity, row index, stored at the region's cursor
 #pragma unroll
         for (int r = 0; r < MR; r++) {
-            if (ent[r] != -2) continue;
-            *(uint64_t*)rec[r] = kk[r];
-            rec[r][PS::valid_off(a)] = vbits[r];
-            if (PS::row_off(a) >= 0)
-                *(uint64_t*)(rec[r] + PS::row_off(a)) = (uint64_t)(b.row_base + row0 + (int64_t)r * PA_BLOCK + threadIdx.x);
+            if (!((rec_m >> r) & 1u)) continue;
+            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
+            uint32_t(&w)[RW] = rec[r];
+#pragma unroll
+            for (int j = 0; j < RW; j++) w[j] = 0;
+            w[0] = klo[r];
+            w[1] = khi[r];
+            if (!fast_key) {
+                uint64_t key = 0, slot;
+                (void)key64_of(u, b, i, &key, &slot);
+                w[0] = (uint32_t)key;
+                w[1] = (uint32_t)(key >> 32);
+            }
+            uint32_t vbits = 0;
+#pragma unroll
+            for (int q = 0; q < PART_MAX_PROG; q++) {
+                if (q >= PS::nprog(a)) break;
+                const ColProg& cp = PS::prog(a, q);
+                if ((vm >> (4 * r + q)) & 1u) vbits |= 1u << cp.col;
+                const int o = cp.rec_off;
+                if (o < 0) continue;
+                put_word<RW>(w, o, lo[r][q]);
+                if (cp.type != VR_INT32) put_word<RW>(w, o + 1, hi[r][q]);
+                if (cp.type == VR_DECIMAL128) {
+                    put_word<RW>(w, o + 2, w2[r][q]);
+                    put_word<RW>(w, o + 3, w3[r][q]);
+                }
+            }
+            put_word<RW>(w, PS::valid_off(a), vbits);
+            if (PS::row_off(a) >= 0) {
+                const uint64_t gi = (uint64_t)(b.row_base + i);
+                put_word<RW>(w, PS::row_off(a), (uint32_t)gi);
+                put_word<RW>(w, PS::row_off(a) + 1, (uint32_t)(gi >> 32));
+            }
+            // a position past the pool (the count pass disagreeing, never expected) is kept in the
+            // pool; the range check after the loop fails the launch
+            rec_at[r] = pos[r] < a.pool_records ? pos[r] : a.pool_records - 1;
+            rec_on |= 1u << r;
+            if (!kDelay) store_record(r);
         }
-        __syncthreads();
-        for (int p = threadIdx.x; p < P; p += blockDim.x) {
-            const uint32_t s0 = start[p], f = fill[p];
-            if (f == s0) continue;
-            const uint32_t have = (s0 + PART_BLK - 1) / PART_BLK;
-            const uint32_t need = (f + PART_BLK - 1) / PART_BLK - have;
-            if (need && first[p] != NO_BLOCK) cur[p] = first[p] + need - 1;
-            start[p] = f;
-        }
-        if (more && threadIdx.x < DESC_WORDS) ((uint64_t*)&lb[cur_buf ^ 1])[threadIdx.x] = next_word;
-        if (threadIdx.x == 0 && s_alt_end) {  // the new range is the share from here on
-            s_next = s_alt;
-            s_end = s_alt_end;
-            s_alt = s_alt_end = 0;
-        }
-        __syncthreads();
-        cur_buf ^= 1;
+        // Every load of the tile has arrived by now (the stores issued before them are done too):
+        // saying so here keeps the compiler from holding the next tile's first instructions for
+        // the loads of rows that took no part, whose values were never read.
+        if (kDelay) vmem_wait_all();
     }
-    // close the chains (the newest block's record count, the heads) and publish the hot partials
-    for (int p = threadIdx.x; p < P; p += blockDim.x) {
-        const uint32_t f = fill[p];
-        const uint32_t c = cur[p];
-        if (f && c != NO_BLOCK) a.block_cnt[c] = f - (f - 1) / PART_BLK * PART_BLK;
-        a.heads[(size_t)blockIdx.x * P + p] = f ? c : NO_BLOCK;
-    }
+    if (kDelay)
+        for (int r = 0; r < MR; r++) store_record(r);
+    __syncthreads();
+    // every range filled exactly (the count pass classified the same rows), and the hot partials
+    const uint32_t* wcount = a.count + (size_t)blockIdx.x * P;
+    for (int p = threadIdx.x; p < P; p += blockDim.x)
+        if (cursor[p] != a.rstart[p] + wbase[p] + wcount[p])
+            atomicOr((unsigned long long*)&u.t.counters[C_ERROR], (unsigned long long)ERR_INTERNAL);
     uint64_t* hp = a.hot_part + (size_t)blockIdx.x * K * S;
     for (int w = threadIdx.x; w < K * S; w += blockDim.x) hp[w] = hrow[(size_t)(w / S) * HW + 1 + w % S];
 }
@@ -619,9 +733,9 @@ __global__ __launch_bounds__(256) void hot_reduce_kernel(PartArgs a) {
 
 // ------------------------------------------------------------------ phase B: merge
 
-// LDS bytes of the merge: the cache (E keys and state rows) and the waves' block lists.
+// LDS bytes of the merge: the cache (E keys and state rows).
 __host__ __device__ inline size_t merge_lds_bytes(int E, int stride) {
-    return (size_t)E * lds_row_words(stride) * 8 + (size_t)PB_GLIST * 4;
+    return (size_t)E * lds_row_words(stride) * 8;
 }
 
 // Entry of `key` in the LDS cache, claiming an empty one; -1 when neither is found within `limit`
@@ -690,13 +804,10 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     const UpdateArgs& u = a.u;
     const int S = u.t.stride;
     const int E = a.cache_e;
-    const int P = a.nparts;
     const uint32_t p = blockIdx.x;
     const int CW = lds_row_words(S);
     uint64_t* crow = lds;  // cache entry e: row e (key, state words)
     const int lane = threadIdx.x & 63;
-    // the workgroup's list of blocks: (records - 1) << PB_ID_BITS | block
-    uint32_t* glist = (uint32_t*)(crow + (size_t)E * CW);
     __shared__ unsigned int s_count, s_inserted, s_rg, s_new, s_more;
     TableView tv = u.t;
     tv.rgroups = nullptr;  // inserts are counted here, once per merge
@@ -858,56 +969,40 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
         return true;
     };
 
-    // The region's blocks, handed out to the waves as they ask (the waves' work evens out whatever
-    // the chains' lengths).  A round: the threads owning a chain (thread c: chain c < grid_a) walk
-    // it, at most `cap` blocks each, into the workgroup's block list; after a barrier the waves
-    // take blocks from the list with an LDS counter; rounds repeat until every chain is walked.
-    __shared__ unsigned int s_gcount, s_gnext, s_walk;
-    const int cap = PB_GLIST / a.grid_a < PB_LANE_BLOCKS ? PB_GLIST / a.grid_a : PB_LANE_BLOCKS;
-    uint32_t ccur = NO_BLOCK;
-    auto walk = [&]() {
-        for (int t = 0; t < cap && ccur != NO_BLOCK; t++) {
-            const uint32_t b = ccur;
-            const uint32_t n = a.block_cnt[b];
-            ccur = a.block_next[b];
-            glist[atomicAdd(&s_gcount, 1u)] = ((n - 1) << PB_ID_BITS) | b;
-        }
-        if (ccur != NO_BLOCK) s_walk = 1;
-    };
-    // the next block of the list for this wave (NO_BLOCK: none) and its record count
+    // The region's blocks (its records are contiguous from rstart[p], PART_BLK to a block, the
+    // last one partly filled), handed out to the waves as they ask with an LDS counter: the waves'
+    // work evens out whatever the records' keys.
+    __shared__ unsigned int s_gnext;
+    const uint32_t rblock0 = a.rstart[p] / PART_BLK, rrecs = a.rtot[p];
+    const uint32_t rblocks = (rrecs + PART_BLK - 1) / PART_BLK;
+    // the next block for this wave (NO_BLOCK: none) and its record count
     auto fetch = [&](uint32_t* n) -> uint32_t {
         uint32_t i = 0;
         if (lane == 0) i = atomicAdd(&s_gnext, 1u);
         i = __shfl(i, 0, 64);
-        if (i >= s_gcount) {
+        if (i >= rblocks) {
             *n = 0;
             return NO_BLOCK;
         }
-        const uint32_t x = glist[i];
-        *n = (x >> PB_ID_BITS) + 1;
-        return x & ((1u << PB_ID_BITS) - 1);
+        *n = i + 1 < rblocks ? (uint32_t)PART_BLK : rrecs - i * PART_BLK;
+        return rblock0 + i;
     };
-    if (threadIdx.x == 0) {
-        s_gcount = 0;
-        s_gnext = 0;
-        s_walk = 0;
-    }
-    __syncthreads();
 
     // Passes.  Pass 0 folds every record; a record that finds no cache entry (the cache is full)
     // is marked in its block's word of the other pending map, and pass k >= 1 folds the records
     // marked by pass k - 1.  Each pass ends with one merge of the cache into the region.  The two
     // maps are used in turn, so a pass whose merge finds the region short stops with its input
-    // intact and resumes there after the table grows.  Within a round the waves run on their own
+    // intact and resumes there after the table grows.  Within a pass the waves run on their own
     // (no barrier): each loads the records of its blocks two blocks ahead, into three buffers
     // used in turn (each unrolled step names its buffers, so no register copy waits for a load in
     // flight).
     int pass = resume - 1;
     uint32_t ra[RW], rb[RW], rc[RW];
     while (!stopped) {
-        const uint64_t* rd = pass ? a.pending + (size_t)(pass & 1) * a.max_blocks : nullptr;
-        uint64_t* wr = a.pending + (size_t)((pass + 1) & 1) * a.max_blocks;
-        ccur = threadIdx.x < a.grid_a ? a.heads[(size_t)threadIdx.x * P + p] : NO_BLOCK;
+        const uint64_t* rd = pass ? a.pending + (size_t)(pass & 1) * a.pool_blocks : nullptr;
+        uint64_t* wr = a.pending + (size_t)((pass + 1) & 1) * a.pool_blocks;
+        if (threadIdx.x == 0) s_gnext = 0;
+        __syncthreads();
         // this lane's record of block b (n records); true when it is pending in this pass
         auto load = [&](uint32_t b, uint32_t n, uint32_t (&w)[RW]) -> bool {
             if (b == NO_BLOCK || (uint32_t)lane >= n) return false;
@@ -942,9 +1037,7 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
             }
             return bY != NO_BLOCK;
         };
-        while (true) {  // rounds
-            walk();
-            __syncthreads();
+        {
             uint32_t ba, bb, bc = NO_BLOCK, n;
             bool ha, hb, hc = false;
             ba = fetch(&n);
@@ -955,16 +1048,6 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
                 while (run_step(ra, ha, ba, bb, rc, bc, hc) && run_step(rb, hb, bb, bc, ra, ba, ha) &&
                        run_step(rc, hc, bc, ba, rb, bb, hb)) {
                 }
-            __syncthreads();
-            const bool again = s_walk != 0;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                s_gcount = 0;
-                s_gnext = 0;
-                s_walk = 0;
-            }
-            __syncthreads();
-            if (!again) break;
         }
         bool more;
         if (!flush(&more)) {

@@ -28,6 +28,10 @@ __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
     return k;
 }
 
+// Waits for every vector memory operation of this wave (loads and stores) to complete: the s_waitcnt
+// encoding of gfx9 with vmcnt 0 and expcnt / lgkmcnt left at their maxima (no wait on them).
+__device__ __forceinline__ void vmem_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Streaming loads of input columns: read once, so they should not displace the hash table from
 // the caches (nontemporal hint).
 template <class T>

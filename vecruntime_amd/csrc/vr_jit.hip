@@ -43,7 +43,7 @@ std::string prog_literal(const ColProg& c) {
     return b;
 }
 
-std::string jit_source(const ColProg* prog, int nprog, const RecLayout& L) {
+std::string jit_source(const ColProg* prog, int nprog, const RecLayout& L, bool fast_key) {
     std::string s = "#include \"vr_agg_part.h\"\nnamespace vr {\n__device__ constexpr ColProg kJitProg[PART_MAX_PROG] = {\n";
     for (int q = 0; q < PART_MAX_PROG; q++) {
         ColProg none{};
@@ -58,8 +58,12 @@ std::string jit_source(const ColProg* prog, int nprog, const RecLayout& L) {
          "; }\n";
     s += "    static __device__ __forceinline__ int row_off(const PartArgs&) { return " + std::to_string(L.row_off) +
          "; }\n";
+    s += std::string("    static __device__ __forceinline__ bool fast_key(const PartArgs&) { return ") +
+         (fast_key ? "true" : "false") + "; }\n";
+    s += "    static __device__ __forceinline__ int rpt(const PartArgs&) { return PA_JIT_RPT; }\n";
     s += "};\n}  // namespace vr\n";
     const std::string rw = std::to_string(L.words);
+    s += "template __global__ void vr::part_count_kernel<vr::JitProgs>(vr::PartArgs);\n";
     s += "template __global__ void vr::part_scatter_kernel<" + rw + ", vr::JitProgs>(vr::PartArgs);\n";
     s += "template __global__ void vr::part_merge_kernel<" + rw + ", vr::JitProgs>(vr::PartArgs);\n";
     return s;
@@ -78,6 +82,8 @@ PartJit* compile(int ordinal, const std::string& src, const RecLayout& L) {
         HIPRTC_SUCCESS)
         return nullptr;
     const std::string nscatter = kernel_name("part_scatter_kernel", L), nmerge = kernel_name("part_merge_kernel", L);
+    const std::string ncount = "vr::part_count_kernel<vr::JitProgs>";
+    hiprtcAddNameExpression(prog, ncount.c_str());
     hiprtcAddNameExpression(prog, nscatter.c_str());
     hiprtcAddNameExpression(prog, nmerge.c_str());
     const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
@@ -85,15 +91,17 @@ PartJit* compile(int ordinal, const std::string& src, const RecLayout& L) {
     const char* opts[] = {arch.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
     const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
     std::vector<char> code;
-    std::string lscatter, lmerge;  // the instantiations' symbol names
+    std::string lcount, lscatter, lmerge;  // the instantiations' symbol names
     if (r == HIPRTC_SUCCESS) {
-        const char *a = nullptr, *b = nullptr;
+        const char *a = nullptr, *b = nullptr, *c = nullptr;
         size_t n = 0;
         if (hiprtcGetLoweredName(prog, nscatter.c_str(), &a) == HIPRTC_SUCCESS &&
-            hiprtcGetLoweredName(prog, nmerge.c_str(), &b) == HIPRTC_SUCCESS && a && b &&
+            hiprtcGetLoweredName(prog, nmerge.c_str(), &b) == HIPRTC_SUCCESS &&
+            hiprtcGetLoweredName(prog, ncount.c_str(), &c) == HIPRTC_SUCCESS && a && b && c &&
             hiprtcGetCodeSize(prog, &n) == HIPRTC_SUCCESS && n) {
             lscatter = a;
             lmerge = b;
+            lcount = c;
             code.resize(n);
             if (hiprtcGetCode(prog, code.data()) != HIPRTC_SUCCESS) code.clear();
         }
@@ -112,6 +120,7 @@ PartJit* compile(int ordinal, const std::string& src, const RecLayout& L) {
     if (cur != ordinal && hipSetDevice(ordinal) != hipSuccess) return nullptr;
     PartJit* j = new PartJit{};
     bool ok = hipModuleLoadData(&j->module, code.data()) == hipSuccess &&
+              hipModuleGetFunction(&j->count, j->module, lcount.c_str()) == hipSuccess &&
               hipModuleGetFunction(&j->scatter, j->module, lscatter.c_str()) == hipSuccess &&
               hipModuleGetFunction(&j->merge, j->module, lmerge.c_str()) == hipSuccess;
     if (cur != ordinal) (void)hipSetDevice(cur);
@@ -125,8 +134,8 @@ PartJit* compile(int ordinal, const std::string& src, const RecLayout& L) {
 
 }  // namespace
 
-const PartJit* part_jit(int ordinal, const ColProg* prog, int nprog, const RecLayout& L) {
-    const std::string src = jit_source(prog, nprog, L);
+const PartJit* part_jit(int ordinal, const ColProg* prog, int nprog, const RecLayout& L, bool fast_key) {
+    const std::string src = jit_source(prog, nprog, L, fast_key);
     const std::string key = std::to_string(ordinal) + "\n" + src;
     std::shared_ptr<JitEntry> e;
     {

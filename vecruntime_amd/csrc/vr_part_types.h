@@ -13,18 +13,13 @@ constexpr int PA_BLOCK = 1024;       // scatter workgroup
 constexpr int PA_MAX_RPT = 2;        // rows per thread per scatter tile (generic kernel)
 constexpr int PA_JIT_RPT = 4;        // the same, kernels compiled for their spec (fewer registers per row)
 constexpr int PART_MAX_PROG = 4;     // value columns the partitioned update folds (programs)
-constexpr int PA_CHUNK = 1024;       // pool blocks a scatter workgroup takes at a time
-constexpr int PA_GAP = PA_BLOCK * PA_JIT_RPT / PART_BLK + 1;  // most blocks one region takes in one tile
 constexpr int PB_BLOCK = 1024;       // merge workgroup
 constexpr int PART_MAX_PBITS = 12;   // regions of the partitioned update: at most 2^12
 constexpr int HS_BLOCK = 256;        // sampling workgroup
 constexpr int HS_ROWS = 2048;        // rows per sample
 constexpr int HS_SLOTS = 4096;       // LDS counting slots of a sample
-constexpr int PB_GLIST = 2048;       // block list entries of a merge workgroup
 constexpr int PB_CLAIM_BITS = 8192;  // merge: regions of at most this many slots claim new slots in LDS
 constexpr int PB_FE = 2;             // merge cache entries per thread (the cache holds PB_FE * PB_BLOCK)
-constexpr int PB_LANE_BLOCKS = 8;    // blocks a thread takes from its chain per walk
-constexpr int PB_ID_BITS = 26;       // block number bits of a list entry (the record count above)
 constexpr uint32_t HOT_EMPTY = 0xFFFFFFFFu;
 
 struct RecLayout {
