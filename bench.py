@@ -687,7 +687,9 @@ def run_join(dev, args):
     for name, jt in (("inner", abi.INNER), ("semi", abi.LEFT_SEMI), ("anti", abi.LEFT_ANTI)):
         bo = [0, 1, 2] if jt == abi.INNER else []
         outs = [vr.device_out_column(t, cap, d) for t, d in zip(types[:3 + len(bo)], outs_all)]
+        payload_before = dev.stats()["payload_join_probes"]
         bt.probe_batches(pb, [0], jt, outputs=outs, probe_out=[0, 1, 2], build_out=bo, capacity=cap)  # warm
+        payload_table = dev.stats()["payload_join_probes"] > payload_before
         reps = 3
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -724,7 +726,8 @@ def run_join(dev, args):
         torch.cuda.synchronize()
         plain_s = time.perf_counter() - t1
         assert ptotal == total, (ptotal, total)
-        res[name] = {"out_rows": total, "batched": {"ms": wall * 1e3, "device_ms": dev_ms, "rows_per_s": npr / wall},
+        res[name] = {"out_rows": total, "payload_table": payload_table,
+                     "batched": {"ms": wall * 1e3, "device_ms": dev_ms, "rows_per_s": npr / wall},
                      "plain": {"ms": plain_s * 1e3, "rows_per_s": npr / plain_s, "calls": len(calls)},
                      "batched_over_plain": plain_s / wall,
                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -133,6 +133,7 @@ typedef struct vr_stats {
     int64_t specialized_updates;  /* of those, calls that ran kernels compiled for their spec */
     int64_t dense_join_builds;    /* join builds that took the direct key index */
     int64_t device_allocations;   /* device blocks the library handed out (new or from its cache) */
+    int64_t payload_join_probes;  /* probe calls that read build columns from the payload table's cells */
 } vr_stats;
 vr_status vr_stats_get(vr_device* dev, vr_stats* out);
 

@@ -117,6 +117,7 @@ class VrStats(C.Structure):
         ("specialized_updates", C.c_int64),
         ("dense_join_builds", C.c_int64),
         ("device_allocations", C.c_int64),
+        ("payload_join_probes", C.c_int64),
     ]
 
 

@@ -212,6 +212,129 @@ __global__ void join_values_kernel(JoinTable t, const uint32_t* cell_count, cons
     }
 }
 
+// ---- cells that carry the build row's payload (unique keys, fixed-width payload lanes, no nulls)
+//
+// For inner and outer joins that write build columns, a second table whose 32-byte cells hold the
+// key, the build row and up to 20 bytes of that row's payload lanes packed in column order, four
+// cells to a 128-byte bucket: the line a probe reads to find the key also holds every build column
+// it writes, so an output row costs no gather (a random line per build column otherwise).  Cell
+// words: key; row (low half, bit 31 of a bucket's first cell = the passed flag) | payload word 0;
+// payload words 1, 2; payload words 3, 4.  Home buckets and probing follow the main table.
+constexpr int PBUCKET = 4;       // cells per payload bucket
+constexpr int PAY_WORDS = 5;     // 32-bit payload words per cell
+constexpr int PAY_MAX_COLS = 8;  // payload lanes of one table
+constexpr uint32_t PROW_PASSED = 1u << 31;
+
+struct PayTable {
+    uint64_t* buckets;  // nb * 16 words
+    uint64_t nb;
+};
+
+struct PaySrc {
+    const void* data[PAY_MAX_COLS];
+    int32_t width[PAY_MAX_COLS];  // 4, 8 or 16 bytes
+    int32_t off[PAY_MAX_COLS];    // byte offset in the payload, a multiple of 4
+    int32_t ncols;
+    int32_t pad;
+};
+
+__global__ void pay_fill_kernel(uint64_t* buckets, uint64_t words) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        buckets[w] = (w & 3) == 0 ? EMPTY_KEY : 0ull;
+}
+
+// Inserts build row i (keys unique, none equal to the empty marker): the claiming lane writes the
+// row (OR-ed in, as other rows' inserts may set the passed flag in the same word) and the payload.
+__global__ void pay_insert_kernel(PayTable t, DCol k0, DCol k1, int nk, int64_t n, PaySrc src) {
+    DCol k[2] = {k0, k1};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t key;
+        if (!key64_of(k, nk, i, &key)) continue;  // a null key never matches
+        uint32_t pw[PAY_WORDS] = {0, 0, 0, 0, 0};
+        for (int c = 0; c < src.ncols; c++) {
+            const int wd = src.width[c] / 4;
+            const uint32_t* from = (const uint32_t*)src.data[c] + (uint64_t)i * wd;
+            for (int q = 0; q < wd; q++) pw[src.off[c] / 4 + q] = from[q];
+        }
+        uint64_t b = home_bucket(key, t.nb);
+        while (true) {
+            uint64_t* cells = t.buckets + b * 16;
+            int claimed = -1;
+            for (int j = 0; j < PBUCKET && claimed < 0; j++)
+                if (cells[4 * j] == EMPTY_KEY && atomic_cas_u64(&cells[4 * j], EMPTY_KEY, key) == EMPTY_KEY) claimed = j;
+            if (claimed >= 0) {
+                uint64_t* c = cells + 4 * claimed;
+                atomicOr((unsigned int*)&c[1], (unsigned int)i);
+                ((uint32_t*)&c[1])[1] = pw[0];
+                c[2] = (uint64_t)pw[1] | ((uint64_t)pw[2] << 32);
+                c[3] = (uint64_t)pw[3] | ((uint64_t)pw[4] << 32);
+                break;
+            }
+            atomicOr((unsigned int*)&cells[1], PROW_PASSED);  // every cell holds another key: go on
+            b = next_bucket(b, t.nb);
+        }
+    }
+}
+
+// The payload cell of `key` walking from bucket b: its row (or -1) and payload words.
+__device__ __forceinline__ int64_t pay_lookup_from(const PayTable& t, uint64_t key, uint64_t b, uint32_t (&pay)[PAY_WORDS]) {
+    while (true) {
+        const uint64_t* cells = t.buckets + b * 16;
+        for (int j = 0; j < PBUCKET; j++)
+            if (cells[4 * j] == key) {
+                const uint64_t w1 = cells[4 * j + 1], w2 = cells[4 * j + 2], w3 = cells[4 * j + 3];
+                pay[0] = (uint32_t)(w1 >> 32);
+                pay[1] = (uint32_t)w2;
+                pay[2] = (uint32_t)(w2 >> 32);
+                pay[3] = (uint32_t)w3;
+                pay[4] = (uint32_t)(w3 >> 32);
+                return (int64_t)((uint32_t)w1 & ~PROW_PASSED);
+            }
+        if (!((uint32_t)cells[1] & PROW_PASSED)) return -1;
+        b = next_bucket(b, t.nb);
+    }
+}
+
+// Wave-cooperative lookup of the wave's 64 rows in the payload table: eight lanes read one bucket
+// (lane j: half j & 1 of cell j >> 1), a ballot over the key halves finds the cell, shuffles hand
+// its row and payload words to the row's lane.  A row whose home bucket lacks the key but was
+// passed goes on by itself.  Returns the build row, -1 when absent (or `active` false).
+__device__ __forceinline__ int64_t coop_pay_lookup(const PayTable& t, bool active, uint64_t key, uint32_t (&pay)[PAY_WORDS]) {
+    const int lane = threadIdx.x & 63, g = lane >> 3, j = lane & 7;
+    const uint32_t home = active ? (uint32_t)home_bucket(key, t.nb) : 0u;
+    const u64x2* q = (const u64x2*)t.buckets;
+    u64x2 x[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const uint32_t bb = __shfl(home, g * 8 + s);
+        x[s] = q[(uint64_t)bb * 8 + j];
+    }
+    int64_t row = -1;
+    bool cont = false;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const uint64_t kk = __shfl(key, g * 8 + s);
+        const uint64_t mb = __ballot((j & 1) == 0 && x[s].x == kk);
+        const uint64_t pb = __ballot(j == 0 && ((uint32_t)x[s].y & PROW_PASSED));
+        const uint32_t gm = (uint32_t)(mb >> (g * 8)) & 0xFFu;
+        const int m = gm ? __ffs(gm) - 1 : 0;
+        const uint64_t a1 = __shfl(x[s].y, g * 8 + m);
+        const uint64_t b0 = __shfl(x[s].x, g * 8 + m + 1);
+        const uint64_t b1 = __shfl(x[s].y, g * 8 + m + 1);
+        if (j == s) {
+            row = gm ? (int64_t)((uint32_t)a1 & ~PROW_PASSED) : -1;
+            cont = !gm && ((pb >> (g * 8)) & 1ull);
+            pay[0] = (uint32_t)(a1 >> 32);
+            pay[1] = (uint32_t)b0;
+            pay[2] = (uint32_t)(b0 >> 32);
+            pay[3] = (uint32_t)b1;
+            pay[4] = (uint32_t)(b1 >> 32);
+        }
+    }
+    if (active && cont) row = pay_lookup_from(t, key, next_bucket(home, t.nb), pay);
+    return active ? row : -1;
+}
+
 // The key's value word (0 when absent), walking from bucket b.
 __device__ __forceinline__ uint64_t lookup_from(const JoinTable& t, uint64_t key, uint64_t b) {
     const u64x2* cells = (const u64x2*)t.buckets;
@@ -312,6 +435,10 @@ struct PBatch {
 // the streamed operands of a residual.
 struct ProbeArgs {
     JoinTable t;
+    PayTable pt;                // buckets != nullptr: look keys up in the payload table (pay_off)
+    int32_t pay_off[8];         // build output c: its byte offset in the payload; -1: not taken from it
+    int32_t pay_words;          // payload words the table's cells use (staged in LDS per row)
+    int32_t pad_pw;
     WideKeys bk;  // wide keys: the build side's key lanes
     DenseIndex dense;
     int32_t nk;
@@ -425,6 +552,26 @@ __device__ __forceinline__ void copy_value(const DCol& src, int64_t i, const Out
     }
 }
 
+// Writes output row o of a build column from a row's payload words staged in LDS (`row_sh`: word w
+// at row_sh[w * PROBE_BLOCK + thread]; the lane starts at byte `off`); payload lanes have no nulls.
+__device__ __forceinline__ void copy_pay(const uint32_t* row_sh, int off, const OutCol& dst, int64_t o) {
+    const uint32_t* p = row_sh + (off / 4) * PROBE_BLOCK + threadIdx.x;
+    if (dst.valid) atomicOr((unsigned int*)dst.valid + (o >> 5), 1u << (o & 31));
+    switch (dst.type) {
+        case VR_INT32: ((uint32_t*)dst.data)[o] = p[0]; break;
+        case VR_INT64:
+        case VR_FLOAT64: ((uint64_t*)dst.data)[o] = (uint64_t)p[0] | ((uint64_t)p[PROBE_BLOCK] << 32); break;
+        case VR_DECIMAL128:
+            ((uint64_t*)dst.data)[2 * o] = (uint64_t)p[0] | ((uint64_t)p[PROBE_BLOCK] << 32);
+            ((uint64_t*)dst.data)[2 * o + 1] = (uint64_t)p[2 * PROBE_BLOCK] | ((uint64_t)p[3 * PROBE_BLOCK] << 32);
+            break;
+        default: break;
+    }
+}
+
+// LDS bytes of a probe launch that reads the payload table: its rows' payload words.
+__host__ __device__ inline size_t pay_lds(int words) { return (size_t)PROBE_ITEMS * words * PROBE_BLOCK * 4; }
+
 constexpr uint64_t ST_AGG = 1ull << 62, ST_PREFIX = 2ull << 62, ST_MASK = 3ull << 62;
 constexpr int PROBE_WAVES = PROBE_BLOCK / 64;
 
@@ -475,13 +622,14 @@ __device__ __forceinline__ uint64_t wave_look_back(uint64_t* tile_state, int64_t
 // offset from a wave-wide decoupled look-back over the preceding tiles, and the rows write pairs
 // or materialized columns.  Outputs of all batches follow each other in batch order, each batch's
 // in the reference's order: streamed rows in order, each row's build rows in build order.
-template <bool MULTI>
+template <bool MULTI, bool PAY>
 __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     __shared__ int64_t tile_id_sh;
     __shared__ int32_t batch_sh;
     __shared__ uint64_t base_sh;
     __shared__ uint32_t wave_tot[PROBE_ITEMS][PROBE_WAVES];  // totals, then their exclusive prefix in the tile
     __shared__ DCol psrc_sh[8], ksrc_sh[MAX_JOIN_KEYS];
+    extern __shared__ uint32_t pay_sh[];  // payload table lookups: row r's word w at [(r * PAY_WORDS + w) * PROBE_BLOCK + thread]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x == 0) {
         const int64_t t = (int64_t)atomic_add_u64(&a.tile_counter[0], 1);
@@ -544,7 +692,14 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
         const bool in = (flags >> r) & 1u, nonnull = (flags >> (16 + r)) & 1u;
         uint64_t v = 0;
         if (a.build_rows) {  // uniform: every lane of the wave takes the same branch
-            if (a.dense.slots) {
+            if (PAY) {  // the payload table (a.pt)
+                uint32_t pw[PAY_WORDS];
+                const int64_t br = coop_pay_lookup(a.pt, nonnull && key[r] != EMPTY_KEY, key[r], pw);
+                v = br >= 0 ? ((1ull << 32) | (uint64_t)br) : 0ull;
+#pragma unroll
+                for (int w = 0; w < PAY_WORDS; w++)
+                    if (w < a.pay_words) pay_sh[(r * a.pay_words + w) * PROBE_BLOCK + threadIdx.x] = pw[w];
+            } else if (a.dense.slots) {
                 if (nonnull) v = dense_lookup(a.dense, a.ctype[0], key[r]);
             } else {
                 v = coop_lookup(a.t, nonnull && key[r] != EMPTY_KEY, key[r]);
@@ -649,6 +804,8 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
             for (int c = 0; c < a.nbuild_out; c++) {
                 const int ks = a.build_from_key[c];
                 if (ks >= 0 && brow >= 0) copy_value(ksrc_sh[ks], i, a.outs[a.nprobe_out + c], (int64_t)o);
+                else if (PAY && a.pay_off[c] >= 0 && brow >= 0)
+                    copy_pay(pay_sh + r * a.pay_words * PROBE_BLOCK, a.pay_off[c], a.outs[a.nprobe_out + c], (int64_t)o);
                 else copy_value(a.build_src[c], brow, a.outs[a.nprobe_out + c], (int64_t)o);
             }
         }
@@ -785,6 +942,12 @@ struct JoinImpl {
     // build rows paired by probes (buildMatched), when the spec asks for them
     bool track = false;
     DevBuf matched, unmatched;
+    // the payload table (unique keys, fixed-width payload lanes without nulls, at most 20 bytes):
+    // pay_off[c] = build column c's byte offset in a cell's payload, -1 when not in it
+    DevBuf pay;
+    uint64_t pay_nb = 0;
+    int32_t pay_words = 0;
+    std::vector<int32_t> pay_off;
     // the direct key index (DenseKeyIndex), when the build allows it
     int32_t dense_mode = 0;  // vr_join_spec.dense_index
     DevBuf dense;
@@ -1052,6 +1215,54 @@ vr_status vr_join_build_append(vr_join* h, const vr_column* columns, int64_t n, 
     return VR_OK;
 }
 
+// The payload table, when every key is unique (no key equal to the empty marker) and the non-key
+// build columns are fixed-width lanes without nulls that fit a cell's 20 payload bytes.
+static vr_status build_payload_table(JoinImpl& j, bool marker, hipStream_t s) {
+    j.pay.reset();
+    j.pay_nb = 0;
+    j.pay_off.assign(j.cols.size(), -1);
+    const int64_t n = j.rows;
+    if (j.wide || marker || n <= 0 || j.max_chain != 1 || n >= (int64_t)PROW_PASSED) return VR_OK;
+    PaySrc src{};
+    int off = 0;
+    for (size_t c = 0; c < j.cols.size(); c++) {
+        if (std::find(j.key_cols.begin(), j.key_cols.end(), (int32_t)c) != j.key_cols.end()) continue;
+        const int32_t t = j.col_types[c];
+        const int w = t == VR_INT32 ? 4 : (t == VR_INT64 || t == VR_FLOAT64) ? 8 : t == VR_DECIMAL128 ? 16 : 0;
+        if (!w || j.cols[c].has_valid || off + w > PAY_WORDS * 4 || src.ncols == PAY_MAX_COLS) return VR_OK;
+        src.data[src.ncols] = j.cols[c].data.p;
+        src.width[src.ncols] = w;
+        src.off[src.ncols] = off;
+        src.ncols++;
+        j.pay_off[c] = off;
+        off += w;
+    }
+    if (!src.ncols) {
+        j.pay_off.assign(j.cols.size(), -1);
+        return VR_OK;
+    }
+    // a quarter of the cells hold keys: with four cells to a bucket, about one home bucket in 270
+    // overflows (at half, one in 19), and a row whose home bucket overflowed holds its wave back
+    const uint64_t cells = (uint64_t)((double)n / 0.25) + 1;
+    const uint64_t nb = (cells + PBUCKET - 1) / PBUCKET;
+    VR_TRY(j.pay.ensure(j.dev, nb * 16 * sizeof(uint64_t)));
+    const int cap = j.dev->num_cus * 16;
+    hipLaunchKernelGGL(pay_fill_kernel, dim3(grid_of(nb * 16, 256, cap)), dim3(256), 0, s, j.pay.as<uint64_t>(), nb * 16);
+    DCol k[2]{};
+    for (size_t q = 0; q < j.key_types.size(); q++) {
+        auto& bc = j.cols[j.key_cols[q]];
+        k[q].type = j.key_types[q];
+        k[q].data = bc.data.p;
+        k[q].valid = bc.has_valid ? bc.valid.as<uint8_t>() : nullptr;
+    }
+    hipLaunchKernelGGL(pay_insert_kernel, dim3(grid_of(n, 256, cap)), dim3(256), 0, s, PayTable{j.pay.as<uint64_t>(), nb}, k[0],
+                       k[1], (int)j.key_types.size(), n, src);
+    VR_TRY_HIP(hipGetLastError());
+    j.pay_nb = nb;
+    j.pay_words = (off + 3) / 4;
+    return VR_OK;
+}
+
 vr_status vr_join_build_finish(vr_join* h, void* stream) {
     if (!h) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     JoinImpl& j = h->j;
@@ -1182,6 +1393,7 @@ vr_status vr_join_build_finish(vr_join* h, void* stream) {
         j.seed = fmix64(j.seed + (uint64_t)attempt + 1);  // build again with another hash
     }
     VR_TRY(build_dense(j, dense_candidate, (int64_t)small[3], (int64_t)small[4], sentinel_rows != 0, s));
+    VR_TRY(build_payload_table(j, sentinel_rows != 0, s));
     if (j.track) {
         const uint64_t words = ((uint64_t)std::max<int64_t>(n, 1) + 63) / 64;
         VR_TRY(j.matched.ensure(j.dev, words * 8));
@@ -1386,6 +1598,24 @@ static vr_status prepare_probe(JoinImpl& j, const vr_column* columns, int32_t nu
         a.res_loff = res->left_offset;
         a.res_roff = res->right_offset;
     }
+    // the payload table, when every build column the probe writes is a key or in its cells
+    for (int q = 0; q < 8; q++) a.pay_off[q] = -1;
+    bool use_pay = j.pay_nb && pairs && !a.res_on, needs_pay = false;
+    for (int q = 0; q < a.nbuild_out && use_pay; q++) {
+        const int bc = out->build_out_columns[q];
+        if (a.build_from_key[q] < 0 && j.pay_off[bc] < 0) use_pay = false;
+        a.pay_off[q] = a.build_from_key[q] < 0 ? j.pay_off[bc] : -1;
+        if (a.pay_off[q] >= 0) needs_pay = true;
+    }
+    use_pay = use_pay && needs_pay;  // build keys alone come from the streamed keys
+    if (use_pay) {
+        a.pt = PayTable{j.pay.as<uint64_t>(), j.pay_nb};
+        a.pay_words = j.pay_words;
+        j.dev->count(&vr_stats::payload_join_probes, 1);
+    } else {
+        a.pt = PayTable{nullptr, 0};
+        for (int q = 0; q < 8; q++) a.pay_off[q] = -1;
+    }
     a.ncb = (int32_t)sh.slot_col.size();
     return VR_OK;
 }
@@ -1443,7 +1673,8 @@ vr_status vr_join_probe(vr_join* h, const vr_column* columns, int32_t num_column
     a.tile_state = tiles.as<uint64_t>() + 2;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(a.ntiles + 2, 256, 1024)), dim3(256), 0, s, tiles.as<uint64_t>(),
                        (uint64_t)(a.ntiles + 2));
-    hipLaunchKernelGGL(join_probe_kernel<false>, dim3((unsigned)a.ntiles), dim3(PROBE_BLOCK), 0, s, a);
+    if (a.pt.buckets) hipLaunchKernelGGL((join_probe_kernel<false, true>), dim3((unsigned)a.ntiles), dim3(PROBE_BLOCK), pay_lds(a.pay_words), s, a);
+    else hipLaunchKernelGGL((join_probe_kernel<false, false>), dim3((unsigned)a.ntiles), dim3(PROBE_BLOCK), 0, s, a);
     VR_TRY_HIP(hipGetLastError());
     uint64_t rows = 0;
     VR_TRY_HIP(hipMemcpyAsync(&rows, tiles.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
@@ -1496,7 +1727,8 @@ vr_status vr_join_probe_noalloc(vr_join* h, const vr_column* columns, int32_t nu
     a.tile_state = sp + 2;
     // cleared by a kernel, not a memset: the call may be captured into a graph
     hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(ntiles + 2, 256, 1024)), dim3(256), 0, s, sp, (uint64_t)(ntiles + 2));
-    hipLaunchKernelGGL(join_probe_kernel<false>, dim3((unsigned)ntiles), dim3(PROBE_BLOCK), 0, s, a);
+    if (a.pt.buckets) hipLaunchKernelGGL((join_probe_kernel<false, true>), dim3((unsigned)ntiles), dim3(PROBE_BLOCK), pay_lds(a.pay_words), s, a);
+    else hipLaunchKernelGGL((join_probe_kernel<false, false>), dim3((unsigned)ntiles), dim3(PROBE_BLOCK), 0, s, a);
     VR_TRY_HIP(hipGetLastError());
     uint64_t rows = 0;
     VR_TRY_HIP(hipMemcpyAsync(&rows, sp + 1, 8, hipMemcpyDeviceToHost, s));
@@ -1580,7 +1812,8 @@ vr_status vr_join_probe_batches(vr_join* h, int32_t num_batches, const vr_column
     hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_of(tiles + 2, 256, cap)), dim3(256), 0, s, a.tile_counter, (uint64_t)(tiles + 2));
     hipLaunchKernelGGL(tile_batch_kernel, dim3(grid_of(num_batches, 256, cap)), dim3(256), 0, s, a.batches, num_batches,
                        (int32_t*)a.tile_batch);
-    hipLaunchKernelGGL(join_probe_kernel<true>, dim3((unsigned)tiles), dim3(PROBE_BLOCK), 0, s, a);
+    if (a.pt.buckets) hipLaunchKernelGGL((join_probe_kernel<true, true>), dim3((unsigned)tiles), dim3(PROBE_BLOCK), pay_lds(a.pay_words), s, a);
+    else hipLaunchKernelGGL((join_probe_kernel<true, false>), dim3((unsigned)tiles), dim3(PROBE_BLOCK), 0, s, a);
     VR_TRY_HIP(hipGetLastError());
     uint64_t rows = 0;
     std::vector<int64_t> ends(batch_rows ? (size_t)num_batches : 0);
@@ -1666,7 +1899,7 @@ vr_status vr_join_unmatched_build(vr_join* h, vr_probe_output* out, void* stream
 vr_status vr_join_memory(vr_join* h, int64_t* bytes) {
     if (!h || !bytes) return fail(VR_ERR_INVALID_ARGUMENT, "null argument");
     JoinImpl& j = h->j;
-    int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.dense.bytes +
+    int64_t b = (int64_t)(j.buckets.bytes + j.range_rows.bytes + j.cub_tmp.bytes + j.small.bytes + j.dense.bytes + j.pay.bytes +
                           j.matched.bytes + j.unmatched.bytes);
     for (auto& c : j.cols) b += (int64_t)(c.data.bytes + c.valid.bytes);
     for (auto& t : j.strs) b += t.memory();
