@@ -170,7 +170,7 @@ def tune_groupby(dev, args, batches, stream):
     def run(c):
         m = c["max_load_percent"]
         if m not in handles:
-            handles[m] = vr.GroupedAggregate(dev, groupby_spec(12_000_000, m))
+            handles[m] = vr.GroupedAggregate(dev, groupby_spec(0, m))
         h = handles[m]
         h.reset()
         torch.cuda.synchronize()
@@ -204,7 +204,7 @@ def run_groupby(dev, args, n):
     torch.cuda.set_stream(stream)
     best, tuning = tune_groupby(dev, args, batches, stream)
     tuned_cfg = autotune.to_launch_config(best)
-    agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000, best["max_load_percent"]))
+    agg = vr.GroupedAggregate(dev, groupby_spec(0, best["max_load_percent"]))
 
     batch_set = vr.GroupedAggregate.prepare_batches(batches)  # as the plain path's prepare(), once
 
@@ -247,7 +247,7 @@ def run_groupby(dev, args, n):
     # the library's defaults beside the tuned configuration (a handle of the default load factor)
     res["tuning"] = dict(tuning, tuned=best)
     if best != autotune.DEFAULT:
-        dh = agg if best["max_load_percent"] == 50 else vr.GroupedAggregate(dev, groupby_spec(12_000_000))
+        dh = agg if best["max_load_percent"] == 50 else vr.GroupedAggregate(dev, groupby_spec(0))
         dcfg = abi.launch_config()
         step_batched(cols, dh, dcfg)  # warm (sizes the table)
         torch.cuda.synchronize()
@@ -339,7 +339,7 @@ def run_pcie(dev, args, host, groups):
     n = len(host["keys"])
     bytes_in = n * (8 + 8 + 8 + 4 + 3 / 8.0)
     res = {"rows": n, "host_bytes": bytes_in}
-    agg = vr.GroupedAggregate(dev, groupby_spec(12_000_000))
+    agg = vr.GroupedAggregate(dev, groupby_spec(0))
     for kind in ("pageable", "pinned"):
         batches, keep = host_groupby_batches(host, pinned=kind == "pinned")
         agg.reset()

@@ -513,20 +513,45 @@ __device__ __forceinline__ void sum_total(const uint64_t* w, bool d128, uint64_t
     *fits = t[2] == (uint64_t)((int64_t)t[1] >> 63);
 }
 
-__global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
+// Emission: one output row per thread.  STAGED: the workgroup's state rows are first copied into
+// LDS with whole rows read by neighbouring lanes (a wave reads 64 consecutive words of several
+// rows at once instead of one word of 64 rows), padded to an odd pitch of 64-bit words.
+constexpr int EMIT_BLOCK = 256;
+constexpr int EMIT_STAGE_MAX_PITCH = 33;  // state rows of at most this many words are staged
+
+__host__ __device__ inline int emit_pitch(int stride) { return stride | 1; }
+
+template <bool STAGED>
+__global__ __launch_bounds__(EMIT_BLOCK) void agg_emit_kernel(EmitArgs a) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = o < a.m;
     uint64_t slot = in ? a.order[a.from + o] : 0;
-    const uint64_t* g = a.t.state + slot * (uint64_t)a.t.stride;
+    const uint64_t* g;
+    if (STAGED) {
+        extern __shared__ uint64_t emit_rows[];
+        __shared__ uint64_t emit_slots[EMIT_BLOCK];
+        const int S = a.t.stride, pitch = emit_pitch(S);
+        emit_slots[threadIdx.x] = slot;
+        __syncthreads();
+        const int64_t left = a.m - (int64_t)blockIdx.x * blockDim.x;
+        const int nrows = left < EMIT_BLOCK ? (int)left : EMIT_BLOCK;
+        for (int i = threadIdx.x; i < nrows * S; i += EMIT_BLOCK) {
+            const int r = i / S, w = i - r * S;
+            emit_rows[r * pitch + w] = a.t.state[emit_slots[r] * (uint64_t)S + w];
+        }
+        __syncthreads();
+        g = emit_rows + threadIdx.x * pitch;
+    } else {
+        g = a.t.state + slot * (uint64_t)a.t.stride;
+    }
     // ---- keys
     const KeySpec& ks = a.ks;
-    uint32_t w[4 * VR_MAX_KEYS + 1];
+    const uint64_t* wide = a.t.cells + slot * (uint64_t)a.t.width;  // KM_WIDE: the key's words, one per cell word
+    auto w = [&](int i) { return (uint32_t)wide[i]; };
     uint64_t key64 = 0;
     bool null64 = false;
     if (in) {
         if (ks.mode == KM_WIDE) {
-            const uint64_t* c = a.t.cells + slot * (uint64_t)a.t.width;
-            for (int i = 0; i < ks.width; i++) w[i] = (uint32_t)c[i];
         } else if (slot == a.t.slots) {
             key64 = EMPTY_KEY;
         } else if (slot == a.t.slots + 1) {
@@ -548,16 +573,16 @@ __global__ __launch_bounds__(256) void agg_emit_kernel(EmitArgs a) {
         } else if (ks.mode == KM_2X32) {
             lo = c == 0 ? (key64 >> 32) : (uint32_t)key64;
         } else if (in) {
-            isnull = ks.null_word >= 0 && ((w[ks.null_word] >> c) & 1u);
+            isnull = ks.null_word >= 0 && ((w(ks.null_word) >> c) & 1u);
             int o2 = ks.word_off[c];
             switch (ks.types[c]) {
                 case VR_INT64:
-                case VR_FLOAT64: lo = (uint64_t)w[o2] | ((uint64_t)w[o2 + 1] << 32); break;
+                case VR_FLOAT64: lo = (uint64_t)w(o2) | ((uint64_t)w(o2 + 1) << 32); break;
                 case VR_DECIMAL128:
-                    lo = (uint64_t)w[o2] | ((uint64_t)w[o2 + 1] << 32);
-                    hi = (uint64_t)w[o2 + 2] | ((uint64_t)w[o2 + 3] << 32);
+                    lo = (uint64_t)w(o2) | ((uint64_t)w(o2 + 1) << 32);
+                    hi = (uint64_t)w(o2 + 2) | ((uint64_t)w(o2 + 3) << 32);
                     break;
-                default: lo = w[o2];
+                default: lo = w(o2);
             }
         }
         if (!in) isnull = true;
@@ -837,6 +862,7 @@ struct AggImpl {
 
     // batched launches
     DevBuf batch_descs;
+    HostBuf desc_host, tile_host;  // the partitioned update's batch descriptors and tile -> batch map
     DevBuf resume;  // guarded launches: the tile each workgroup resumes at
 
     // partitioned update: hot-set sampling, record counts and ranges, record pool, merge resume points
@@ -1848,16 +1874,23 @@ static vr_status update_partitioned(AggImpl& h, BatchDesc* descs, int nb, int64_
         tiles += (descs[b].n + T - 1) / T;
         seen += descs[b].n;
     }
-    VR_TRY(h.batch_descs.ensure(h.dev, sizeof(BatchDesc) * (size_t)nb));
-    VR_TRY_HIP(hipMemcpyAsync(h.batch_descs.p, descs, sizeof(BatchDesc) * (size_t)nb, hipMemcpyHostToDevice, s));
-    static thread_local std::vector<uint32_t> tile_batch;
-    tile_batch.resize((size_t)tiles);
+    // descriptors and the tile -> batch map go up from page-locked buffers (descs may already be
+    // the handle's own page-locked buffer: the batched entry point describes into it)
+    const size_t dbytes = sizeof(BatchDesc) * (size_t)nb;
+    if (descs != h.desc_host.as<BatchDesc>()) {
+        VR_TRY(h.desc_host.acquire(dbytes));
+        std::memcpy(h.desc_host.p, descs, dbytes);
+    }
+    VR_TRY(h.batch_descs.ensure(h.dev, dbytes));
+    VR_TRY(h.desc_host.copy_to(h.batch_descs.p, dbytes, s));
+    VR_TRY(h.tile_host.acquire(sizeof(uint32_t) * (size_t)std::max<int64_t>(tiles, 1)));
+    uint32_t* tile_batch = h.tile_host.as<uint32_t>();
     for (int b = 0; b < nb; b++) {
         const int64_t t1 = b + 1 < nb ? descs[b + 1].tile_begin : tiles;
-        for (int64_t t = descs[b].tile_begin; t < t1; t++) tile_batch[(size_t)t] = (uint32_t)b;
+        for (int64_t t = descs[b].tile_begin; t < t1; t++) tile_batch[t] = (uint32_t)b;
     }
     VR_TRY(h.p_tile_batch.ensure(h.dev, sizeof(uint32_t) * (size_t)std::max<int64_t>(tiles, 1)));
-    VR_TRY_HIP(hipMemcpyAsync(h.p_tile_batch.p, tile_batch.data(), sizeof(uint32_t) * (size_t)tiles, hipMemcpyHostToDevice, s));
+    VR_TRY(h.tile_host.copy_to(h.p_tile_batch.p, sizeof(uint32_t) * (size_t)tiles, s));
     a.batches = h.batch_descs.as<BatchDesc>();
     a.num_batches = nb;
     a.one = descs[0];
@@ -2301,8 +2334,8 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
     RecLayout layout;
     if (partitioned_applies(h, total, &layout, cfg)) {
         VR_TRY(enter(h, s));
-        static thread_local std::vector<BatchDesc> pdescs;
-        pdescs.resize(nb);
+        VR_TRY(h.desc_host.acquire(sizeof(BatchDesc) * (size_t)nb));  // described in place, copied by DMA
+        BatchDesc* pdescs = h.desc_host.as<BatchDesc>();
         KeyIds ids(s);
         for (int b = 0; b < nb; b++) {
             StagePlan plan;
@@ -2310,7 +2343,7 @@ vr_status vr_agg_update_batches(vr_agg* agg, const vr_batch_set* bs, const vr_la
             VR_TRY(describe(h, bs->keys + (size_t)b * nk, bs->values + (size_t)b * nv, bs->lengths[b], nullptr, pdescs[b],
                             plan, ids.bufs, &mem, s));
         }
-        VR_TRY(update_partitioned(h, pdescs.data(), nb, total, bs->filter_col, bs->filter_threshold, layout, cfg, s));
+        VR_TRY(update_partitioned(h, pdescs, nb, total, bs->filter_col, bs->filter_threshold, layout, cfg, s));
         h.dev->count(&vr_stats::calls, 1);
         h.dev->count(&vr_stats::input_rows, total);
         return leave(h, s);
@@ -2476,7 +2509,12 @@ vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys
         for (int c = 0; c < nk; c++) bind(keys[c], a.keys[c]);
         for (int c = 0; c < no; c++) bind(out[c], a.outs[c]);
     }
-    hipLaunchKernelGGL(agg_emit_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, a);
+    const unsigned eg = (unsigned)((m + EMIT_BLOCK - 1) / EMIT_BLOCK);
+    if (emit_pitch(h.stride) <= EMIT_STAGE_MAX_PITCH)
+        hipLaunchKernelGGL(agg_emit_kernel<true>, dim3(eg), dim3(EMIT_BLOCK),
+                           (size_t)EMIT_BLOCK * emit_pitch(h.stride) * sizeof(uint64_t), s, a);
+    else
+        hipLaunchKernelGGL(agg_emit_kernel<false>, dim3(eg), dim3(EMIT_BLOCK), 0, s, a);
     VR_TRY_HIP(hipGetLastError());
     for (auto& bk : back) {
         VR_TRY_HIP(hipMemcpyAsync(bk.host, h.out_stage.as<uint8_t>() + bk.off, bk.bytes, hipMemcpyDeviceToHost, s));

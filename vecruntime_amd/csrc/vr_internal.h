@@ -112,6 +112,49 @@ struct DevBuf {
     T* as() const { return (T*)p; }
 };
 
+// A page-locked host buffer for descriptors that a call writes and copies to the device: the copy
+// is a DMA (no bounce through the runtime's own staging, as pageable memory takes), and the next
+// call waits on the event recorded after the copy before it writes the buffer again.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipEvent_t copied = nullptr;
+    bool pending = false;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() {
+        if (pending) (void)hipEventSynchronize(copied);
+        if (p) (void)hipHostFree(p);
+        if (copied) (void)hipEventDestroy(copied);
+    }
+    // Waits for the last copy out of the buffer, then grows it (contents dropped) to `want` bytes.
+    vr_status acquire(size_t want) {
+        if (pending) {
+            VR_TRY_HIP(hipEventSynchronize(copied));
+            pending = false;
+        }
+        if (want <= bytes && p) return VR_OK;
+        if (p) VR_TRY_HIP(hipHostFree(p));
+        p = nullptr;
+        bytes = 0;
+        const size_t b = want + want / 4 + 4096;
+        VR_TRY_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+        return VR_OK;
+    }
+    // Queues the copy of the first `n` bytes to `dst` on `s` and marks the buffer busy until it ends.
+    vr_status copy_to(void* dst, size_t n, hipStream_t s) {
+        if (!copied) VR_TRY_HIP(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+        VR_TRY_HIP(hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, s));
+        VR_TRY_HIP(hipEventRecord(copied, s));
+        pending = true;
+        return VR_OK;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
 // A host buffer to stage: its bytes and the device pointer to patch with its staged copy.
 struct StagePiece {
     const void* src;
