@@ -1738,7 +1738,10 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     const uint64_t nblocks = pool_blocks(rows, (uint64_t)P);
     if (nblocks * PART_BLK >= (1ull << 32)) return fail(VR_ERR_UNSUPPORTED, "too many rows for one partitioned update");
     VR_TRY(h.p_pool.ensure(h.dev, nblocks * PART_BLK * L.words * sizeof(uint32_t)));
-    VR_TRY(h.p_count.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
+    // the count pass runs two workgroups per scatter workgroup when two fit on a CU
+    const size_t lds_c = count_lds_bytes(K, H, P);
+    pa.count_split = 2 * (int64_t)lds_c + 2048 <= h.dev->lds_per_block ? 2 : 1;
+    VR_TRY(h.p_count.ensure(h.dev, (size_t)pa.count_split * grid_a * P * sizeof(uint32_t)));
     VR_TRY(h.p_base.ensure(h.dev, (size_t)grid_a * P * sizeof(uint32_t)));
     VR_TRY(h.p_rtot.ensure(h.dev, (size_t)P * sizeof(uint32_t)));
     VR_TRY(h.p_rstart.ensure(h.dev, (size_t)(P + 1) * sizeof(uint32_t)));
@@ -1755,11 +1758,10 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
     pa.pool_blocks = nblocks;
     pa.pool_records = (uint32_t)(nblocks * PART_BLK);
     pa.u = a;
-    const size_t lds_c = count_lds_bytes(K, H, P);
     const size_t lds_a = scatter_lds_bytes(K, H, S, P);
     const size_t lds_b = merge_lds_bytes(E, S);
     if (jit) {
-        if (launch_jit(jit->count, (unsigned)grid_a, PA_BLOCK, lds_c, s, pa) != VR_OK) {
+        if (launch_jit(jit->count, (unsigned)(grid_a * pa.count_split), PA_BLOCK, lds_c, s, pa) != VR_OK) {
             // the module's kernels do not launch here: the generic kernels from the next call on
             (void)hipGetLastError();
             part_jit_disable();
@@ -1767,7 +1769,7 @@ static vr_status run_partitioned(AggImpl& h, UpdateArgs& a, int64_t rows, const 
         }
     } else {
         VR_TRY(allow_lds(part_count_kernel<RuntimeProgs>, lds_c));
-        hipLaunchKernelGGL(part_count_kernel<RuntimeProgs>, dim3(grid_a), dim3(PA_BLOCK), lds_c, s, pa);
+        hipLaunchKernelGGL(part_count_kernel<RuntimeProgs>, dim3(grid_a * pa.count_split), dim3(PA_BLOCK), lds_c, s, pa);
     }
     hipLaunchKernelGGL(part_col_scan_kernel, dim3((unsigned)((P + 63) / 64)), dim3(1024), 0, s, pa);
     hipLaunchKernelGGL(part_region_scan_kernel, dim3(1), dim3(1024), 0, s, pa);

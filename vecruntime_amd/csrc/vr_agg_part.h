@@ -40,14 +40,14 @@ namespace vr {
 // Stand-ins read when a batch has no bitmap, selection or filter, so that every load of a tile is
 // issued whatever the batch holds (no branch between a load and the next).
 __device__ const uint8_t k_all_ones[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF};
-__device__ const uint32_t k_zero_words[4] = {0, 0, 0, 0};
+__device__ __align__(16) const uint32_t k_zero_words[4] = {0, 0, 0, 0};
 
 struct PartArgs {
     UpdateArgs u;
     RecLayout rl;
     ColProg prog[VR_MAX_VALUES];
     int32_t nprog;
-    int32_t pad2;
+    int32_t count_split;    // count-pass workgroups per scatter workgroup (each a share of its tiles)
     int32_t nparts;         // regions = 2^u.t.pbits
     int32_t nhot;           // hot-set entries (unused entries hold EMPTY_KEY)
     int32_t hot_slots;      // LDS index slots of the hot set (power of two >= 2 * nhot)
@@ -60,7 +60,9 @@ struct PartArgs {
     uint64_t* hot_part;        // [grid_a][nhot][stride]
     uint64_t* hot_total;       // [nhot][stride]
     uint32_t* pool;            // block b holds records [b * 64, b * 64 + 64) of rl.words words
-    uint32_t* count;           // [grid_a][nparts] records of each scatter workgroup per region
+    uint32_t* count;           // [count_split][grid_a][nparts] records of each scatter workgroup per
+                               // region, by count workgroup (the column scan leaves the totals in
+                               // the first [grid_a][nparts])
     uint32_t* base;            // [grid_a][nparts] where they start inside the region's records
     uint32_t* rtot;            // [nparts] records per region
     uint32_t* rstart;          // [nparts + 1] first record of each region (a multiple of PART_BLK)
@@ -257,15 +259,30 @@ __global__ __launch_bounds__(256) void hot_pick_kernel(const uint64_t* cand, con
 __host__ __device__ inline int lds_row_words(int stride) { return (stride + 1) | 1; }
 
 // The hot set's LDS index: buckets of HB entries, an entry = 16 bits of the key's hash (bits
-// 32-47, not the bucket's) above the key's number in the hot set.  Entries fill each bucket from
-// its start, so a bucket with an empty entry ends the search; one bucket read answers almost
-// every lookup.
+// 32-47, not the bucket's; never 0xFFFF, so no entry of a key reads as an empty one) above the
+// key's number in the hot set.  Entries fill each bucket from its start, so a bucket with an empty
+// entry ends the search; one bucket read answers almost every lookup.
 constexpr int HB = 8;
-__device__ __forceinline__ uint32_t hot_tag(uint64_t h) { return (uint32_t)(h >> 16) & 0xFFFF0000u; }
+__device__ __forceinline__ uint32_t hot_tag(uint64_t h) {
+    const uint32_t t = (uint32_t)(h >> 16) & 0xFFFF0000u;
+    return t < 0xFFFF0000u ? t : 0xFFFE0000u;
+}
 
-// Number of `key` in the hot set (keys hrow[k * HW]), or HOT_EMPTY.  A bucket's entries are tested
-// without branches (a mask of tag matches, a mask of empty entries); only a tag match reads a key,
-// and a second match in one bucket (a 16-bit tag collision) takes the slow loop.
+// The entry of `key` among bucket b's entries whose tag matches (a tag collision): a call of its
+// own, so that none of its work is hoisted into the common path.
+__device__ __noinline__ uint32_t hot_find_collided(const uint32_t* hidx, const uint64_t* hrow, int HW, uint32_t b,
+                                                   uint32_t tag, uint64_t key) {
+    for (int j = 0; j < HB; j++) {
+        const uint32_t e = hidx[b * HB + j];
+        if ((e ^ tag) < 0x10000u && hrow[(size_t)(e & 0xFFFFu) * HW] == key) return e & 0xFFFFu;
+    }
+    return HOT_EMPTY;
+}
+
+// Number of `key` in the hot set (keys hrow[k * HW]), or HOT_EMPTY.  A bucket's entries are
+// compared without branches (three instructions an entry: the tag's half of the entry against the
+// key's, the last match kept); the one candidate's key is read back, and only a second key of the
+// bucket with the same 16-bit tag (a tag collision) takes the slow loop.
 __device__ __forceinline__ uint32_t hot_find(const uint32_t* hidx, const uint64_t* hrow, int HW, uint32_t bmask,
                                              uint64_t h, uint64_t key) {
     const uint32_t tag = hot_tag(h);
@@ -274,23 +291,17 @@ __device__ __forceinline__ uint32_t hot_find(const uint32_t* hidx, const uint64_
         const uint4* q = (const uint4*)(hidx + b * HB);
         const uint4 x = q[0], y = q[1];
         const uint32_t e[HB] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
-        uint32_t match = 0, empty = 0, cand = HOT_EMPTY;
+        uint32_t cand = HOT_EMPTY;
 #pragma unroll
-        for (int j = HB - 1; j >= 0; j--) {
-            const bool m = e[j] != HOT_EMPTY && (e[j] & 0xFFFF0000u) == tag;
-            match |= (uint32_t)m << j;
-            empty |= (uint32_t)(e[j] == HOT_EMPTY) << j;
-            cand = m ? (e[j] & 0xFFFFu) : cand;  // the first match
+        for (int j = HB - 1; j >= 0; j--) cand = (e[j] ^ tag) < 0x10000u ? e[j] : cand;  // the first match
+        const uint32_t k = cand != HOT_EMPTY ? (cand & 0xFFFFu) : 0u;
+        const bool hit = cand != HOT_EMPTY && hrow[(size_t)k * HW] == key;
+        if (hit) return k;
+        if (cand != HOT_EMPTY) {  // a tag collision: every matching entry's key
+            const uint32_t c = hot_find_collided(hidx, hrow, HW, b, tag, key);
+            if (c != HOT_EMPTY) return c;
         }
-        if (match) {
-            if (hrow[(size_t)cand * HW] == key) return cand;
-            if (match & (match - 1)) {
-#pragma unroll
-                for (int j = 0; j < HB; j++)
-                    if (((match >> j) & 1u) && hrow[(size_t)(e[j] & 0xFFFFu) * HW] == key) return e[j] & 0xFFFFu;
-            }
-        }
-        if (empty) return HOT_EMPTY;
+        if (e[HB - 1] == HOT_EMPTY) return HOT_EMPTY;  // the bucket has room: the key is not in the set
         b = (b + 1) & bmask;
     }
     return HOT_EMPTY;
@@ -317,6 +328,50 @@ __device__ __forceinline__ void build_hot_index(uint32_t* hidx, const uint64_t* 
 __device__ __forceinline__ const BatchDesc& tile_desc(const PartArgs& a, int64_t tile) {
     const __attribute__((address_space(4))) uint32_t* tb = (const __attribute__((address_space(4))) uint32_t*)a.tile_batch;
     return batch_at(a.u.batches, tb[tile]);
+}
+
+// A lane array of a tile's rows, addressed from the tile's first row: a uniform base (scalar
+// registers) and a 32-bit offset per row, so that each load is one instruction with no 64-bit
+// address arithmetic per row.  An array the batch lacks reads its stand-in at offset 0 (mask 0).
+// Rows are clamped to the tile's last row inside the batch, so every offset stays inside the
+// batch's array (and a bitmap's byte offset is the row's >> 3: tiles start on multiples of 8 rows).
+struct TileLane {
+    const uint8_t* base;
+    uint32_t mask;
+};
+__device__ __forceinline__ TileLane tile_lane(const void* p, bool on, const void* stand_in, int64_t first_byte) {
+    TileLane t;
+    t.base = on ? (const uint8_t*)p + first_byte : (const uint8_t*)stand_in;
+    t.mask = on ? ~0u : 0u;
+    return t;
+}
+__device__ __forceinline__ uint32_t tl_u8(const TileLane& t, uint32_t off) { return *gptr(t.base + (off & t.mask)); }
+__device__ __forceinline__ uint32_t tl_u32(const TileLane& t, uint32_t off) {
+    return *gptr((const uint32_t*)(t.base + (off & t.mask)));
+}
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 tl_u64(const TileLane& t, uint32_t off) {
+    const u32x2_t v = *gptr((const u32x2_t*)(t.base + (off & t.mask)));
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 tl_u128(const TileLane& t, uint32_t off) {
+    const u32x4_t v = *gptr((const u32x4_t*)(t.base + (off & t.mask)));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+// The words of a value lane of `width` bytes (0: none, validity only) at row offset lr.
+__device__ __forceinline__ uint4 tl_value(const TileLane& t, int width, uint32_t lr) {
+    uint4 x = make_uint4(0u, 0u, 0u, 0u);
+    if (width == 16) {
+        x = tl_u128(t, lr * 16u);
+    } else if (width == 8) {
+        const uint2 y = tl_u64(t, lr * 8u);
+        x.x = y.x;
+        x.y = y.y;
+    } else if (width == 4) {
+        x.x = tl_u32(t, lr * 4u);
+    }
+    return x;
 }
 
 // Whether row i of a tile takes part: inside its batch, selected, and passing the fused filter.
@@ -346,12 +401,14 @@ __host__ __device__ inline size_t count_lds_bytes(int nhot, int hot_slots, int n
     return (size_t)nhot * 8 + (size_t)hot_slots * 4 + (size_t)nparts * 4;
 }
 
-// Phase A, first pass: the records each scatter workgroup will write to each region.  Workgroup w
-// walks the tiles scatter workgroup w walks and classifies their rows as the scatter does, reading
-// only the keys, the selection and the filter lane; count[w * nparts + p] = its rows that become
-// records of region p.  Rows of the reserved rows (null keys, the empty marker) are folded here.
+// Phase A, first pass: the records each scatter workgroup will write to each region.  Workgroup
+// c = sub * grid_a + w walks every count_split-th of the tiles scatter workgroup w walks (from its
+// sub-th on) and classifies their rows as the scatter does, reading only the keys, the selection
+// and the filter lane; count[c * nparts + p] = its rows that become records of region p.  Rows of
+// the reserved rows (null keys, the empty marker) are folded here.  Its LDS and registers allow two
+// workgroups per CU, so that twice the scatter's waves have loads in flight.
 template <class PS>
-__global__ __launch_bounds__(PA_BLOCK) void part_count_kernel(PartArgs a) {
+__global__ __launch_bounds__(PA_BLOCK, 8) void part_count_kernel(PartArgs a) {
     constexpr int MR = PA_JIT_RPT;
     extern __shared__ uint64_t lds[];
     const UpdateArgs& u = a.u;
@@ -373,11 +430,20 @@ __global__ __launch_bounds__(PA_BLOCK) void part_count_kernel(PartArgs a) {
     const int pbits = u.t.pbits;
     const bool fast_key = PS::fast_key(a);
     const bool has_filter = u.filter_col >= 0;
-    for (int64_t tile = u.tile_lo + blockIdx.x; tile < u.total_tiles; tile += gridDim.x) {
+    const int64_t G = a.grid_a;
+    const int64_t w = blockIdx.x % a.grid_a, sub = blockIdx.x / a.grid_a;
+    for (int64_t tile = u.tile_lo + w + G * sub; tile < u.total_tiles; tile += G * a.count_split) {
         const BatchDesc& b = tile_desc(a, tile);
         const int64_t row0 = (tile - b.tile_begin) * T;
+        const int64_t left = b.n - row0;  // >= 1: a tile holds rows of its batch
+        const uint32_t last = (uint32_t)(left < T ? left : T) - 1u;
         const DCol& fcol = b.vals[has_filter ? u.filter_col : 0];
         const DCol& kcol = b.keys[0];
+        const TileLane sel = tile_lane(b.sel, b.sel != nullptr, k_all_ones, row0 >> 3);
+        const TileLane fvalid = tile_lane(fcol.valid, has_filter && fcol.valid, k_all_ones, row0 >> 3);
+        const TileLane fdata = tile_lane(fcol.data, has_filter, k_zero_words, row0 * 4);
+        const TileLane kvalid = tile_lane(kcol.valid, fast_key && kcol.valid, k_all_ones, row0 >> 3);
+        const TileLane kdata = tile_lane(kcol.data, fast_key, k_zero_words, row0 * 8);
         uint32_t sbyte[MR], fbyte[MR], kbyte[MR], klo[MR], khi[MR];
         int32_t fval[MR];
 #pragma unroll
@@ -385,15 +451,14 @@ __global__ __launch_bounds__(PA_BLOCK) void part_count_kernel(PartArgs a) {
             sbyte[r] = fbyte[r] = kbyte[r] = klo[r] = khi[r] = 0;
             fval[r] = 0;
             if (r >= RPT) continue;
-            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-            const int64_t ci = i < b.n ? i : b.n - 1;
-            sbyte[r] = *gptr(b.sel ? b.sel + (ci >> 3) : k_all_ones);
-            fbyte[r] = *gptr(has_filter && fcol.valid ? fcol.valid + (ci >> 3) : k_all_ones);
-            fval[r] = (int32_t)*gptr(has_filter ? (const uint32_t*)fcol.data + ci : k_zero_words);
-            const uint32_t* kp = fast_key ? (const uint32_t*)kcol.data + 2 * ci : k_zero_words;
-            kbyte[r] = *gptr(fast_key && kcol.valid ? kcol.valid + (ci >> 3) : k_all_ones);
-            klo[r] = *gptr(kp);
-            khi[r] = *gptr(kp + 1);
+            const uint32_t lr = min((uint32_t)(r * PA_BLOCK) + threadIdx.x, last);
+            sbyte[r] = tl_u8(sel, lr >> 3);
+            fbyte[r] = tl_u8(fvalid, lr >> 3);
+            fval[r] = (int32_t)tl_u32(fdata, lr * 4u);
+            kbyte[r] = tl_u8(kvalid, lr >> 3);
+            const uint2 kw = tl_u64(kdata, lr * 8u);
+            klo[r] = kw.x;
+            khi[r] = kw.y;
         }
 #pragma unroll
         for (int r = 0; r < MR; r++) {
@@ -430,25 +495,32 @@ __global__ __launch_bounds__(PA_BLOCK) void part_count_kernel(PartArgs a) {
 }
 
 // base[w * nparts + p] = records of region p that the scatter workgroups before w write (a scan
-// down each column of count); rtot[p] = the column's total.  A workgroup takes 64 regions, each of
-// its 16 waves a sixteenth of the workgroups.
+// down each column of count, the count workgroups of one scatter workgroup summed first, their sum
+// left in count[w * nparts + p]); rtot[p] = the column's total.  A workgroup takes 64 regions, each
+// of its 16 waves a sixteenth of the scatter workgroups.
 __global__ __launch_bounds__(1024) void part_col_scan_kernel(PartArgs a) {
     __shared__ uint32_t part[16][64];
-    const int G = a.grid_a, P = a.nparts;
+    const int G = a.grid_a, P = a.nparts, CS = a.count_split;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
     const int per = (G + 15) / 16;
     const int w0 = wv * per < G ? wv * per : G, w1 = w0 + per < G ? w0 + per : G;
+    auto total = [&](int w) {
+        uint32_t c = 0;
+        for (int q = 0; q < CS; q++) c += a.count[((size_t)q * G + w) * P + p];
+        return c;
+    };
     uint32_t s = 0;
     if (p < P)
-        for (int w = w0; w < w1; w++) s += a.count[(size_t)w * P + p];
+        for (int w = w0; w < w1; w++) s += total(w);
     part[wv][lane] = s;
     __syncthreads();
     uint32_t off = 0;
     for (int j = 0; j < wv; j++) off += part[j][lane];
     if (p >= P) return;
     for (int w = w0; w < w1; w++) {
-        const uint32_t c = a.count[(size_t)w * P + p];
+        const uint32_t c = total(w);
+        if (CS > 1) a.count[(size_t)w * P + p] = c;
         a.base[(size_t)w * P + p] = off;
         off += c;
     }
@@ -571,30 +643,43 @@ __global__ __launch_bounds__(PA_BLOCK) void part_scatter_kernel(PartArgs a) {
         uint32_t klo[MR], khi[MR];
         const DCol& fcol = b.vals[has_filter ? u.filter_col : 0];
         const DCol& kcol = b.keys[0];
+        const int64_t left = b.n - row0;  // >= 1: a tile holds rows of its batch
+        const uint32_t last = (uint32_t)(left < T ? left : T) - 1u;
+        const TileLane sel = tile_lane(b.sel, b.sel != nullptr, k_all_ones, row0 >> 3);
+        const TileLane fvalid = tile_lane(fcol.valid, has_filter && fcol.valid, k_all_ones, row0 >> 3);
+        const TileLane fdata = tile_lane(fcol.data, has_filter, k_zero_words, row0 * 4);
+        // the 64-bit key lane (other key modes read their lanes while classifying)
+        const TileLane kvalid = tile_lane(kcol.valid, fast_key && kcol.valid, k_all_ones, row0 >> 3);
+        const TileLane kdata = tile_lane(kcol.data, fast_key, k_zero_words, row0 * 8);
+        TileLane vvalid[PART_MAX_PROG], vdata[PART_MAX_PROG];
+        int vwidth[PART_MAX_PROG];
+#pragma unroll
+        for (int q = 0; q < PART_MAX_PROG; q++) {
+            const bool on = q < PS::nprog(a);
+            const ColProg& cp = PS::prog(a, on ? q : 0);
+            const DCol& d = b.vals[cp.col];
+            vwidth[q] = on ? cp.width : 0;
+            vvalid[q] = tile_lane(d.valid, on && d.valid, k_all_ones, row0 >> 3);
+            vdata[q] = tile_lane(d.data, vwidth[q] != 0, k_zero_words, row0 * vwidth[q]);
+        }
 #pragma unroll
         for (int r = 0; r < MR; r++) {
-            const int64_t i = row0 + (int64_t)r * PA_BLOCK + threadIdx.x;
-            const int64_t ci = i < b.n ? i : b.n - 1;
-            sbyte[r] = *gptr(b.sel ? b.sel + (ci >> 3) : k_all_ones);
-            fbyte[r] = *gptr(has_filter && fcol.valid ? fcol.valid + (ci >> 3) : k_all_ones);
-            fval[r] = (int32_t)*gptr(has_filter ? (const uint32_t*)fcol.data + ci : k_zero_words);
-            // the 64-bit key lane (other key modes read their lanes while classifying)
-            const uint32_t* kp = fast_key ? (const uint32_t*)kcol.data + 2 * ci : k_zero_words;
-            kbyte[r] = *gptr(fast_key && kcol.valid ? kcol.valid + (ci >> 3) : k_all_ones);
-            klo[r] = *gptr(kp);
-            khi[r] = *gptr(kp + 1);
+            const uint32_t lr = min((uint32_t)(r * PA_BLOCK) + threadIdx.x, last);
+            sbyte[r] = tl_u8(sel, lr >> 3);
+            fbyte[r] = tl_u8(fvalid, lr >> 3);
+            fval[r] = (int32_t)tl_u32(fdata, lr * 4u);
+            kbyte[r] = tl_u8(kvalid, lr >> 3);
+            const uint2 kw = tl_u64(kdata, lr * 8u);
+            klo[r] = kw.x;
+            khi[r] = kw.y;
 #pragma unroll
             for (int q = 0; q < PART_MAX_PROG; q++) {
-                const bool on = q < PS::nprog(a);
-                const ColProg& cp = PS::prog(a, on ? q : 0);
-                const DCol& d = b.vals[cp.col];
-                vbyte[r][q] = *gptr(on && d.valid ? d.valid + (ci >> 3) : k_all_ones);
-                const int wd = on ? cp.width : 0;
-                const uint32_t* vp = wd ? (const uint32_t*)((const uint8_t*)d.data + ci * wd) : k_zero_words;
-                lo[r][q] = *gptr(vp);
-                hi[r][q] = *gptr(vp + (wd >= 8 ? 1 : 0));
-                w2[r][q] = *gptr(vp + (wd >= 16 ? 2 : 0));
-                w3[r][q] = *gptr(vp + (wd >= 16 ? 3 : 0));
+                vbyte[r][q] = tl_u8(vvalid[q], lr >> 3);
+                const uint4 x = tl_value(vdata[q], vwidth[q], lr);
+                lo[r][q] = x.x;
+                hi[r][q] = x.y;
+                w2[r][q] = x.z;
+                w3[r][q] = x.w;
             }
         }
         // The rows' fates, with no branch on the row: hot entry (hot_m), record (rec_m), neither.
@@ -973,8 +1058,16 @@ __global__ __launch_bounds__(PB_BLOCK) void part_merge_kernel(PartArgs a) {
     // last one partly filled), handed out to the waves as they ask with an LDS counter: the waves'
     // work evens out whatever the records' keys.
     __shared__ unsigned int s_gnext;
-    const uint32_t rblock0 = a.rstart[p] / PART_BLK, rrecs = a.rtot[p];
-    const uint32_t rblocks = (rrecs + PART_BLK - 1) / PART_BLK;
+    const uint32_t rblock0 = a.rstart[p] / PART_BLK;
+    uint32_t rrecs = a.rtot[p];
+    uint32_t rblocks = (rrecs + PART_BLK - 1) / PART_BLK;
+    if ((uint64_t)rblock0 + rblocks > a.pool_blocks) {
+        // ranges past the pool (the counts disagreeing with the pool's bound, never expected): the
+        // blocks inside it are read, and the launch fails
+        if (threadIdx.x == 0) atomicOr((unsigned long long*)&a.u.t.counters[C_ERROR], (unsigned long long)ERR_INTERNAL);
+        rblocks = rblock0 < a.pool_blocks ? (uint32_t)(a.pool_blocks - rblock0) : 0u;
+        rrecs = rblocks * PART_BLK;
+    }
     // the next block for this wave (NO_BLOCK: none) and its record count
     auto fetch = [&](uint32_t* n) -> uint32_t {
         uint32_t i = 0;
