@@ -27,12 +27,11 @@ BATCH = 65536
 # the uneven batched workload's columns: filter int32, keys int32 + int64 + int32 dictionary code,
 # decimal(15,2) as int64, double (every row's line is read at 30% selectivity)
 UNEVEN_BYTES_PER_ROW = 4 + 4 + 8 + 4 + 8 + 8
-# HBM bytes per launch of the batched update kernel on the 200M-row group-by, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs scaled by the same counters on a 1 GiB device copy
-# (profiles/agg_update_groupby200M_pmc_traffic.md).  Measured for the kernel as committed with it.
-# HBM bytes moved by one partitioned update of the group-by (sampling, scatter, hot reduce, merge),
-# from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a device copy of known size
-PMC_TRAFFIC_BYTES = {200_000_000: 12.10e9}
+# HBM bytes moved by one partitioned update of the 200M-row group-by (sampling, count pass, scans,
+# scatter, hot reduce, merge), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the same
+# counters on a 1 GiB device copy (profiles/partitioned_update_groupby200M_pmc_traffic.md).
+# Measured for the kernels as committed with it.
+PMC_TRAFFIC_BYTES = {200_000_000: 12.93e9}
 
 
 def log(*a):
@@ -1230,8 +1229,9 @@ def main():
                    "groups": gb["groups"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get(n),
-                     "kernel": "partitioned update: hot_sample_kernel, part_scatter_kernel, hot_reduce_kernel, "
-                               "part_merge_kernel", "kernel_ms_per_step": gb["update_kernel_ms"],
+                     "kernel": "partitioned update: hot_sample_kernel, part_count_kernel, part_col_scan_kernel, "
+                               "part_region_scan_kernel, part_scatter_kernel, hot_reduce_kernel, part_merge_kernel",
+                     "kernel_ms_per_step": gb["update_kernel_ms"],
                      "launches_per_step": gb["update_launches_per_step"],
                      "avg_launch_ms": gb["update_kernel_avg_launch_ms"], "algorithmic_bytes_per_step": alg_bytes},
         "cpu_baseline": cpu,
