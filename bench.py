@@ -158,27 +158,29 @@ def verify_groupby(keys, dec, dbl, i32, valid, n, groups, ek, eo):
 
 def tune_groupby(dev, args, batches, stream):
     """The autotuner's descent over the partitioned path's knobs and the load factor, each
-    configuration timed (median of 3) on the first quarter of the batches -- outside the timed
-    steps; cached per device and workload."""
+    configuration timed (median of 3) on what a step does to the table -- its reset and the update
+    of every batch (the load factor sets the table's size, which the reset pays for) -- outside the
+    timed steps; cached per device and workload."""
     import torch
     from vecruntime_amd import autotune, vr
-    sample = vr.GroupedAggregate.prepare_batches(batches[: max(1, len(batches) // 4)])
-    srows = sum(b[2] for b in batches[: max(1, len(batches) // 4)])
+    sample = vr.GroupedAggregate.prepare_batches(batches)
+    srows = sum(b[2] for b in batches)
     handles = {}
 
     def run(c):
         m = c["max_load_percent"]
         if m not in handles:
             handles[m] = vr.GroupedAggregate(dev, groupby_spec(0, m))
+            handles[m].update_batches(sample, stream=stream)  # grows the table to the workload's groups
         h = handles[m]
-        h.reset()
         torch.cuda.synchronize()
         t = time.perf_counter()
+        h.reset()
         h.update_batches(sample, config=autotune.to_launch_config(c), stream=stream)
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
-    key = autotune.signature(torch.cuda.get_device_properties(0).name, groupby_spec(0), "groupby-zipf1.1-10M")
+    key = autotune.signature(torch.cuda.get_device_properties(0).name, groupby_spec(0), "groupby-zipf1.1-10M-step")
     t0 = time.time()
     best, trials = autotune.tuned(key, run, autotune.SPACES["partitioned"], reps=3, budget_s=args.tune_budget)
     for h in handles.values():
@@ -494,25 +496,31 @@ def run_uneven(dev, args):
     passing = int((cols[0] < thr).sum().item())
     log("generated %d rows in %d batches (%d pass the filter) in %.1f s" % (total, len(batches), passing, time.time() - t0))
     stream = torch.cuda.current_stream()
-    # autotune the batched launch on a sample (the first ~6% of the batches)
-    sample = batches[: max(8, len(batches) // 16)]
+    # autotune the batched launch on what a step does: the table's reset (its size follows the load
+    # factor) and the update of every batch, on a handle sized as the timed ones; one handle at a
+    # time (at 35% load the table takes 96 GB)
+    sample = batches
     srows = sum(b[2] for b in sample)
     handles = {}
 
     def run(c):
         m = c["max_load_percent"]
         if m not in handles:
-            handles[m] = vr.GroupedAggregate(dev, uneven_spec(int(srows * 0.3), m))
+            for h in handles.values():
+                h.close()
+            handles.clear()
+            torch.cuda.synchronize()
+            handles[m] = vr.GroupedAggregate(dev, uneven_spec(int(passing * 0.8), m))
         h = handles[m]
-        h.reset()
         torch.cuda.synchronize()
         t = time.perf_counter()
+        h.reset()
         h.update_batches(sample, filter_col=0, threshold=thr, config=autotune.to_launch_config(c), stream=stream)
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
     props = torch.cuda.get_device_properties(0)
-    key = autotune.signature(props.name, uneven_spec(0), "uneven-filter30")
+    key = autotune.signature(props.name, uneven_spec(0), "uneven-filter30-step")
     t0 = time.time()
     best, results = autotune.tuned(key, run, autotune.SPACES["persistent"], reps=3, budget_s=args.tune_budget)
     for h in handles.values():

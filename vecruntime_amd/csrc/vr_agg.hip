@@ -1197,7 +1197,9 @@ static TableView view_of(AggImpl& h) {
     t.cells = h.cells.as<uint64_t>();
     t.state = h.state.as<uint64_t>();
     t.counters = h.counters.as<uint64_t>();
-    t.rgroups = h.rgroups.as<uint32_t>();
+    // groups per region are read by the partitioned merge only, which wide keys never take: their
+    // inserts skip the count (one device atomic per new group on a few hundred counters)
+    t.rgroups = h.proto.ks.mode == KM_WIDE ? nullptr : h.rgroups.as<uint32_t>();
     t.slots = h.slots;
     t.mask = h.slots - 1;
     t.pbits = h.pbits;
