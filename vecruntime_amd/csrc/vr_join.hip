@@ -574,6 +574,8 @@ __host__ __device__ inline size_t pay_lds(int words) { return (size_t)PROBE_ITEM
 
 constexpr uint64_t ST_AGG = 1ull << 62, ST_PREFIX = 2ull << 62, ST_MASK = 3ull << 62;
 constexpr int PROBE_WAVES = PROBE_BLOCK / 64;
+// Read in place of a selection or validity bitmap a batch does not have (every bit set).
+__device__ const uint8_t k_probe_ones[8] = {0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF};
 
 // Exclusive prefix sum of v over the wave's lanes; *total = the wave's sum.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
@@ -666,22 +668,62 @@ __global__ __launch_bounds__(PROBE_BLOCK) void join_probe_kernel(ProbeArgs a) {
     // the keys of all the thread's rows first: their loads overlap
     uint64_t key[PROBE_ITEMS];
     uint32_t flags = 0;  // bit r: row r is in (selected), bit 16 + r: its key is non-null
+    if (a.t.wide) {
 #pragma unroll
-    for (int r = 0; r < PROBE_ITEMS; r++) {
-        const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
-        const bool in = i < B.n && (!B.sel || bit_at(B.sel, i));
-        key[r] = 0;
-        bool nonnull;
-        if (a.t.wide) {  // the key's hash; a null lane makes the key null
-            nonnull = in;
+        for (int r = 0; r < PROBE_ITEMS; r++) {
+            const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+            const bool in = i < B.n && (!B.sel || bit_at(B.sel, i));
+            key[r] = 0;
+            bool nonnull = in;  // the key's hash; a null lane makes the key null
             for (int c = 0; c < a.nk; c++)
                 if (pv[c] && !bit_at(pv[c], i)) nonnull = false;
             if (nonnull) key[r] = wide_hash(pd, a.bk.width, a.nk, a.bk.seed, i);
-        } else {
-            nonnull = in && probe_key(k0, k1, a.ctype[0], a.nk, i, &key[r]);
+            flags |= (in ? 1u : 0u) << r;
+            flags |= (nonnull ? 1u : 0u) << (16 + r);
         }
-        flags |= (in ? 1u : 0u) << r;
-        flags |= (nonnull ? 1u : 0u) << (16 + r);
+    } else {
+        // One INT32/INT64/FLOAT64 key or two INT32 keys (probe_key's cases): every load of the
+        // rows issued before any is used, with no branch between them -- the row clamped into the
+        // batch, bitmaps the batch lacks read from a constant byte (offset masked to 0), the second
+        // word of a one-lane key read from its own lane -- so the tile waits for one round trip,
+        // not for a load per bitmap and row.
+        const int64_t n = B.n;  // >= 1: a tile holds rows of its batch
+        const bool two = a.nk > 1;
+        const bool w8 = !two && key_lane_width(a.ctype[0]) == 8;
+        const uint8_t* sp = B.sel ? B.sel : k_probe_ones;
+        const int64_t sm = B.sel ? -1 : 0;
+        const uint8_t* v0 = k0.valid ? k0.valid : k_probe_ones;
+        const int64_t m0 = k0.valid ? -1 : 0;
+        const uint8_t* v1 = two && k1.valid ? k1.valid : k_probe_ones;
+        const int64_t m1 = two && k1.valid ? -1 : 0;
+        const uint32_t* d0 = (const uint32_t*)k0.data;
+        const uint32_t* d1 = two ? (const uint32_t*)k1.data : d0;
+        uint32_t sb[PROBE_ITEMS], b0[PROBE_ITEMS], b1[PROBE_ITEMS], lo[PROBE_ITEMS], hi[PROBE_ITEMS];
+#pragma unroll
+        for (int r = 0; r < PROBE_ITEMS; r++) {
+            const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+            const int64_t ci = i < n ? i : n - 1;
+            sb[r] = *gptr(sp + ((ci >> 3) & sm));
+            b0[r] = *gptr(v0 + ((ci >> 3) & m0));
+            b1[r] = *gptr(v1 + ((ci >> 3) & m1));
+            const int64_t e0 = w8 ? 2 * ci : ci;
+            lo[r] = ld_stream(d0 + e0);
+            hi[r] = ld_stream(w8 ? d0 + e0 + 1 : d1 + ci);
+        }
+#pragma unroll
+        for (int r = 0; r < PROBE_ITEMS; r++) {
+            const int64_t i = row0 + (int64_t)r * PROBE_BLOCK + threadIdx.x;
+            const int bit = (int)(i & 7);
+            // bitwise, not short-circuit: a load whose value is needed only behind a branch would
+            // be sunk into it (and waited for there)
+            const uint32_t in_bits = (uint32_t)(i < n) & (sb[r] >> bit);
+            const bool in = in_bits & 1u;
+            const bool nonnull = (in_bits & (b0[r] >> bit) & (b1[r] >> bit)) & 1u;
+            const uint64_t k = two ? (((uint64_t)lo[r] << 32) | hi[r]) : (w8 ? ((uint64_t)lo[r] | ((uint64_t)hi[r] << 32)) : lo[r]);
+            key[r] = nonnull ? k : 0;
+            flags |= (in ? 1u : 0u) << r;
+            flags |= (nonnull ? 1u : 0u) << (16 + r);
+        }
     }
     uint32_t cnt[PROBE_ITEMS], off[PROBE_ITEMS];
     uint32_t payload[PROBE_ITEMS], found[PROBE_ITEMS], npass[PROBE_ITEMS];
