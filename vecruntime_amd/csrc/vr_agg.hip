@@ -1483,6 +1483,12 @@ static void launch_shape(AggImpl& h, const vr_launch_config* cfg, int64_t n, Upd
     *grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
     bool lds = c.lds_preaggregate != 0;
     int E = lds ? std::max(64, c.lds_entries) : 0;
+    if (!multi && lds) {
+        // one batch: each workgroup folds one tile, so the cache it clears and flushes need not
+        // hold more entries than the tile has rows (65,536-row batches: 256 rows, 256 entries, a
+        // fifth less time per call than 512)
+        E = std::min<int64_t>(E, std::max<int64_t>(64, next_pow2((uint64_t)T)));
+    }
     while (E & (E - 1)) E &= E - 1;  // a power of two
     if (E > 2048) E = 2048;
     size_t bytes = (size_t)E * (1 + h.stride) * sizeof(uint64_t);
