@@ -211,6 +211,26 @@ def test_wide_sums_and_decimal_average(dev):
         assert a.values(G) == b.values(G)
 
 
+def test_wide_state_rows_emit_from_global_memory(dev):
+    """State rows wider than the emission stages in LDS (MIN and MAX over four double columns,
+    four words each, two double sums with their null counts, COUNT(*): 37 words) are emitted from
+    global memory; every output equals the oracle's (the doubles are whole numbers, so sums are
+    exact in any order).  The last workgroup of the emission holds fewer than 256 groups."""
+    rng = np.random.default_rng(77)
+    spec = abi.AggSpec([abi.INT64], [abi.FLOAT64] * 4,
+                       [abi.agg_desc(abi.MIN, c) for c in range(4)] + [abi.agg_desc(abi.MAX, c) for c in range(4)] +
+                       [abi.agg_desc(abi.SUM_DOUBLE, 0), abi.agg_desc(abi.SUM_DOUBLE, 3), abi.agg_desc(abi.COUNT_STAR)],
+                       key_nullable=[False])
+    batches = []
+    for n in (1000, 777, 3001):
+        keys = rng.integers(0, 1499, n)
+        vals = [Column.doubles(rng.integers(-1000, 1000, n).astype(np.float64), nulls=rng.random(n) < 0.1)
+                for _ in range(4)]
+        batches.append(([Column.longs(keys)], vals, n, None))
+    g, o = run_both(dev, spec, batches)
+    assert compare(g, o, spec, rel=0.0) == 0.0
+
+
 def test_checked_long_sum_overflow(dev):
     from vecruntime_amd import vr
     big = (1 << 63) - 1
