@@ -2618,7 +2618,8 @@ vr_status vr_agg_reset(vr_agg* agg) {
     h.groups_upper = 0;
     h.rows_since_reset = 0;
     if (h.ungrouped) VR_TRY(insert_single_group(h, s));
-    VR_TRY_HIP(hipStreamSynchronize(s));
+    // no wait here: the next call on any stream is ordered after these fills (enter), and the
+    // host can describe that call's batches while they run
     h.order_count = -1;
     return leave(h, s);
 }
