@@ -24,6 +24,15 @@ LIB_PATH = os.path.join(HERE, "libvrgpu.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError("libvrgpu.so is not built (run __graft_entry__.build() or python -m vecruntime_amd.build)")
 
+try:
+    # PyTorch ships its own HIP runtime and links to it by the unversioned name
+    # (libamdhip64.so), which the dynamic loader does not match with an already loaded
+    # libamdhip64.so.7.  Loaded after this library, torch would start a second runtime that
+    # sees no GPU; loaded first, its runtime is the one this library binds to.
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _P = C.c_void_p
 _lib = C.CDLL(LIB_PATH)
 
