@@ -398,7 +398,8 @@ __global__ __launch_bounds__(256) void count_occupied_kernel(const uint64_t* cel
 }
 
 __global__ __launch_bounds__(256) void write_occupied_kernel(const uint64_t* cells, int W, uint64_t slots,
-                                                             const uint64_t* block_offsets, uint64_t* order) {
+                                                             const uint64_t* block_offsets, uint64_t* order,
+                                                             uint64_t cap) {
     typedef hipcub::BlockScan<uint32_t, 256> Scan;
     __shared__ typename Scan::TempStorage tmp;
     uint64_t base = (uint64_t)blockIdx.x * ORDER_CHUNK;
@@ -408,10 +409,22 @@ __global__ __launch_bounds__(256) void write_occupied_kernel(const uint64_t* cel
         uint32_t flag = (s < slots && occupied(cells, W, s)) ? 1u : 0u;
         uint32_t pos, total;
         Scan(tmp).ExclusiveSum(flag, pos, total);
-        if (flag) order[out + pos] = s;
+        if (flag && out + pos < cap) order[out + pos] = s;
         out += total;
         __syncthreads();
     }
+}
+
+// Checks the order's length (the scan's total) against the group count and appends the reserved
+// rows (the sentinel key's, the null key's) after the occupied slots.
+__global__ void order_tail_kernel(const uint64_t* block_counts, const uint64_t* block_offsets, uint64_t nblocks,
+                                  uint64_t selected, uint64_t* order, uint64_t x0, uint64_t x1, int nx,
+                                  uint64_t* counters) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (block_counts[nblocks - 1] + block_offsets[nblocks - 1] != selected)
+        atomicOr((unsigned long long*)&counters[C_ERROR], (unsigned long long)ERR_ORDER);
+    if (nx > 0) order[selected] = x0;
+    if (nx > 1) order[selected + 1] = x1;
 }
 
 __global__ void first_rows_kernel(const uint64_t* order, int64_t m, const uint64_t* state, int stride, int off,
@@ -1254,6 +1267,7 @@ static vr_status read_counters(AggImpl& h, uint64_t* out, hipStream_t s) {
     if (out[C_ERROR] & ERR_TABLE_FULL)
         return fail(VR_ERR_OOM, "a hash table region is full (keys that do not spread over the table)");
     if (out[C_ERROR] & ERR_INTERNAL) return fail(VR_ERR_DEVICE, "the partitioned update's record counts disagree");
+    if (out[C_ERROR] & ERR_ORDER) return fail(VR_ERR_DEVICE, "the table's occupied slots differ from its group count");
     return VR_OK;
 }
 
@@ -2030,21 +2044,18 @@ static vr_status build_order(AggImpl& h, hipStream_t s) {
     VR_TRY(h.cub_tmp.ensure(h.dev, tmp));
     VR_TRY_HIP(hipcub::DeviceScan::ExclusiveSum(h.cub_tmp.p, tmp, counts, offsets, (int)nblocks, s));
     hipLaunchKernelGGL(write_occupied_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, h.cells.as<uint64_t>(), W,
-                       h.slots, offsets, h.order.as<uint64_t>());
+                       h.slots, offsets, h.order.as<uint64_t>(), (uint64_t)(g + 2));
+    // the occupied slots are the groups counted at insertion; the order's own count is checked on
+    // the device (a difference fails the emission's closing read of the counters), with no wait here
+    const uint64_t selected = c[C_GROUPS];
+    uint64_t extra[2] = {0, 0};
+    int nx = 0;
+    if (c[C_RESERVED] & RES_SENTINEL) extra[nx++] = h.slots;
+    if (c[C_RESERVED] & RES_NULL) extra[nx++] = h.slots + 1;
+    hipLaunchKernelGGL(order_tail_kernel, dim3(1), dim3(64), 0, s, counts, offsets, nblocks, selected,
+                       h.order.as<uint64_t>(), extra[0], extra[1], nx, h.counters.as<uint64_t>());
     VR_TRY_HIP(hipGetLastError());
-    uint64_t tail[2] = {0, 0};
-    VR_TRY_HIP(hipMemcpyAsync(tail, counts + nblocks - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    VR_TRY_HIP(hipMemcpyAsync(tail + 1, offsets + nblocks - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    VR_TRY_HIP(hipStreamSynchronize(s));
-    uint64_t selected = tail[0] + tail[1];
-    std::vector<uint64_t> extra;
-    if (c[C_RESERVED] & RES_SENTINEL) extra.push_back(h.slots);
-    if (c[C_RESERVED] & RES_NULL) extra.push_back(h.slots + 1);
-    if (!extra.empty())
-        VR_TRY_HIP(hipMemcpy(h.order.as<uint64_t>() + selected, extra.data(), extra.size() * sizeof(uint64_t),
-                             hipMemcpyHostToDevice));
-    int64_t m = (int64_t)selected + (int64_t)extra.size();
-    if (m != g) return fail(VR_ERR_DEVICE, "group count mismatch: table holds " + std::to_string(m) + ", counted " + std::to_string(g));
+    const int64_t m = (int64_t)selected + nx;  // == g
     if (h.det && m > 1) {
         int off = -1;
         for (int k = 0; k < h.proto.nops; k++)
@@ -2063,7 +2074,8 @@ static vr_status build_order(AggImpl& h, hipStream_t s) {
         std::swap(h.order.p, h.order_tmp.p);
         std::swap(h.order.bytes, h.order_tmp.bytes);
     }
-    VR_TRY_HIP(hipStreamSynchronize(s));
+    // no wait: the emission's kernel is queued after these on the same stream, and the emission
+    // ends with a synchronous read of the counters
     h.order_count = m;
     return VR_OK;
 }
@@ -2482,9 +2494,12 @@ vr_status vr_agg_emit(vr_agg* agg, int64_t from, int64_t to, vr_out_column* keys
     hipStream_t s = h.dev->pick(stream);
     VR_TRY(enter(h, s));
     VR_TRY(build_order(h, s));
-    if (from < 0 || to < from || to > h.order_count) return fail(VR_ERR_INVALID_ARGUMENT, "emission range outside the groups");
+    if (from < 0 || to < from || to > h.order_count) {
+        (void)leave(h, s);  // the order's kernels are queued: later calls on other streams wait for them
+        return fail(VR_ERR_INVALID_ARGUMENT, "emission range outside the groups");
+    }
     const int64_t m = to - from;
-    if (m == 0) return VR_OK;
+    if (m == 0) return leave(h, s);
     const int nk = h.ungrouped ? 0 : (int)h.key_types.size(), no = (int)h.out_types.size();
     EmitArgs a{};
     a.t = view_of(h);

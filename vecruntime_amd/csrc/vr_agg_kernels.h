@@ -73,7 +73,8 @@ enum {
     ERR_TABLE_FULL = 8,
     ERR_SUM_OF_OVERFLOW = 16,  // a merged decimal sum that a partial left null or that overflowed
     ERR_WIDE_TOTAL = 32,       // an ungrouped final average whose exact total needs more than 128 bits
-    ERR_INTERNAL = 64          // the partitioned update's count pass and scatter disagreed (never expected)
+    ERR_INTERNAL = 64,         // the partitioned update's count pass and scatter disagreed (never expected)
+    ERR_ORDER = 128            // the emission order's occupied slots differ from the group count (never expected)
 };
 
 // How two partial states of a group combine, word by word.
